@@ -1,0 +1,77 @@
+"""Initial data for the workloads.
+
+:func:`interp` is ``MeshFn::interp`` (``src/MeshFn.cpp:56-145``): a least-squares fit
+of the Legendre basis to samples on the ``(p+1) x (p+1)`` Gauss-Legendre tensor grid
+of each cell's bounding box, vectorised over cells.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .mesh import PolyMesh
+from .spec import bs_of, isentropic_vortex
+
+
+def legendre_basis(deg: int, xi: np.ndarray, eta: np.ndarray) -> np.ndarray:
+    """phi_k(xi, eta) = P_a(xi) P_b(eta), k over a = 0..p, b = 0..p-a
+    (``src/Legendre.h:20-35``).  Returns ``(..., bs)``."""
+    xi = np.asarray(xi, np.float64)
+    eta = np.asarray(eta, np.float64)
+    Px = [np.ones_like(xi), xi]
+    Py = [np.ones_like(eta), eta]
+    for n in range(1, deg):
+        Px.append(((2 * n + 1) * xi * Px[n] - n * Px[n - 1]) / (n + 1))
+        Py.append(((2 * n + 1) * eta * Py[n] - n * Py[n - 1]) / (n + 1))
+    out = []
+    for a in range(deg + 1):
+        for b in range(deg + 1 - a):
+            out.append(Px[a] * Py[b])
+    return np.stack(out, axis=-1)
+
+
+def interp(mesh: PolyMesh, deg: int, fn, nc: int) -> np.ndarray:
+    """Project ``fn(x, y) -> (..., nc)`` onto the DG space; returns the flat
+    coefficient vector ``u[i*bl + c*bs + k]``."""
+    bs = bs_of(deg)
+    q, _ = np.polynomial.legendre.leggauss(deg + 1)
+    qi, qj = np.meshgrid(q, q, indexing="ij")
+    G = legendre_basis(deg, qi.ravel(), qj.ravel())           # (Q, bs)
+    bb = mesh.bb
+    x = (qi.ravel()[None, :] + 1) * bb[:, 2:3] / 2.0 + bb[:, 0:1]
+    y = (qj.ravel()[None, :] + 1) * bb[:, 3:4] / 2.0 + bb[:, 1:2]
+    vals = np.asarray(fn(x, y), np.float64)
+    if nc == 1 and vals.ndim == 2:
+        vals = vals[..., None]
+    coef = np.linalg.lstsq(G, vals.transpose(1, 0, 2).reshape(G.shape[0], -1), rcond=None)[0]
+    coef = coef.reshape(bs, mesh.n_cells, nc).transpose(1, 2, 0)  # (np, nc, bs)
+    return np.ascontiguousarray(coef.reshape(-1))
+
+
+def gaussian(cx=0.5, cy=0.5, a=100.0):
+    return lambda x, y: np.exp(-a * ((x - cx) ** 2 + (y - cy) ** 2))
+
+
+def vortex(beta=5.0, x0=5.0, y0=5.0, uinf=1.0, vinf=1.0, gamma=1.4, t=0.0):
+    return lambda x, y: isentropic_vortex(x, y, t, beta, x0, y0, uinf, vinf, gamma)
+
+
+def sine_density(ux=1.0, uy=0.5, p=1.0, gamma=1.4, amp=0.2):
+    """rho = 1 + amp sin(2 pi x) sin(2 pi y), velocity (ux, uy), pressure p."""
+    def f(x, y):
+        rho = 1 + amp * np.sin(2 * math.pi * x) * np.sin(2 * math.pi * y)
+        return np.stack([rho, rho * ux, rho * uy, p / (gamma - 1) + 0.5 * rho * (ux * ux + uy * uy)], axis=-1)
+    return f
+
+
+def euler_max_speed(u: np.ndarray, deg: int, gamma: float = 1.4) -> float:
+    """max(|v| + c) over the cell means (coefficient 0 of each component)."""
+    bs = bs_of(deg)
+    U = u.reshape(-1, 4, bs)[:, :, 0]
+    rho = U[:, 0]
+    vx = U[:, 1] / rho
+    vy = U[:, 2] / rho
+    P = (gamma - 1) * (U[:, 3] - 0.5 * rho * (vx * vx + vy * vy))
+    c = np.sqrt(gamma * P / rho)
+    return float(np.max(np.hypot(vx, vy) + c))
