@@ -1,0 +1,223 @@
+"""Benchmark: DIRK3 steps/s for compressible Euler, p=3, 65,536-cell periodic Voronoi mesh.
+
+Workload (one "step" = one DIRK3 time step, i.e. three Newton solves on the device):
+  isentropic vortex (strength 5, centre (5,5), free stream (1,1), gamma 1.4) on [0,10]^2,
+  periodic; Voronoi mesh from a 256x256 lattice with uniform jitter 0.3h (seed 3),
+  multicolour-ordered; Legendre p=3 (40x40 blocks); DIRK3 at CFL 20
+  (dt = 20 h / max(|v|+c), h = 10/256); Newton to 5e-13 on the L2 residual norm;
+  flexible GMRES(50) to 1e-10 with block DILU.
+The headline value is the mixed-precision path (DILU factors stored in BF16 with one
+power-of-two scale per block, FP32 accumulation, inside FP64 flexible GMRES); the FP64
+path (FP64 factors, same solver) is timed in the same run and reported beside it,
+with the difference between the two solutions.
+
+Prints ONE JSON line.  Only one GPU is used: --gpus above 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DIRK steps/s, Euler p=3 65k cells: mixed-prec vs FP64 on GPU at 1e-10 resid"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--nx", type=int, default=256, help="lattice size (cells = nx^2)")
+    p.add_argument("--deg", type=int, default=3)
+    p.add_argument("--cfl", type=float, default=20.0)
+    p.add_argument("--restart", type=int, default=50)
+    p.add_argument("--mixed", choices=["bf16", "fp32"], default="bf16")
+    p.add_argument("--jacobian", type=int, default=1, help="1: exact off-diagonal Jacobian, 0: reference form")
+    p.add_argument("--trace", type=int, default=0, help="periodic trace: 0 reference, 1 exact")
+    p.add_argument("--no-fp64", action="store_true")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-nx", type=int, default=128, help="lattice size of the CPU-baseline sample")
+    return p.parse_args()
+
+
+def workload(nx, deg):
+    from cpolyg_amd import setups
+    from cpolyg_amd.mesh import jittered_lattice_seeds, periodic_voronoi
+
+    mesh = periodic_voronoi(jittered_lattice_seeds(nx, nx, 10.0, 10.0, 0.3, 3), 10.0, 10.0)
+    u0 = setups.interp(mesh, deg, setups.vortex(beta=5.0, x0=5.0, y0=5.0, uinf=1.0, vinf=1.0, gamma=1.4), 4)
+    return mesh, u0
+
+
+def algorithmic_bytes(n, nnzb, bl, elem, scales):
+    """matvec: blocks + colind + rowptr + x + y;  DILU apply: factor blocks (+ scales) + x in + x out."""
+    N = n * bl
+    matvec = nnzb * bl * bl * 8 + nnzb * 4 + (n + 1) * 4 + 2 * N * 8
+    apply = nnzb * bl * bl * elem + (nnzb * 4 if scales else 0) + 2 * N * 8
+    return matvec, apply
+
+
+def run_path(E, u0, dt, steps, warmup, solver, timing):
+    import torch
+    from cpolyg_amd import hip
+
+    S = hip.Stepper(E, 0, 3, solver)
+    stream = torch.cuda.current_stream()
+    u = torch.from_numpy(u0).cuda()
+    nxt = torch.empty_like(u)
+    t = 0.0
+    for _ in range(warmup):
+        S.advance(u, dt, t, nxt, stream)
+        u, nxt = nxt, u
+        t += dt
+    if timing:
+        S.enable_kernel_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(steps):
+        st = S.advance(u, dt, t, nxt, stream)
+        stats.append(st.as_dict())
+        u, nxt = nxt, u
+        t += dt
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kt = S.kernel_timing() if timing else None
+    return u.cpu().numpy(), wall, stats, kt
+
+
+def cpu_baseline(nx, deg, cfl, counts, n_target):
+    """Oracle (single-threaded C port of the reference algorithm) timed per phase on a
+    bounded sample of the same workload at nx^2 cells, scaled linearly to n_target cells
+    and composed with the per-step phase counts of the GPU run."""
+    import numpy as np
+
+    import oracle as O
+    from cpolyg_amd import setups
+    from cpolyg_amd.spec import EquationSpec
+
+    mesh, u = workload(nx, deg)
+    orc = O.Oracle(mesh)
+    spec = EquationSpec.euler(periodic_trace=counts["trace"], jacobian=counts["jacobian"])
+    h = 10.0 / nx
+    dt = cfl * h / setups.euler_max_speed(u, deg)
+    a = 0.5 * (1 + 0.435866521508459)
+    t0 = time.perf_counter()
+    b = orc.assemble(spec, deg, u, 0.0)
+    t_res = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J = orc.jacobian(spec, deg, u, 0.0)
+    t_jac = time.perf_counter() - t0
+    M = orc.mass(deg, 4)
+    B = -dt * a * J
+    B[orc.rowptr[:-1]] += M
+    t0 = time.perf_counter()
+    O.dilu_factor(orc.rowptr, orc.colind, B)
+    t_fac = time.perf_counter() - t0
+    it = 6
+    t0 = time.perf_counter()
+    O.gmres(orc.rowptr, orc.colind, B, b, m=50, tol=1e-30, maxit=it, pc_kind=2, mode=1)
+    t_gm = (time.perf_counter() - t0 - t_fac) / it
+    scale = n_target / mesh.n_cells
+    step = scale * (counts["residuals"] * t_res + counts["newton"] * (t_jac + t_fac) + counts["gmres"] * t_gm)
+    sample = (f"oracle on the same workload at {mesh.n_cells} cells (lattice {nx}^2): 1 residual "
+              f"{t_res:.2f}s, 1 Jacobian {t_jac:.2f}s, 1 DILU factor {t_fac:.2f}s, {it} FGMRES(50) iterations "
+              f"{t_gm:.3f}s each; scaled x{scale:g} to {n_target} cells and composed with the GPU run's per-step "
+              f"counts ({counts['residuals']:.1f} residuals, {counts['newton']:.1f} Jacobians+factors, "
+              f"{counts['gmres']:.1f} GMRES iterations)")
+    return {"value": 1.0 / step, "unit": "DIRK steps/s", "cores": 1, "kind": "port", "sample": sample,
+            "s_per_step": step}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is described in "
+              f"DESIGN.md and not measured.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from cpolyg_amd import hip, setups
+    from cpolyg_amd.spec import (EquationSpec, GMRES_FLEXIBLE, PC_DILU, PREC_BF16, PREC_FP32, PREC_FP64,
+                                 SolverSpec)
+
+    deg = args.deg
+    mesh, u0 = workload(args.nx, deg)
+    n = mesh.n_cells
+    h = 10.0 / args.nx
+    dt = args.cfl * h / setups.euler_max_speed(u0, deg)
+    spec = EquationSpec.euler(periodic_trace=args.trace, jacobian=args.jacobian)
+    E = hip.Equation(hip.Mesh(mesh), spec, deg)
+    bl, nnzb = E.bl, E.nnzb
+
+    def solver(prec):
+        return SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=args.restart, rtol=1e-10, maxit=20000,
+                          pc_kind=PC_DILU, newton_tol=5e-13, newton_maxit=100, precision=prec)
+
+    mixed_prec = PREC_BF16 if args.mixed == "bf16" else PREC_FP32
+    u_mix, wall_mix, st_mix, kt = run_path(E, u0, dt, args.steps, args.warmup, solver(mixed_prec), True)
+    res = {}
+    if not args.no_fp64:
+        u64, wall64, st64, kt64 = run_path(E, u0, dt, args.steps, args.warmup, solver(PREC_FP64), True)
+        res["fp64_steps_per_s"] = args.steps / wall64
+        res["fp64_ms_per_step"] = 1e3 * wall64 / args.steps
+        res["speedup_mixed_vs_fp64"] = wall64 / wall_mix
+        res["mixed_vs_fp64_max_rel_diff"] = float(np.max(np.abs(u_mix - u64)) / np.max(np.abs(u64)))
+        res["fp64_gmres_iters_per_step"] = sum(s["gmres_iters"] for s in st64) / args.steps
+        res["fp64_newton_iters_per_step"] = sum(s["newton_iters"] for s in st64) / args.steps
+        nm64, msm64, na64, msa64 = kt64
+        res["fp64_apply_ms"] = msa64 / max(na64, 1)
+    steps = args.steps
+    nm, msm, na, msa = kt
+    elem = {PREC_BF16: 2, PREC_FP32: 4}[mixed_prec]
+    b_mv, b_ap = algorithmic_bytes(n, nnzb, bl, elem, mixed_prec == PREC_BF16)
+    mv_ms = msm / max(nm, 1)
+    ap_ms = msa / max(na, 1)
+    gbs = b_mv / (mv_ms * 1e-3) / 1e9
+    newton = sum(s["newton_iters"] for s in st_mix) / steps
+    gm = sum(s["gmres_iters"] for s in st_mix) / steps
+    out = {
+        "metric": METRIC,
+        "value": steps / wall_mix,
+        "unit": "DIRK steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall_mix / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"f64 outer / {args.mixed} preconditioner factors",
+        "data": "synthetic",
+        "config": {"workload": f"Euler isentropic vortex, periodic Voronoi {n} cells ({args.nx}^2 jittered "
+                               f"lattice), p={deg} ({bl}x{bl} blocks), DIRK3 CFL {args.cfl:g}, FGMRES({args.restart}) "
+                               f"rtol 1e-10, block DILU", "cells": n, "deg": deg, "block": bl, "nnzb": nnzb,
+                   "dt": dt, "cfl": args.cfl, "mixed": args.mixed, "offdiag_jacobian": "exact" if args.jacobian else "reference",
+                   "periodic_trace": "exact" if args.trace else "reference", "colours": mesh.n_colours},
+        "roofline": {"bound": "hbm", "kernel": "bsr_matvec (FP64, 40x40 blocks)", "achieved": gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": b_mv, "avg_ms": mv_ms, "launches": nm},
+        "dilu_apply": {"precision": args.mixed, "algorithmic_bytes": b_ap, "avg_ms": ap_ms,
+                       "achieved_gbs": b_ap / (ap_ms * 1e-3) / 1e9 if ap_ms > 0 else None, "launches": na},
+        "newton_iters_per_step": newton,
+        "gmres_iters_per_step": gm,
+        "phase_ms_per_step": {k: sum(s[k] for s in st_mix) / steps for k in
+                              ("ms_residual", "ms_jacobian", "ms_factor", "ms_gmres")},
+        **res,
+    }
+    if not args.no_cpu:
+        counts = {"residuals": newton + 3, "newton": newton, "gmres": gm, "trace": args.trace,
+                  "jacobian": args.jacobian}
+        out["cpu_baseline"] = cpu_baseline(args.cpu_nx, deg, args.cfl, counts, n)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,841 @@
+// DG residual and Jacobian assembly on polygon sub-triangulations.
+//
+// Reference: Equation::assemble / Equation::jacobian (src/Equation.cpp:152-279).
+// The reference re-runs the whole quadrature for every test function (residual)
+// and every (test, trial) pair (Jacobian).  Here each quadrature point is
+// evaluated once per cell: state, flux and flux Jacobian are computed at the point
+// and contracted against all test/trial basis values.
+//
+//   residual  b_i[c*bs+k] = sum_vol W (Fx_c d_x psi_k + Fy_c d_y psi_k)
+//                          - sum_edges sum_q W psi_k Fhat_c(U-, U+, n)
+//   Jacobian  J_ii[(c1,k),(c2,l)] = sum_vol W phi_l (Ax d_x psi_k + Ay d_y psi_k)_{c1c2}
+//                                 - sum_edges sum_q W psi_k phi_l dFhat/dU-_{c1c2}
+//             J_i,nb[(c1,k),(c2,l)] = - sum_q W psi_k phi^nb_l dFhat/dU+_{c1c2}
+//
+// Jacobian blocks are written straight into the BSR slots of the pattern of
+// Jacobian::Jacobian (src/Jacobian.cpp:7-47): the diagonal block first, then one
+// block per non-Dirichlet edge in edge order.  The Newton matrix
+// scale*J + M (B *= -dt a; B += M, src/TimeIntegration.cpp:128-129) is produced in
+// the same pass.
+#include "cpg_internal.hpp"
+#include "dg_device.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace cpg {
+
+// triangle rules, values of src/Quadrature.cpp:126-171 (r, s, weight)
+static const double TRI2[] = {0.16666666666667, 0.16666666666667, 0.33333333333333,
+                              0.16666666666667, 0.66666666666667, 0.33333333333333,
+                              0.66666666666667, 0.16666666666667, 0.33333333333333};
+static const double TRI4[] = {0.44594849091597, 0.44594849091597, 0.22338158967801,
+                              0.44594849091597, 0.10810301816807, 0.22338158967801,
+                              0.10810301816807, 0.44594849091597, 0.22338158967801,
+                              0.09157621350977, 0.09157621350977, 0.10995174365532,
+                              0.09157621350977, 0.81684757298046, 0.10995174365532,
+                              0.81684757298046, 0.09157621350977, 0.10995174365532};
+static const double TRI6[] = {0.24928674517091, 0.24928674517091, 0.11678627572638,
+                              0.24928674517091, 0.50142650965818, 0.11678627572638,
+                              0.50142650965818, 0.24928674517091, 0.11678627572638,
+                              0.06308901449150, 0.06308901449150, 0.05084490637021,
+                              0.06308901449150, 0.87382197101700, 0.05084490637021,
+                              0.87382197101700, 0.06308901449150, 0.05084490637021,
+                              0.31035245103378, 0.63650249912140, 0.08285107561837,
+                              0.63650249912140, 0.05314504984482, 0.08285107561837,
+                              0.05314504984482, 0.31035245103378, 0.08285107561837,
+                              0.63650249912140, 0.31035245103378, 0.08285107561837,
+                              0.31035245103378, 0.05314504984482, 0.08285107561837,
+                              0.05314504984482, 0.63650249912140, 0.08285107561837};
+static const double TRI10[] = {0.33333333333333, 0.33333333333333, 0.09081799038275,
+                               0.48557763338366, 0.48557763338366, 0.03672595775647,
+                               0.48557763338366, 0.02884473323269, 0.03672595775647,
+                               0.02884473323269, 0.48557763338366, 0.03672595775647,
+                               0.10948157548504, 0.10948157548504, 0.04532105943553,
+                               0.10948157548504, 0.78103684902993, 0.04532105943553,
+                               0.78103684902993, 0.10948157548504, 0.04532105943553,
+                               0.30793983876412, 0.55035294182100, 0.07275791684542,
+                               0.55035294182100, 0.14170721941488, 0.07275791684542,
+                               0.14170721941488, 0.30793983876412, 0.07275791684542,
+                               0.55035294182100, 0.30793983876412, 0.07275791684542,
+                               0.30793983876412, 0.14170721941488, 0.07275791684542,
+                               0.14170721941488, 0.55035294182100, 0.07275791684542,
+                               0.24667256063990, 0.72832390459741, 0.02832724253106,
+                               0.72832390459741, 0.02500353476269, 0.02832724253106,
+                               0.02500353476269, 0.24667256063990, 0.02832724253106,
+                               0.72832390459741, 0.24667256063990, 0.02832724253106,
+                               0.24667256063990, 0.02500353476269, 0.02832724253106,
+                               0.02500353476269, 0.72832390459741, 0.02832724253106,
+                               0.06680325101220, 0.92365593358750, 0.00942166696373,
+                               0.92365593358750, 0.00954081540030, 0.00942166696373,
+                               0.00954081540030, 0.06680325101220, 0.00942166696373,
+                               0.92365593358750, 0.06680325101220, 0.00942166696373,
+                               0.06680325101220, 0.00954081540030, 0.00942166696373,
+                               0.00954081540030, 0.92365593358750, 0.00942166696373};
+
+// edge record (host-computed geometry)
+struct Edge {
+    double x1, y1, x2, y2;  // endpoints (polygon edge pv[j] -> pv[j+1])
+    double nx, ny;          // outward normal (centroid sign test, src/PolyMesh.cpp:248-273)
+    double tx, ty;          // periodic translation v[a2] - v[a1] (src/PolyMesh.cpp:307-325)
+    double len;
+    int nb;                 // neighbour cell (interior or periodic partner), -1 Dirichlet
+    int kind;               // 0 interior, 1 periodic, 2 Dirichlet
+    int slot;               // BSR slot of the off-diagonal block, -1 if none
+    int pad;
+};
+
+struct Geo {
+    const double4 *bb;      // x0, y0, w, h
+    const int *vptr;        // volume points per cell
+    const double4 *vpts;    // x, y, W
+    const int *eptr;        // edges per cell
+    const Edge *edges;
+};
+
+// ---------------------------------------------------------------------------------------
+template <int DEG, int NC>
+__device__ inline void outer_state(const DevEq &eq, const Geo &g, const Edge &E, const double *__restrict__ u,
+                                   double x, double y, double t, double *Up, double *phin) {
+    constexpr int BS = Deg<DEG>::BS, BL = NC * BS;
+    if (E.kind == 2) {
+        eval_bc<NC>(eq, x, y, t, Up);
+#pragma unroll
+        for (int k = 0; k < BS; k++) phin[k] = 0.0;
+        return;
+    }
+    const double4 b = g.bb[E.nb];
+    double xi, eta;
+    if (E.kind == 0) {
+        xi = 2.0 * (x - b.x) / b.z - 1.0;
+        eta = 2.0 * (y - b.y) / b.w - 1.0;
+        basis<DEG>(xi, eta, phin, nullptr, nullptr);
+    } else {
+        xi = 2.0 * (x + E.tx - b.x) / b.z - 1.0;
+        eta = 2.0 * (y + E.ty - b.y) / b.w - 1.0;
+        trace_basis<DEG>(eq.periodic_trace, xi, eta, phin);
+    }
+    eval_state<BS, NC>(u + (size_t)E.nb * BL, phin, Up);
+}
+
+// one wavefront per cell; lanes take quadrature points, partial sums reduced by shuffles
+template <int DEG, int NC>
+__global__ __launch_bounds__(64) void k_residual(DevEq eq, Geo g, const double *__restrict__ u, double t,
+                                                 double *__restrict__ b) {
+    constexpr int BS = Deg<DEG>::BS, BL = NC * BS, NQL = Deg<DEG>::NQL;
+    __shared__ double sU[BL];
+    __shared__ double sOut[BL];
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    for (int r = lane; r < BL; r += 64) sU[r] = u[(size_t)i * BL + r];
+    __syncthreads();
+    const double4 bb = g.bb[i];
+    const double sx = 2.0 / bb.z, sy = 2.0 / bb.w;
+    double acc[BL];
+#pragma unroll
+    for (int r = 0; r < BL; r++) acc[r] = 0.0;
+    double phi[BS], dxi[BS], deta[BS], Um[NC], Fx[NC], Fy[NC];
+    for (int q = g.vptr[i] + lane; q < g.vptr[i + 1]; q += 64) {
+        const double4 P = g.vpts[q];
+        const double xi = 2.0 * (P.x - bb.x) / bb.z - 1.0, eta = 2.0 * (P.y - bb.y) / bb.w - 1.0;
+        basis<DEG>(xi, eta, phi, dxi, deta);
+        eval_state<BS, NC>(sU, phi, Um);
+        flux<NC>(eq, Um, P.x, P.y, Fx, Fy);
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int k = 0; k < BS; k++) acc[c * BS + k] += P.z * (Fx[c] * dxi[k] * sx + Fy[c] * deta[k] * sy);
+    }
+    const double *lr = line_rule<NQL>();
+    const int e0 = g.eptr[i], ne = g.eptr[i + 1] - e0;
+    double phin[BS], Up[NC], F[NC];
+    for (int idx = lane; idx < ne * NQL; idx += 64) {
+        const int e = idx / NQL, q = idx - e * NQL;
+        const Edge E = g.edges[e0 + e];
+        const double s = 0.5 * (lr[2 * q] + 1.0);
+        const double x = E.x1 + s * (E.x2 - E.x1), y = E.y1 + s * (E.y2 - E.y1);
+        const double W = 0.5 * E.len * lr[2 * q + 1];
+        const double xi = 2.0 * (x - bb.x) / bb.z - 1.0, eta = 2.0 * (y - bb.y) / bb.w - 1.0;
+        basis<DEG>(xi, eta, phi, nullptr, nullptr);
+        eval_state<BS, NC>(sU, phi, Um);
+        outer_state<DEG, NC>(eq, g, E, u, x, y, t, Up, phin);
+        numflux<NC>(eq, Um, Up, x, y, E.nx, E.ny, F);
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int k = 0; k < BS; k++) acc[c * BS + k] -= W * phi[k] * F[c];
+    }
+#pragma unroll
+    for (int r = 0; r < BL; r++) {
+        double v = acc[r];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) sOut[r] = v;
+    }
+    __syncthreads();
+    for (int r = lane; r < BL; r += 64) b[(size_t)i * BL + r] = sOut[r];
+}
+
+// ---------------------------------------------------------------------------------------
+// Jacobian: one 256-thread workgroup per cell.  Point data (basis, scaled gradients,
+// flux Jacobians) is staged in LDS in chunks; each thread owns NE entries of the
+// diagonal block and accumulates them over all points, and computes its entries of
+// each off-diagonal block edge by edge.
+constexpr int JWG = 256;
+constexpr int JCH = 64;
+
+template <int DEG, int NC>
+struct JacLayout {
+    static constexpr int BS = Deg<DEG>::BS, BL = NC * BS, NQL = Deg<DEG>::NQL;
+    static constexpr int NC2 = NC * NC;
+    static constexpr int PV = 3 * BS + 2 * NC2;   // phi, gx, gy, Ax, Ay
+    static constexpr int NE = (BL * BL + JWG - 1) / JWG;
+    static constexpr int LDS_VOL = JCH * PV;
+    static constexpr int LDS_EDGE = NQL * PV;       // psiW, phi, phin, Jd, Jo
+    static constexpr int LDS = (LDS_VOL > LDS_EDGE ? LDS_VOL : LDS_EDGE) + BL;
+};
+
+template <int DEG, int NC>
+__global__ __launch_bounds__(JWG) void k_jacobian(DevEq eq, Geo g, const double *__restrict__ u, double t,
+                                                  double scale, int add_mass, const double *__restrict__ mass,
+                                                  const int *__restrict__ rowptr, double *__restrict__ J) {
+    using L = JacLayout<DEG, NC>;
+    constexpr int BS = L::BS, BL = L::BL, NQL = L::NQL, NC2 = L::NC2, PV = L::PV, NE = L::NE;
+    __shared__ double lds[L::LDS];
+    double *sU = lds + (L::LDS - BL);
+    const int i = blockIdx.x;
+    const int tid = threadIdx.x;
+    for (int r = tid; r < BL; r += JWG) sU[r] = u[(size_t)i * BL + r];
+    const double4 bb = g.bb[i];
+    const double sx = 2.0 / bb.z, sy = 2.0 / bb.w;
+
+    int ek[NE], el[NE], ecc[NE];
+    bool ev[NE];
+    double acc[NE];
+#pragma unroll
+    for (int j = 0; j < NE; j++) {
+        const int e = tid + JWG * j;
+        ev[j] = e < BL * BL;
+        const int r = ev[j] ? e % BL : 0, col = ev[j] ? e / BL : 0;
+        const int c1 = r / BS, c2 = col / BS;
+        ek[j] = r - c1 * BS;
+        el[j] = col - c2 * BS;
+        ecc[j] = c1 * NC + c2;
+        acc[j] = 0.0;
+    }
+    __syncthreads();
+
+    // ---- volume terms
+    const int v0 = g.vptr[i], v1 = g.vptr[i + 1];
+    for (int q0 = v0; q0 < v1; q0 += JCH) {
+        const int nq = min(JCH, v1 - q0);
+        if (tid < nq) {
+            const double4 P = g.vpts[q0 + tid];
+            double *d = lds + tid * PV;
+            double phi[BS], dxi[BS], deta[BS], Um[NC], Ax[NC2], Ay[NC2];
+            const double xi = 2.0 * (P.x - bb.x) / bb.z - 1.0, eta = 2.0 * (P.y - bb.y) / bb.w - 1.0;
+            basis<DEG>(xi, eta, phi, dxi, deta);
+            eval_state<BS, NC>(sU, phi, Um);
+            fluxjac<NC>(eq, Um, P.x, P.y, Ax, Ay);
+#pragma unroll
+            for (int k = 0; k < BS; k++) {
+                d[k] = phi[k];
+                d[BS + k] = P.z * dxi[k] * sx;
+                d[2 * BS + k] = P.z * deta[k] * sy;
+            }
+#pragma unroll
+            for (int c = 0; c < NC2; c++) {
+                d[3 * BS + c] = Ax[c];
+                d[3 * BS + NC2 + c] = Ay[c];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < NE; j++) {
+            if (!ev[j]) continue;
+            double s = 0.0;
+            const int k = ek[j], l = el[j], cc = ecc[j];
+            for (int q = 0; q < nq; q++) {
+                const double *d = lds + q * PV;
+                s = fma(d[l], d[3 * BS + cc] * d[BS + k] + d[3 * BS + NC2 + cc] * d[2 * BS + k], s);
+            }
+            acc[j] += s;
+        }
+        __syncthreads();
+    }
+
+    // ---- edge terms
+    const double *lr = line_rule<NQL>();
+    const size_t bb2 = (size_t)BL * BL;
+    for (int e = g.eptr[i]; e < g.eptr[i + 1]; e++) {
+        const Edge E = g.edges[e];
+        if (tid < NQL) {
+            const int q = tid;
+            double *d = lds + q * PV;  // psiW[BS] phi[BS] phin[BS] Jd[NC2] Jo[NC2]
+            const double s = 0.5 * (lr[2 * q] + 1.0);
+            const double x = E.x1 + s * (E.x2 - E.x1), y = E.y1 + s * (E.y2 - E.y1);
+            const double W = 0.5 * E.len * lr[2 * q + 1];
+            const double xi = 2.0 * (x - bb.x) / bb.z - 1.0, eta = 2.0 * (y - bb.y) / bb.w - 1.0;
+            double phi[BS], phin[BS], Um[NC], Up[NC], Jd[NC2], Jo[NC2];
+            basis<DEG>(xi, eta, phi, nullptr, nullptr);
+            eval_state<BS, NC>(sU, phi, Um);
+            outer_state<DEG, NC>(eq, g, E, u, x, y, t, Up, phin);
+            numfluxjac<NC>(eq, Um, Up, x, y, E.nx, E.ny, +1, Jd);
+            if (E.kind != 2) {
+                if constexpr (NC == 4) {
+                    if (eq.jacobian) euler_offdiag_exact(eq.gamma, Um, Up, E.nx, E.ny, Jo);
+                    else numfluxjac<NC>(eq, Um, Up, x, y, E.nx, E.ny, -1, Jo);
+                } else {
+                    numfluxjac<NC>(eq, Um, Up, x, y, E.nx, E.ny, -1, Jo);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < NC2; c++) Jo[c] = 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < BS; k++) {
+                d[k] = W * phi[k];
+                d[BS + k] = phi[k];
+                d[2 * BS + k] = phin[k];
+            }
+#pragma unroll
+            for (int c = 0; c < NC2; c++) {
+                d[3 * BS + c] = Jd[c];
+                d[3 * BS + NC2 + c] = Jo[c];
+            }
+        }
+        __syncthreads();
+        double *O = E.slot >= 0 ? J + (size_t)E.slot * bb2 : nullptr;
+#pragma unroll
+        for (int j = 0; j < NE; j++) {
+            if (!ev[j]) continue;
+            const int k = ek[j], l = el[j], cc = ecc[j];
+            double sd = 0.0, so = 0.0;
+#pragma unroll
+            for (int q = 0; q < NQL; q++) {
+                const double *d = lds + q * PV;
+                const double pw = d[k];
+                sd = fma(pw * d[BS + l], d[3 * BS + cc], sd);
+                so = fma(pw * d[2 * BS + l], d[3 * BS + NC2 + cc], so);
+            }
+            acc[j] -= sd;
+            if (O) O[tid + JWG * j] = -scale * so;
+        }
+        __syncthreads();
+    }
+
+    // ---- diagonal block
+    double *D = J + (size_t)rowptr[i] * bb2;
+#pragma unroll
+    for (int j = 0; j < NE; j++) {
+        if (!ev[j]) continue;
+        double v = scale * acc[j];
+        if (add_mass && (ecc[j] / NC == ecc[j] % NC)) v += mass[(size_t)i * BS * BS + el[j] * BS + ek[j]];
+        D[tid + JWG * j] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// mass blocks M_i[k,l] = sum_q W phi_k phi_l  (MassMatrix::MassMatrix, src/MassMatrix.cpp:30-94)
+template <int DEG>
+__global__ __launch_bounds__(64) void k_mass(Geo g, double *__restrict__ M) {
+    constexpr int BS = Deg<DEG>::BS;
+    const int i = blockIdx.x;
+    const double4 bb = g.bb[i];
+    double acc[(BS * BS + 63) / 64];
+#pragma unroll
+    for (int j = 0; j < (BS * BS + 63) / 64; j++) acc[j] = 0.0;
+    double phi[BS];
+    for (int q = g.vptr[i]; q < g.vptr[i + 1]; q++) {
+        const double4 P = g.vpts[q];
+        const double xi = 2.0 * (P.x - bb.x) / bb.z - 1.0, eta = 2.0 * (P.y - bb.y) / bb.w - 1.0;
+        basis<DEG>(xi, eta, phi, nullptr, nullptr);
+#pragma unroll
+        for (int j = 0; j < (BS * BS + 63) / 64; j++) {
+            const int e = threadIdx.x + 64 * j;
+            if (e < BS * BS) acc[j] += P.z * phi[e % BS] * phi[e / BS];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < (BS * BS + 63) / 64; j++) {
+        const int e = threadIdx.x + 64 * j;
+        if (e < BS * BS) M[(size_t)i * BS * BS + e] = acc[j];
+    }
+}
+
+// y_i[c*bs+k] = sum_l M_i[k,l] x_i[c*bs+l]
+template <int BS, int NC>
+__global__ void k_mass_matvec(int n, const double *__restrict__ M, const double *__restrict__ x,
+                              double *__restrict__ y) {
+    constexpr int BL = NC * BS;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * BL) return;
+    const size_t i = e / BL;
+    const int r = (int)(e - i * BL);
+    const int c = r / BS, k = r - c * BS;
+    const double *Mi = M + i * BS * BS;
+    const double *xi = x + i * BL + c * BS;
+    double s = 0.0;
+#pragma unroll
+    for (int l = 0; l < BS; l++) s = fma(Mi[l * BS + k], xi[l], s);
+    y[e] = s;
+}
+
+// r_i = M_i k_i - dt b_i (or r = x when k is null); normc[i*NC + c] = r_c^T M_i r_c
+template <int BS, int NC>
+__global__ void k_newton_resid(int n, const double *__restrict__ M, const double *__restrict__ kv,
+                               const double *__restrict__ b, double dt, double *__restrict__ r,
+                               double *__restrict__ normc) {
+    constexpr int BL = NC * BS;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // (cell, component)
+    if (e >= (size_t)n * NC) return;
+    const size_t i = e / NC;
+    const int c = (int)(e - i * NC);
+    const double *Mi = M + i * BS * BS;
+    double rv[BS];
+    if (kv) {
+        const double *ki = kv + i * BL + c * BS;
+#pragma unroll
+        for (int k = 0; k < BS; k++) {
+            double s = 0.0;
+#pragma unroll
+            for (int l = 0; l < BS; l++) s = fma(Mi[l * BS + k], ki[l], s);
+            rv[k] = s - dt * b[i * BL + c * BS + k];
+            r[i * BL + c * BS + k] = rv[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < BS; k++) rv[k] = b[i * BL + c * BS + k];
+    }
+    double q = 0.0;
+#pragma unroll
+    for (int k = 0; k < BS; k++) {
+        double s = 0.0;
+#pragma unroll
+        for (int l = 0; l < BS; l++) s = fma(Mi[l * BS + k], rv[l], s);
+        q = fma(rv[k], s, q);
+    }
+    normc[e] = q;
+}
+
+// column sums of normc (n x NC) in two fixed-order stages
+__global__ __launch_bounds__(256) void k_colsum_partial(const double *__restrict__ a, int n, int nc,
+                                                        double *__restrict__ partial) {
+    __shared__ double red[4][4];
+    const int c = blockIdx.y;
+    double s = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += a[(size_t)i * nc + c];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][0] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[(size_t)c * gridDim.x + blockIdx.x] = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+}
+
+__global__ void k_colsum_final(const double *__restrict__ partial, int nb, int nc, double *__restrict__ out) {
+    const int c = threadIdx.x;
+    if (c >= nc) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; b++) s += partial[(size_t)c * nb + b];
+    out[c] = s;
+}
+
+}  // namespace cpg
+
+using namespace cpg;
+
+// ---- handles -------------------------------------------------------------------------
+struct cpg_mesh_s {
+    int np = 0, nv = 0, nbc = 0;
+    std::vector<double> v, txy, bb;
+    std::vector<int> pptr, pv, p2p, bc, tptr;
+};
+
+struct cpg_equation_s {
+    cpg_mesh_s *mesh = nullptr;
+    DevEq eq{};
+    int deg = 0, bs = 0, nc = 0, bl = 0, n = 0, nnzb = 0;
+    std::vector<int> rowptr, colind;
+    DevBuf<int> d_rowptr;
+    DevBuf<double4> d_bb, d_vpts;
+    DevBuf<int> d_vptr, d_eptr;
+    DevBuf<Edge> d_edges;
+    DevBuf<double> mass;        // n x bs x bs
+    DevBuf<double> normc, partial, sums;
+    int npartial = 0;
+    Geo geo() const { return Geo{d_bb.p, d_vptr.p, d_vpts.p, d_eptr.p, d_edges.p}; }
+};
+
+namespace cpg {
+
+// Launch helpers shared with the stepper
+template <int DEG, int NC>
+static void launch_residual(const cpg_equation_s &E, const double *u, double t, double *b, hipStream_t s) {
+    hipLaunchKernelGGL((k_residual<DEG, NC>), dim3(E.n), dim3(64), 0, s, E.eq, E.geo(), u, t, b);
+    CPG_LAUNCH_CHECK();
+}
+
+template <int DEG, int NC>
+static void launch_jacobian(const cpg_equation_s &E, const double *u, double t, double scale, int add_mass,
+                            double *J, hipStream_t s) {
+    hipLaunchKernelGGL((k_jacobian<DEG, NC>), dim3(E.n), dim3(JWG), 0, s, E.eq, E.geo(), u, t, scale, add_mass,
+                       E.mass.p, E.d_rowptr.p, J);
+    CPG_LAUNCH_CHECK();
+}
+
+#define CPG_DISPATCH(E, FN, ...)                                                          \
+    do {                                                                                  \
+        if ((E).nc == 4) {                                                                \
+            switch ((E).deg) {                                                            \
+            case 1: FN<1, 4>(__VA_ARGS__); break;                                         \
+            case 2: FN<2, 4>(__VA_ARGS__); break;                                         \
+            case 3: FN<3, 4>(__VA_ARGS__); break;                                         \
+            case 4: FN<4, 4>(__VA_ARGS__); break;                                         \
+            default: CPG_CHECK(false, CPG_ERR_UNSUPPORTED, "degree %d", (E).deg);         \
+            }                                                                             \
+        } else {                                                                          \
+            switch ((E).deg) {                                                            \
+            case 1: FN<1, 1>(__VA_ARGS__); break;                                         \
+            case 2: FN<2, 1>(__VA_ARGS__); break;                                         \
+            case 3: FN<3, 1>(__VA_ARGS__); break;                                         \
+            case 4: FN<4, 1>(__VA_ARGS__); break;                                         \
+            default: CPG_CHECK(false, CPG_ERR_UNSUPPORTED, "degree %d", (E).deg);         \
+            }                                                                             \
+        }                                                                                 \
+    } while (0)
+
+void eq_residual(const cpg_equation_s &E, const double *u, double t, double *b, hipStream_t s) {
+    CPG_DISPATCH(E, launch_residual, E, u, t, b, s);
+}
+
+void eq_jacobian(const cpg_equation_s &E, const double *u, double t, double scale, int add_mass, double *J,
+                 hipStream_t s) {
+    CPG_DISPATCH(E, launch_jacobian, E, u, t, scale, add_mass, J, s);
+}
+
+template <int BS, int NC>
+static void launch_mass_matvec(const cpg_equation_s &E, const double *x, double *y, hipStream_t s) {
+    const size_t cnt = (size_t)E.n * NC * BS;
+    hipLaunchKernelGGL((k_mass_matvec<BS, NC>), dim3(ceil_div(cnt, 256)), dim3(256), 0, s, E.n, E.mass.p, x, y);
+    CPG_LAUNCH_CHECK();
+}
+
+template <int BS, int NC>
+static void launch_newton_resid(const cpg_equation_s &E, const double *kv, const double *b, double dt, double *r,
+                                hipStream_t s) {
+    const size_t cnt = (size_t)E.n * NC;
+    hipLaunchKernelGGL((k_newton_resid<BS, NC>), dim3(ceil_div(cnt, 128)), dim3(128), 0, s, E.n, E.mass.p, kv, b,
+                       dt, r, E.normc.p);
+    CPG_LAUNCH_CHECK();
+}
+
+#define CPG_DISPATCH_BS(E, FN, ...)                                                       \
+    do {                                                                                  \
+        const int bs_ = (E).bs;                                                           \
+        if ((E).nc == 4) {                                                                \
+            if (bs_ == 3) FN<3, 4>(__VA_ARGS__);                                          \
+            else if (bs_ == 6) FN<6, 4>(__VA_ARGS__);                                     \
+            else if (bs_ == 10) FN<10, 4>(__VA_ARGS__);                                   \
+            else FN<15, 4>(__VA_ARGS__);                                                  \
+        } else {                                                                          \
+            if (bs_ == 3) FN<3, 1>(__VA_ARGS__);                                          \
+            else if (bs_ == 6) FN<6, 1>(__VA_ARGS__);                                     \
+            else if (bs_ == 10) FN<10, 1>(__VA_ARGS__);                                   \
+            else FN<15, 1>(__VA_ARGS__);                                                  \
+        }                                                                                 \
+    } while (0)
+
+void eq_mass_matvec(const cpg_equation_s &E, const double *x, double *y, hipStream_t s) {
+    CPG_DISPATCH_BS(E, launch_mass_matvec, E, x, y, s);
+}
+
+// r = M k - dt b (k non-null) and returns max_c sqrt(r_c^T M r_c) summed over cells (synchronous);
+// with k null, the norm of b itself.  out_nc (optional) receives the per-component norms.
+double eq_newton_resid(const cpg_equation_s &E, const double *kv, const double *b, double dt, double *r,
+                       double *out_nc, hipStream_t s) {
+    CPG_DISPATCH_BS(E, launch_newton_resid, E, kv, b, dt, r, s);
+    hipLaunchKernelGGL(k_colsum_partial, dim3(E.npartial, E.nc), dim3(256), 0, s, E.normc.p, E.n, E.nc, E.partial.p);
+    CPG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(64), 0, s, E.partial.p, E.npartial, E.nc, E.sums.p);
+    CPG_LAUNCH_CHECK();
+    double h[4] = {0, 0, 0, 0};
+    CPG_HIP(hipMemcpyAsync(h, E.sums.p, sizeof(double) * E.nc, hipMemcpyDeviceToHost, s));
+    CPG_HIP(hipStreamSynchronize(s));
+    double mx = 0.0;
+    for (int c = 0; c < E.nc; c++) {
+        const double v = std::sqrt(h[c]);
+        if (out_nc) out_nc[c] = v;
+        if (!(v <= mx)) mx = v;  // NaN propagates
+    }
+    return mx;
+}
+
+}  // namespace cpg
+
+namespace {
+
+template <int DEG>
+void launch_mass(cpg_equation_s &E) {
+    hipLaunchKernelGGL(k_mass<DEG>, dim3(E.n), dim3(64), 0, nullptr, E.geo(), E.mass.p);
+    CPG_LAUNCH_CHECK();
+}
+
+void pick_tri(int deg2, const double **tri, int *nt, int *nl) {
+    if (deg2 <= 2) { *tri = TRI2; *nt = 3; *nl = 2; }
+    else if (deg2 <= 4) { *tri = TRI4; *nt = 6; *nl = 4; }
+    else if (deg2 <= 6) { *tri = TRI6; *nt = 12; *nl = 6; }
+    else { *tri = TRI10; *nt = 25; *nl = 10; }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpg_mesh_create(int np, int nv, const double *v, const int *pptr, const int *pv, const int *p2p, int nbc,
+                    const int *bc, const int *tptr, const double *txy, const double *bb, cpg_mesh *out) {
+    return guard([&] {
+        CPG_CHECK(out && v && pptr && pv && p2p && tptr && txy && np > 0 && nv > 0, CPG_ERR_ARG, "bad mesh arguments");
+        CPG_CHECK(nbc == 0 || bc, CPG_ERR_ARG, "null bc table");
+        auto *M = new cpg_mesh_s;
+        try {
+            M->np = np;
+            M->nv = nv;
+            M->nbc = nbc;
+            M->v.assign(v, v + 2 * nv);
+            M->pptr.assign(pptr, pptr + np + 1);
+            const int ne = pptr[np];
+            M->pv.assign(pv, pv + ne);
+            M->p2p.assign(p2p, p2p + ne);
+            M->bc.assign(bc, bc + 7 * nbc);
+            M->tptr.assign(tptr, tptr + np + 1);
+            M->txy.assign(txy, txy + 6 * (size_t)tptr[np]);
+            for (int e = 0; e < ne; e++) {
+                CPG_CHECK(pv[e] >= 0 && pv[e] < nv, CPG_ERR_ARG, "vertex index out of range");
+                CPG_CHECK(p2p[e] < np && (p2p[e] >= 0 || -1 - p2p[e] < nbc), CPG_ERR_ARG,
+                          "neighbour id %d out of range", p2p[e]);
+            }
+            if (bb) {
+                M->bb.assign(bb, bb + 4 * (size_t)np);
+            } else {
+                M->bb.resize(4 * (size_t)np);
+                for (int i = 0; i < np; i++) {
+                    double x0 = v[2 * pv[pptr[i]]], y0 = v[2 * pv[pptr[i]] + 1], x1 = x0, y1 = y0;
+                    for (int j = pptr[i]; j < pptr[i + 1]; j++) {
+                        x0 = std::min(x0, v[2 * pv[j]]); x1 = std::max(x1, v[2 * pv[j]]);
+                        y0 = std::min(y0, v[2 * pv[j] + 1]); y1 = std::max(y1, v[2 * pv[j] + 1]);
+                    }
+                    M->bb[4 * i] = x0; M->bb[4 * i + 1] = y0; M->bb[4 * i + 2] = x1 - x0; M->bb[4 * i + 3] = y1 - y0;
+                }
+            }
+        } catch (...) {
+            delete M;
+            throw;
+        }
+        *out = M;
+    });
+}
+
+int cpg_mesh_destroy(cpg_mesh M) {
+    return guard([&] { delete M; });
+}
+
+int cpg_equation_create(cpg_mesh M, const cpg_eq_params *p, int deg, cpg_equation *out) {
+    return guard([&] {
+        CPG_CHECK(M && p && out, CPG_ERR_ARG, "null argument");
+        CPG_CHECK(deg >= 1 && deg <= 4, CPG_ERR_UNSUPPORTED, "degree %d not in 1..4", deg);
+        CPG_CHECK(p->kind == CPG_ADVECTION || p->kind == CPG_EULER, CPG_ERR_ARG, "bad equation kind");
+        auto *E = new cpg_equation_s;
+        try {
+            E->mesh = M;
+            E->eq.kind = p->kind;
+            for (int j = 0; j < 6; j++) E->eq.beta[j] = p->beta[j];
+            E->eq.gamma = p->gamma;
+            E->eq.bc_kind = p->bc_kind;
+            for (int j = 0; j < 8; j++) E->eq.bc_par[j] = p->bc_par[j];
+            E->eq.periodic_trace = p->periodic_trace;
+            E->eq.jacobian = p->jacobian;
+            E->deg = deg;
+            E->bs = (deg + 1) * (deg + 2) / 2;
+            E->nc = p->kind == CPG_EULER ? 4 : 1;
+            E->bl = E->nc * E->bs;
+            const int n = M->np;
+            E->n = n;
+            const auto &v = M->v;
+            // Jacobian pattern (src/Jacobian.cpp:7-47) and edge records
+            E->rowptr.assign(n + 1, 0);
+            std::vector<Edge> edges;
+            std::vector<int> eptr(n + 1, 0);
+            for (int i = 0; i < n; i++) {
+                E->rowptr[i] = (int)E->colind.size();
+                E->colind.push_back(i);
+                const int nv = M->pptr[i + 1] - M->pptr[i];
+                double cx = 0, cy = 0;
+                for (int j = 0; j < nv; j++) { cx += v[2 * M->pv[M->pptr[i] + j]]; cy += v[2 * M->pv[M->pptr[i] + j] + 1]; }
+                cx /= nv;
+                cy /= nv;
+                for (int j = 0; j < nv; j++) {
+                    const int a = M->pv[M->pptr[i] + j], b = M->pv[M->pptr[i] + (j + 1) % nv];
+                    const int nb = M->p2p[M->pptr[i] + j];
+                    Edge Ed{};
+                    Ed.x1 = v[2 * a]; Ed.y1 = v[2 * a + 1]; Ed.x2 = v[2 * b]; Ed.y2 = v[2 * b + 1];
+                    const double dx = v[2 * a] - v[2 * b], dy = v[2 * a + 1] - v[2 * b + 1];
+                    const double len = std::sqrt(dx * dx + dy * dy);
+                    const double sv = -dy * (v[2 * a] - cx) + dx * (v[2 * a + 1] - cy);
+                    const int sg = (0.0 < sv) - (sv < 0.0);
+                    Ed.nx = -sg * dy / len;
+                    Ed.ny = sg * dx / len;
+                    Ed.len = std::sqrt((Ed.x2 - Ed.x1) * (Ed.x2 - Ed.x1) + (Ed.y2 - Ed.y1) * (Ed.y2 - Ed.y1));
+                    Ed.slot = -1;
+                    if (nb >= 0) {
+                        Ed.kind = 0;
+                        Ed.nb = nb;
+                    } else {
+                        const int *row = &M->bc[7 * (-1 - nb)];
+                        if (row[0]) {
+                            Ed.kind = 1;
+                            Ed.nb = row[2];
+                            Ed.tx = v[2 * row[5]] - v[2 * row[3]];
+                            Ed.ty = v[2 * row[5] + 1] - v[2 * row[3] + 1];
+                        } else {
+                            Ed.kind = 2;
+                            Ed.nb = -1;
+                        }
+                    }
+                    if (Ed.kind != 2) {
+                        Ed.slot = (int)E->colind.size();
+                        E->colind.push_back(Ed.nb);
+                    }
+                    edges.push_back(Ed);
+                }
+                eptr[i + 1] = (int)edges.size();
+            }
+            E->rowptr[n] = (int)E->colind.size();
+            E->nnzb = E->rowptr[n];
+            // volume quadrature points (Quadrature::polygonIntegral, src/Quadrature.cpp:13-57)
+            const double *tr;
+            int nt, nl;
+            pick_tri(2 * deg, &tr, &nt, &nl);
+            std::vector<int> vptr(n + 1, 0);
+            std::vector<double4> vpts;
+            for (int i = 0; i < n; i++) {
+                for (int tt = M->tptr[i]; tt < M->tptr[i + 1]; tt++) {
+                    const double *T = &M->txy[6 * (size_t)tt];
+                    const double area = 0.5 * std::fabs(T[0] * T[3] + T[2] * T[5] + T[4] * T[1] - T[0] * T[5] -
+                                                        T[4] * T[3] - T[2] * T[1]);
+                    for (int q = 0; q < nt; q++) {
+                        const double r = tr[3 * q], s = tr[3 * q + 1];
+                        double4 P;
+                        P.x = T[0] * (1 - r - s) + T[2] * r + T[4] * s;
+                        P.y = T[1] * (1 - r - s) + T[3] * r + T[5] * s;
+                        P.z = tr[3 * q + 2] * area;
+                        P.w = 0.0;
+                        vpts.push_back(P);
+                    }
+                }
+                vptr[i + 1] = (int)vpts.size();
+            }
+            std::vector<double4> bbv(n);
+            for (int i = 0; i < n; i++)
+                bbv[i] = make_double4(M->bb[4 * i], M->bb[4 * i + 1], M->bb[4 * i + 2], M->bb[4 * i + 3]);
+            E->d_rowptr.upload(E->rowptr);
+            E->d_bb.upload(bbv);
+            E->d_vptr.upload(vptr);
+            E->d_vpts.upload(vpts);
+            E->d_eptr.upload(eptr);
+            E->d_edges.upload(edges);
+            E->mass.alloc((size_t)n * E->bs * E->bs);
+            switch (deg) {
+            case 1: launch_mass<1>(*E); break;
+            case 2: launch_mass<2>(*E); break;
+            case 3: launch_mass<3>(*E); break;
+            default: launch_mass<4>(*E); break;
+            }
+            E->normc.alloc((size_t)n * E->nc);
+            E->npartial = std::max(1, std::min(1024, ceil_div(n, 2048)));
+            E->partial.alloc((size_t)E->npartial * E->nc);
+            E->sums.alloc(4);
+            CPG_HIP(hipDeviceSynchronize());
+        } catch (...) {
+            delete E;
+            throw;
+        }
+        *out = E;
+    });
+}
+
+int cpg_equation_destroy(cpg_equation E) {
+    return guard([&] { delete E; });
+}
+
+int cpg_equation_info(cpg_equation E, int *n_cells, int *bl, int *nnzb) {
+    return guard([&] {
+        CPG_CHECK(E, CPG_ERR_ARG, "null equation");
+        if (n_cells) *n_cells = E->n;
+        if (bl) *bl = E->bl;
+        if (nnzb) *nnzb = E->nnzb;
+    });
+}
+
+int cpg_jacobian_pattern(cpg_equation E, int *rowptr, int *colind) {
+    return guard([&] {
+        CPG_CHECK(E && rowptr && colind, CPG_ERR_ARG, "null argument");
+        std::memcpy(rowptr, E->rowptr.data(), sizeof(int) * (E->n + 1));
+        std::memcpy(colind, E->colind.data(), sizeof(int) * E->nnzb);
+    });
+}
+
+int cpg_jacobian_create(cpg_equation E, cpg_bsr *out) {
+    if (!E || !out) {
+        set_error("null argument");
+        return CPG_ERR_ARG;
+    }
+    return cpg_bsr_create(E->n, E->bl, E->nnzb, E->rowptr.data(), E->colind.data(), out);
+}
+
+int cpg_assemble_residual(cpg_equation E, const double *u, double t, double *b, void *stream) {
+    return guard([&] {
+        CPG_CHECK(E && u && b, CPG_ERR_ARG, "null argument");
+        CPG_CHECK(u != b, CPG_ERR_ARG, "u and b alias");
+        eq_residual(*E, u, t, b, S(stream));
+    });
+}
+
+int cpg_assemble_jacobian(cpg_equation E, const double *u, double t, double scale, int add_mass, cpg_bsr J,
+                          void *stream) {
+    return guard([&] {
+        CPG_CHECK(E && u && J, CPG_ERR_ARG, "null argument");
+        CPG_CHECK(J->n == E->n && J->bl == E->bl && J->nnzb == E->nnzb && J->h_colind == E->colind, CPG_ERR_ARG,
+                  "matrix does not have the Jacobian pattern");
+        eq_jacobian(*E, u, t, scale, add_mass, J->blocks.p, S(stream));
+    });
+}
+
+int cpg_mass_matvec(cpg_equation E, const double *x, double *y, void *stream) {
+    return guard([&] {
+        CPG_CHECK(E && x && y && x != y, CPG_ERR_ARG, "bad argument");
+        eq_mass_matvec(*E, x, y, S(stream));
+    });
+}
+
+int cpg_mass_download(cpg_equation E, double *host_blocks) {
+    return guard([&] {
+        CPG_CHECK(E && host_blocks, CPG_ERR_ARG, "null argument");
+        std::vector<double> m((size_t)E->n * E->bs * E->bs);
+        CPG_HIP(hipMemcpy(m.data(), E->mass.p, m.size() * sizeof(double), hipMemcpyDeviceToHost));
+        const int bs = E->bs, bl = E->bl;
+        std::memset(host_blocks, 0, sizeof(double) * (size_t)E->n * bl * bl);
+        for (int i = 0; i < E->n; i++)
+            for (int c = 0; c < E->nc; c++)
+                for (int l = 0; l < bs; l++)
+                    for (int k = 0; k < bs; k++)
+                        host_blocks[(size_t)i * bl * bl + (size_t)(c * bs + l) * bl + c * bs + k] =
+                            m[(size_t)i * bs * bs + l * bs + k];
+    });
+}
+
+int cpg_l2norm(cpg_equation E, const double *u, double *out, void *stream) {
+    return guard([&] {
+        CPG_CHECK(E && u && out, CPG_ERR_ARG, "null argument");
+        eq_newton_resid(*E, nullptr, u, 0.0, nullptr, out, S(stream));
+    });
+}
+
+}  // extern "C"

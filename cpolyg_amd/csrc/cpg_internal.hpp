@@ -1,0 +1,112 @@
+// Internal definitions shared by the translation units of libcpolyg_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/cpolyg.h"
+
+namespace cpg {
+
+// ---- error plumbing ----------------------------------------------------------------
+void set_error(const char *fmt, ...);
+
+struct Error {
+    int status;
+};
+
+#define CPG_HIP(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            ::cpg::set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_)); \
+            throw ::cpg::Error{e_ == hipErrorOutOfMemory ? CPG_ERR_OOM : CPG_ERR_HIP};          \
+        }                                                                                      \
+    } while (0)
+
+#define CPG_CHECK(cond, status, ...)        \
+    do {                                    \
+        if (!(cond)) {                      \
+            ::cpg::set_error(__VA_ARGS__);  \
+            throw ::cpg::Error{status};     \
+        }                                   \
+    } while (0)
+
+#define CPG_LAUNCH_CHECK() CPG_HIP(hipGetLastError())
+
+template <class F>
+int guard(F &&f) {
+    try {
+        f();
+        return CPG_OK;
+    } catch (const Error &e) {
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return CPG_ERR_OOM;
+    } catch (...) {
+        set_error("unexpected C++ exception");
+        return CPG_ERR_HIP;
+    }
+}
+
+// ---- device buffers ----------------------------------------------------------------
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count) CPG_HIP(hipMalloc((void **)&p, count * sizeof(T)));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T *h, size_t count, hipStream_t s) {
+        CPG_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T> &h, hipStream_t s = nullptr) {
+        if (h.size() > n) alloc(h.size());
+        if (!h.empty()) {
+            CPG_HIP(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+            CPG_HIP(hipStreamSynchronize(s));
+        }
+    }
+};
+
+inline hipStream_t S(void *stream) { return reinterpret_cast<hipStream_t>(stream); }
+
+inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace cpg
+
+// ---- handle types ------------------------------------------------------------------
+struct cpg_bsr_s {
+    int n = 0, bl = 0, nnzb = 0;
+    std::vector<int> h_rowptr, h_colind;
+    cpg::DevBuf<int> rowptr, colind;
+    cpg::DevBuf<double> blocks;
+};
+
+namespace cpg {
+// shared kernels-launchers used across translation units
+void bsr_matvec(const cpg_bsr_s &A, const double *x, double *y, hipStream_t s);
+void vec_fill(double *x, double v, size_t n, hipStream_t s);
+}  // namespace cpg

@@ -1,0 +1,774 @@
+// Block DILU ("BlockILU0") and block-Jacobi preconditioners on the device.
+//
+// Reference: BlockILU0 (src/Preconditioners.cpp:49-158) is block DILU:
+//   D_j = A_jj - sum_{i<j} A_ji D_i^-1 A_ij,  L_ji = A_ji D_i^-1,  DU_ij = D_i^-1 A_ij,
+// applied as x <- (I + DU)^-1 D^-1 (I + L)^-1 x.  Rows are processed in levels:
+// a row's forward level is one more than the deepest lower neighbour's, its backward
+// level one more than the deepest upper neighbour's.  For a colour-ordered matrix
+// (every neighbour pair i<j has colour(i) < colour(j)) the levels are the colours.
+//
+// Factor (left-looking, one workgroup per row of a forward level):
+//   AD = A_ii - sum_k L_ik A_ki with L_ik = A_ik D_k^-1 (k lower neighbour),
+//   D_i^-1 by Gauss-Jordan with partial pivoting in LDS, DU_ij = D_i^-1 A_ij.
+// The factor runs in FP64 and rounds once when storing to the requested precision;
+// an FP64 copy of D^-1 is kept for the L products of later levels.
+//
+// Apply (one workgroup per row, each level a launch): forward y_i = x_i - sum L_ik y_k,
+// z_i = D_i^-1 y_i (same kernel), backward x_i = z_i - sum DU_ij x_j.  Blocks are
+// streamed column by column exactly like the BSR matvec.
+#include "cpg_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace cpg {
+
+// ---- storage types -----------------------------------------------------------------
+struct bf16 {
+    uint16_t v;
+};
+
+__host__ __device__ inline float bf16_to_f(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// round to nearest even; NaN stays NaN
+__host__ __device__ inline uint16_t f_to_bf16(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return (uint16_t)(u >> 16);
+}
+
+template <class T>
+struct Store;
+template <>
+struct Store<double> {
+    using acc = double;
+    static constexpr int prec = CPG_FP64;
+    __device__ static void put(double *p, size_t e, double v, float) { p[e] = v; }
+    __device__ static double get(const double *p, size_t e) { return p[e]; }
+};
+template <>
+struct Store<float> {
+    using acc = float;
+    static constexpr int prec = CPG_FP32;
+    __device__ static void put(float *p, size_t e, double v, float) { p[e] = (float)v; }
+    __device__ static float get(const float *p, size_t e) { return p[e]; }
+};
+template <>
+struct Store<bf16> {
+    using acc = float;
+    static constexpr int prec = CPG_BF16;
+    __device__ static void put(bf16 *p, size_t e, double v, float inv_scale) {
+        p[e].v = f_to_bf16((float)(v * (double)inv_scale));
+    }
+    __device__ static float get(const bf16 *p, size_t e) { return bf16_to_f(p[e].v); }
+};
+
+// ---- workgroup helpers ---------------------------------------------------------------
+constexpr int WG = 256;
+
+__device__ inline double wg_max_abs(double v, double *scratch) {
+    // block-wide max of |v| (WG threads, 4 wavefronts)
+    double a = fabs(v);
+    for (int off = 32; off > 0; off >>= 1) a = fmax(a, __shfl_xor(a, off));
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) scratch[w] = a;
+    __syncthreads();
+    double r = scratch[0];
+    for (int k = 1; k < WG / 64; k++) r = fmax(r, scratch[k]);
+    __syncthreads();
+    return r;
+}
+
+// power-of-two scale s with max|v|/s in (0.5, 1]; 1 for a zero block
+__device__ inline float pow2_scale(double mx) {
+    if (!(mx > 0.0) || !isfinite(mx)) return 1.0f;
+    int e;
+    frexp(mx, &e);  // mx = f * 2^e, f in [0.5, 1)
+    return ldexpf(1.0f, e);
+}
+
+// C(entries of this thread) = P * Q, all bl x bl column-major in LDS
+template <int NE>
+__device__ inline void gemm_lds(int bl, const double *P, const double *Q, double (&c)[NE]) {
+    const int bb = bl * bl;
+#pragma unroll
+    for (int j = 0; j < NE; j++) {
+        const int e = threadIdx.x + WG * j;
+        double s = 0.0;
+        if (e < bb) {
+            const int r = e % bl, col = e / bl;
+            const double *q = Q + col * bl;
+            for (int k = 0; k < bl; k++) s = fma(P[k * bl + r], q[k], s);
+        }
+        c[j] = s;
+    }
+}
+
+template <int NE>
+__device__ inline void load_block(int bl, double *dst, const double *src) {
+    const int bb = bl * bl;
+#pragma unroll
+    for (int j = 0; j < NE; j++) {
+        const int e = threadIdx.x + WG * j;
+        if (e < bb) dst[e] = src[e];
+    }
+}
+
+// store a bl x bl block held in registers (entries threadIdx.x + WG*j) in precision T
+template <class T, int NE>
+__device__ inline void store_block(int bl, T *dst, float *scale_out, const double (&c)[NE], double *scratch) {
+    const int bb = bl * bl;
+    float inv = 1.0f;
+    if constexpr (Store<T>::prec == CPG_BF16) {
+        double mx = 0.0;
+#pragma unroll
+        for (int j = 0; j < NE; j++)
+            if (threadIdx.x + WG * j < bb) mx = fmax(mx, fabs(c[j]));
+        mx = wg_max_abs(mx, scratch);
+        const float s = pow2_scale(mx);
+        inv = 1.0f / s;
+        if (threadIdx.x == 0) *scale_out = s;
+    }
+#pragma unroll
+    for (int j = 0; j < NE; j++) {
+        const int e = threadIdx.x + WG * j;
+        if (e < bb) Store<T>::put(dst, e, c[j], inv);
+    }
+}
+
+// Gauss-Jordan inversion with partial pivoting (first maximal |pivot|) of the bl x bl
+// left half of AUG (column-major, bl rows, 2*bl columns); the right half must hold I.
+// Returns false (uniformly) if a zero pivot was met.
+__device__ bool gauss_jordan(int bl, double *AUG, double *fac, int *piv_row) {
+    bool ok = true;
+    const int t = threadIdx.x;
+    const int w2 = 2 * bl;
+    for (int p = 0; p < bl; p++) {
+        if (t < 64) {
+            // wavefront 0 finds the pivot row among rows p..bl-1 (bl <= 64 handled per lane; larger in a loop)
+            double best = -1.0;
+            int bi = bl;
+            for (int r = p + t; r < bl; r += 64) {
+                const double a = fabs(AUG[p * bl + r]);
+                if (a > best) { best = a; bi = r; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(best, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            }
+            if (t == 0) *piv_row = bi;
+        }
+        __syncthreads();
+        const int pr = *piv_row;
+        if (pr != p) {
+            for (int c = t; c < w2; c += WG) {
+                const double tmp = AUG[c * bl + p];
+                AUG[c * bl + p] = AUG[c * bl + pr];
+                AUG[c * bl + pr] = tmp;
+            }
+        }
+        __syncthreads();
+        double d = AUG[p * bl + p];
+        if (d == 0.0) { ok = false; d = 1.0; }
+        const double invd = 1.0 / d;
+        for (int c = t; c < w2; c += WG) AUG[c * bl + p] *= invd;
+        for (int r = t; r < bl; r += WG) fac[r] = (r == p) ? 0.0 : AUG[p * bl + r];
+        __syncthreads();
+        const int ncell = bl * w2;
+        for (int e = t; e < ncell; e += WG) {
+            const int c = e / bl, r = e - c * bl;
+            if (r != p) AUG[c * bl + r] = fma(-fac[r], AUG[c * bl + p], AUG[c * bl + r]);
+        }
+        __syncthreads();
+    }
+    return ok;
+}
+
+// ---- factor kernels ----------------------------------------------------------------
+struct RowLists {
+    const int *rows;      // rows of this level
+    const int *diag;      // diagonal slot per row
+    const int *lptr, *lslot, *lcol, *ltrans;  // lower slots, their columns, transposed slot
+    const int *uptr, *uslot, *ucol;           // upper slots and columns
+};
+
+// LDS: AUG (2 bl^2) + P (bl^2) + Q (bl^2) + fac (bl) + scratch
+template <class T, int NE>
+__global__ __launch_bounds__(WG) void k_dilu_factor(int bl, RowLists L, const double *__restrict__ A,
+                                                    T *__restrict__ F, float *__restrict__ scale,
+                                                    double *__restrict__ DD64, int *__restrict__ status) {
+    extern __shared__ double lds[];
+    const int bb = bl * bl;
+    double *AUG = lds;
+    double *P = AUG + 2 * bb;
+    double *Q = P + bb;
+    double *fac = Q + bb;
+    double *scratch = fac + bl;
+    __shared__ int piv_row;
+    const int i = L.rows[blockIdx.x];
+    const int d = L.diag[i];
+    const int t = threadIdx.x;
+    // AUG = [A_ii | I]
+    for (int e = t; e < bb; e += WG) {
+        AUG[e] = A[(size_t)d * bb + e];
+        const int r = e % bl, c = e / bl;
+        AUG[bb + e] = (r == c) ? 1.0 : 0.0;
+    }
+    double c[NE];
+    for (int q = L.lptr[i]; q < L.lptr[i + 1]; q++) {
+        const int s = L.lslot[q], k = L.lcol[q], st = L.ltrans[q];
+        __syncthreads();
+        load_block<NE>(bl, P, A + (size_t)s * bb);
+        load_block<NE>(bl, Q, DD64 + (size_t)k * bb);
+        __syncthreads();
+        gemm_lds<NE>(bl, P, Q, c);  // L_ik = A_ik D_k^-1
+        store_block<T, NE>(bl, F + (size_t)s * bb, scale ? scale + s : nullptr, c, scratch);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < NE; j++) {
+            const int e = t + WG * j;
+            if (e < bb) P[e] = c[j];
+        }
+        load_block<NE>(bl, Q, A + (size_t)st * bb);
+        __syncthreads();
+        gemm_lds<NE>(bl, P, Q, c);  // L_ik A_ki
+#pragma unroll
+        for (int j = 0; j < NE; j++) {
+            const int e = t + WG * j;
+            if (e < bb) AUG[e] -= c[j];
+        }
+    }
+    __syncthreads();
+    const bool ok = gauss_jordan(bl, AUG, fac, &piv_row);
+    if (!ok && t == 0) atomicOr(status, 1);
+    // D_i^-1 = right half of AUG
+    double *Dinv = AUG + bb;
+#pragma unroll
+    for (int j = 0; j < NE; j++) {
+        const int e = t + WG * j;
+        c[j] = e < bb ? Dinv[e] : 0.0;
+        if (e < bb) DD64[(size_t)i * bb + e] = c[j];
+    }
+    store_block<T, NE>(bl, F + (size_t)d * bb, scale ? scale + d : nullptr, c, scratch);
+    for (int q = L.uptr[i]; q < L.uptr[i + 1]; q++) {
+        const int s = L.uslot[q];
+        __syncthreads();
+        load_block<NE>(bl, Q, A + (size_t)s * bb);
+        __syncthreads();
+        gemm_lds<NE>(bl, Dinv, Q, c);  // DU_ij = D_i^-1 A_ij
+        store_block<T, NE>(bl, F + (size_t)s * bb, scale ? scale + s : nullptr, c, scratch);
+    }
+}
+
+// block Jacobi: inverse of each diagonal block
+template <class T, int NE>
+__global__ __launch_bounds__(WG) void k_bjacobi_factor(int bl, const int *__restrict__ diag,
+                                                       const double *__restrict__ A, T *__restrict__ F,
+                                                       float *__restrict__ scale, int *__restrict__ status) {
+    extern __shared__ double lds[];
+    const int bb = bl * bl;
+    double *AUG = lds;
+    double *fac = AUG + 2 * bb;
+    double *scratch = fac + bl;
+    __shared__ int piv_row;
+    const int i = blockIdx.x;
+    const int d = diag[i];
+    for (int e = threadIdx.x; e < bb; e += WG) {
+        AUG[e] = A[(size_t)d * bb + e];
+        AUG[bb + e] = ((e % bl) == (e / bl)) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const bool ok = gauss_jordan(bl, AUG, fac, &piv_row);
+    if (!ok && threadIdx.x == 0) atomicOr(status, 1);
+    double c[NE];
+#pragma unroll
+    for (int j = 0; j < NE; j++) {
+        const int e = threadIdx.x + WG * j;
+        c[j] = e < bb ? AUG[bb + e] : 0.0;
+    }
+    store_block<T, NE>(bl, F + (size_t)i * bb, scale ? scale + i : nullptr, c, scratch);
+}
+
+// ---- apply kernels -----------------------------------------------------------------
+template <class T>
+struct VecW {
+    // elements per lane-load: 16 bytes when bl allows it
+    static int pick(int bl) {
+        const int maxw = 16 / (int)sizeof(T);
+        for (int w = maxw; w > 1; w >>= 1)
+            if (bl % w == 0) return w;
+        return 1;
+    }
+};
+
+template <class T, int VW>
+__device__ inline void load_vec(const T *p, typename Store<T>::acc (&a)[VW]) {
+    using A = typename Store<T>::acc;
+    if constexpr (sizeof(T) * VW == 16) {
+        const uint4 raw = *reinterpret_cast<const uint4 *>(p);
+        const T *e = reinterpret_cast<const T *>(&raw);
+#pragma unroll
+        for (int v = 0; v < VW; v++) a[v] = (A)Store<T>::get(e, v);
+    } else if constexpr (sizeof(T) * VW == 8) {
+        const uint2 raw = *reinterpret_cast<const uint2 *>(p);
+        const T *e = reinterpret_cast<const T *>(&raw);
+#pragma unroll
+        for (int v = 0; v < VW; v++) a[v] = (A)Store<T>::get(e, v);
+    } else {
+#pragma unroll
+        for (int v = 0; v < VW; v++) a[v] = (A)Store<T>::get(p, v);
+    }
+}
+
+// acc(rows rr..rr+VW) += sum over the listed blocks of blk * vec(col block)
+template <class T, int VW, class SL, class CL, class XV>
+__device__ inline void stream_blocks(int bl, int nblk, SL slot_of, CL col_of, const T *F,
+                                     const float *scale, XV xval, typename Store<T>::acc (&acc)[VW]) {
+    using A = typename Store<T>::acc;
+    const int tpc = bl / VW;
+    const int G = WG / tpc;
+    const int g = threadIdx.x / tpc;
+    const int rr = (threadIdx.x - g * tpc) * VW;
+    if (g >= G) return;
+    const int ncols = nblk * bl;
+#pragma unroll 4
+    for (int cg = g; cg < ncols; cg += G) {
+        const int blk = cg / bl;
+        const int col = cg - blk * bl;
+        const int s = slot_of(blk);
+        A xv = xval(col_of(blk), col);
+        if constexpr (Store<T>::prec == CPG_BF16) xv *= scale[s];
+        A a[VW];
+        load_vec<T, VW>(F + (size_t)s * bl * bl + (size_t)col * bl + rr, a);
+#pragma unroll
+        for (int v = 0; v < VW; v++) acc[v] = fma(a[v], xv, acc[v]);
+    }
+}
+
+template <class A, int VW>
+__device__ inline void reduce_groups(int bl, A (&acc)[VW], A *red) {
+    const int tpc = bl / VW;
+    const int G = WG / tpc;
+    const int g = threadIdx.x / tpc;
+    const int rr = (threadIdx.x - g * tpc) * VW;
+    if (g < G)
+#pragma unroll
+        for (int v = 0; v < VW; v++) red[g * bl + rr + v] = acc[v];
+    __syncthreads();
+    if (threadIdx.x < bl) {
+        A s = 0;
+        for (int gg = 0; gg < G; gg++) s += red[gg * bl + threadIdx.x];
+        red[G * bl + threadIdx.x] = s;  // result row
+    }
+    __syncthreads();
+}
+
+// forward level: y_i = x_i - sum_lower L y ; z_i = D_i^-1 y_i
+template <class T, int VW>
+__global__ __launch_bounds__(WG) void k_dilu_forward(int bl, RowLists L, const T *__restrict__ F,
+                                                     const float *__restrict__ scale, const double *__restrict__ x,
+                                                     typename Store<T>::acc *__restrict__ y,
+                                                     typename Store<T>::acc *__restrict__ z) {
+    using A = typename Store<T>::acc;
+    extern __shared__ unsigned char raw[];
+    A *red = reinterpret_cast<A *>(raw);
+    const int tpc = bl / VW;
+    const int G = WG / tpc;
+    A *yi = red + (G + 1) * bl;
+    const int i = L.rows[blockIdx.x];
+    A acc[VW];
+#pragma unroll
+    for (int v = 0; v < VW; v++) acc[v] = 0;
+    const int l0 = L.lptr[i];
+    stream_blocks<T, VW>(bl, L.lptr[i + 1] - l0, [&](int b) { return L.lslot[l0 + b]; },
+                         [&](int b) { return L.lcol[l0 + b]; }, F, scale,
+                         [&](int cb, int col) { return y[(size_t)cb * bl + col]; }, acc);
+    reduce_groups<A, VW>(bl, acc, red);
+    if (threadIdx.x < bl) {
+        const A v = (A)x[(size_t)i * bl + threadIdx.x] - red[G * bl + threadIdx.x];
+        yi[threadIdx.x] = v;
+        y[(size_t)i * bl + threadIdx.x] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < VW; v++) acc[v] = 0;
+    const int d = L.diag[i];
+    stream_blocks<T, VW>(bl, 1, [&](int) { return d; }, [&](int) { return i; }, F, scale,
+                         [&](int, int col) { return yi[col]; }, acc);
+    reduce_groups<A, VW>(bl, acc, red);
+    if (threadIdx.x < bl) z[(size_t)i * bl + threadIdx.x] = red[G * bl + threadIdx.x];
+}
+
+// backward level: x_i = z_i - sum_upper DU x
+template <class T, int VW>
+__global__ __launch_bounds__(WG) void k_dilu_backward(int bl, RowLists L, const T *__restrict__ F,
+                                                      const float *__restrict__ scale,
+                                                      const typename Store<T>::acc *__restrict__ z,
+                                                      double *__restrict__ x) {
+    using A = typename Store<T>::acc;
+    extern __shared__ unsigned char raw[];
+    A *red = reinterpret_cast<A *>(raw);
+    const int tpc = bl / VW;
+    const int G = WG / tpc;
+    const int i = L.rows[blockIdx.x];
+    A acc[VW];
+#pragma unroll
+    for (int v = 0; v < VW; v++) acc[v] = 0;
+    const int u0 = L.uptr[i];
+    stream_blocks<T, VW>(bl, L.uptr[i + 1] - u0, [&](int b) { return L.uslot[u0 + b]; },
+                         [&](int b) { return L.ucol[u0 + b]; }, F, scale,
+                         [&](int cb, int col) { return (A)x[(size_t)cb * bl + col]; }, acc);
+    reduce_groups<A, VW>(bl, acc, red);
+    if (threadIdx.x < bl)
+        x[(size_t)i * bl + threadIdx.x] = (double)(z[(size_t)i * bl + threadIdx.x] - red[G * bl + threadIdx.x]);
+}
+
+// block Jacobi apply: x_i = D_i^-1 x_i
+template <class T, int VW>
+__global__ __launch_bounds__(WG) void k_bjacobi_apply(int bl, const T *__restrict__ F, const float *__restrict__ scale,
+                                                      double *__restrict__ x) {
+    using A = typename Store<T>::acc;
+    extern __shared__ unsigned char raw[];
+    A *red = reinterpret_cast<A *>(raw);
+    const int tpc = bl / VW;
+    const int G = WG / tpc;
+    A *xi = red + (G + 1) * bl;
+    const int i = blockIdx.x;
+    if (threadIdx.x < bl) xi[threadIdx.x] = (A)x[(size_t)i * bl + threadIdx.x];
+    __syncthreads();
+    A acc[VW];
+#pragma unroll
+    for (int v = 0; v < VW; v++) acc[v] = 0;
+    stream_blocks<T, VW>(bl, 1, [&](int) { return i; }, [&](int) { return i; }, F, scale,
+                         [&](int, int col) { return xi[col]; }, acc);
+    reduce_groups<A, VW>(bl, acc, red);
+    if (threadIdx.x < bl) x[(size_t)i * bl + threadIdx.x] = (double)red[G * bl + threadIdx.x];
+}
+
+// factors to FP64 (for download)
+template <class T>
+__global__ void k_to_fp64(const T *__restrict__ F, const float *__restrict__ scale, size_t bb, size_t n,
+                          double *__restrict__ out) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double v = (double)Store<T>::get(F, e);
+    if (scale) v *= (double)scale[e / bb];
+    out[e] = v;
+}
+
+}  // namespace cpg
+
+using namespace cpg;
+
+// ---- handle --------------------------------------------------------------------------
+struct cpg_prec_s {
+    int kind = CPG_PC_NONE, precision = CPG_FP64;
+    int n = 0, bl = 0, nnzb = 0;
+    // pattern-derived lists
+    DevBuf<int> diag, lptr, lslot, lcol, ltrans, uptr, uslot, ucol;
+    DevBuf<int> frows, brows;
+    std::vector<int> flev, blev;  // level pointers into frows / brows
+    // storage
+    DevBuf<unsigned char> F;      // factor blocks in `precision`
+    DevBuf<float> scale;          // per-block scale (BF16)
+    DevBuf<double> DD64;          // FP64 D^-1 (DILU)
+    DevBuf<unsigned char> ybuf, zbuf;
+    DevBuf<int> status;
+    size_t elem = 8;
+
+    RowLists lists(bool fwd) const {
+        RowLists L;
+        L.rows = fwd ? frows.p : brows.p;
+        L.diag = diag.p;
+        L.lptr = lptr.p; L.lslot = lslot.p; L.lcol = lcol.p; L.ltrans = ltrans.p;
+        L.uptr = uptr.p; L.uslot = uslot.p; L.ucol = ucol.p;
+        return L;
+    }
+};
+
+namespace {
+
+size_t elem_size(int prec) { return prec == CPG_FP64 ? 8 : prec == CPG_FP32 ? 4 : 2; }
+
+template <class T>
+T *Fp(cpg_prec_s &P) { return reinterpret_cast<T *>(P.F.p); }
+
+template <class K>
+void set_lds(K kernel, size_t bytes) {
+    if (bytes > 64 * 1024)
+        CPG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)bytes));
+}
+
+template <class T, int NE>
+void factor_dilu_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
+    const int bl = P.bl;
+    const size_t lds = (size_t)(4 * bl * bl + bl + 8) * sizeof(double);
+    CPG_CHECK(lds <= 160 * 1024, CPG_ERR_UNSUPPORTED, "block size %d too large for the LDS factor kernel", bl);
+    auto kern = k_dilu_factor<T, NE>;
+    set_lds(kern, lds);
+    RowLists L = P.lists(true);
+    for (size_t lv = 0; lv + 1 < P.flev.size(); lv++) {
+        const int nr = P.flev[lv + 1] - P.flev[lv];
+        if (!nr) continue;
+        RowLists Ll = L;
+        Ll.rows = P.frows.p + P.flev[lv];
+        hipLaunchKernelGGL(kern, dim3(nr), dim3(WG), lds, s, bl, Ll, A.blocks.p, Fp<T>(P),
+                           P.precision == CPG_BF16 ? P.scale.p : nullptr, P.DD64.p, P.status.p);
+        CPG_LAUNCH_CHECK();
+    }
+}
+
+template <class T, int NE>
+void factor_bj_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
+    const int bl = P.bl;
+    const size_t lds = (size_t)(2 * bl * bl + bl + 8) * sizeof(double);
+    CPG_CHECK(lds <= 160 * 1024, CPG_ERR_UNSUPPORTED, "block size %d too large for the LDS factor kernel", bl);
+    auto kern = k_bjacobi_factor<T, NE>;
+    set_lds(kern, lds);
+    hipLaunchKernelGGL(kern, dim3(P.n), dim3(WG), lds, s, bl, P.diag.p, A.blocks.p, Fp<T>(P),
+                       P.precision == CPG_BF16 ? P.scale.p : nullptr, P.status.p);
+    CPG_LAUNCH_CHECK();
+}
+
+template <class T>
+void factor_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
+    const int ne = ceil_div((long long)P.bl * P.bl, WG);
+#define CPG_NE_CASE(NEV)                                                    \
+    if (ne <= NEV) {                                                        \
+        if (P.kind == CPG_PC_DILU) factor_dilu_t<T, NEV>(P, A, s);          \
+        else factor_bj_t<T, NEV>(P, A, s);                                  \
+        return;                                                             \
+    }
+    CPG_NE_CASE(1) CPG_NE_CASE(2) CPG_NE_CASE(4) CPG_NE_CASE(7) CPG_NE_CASE(15) CPG_NE_CASE(29) CPG_NE_CASE(57)
+#undef CPG_NE_CASE
+    CPG_CHECK(false, CPG_ERR_UNSUPPORTED, "block size %d too large", P.bl);
+}
+
+template <class T, int VW>
+void apply_t(cpg_prec_s &P, double *x, hipStream_t s) {
+    using A = typename Store<T>::acc;
+    const int bl = P.bl;
+    const int tpc = bl / VW;
+    const int G = WG / tpc;
+    const size_t lds = (size_t)(G + 2) * bl * sizeof(A);
+    const float *sc = P.precision == CPG_BF16 ? P.scale.p : nullptr;
+    if (P.kind == CPG_PC_BJACOBI) {
+        hipLaunchKernelGGL((k_bjacobi_apply<T, VW>), dim3(P.n), dim3(WG), lds, s, bl, Fp<T>(P), sc, x);
+        CPG_LAUNCH_CHECK();
+        return;
+    }
+    A *y = reinterpret_cast<A *>(P.ybuf.p);
+    A *z = reinterpret_cast<A *>(P.zbuf.p);
+    RowLists Lf = P.lists(true);
+    for (size_t lv = 0; lv + 1 < P.flev.size(); lv++) {
+        const int nr = P.flev[lv + 1] - P.flev[lv];
+        if (!nr) continue;
+        RowLists Ll = Lf;
+        Ll.rows = P.frows.p + P.flev[lv];
+        hipLaunchKernelGGL((k_dilu_forward<T, VW>), dim3(nr), dim3(WG), lds, s, bl, Ll, Fp<T>(P), sc, x, y, z);
+        CPG_LAUNCH_CHECK();
+    }
+    RowLists Lb = P.lists(false);
+    for (size_t lv = 0; lv + 1 < P.blev.size(); lv++) {
+        const int nr = P.blev[lv + 1] - P.blev[lv];
+        if (!nr) continue;
+        RowLists Ll = Lb;
+        Ll.rows = P.brows.p + P.blev[lv];
+        hipLaunchKernelGGL((k_dilu_backward<T, VW>), dim3(nr), dim3(WG), lds, s, bl, Ll, Fp<T>(P), sc, z, x);
+        CPG_LAUNCH_CHECK();
+    }
+}
+
+template <class T>
+void apply_dispatch(cpg_prec_s &P, double *x, hipStream_t s) {
+    switch (VecW<T>::pick(P.bl)) {
+    case 8: if constexpr (16 / sizeof(T) >= 8) { apply_t<T, 8>(P, x, s); return; } break;
+    case 4: if constexpr (16 / sizeof(T) >= 4) { apply_t<T, 4>(P, x, s); return; } break;
+    case 2: apply_t<T, 2>(P, x, s); return;
+    default: apply_t<T, 1>(P, x, s); return;
+    }
+    apply_t<T, 1>(P, x, s);
+}
+
+// level schedule from the pattern: level(i) = 1 + max level(j) over dependencies
+void levels(int n, const std::vector<int> &ptr, const std::vector<int> &cols, bool forward, std::vector<int> &lev_ptr,
+            std::vector<int> &rows) {
+    std::vector<int> lev(n, 0);
+    int nl = 0;
+    if (forward) {
+        for (int i = 0; i < n; i++) {
+            int l = 0;
+            for (int q = ptr[i]; q < ptr[i + 1]; q++) l = std::max(l, lev[cols[q]] + 1);
+            lev[i] = l;
+            nl = std::max(nl, l + 1);
+        }
+    } else {
+        for (int i = n - 1; i >= 0; i--) {
+            int l = 0;
+            for (int q = ptr[i]; q < ptr[i + 1]; q++) l = std::max(l, lev[cols[q]] + 1);
+            lev[i] = l;
+            nl = std::max(nl, l + 1);
+        }
+    }
+    lev_ptr.assign(nl + 1, 0);
+    for (int i = 0; i < n; i++) lev_ptr[lev[i] + 1]++;
+    for (int l = 0; l < nl; l++) lev_ptr[l + 1] += lev_ptr[l];
+    rows.assign(n, 0);
+    std::vector<int> fill(lev_ptr.begin(), lev_ptr.end() - 1);
+    for (int i = 0; i < n; i++) rows[fill[lev[i]]++] = i;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
+    return guard([&] {
+        CPG_CHECK(A && out, CPG_ERR_ARG, "null argument");
+        CPG_CHECK(kind == CPG_PC_NONE || kind == CPG_PC_BJACOBI || kind == CPG_PC_DILU, CPG_ERR_ARG, "bad kind %d", kind);
+        CPG_CHECK(precision >= CPG_FP64 && precision <= CPG_BF16, CPG_ERR_ARG, "bad precision %d", precision);
+        auto *P = new cpg_prec_s;
+        try {
+            P->kind = kind;
+            P->precision = precision;
+            P->n = A->n;
+            P->bl = A->bl;
+            P->nnzb = A->nnzb;
+            P->elem = elem_size(precision);
+            const int n = A->n, bl = A->bl;
+            const size_t bb = (size_t)bl * bl;
+            const auto &rp = A->h_rowptr;
+            const auto &ci = A->h_colind;
+            std::vector<int> diag(n, -1), lptr(n + 1, 0), uptr(n + 1, 0), lslot, lcol, ltrans, uslot, ucol;
+            for (int i = 0; i < n; i++) {
+                for (int s = rp[i]; s < rp[i + 1]; s++) {
+                    const int j = ci[s];
+                    if (j == i) {
+                        if (diag[i] < 0) diag[i] = s;
+                    } else if (j < i) {
+                        lslot.push_back(s);
+                        lcol.push_back(j);
+                        int st = -1;
+                        for (int s2 = rp[j]; s2 < rp[j + 1]; s2++)
+                            if (ci[s2] == i) { st = s2; break; }
+                        CPG_CHECK(kind != CPG_PC_DILU || st >= 0, CPG_ERR_ARG,
+                                  "pattern is not structurally symmetric at (%d,%d)", i, j);
+                        ltrans.push_back(st);
+                    } else {
+                        uslot.push_back(s);
+                        ucol.push_back(j);
+                    }
+                }
+                CPG_CHECK(kind == CPG_PC_NONE || diag[i] >= 0, CPG_ERR_ARG, "row %d has no diagonal block", i);
+                lptr[i + 1] = (int)lslot.size();
+                uptr[i + 1] = (int)uslot.size();
+            }
+            P->diag.upload(diag);
+            P->lptr.upload(lptr);
+            P->uptr.upload(uptr);
+            P->lslot.upload(lslot);
+            P->lcol.upload(lcol);
+            P->ltrans.upload(ltrans);
+            P->uslot.upload(uslot);
+            P->ucol.upload(ucol);
+            std::vector<int> frows, brows;
+            levels(n, lptr, lcol, true, P->flev, frows);
+            levels(n, uptr, ucol, false, P->blev, brows);
+            P->frows.upload(frows);
+            P->brows.upload(brows);
+            P->status.alloc(1);
+            if (kind == CPG_PC_DILU) {
+                P->F.alloc((size_t)A->nnzb * bb * P->elem);
+                if (precision == CPG_BF16) P->scale.alloc(A->nnzb);
+                P->DD64.alloc((size_t)n * bb);
+                const size_t ae = precision == CPG_FP64 ? 8 : 4;
+                P->ybuf.alloc((size_t)n * bl * ae);
+                P->zbuf.alloc((size_t)n * bl * ae);
+            } else if (kind == CPG_PC_BJACOBI) {
+                P->F.alloc((size_t)n * bb * P->elem);
+                if (precision == CPG_BF16) P->scale.alloc(n);
+            }
+        } catch (...) {
+            delete P;
+            throw;
+        }
+        *out = P;
+    });
+}
+
+int cpg_prec_destroy(cpg_prec P) {
+    return guard([&] { delete P; });
+}
+
+int cpg_prec_levels(cpg_prec P, int *n_levels) {
+    return guard([&] {
+        CPG_CHECK(P && n_levels, CPG_ERR_ARG, "null argument");
+        *n_levels = (int)P->flev.size() - 1;
+    });
+}
+
+int cpg_prec_factor(cpg_prec P, cpg_bsr A, void *stream) {
+    return guard([&] {
+        CPG_CHECK(P && A, CPG_ERR_ARG, "null argument");
+        CPG_CHECK(A->n == P->n && A->bl == P->bl && A->nnzb == P->nnzb, CPG_ERR_ARG, "matrix does not match");
+        if (P->kind == CPG_PC_NONE || P->n == 0) return;
+        hipStream_t s = S(stream);
+        CPG_HIP(hipMemsetAsync(P->status.p, 0, sizeof(int), s));
+        switch (P->precision) {
+        case CPG_FP64: factor_t<double>(*P, *A, s); break;
+        case CPG_FP32: factor_t<float>(*P, *A, s); break;
+        default: factor_t<bf16>(*P, *A, s); break;
+        }
+        int st = 0;
+        CPG_HIP(hipMemcpyAsync(&st, P->status.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        CPG_HIP(hipStreamSynchronize(s));
+        CPG_CHECK(st == 0, CPG_ERR_SINGULAR, "zero pivot in a diagonal block inverse");
+    });
+}
+
+int cpg_prec_apply(cpg_prec P, double *x, void *stream) {
+    return guard([&] {
+        CPG_CHECK(P && x, CPG_ERR_ARG, "null argument");
+        if (P->kind == CPG_PC_NONE || P->n == 0) return;
+        hipStream_t s = S(stream);
+        switch (P->precision) {
+        case CPG_FP64: apply_dispatch<double>(*P, x, s); break;
+        case CPG_FP32: apply_dispatch<float>(*P, x, s); break;
+        default: apply_dispatch<bf16>(*P, x, s); break;
+        }
+    });
+}
+
+int cpg_prec_download(cpg_prec P, double *host_blocks) {
+    return guard([&] {
+        CPG_CHECK(P && host_blocks, CPG_ERR_ARG, "null argument");
+        if (P->kind == CPG_PC_NONE) return;
+        const size_t bb = (size_t)P->bl * P->bl;
+        const size_t nblk = P->kind == CPG_PC_DILU ? P->nnzb : P->n;
+        const size_t cnt = nblk * bb;
+        DevBuf<double> tmp(cnt);
+        const float *sc = P->precision == CPG_BF16 ? P->scale.p : nullptr;
+        const int g = ceil_div(cnt, 256);
+        switch (P->precision) {
+        case CPG_FP64: hipLaunchKernelGGL(k_to_fp64<double>, dim3(g), dim3(256), 0, nullptr, Fp<double>(*P), sc, bb, cnt, tmp.p); break;
+        case CPG_FP32: hipLaunchKernelGGL(k_to_fp64<float>, dim3(g), dim3(256), 0, nullptr, Fp<float>(*P), sc, bb, cnt, tmp.p); break;
+        default: hipLaunchKernelGGL(k_to_fp64<bf16>, dim3(g), dim3(256), 0, nullptr, Fp<bf16>(*P), sc, bb, cnt, tmp.p); break;
+        }
+        CPG_LAUNCH_CHECK();
+        CPG_HIP(hipMemcpy(host_blocks, tmp.p, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,323 @@
+"""ctypes binding of ``libcpolyg_hip.so`` (the C ABI in ``include/cpolyg.h``).
+
+Device vectors are passed as torch tensors on ``cuda`` (torch is only the
+allocator and stream provider here) or as raw device addresses.  Every call
+checks the returned status and raises :class:`CpgError` with the library's
+message.  There is no CPU fallback: loading fails loudly when the library has
+not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import spec as _spec
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcpolyg_hip.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "cpolyg.h")
+
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+
+class CpgError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"cpolyg status {status}: {msg}")
+        self.status = status
+
+
+class EqParams(C.Structure):
+    _fields_ = [("kind", C.c_int), ("beta", C.c_double * 6), ("gamma", C.c_double), ("bc_kind", C.c_int),
+                ("bc_par", C.c_double * 8), ("periodic_trace", C.c_int), ("jacobian", C.c_int)]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("gmres_mode", C.c_int), ("restart", C.c_int), ("rtol", C.c_double), ("maxit", C.c_int),
+                ("pc_kind", C.c_int), ("precision", C.c_int), ("newton_tol", C.c_double),
+                ("newton_maxit", C.c_int)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("newton_iters", C.c_int), ("gmres_iters", C.c_int), ("linear_solves", C.c_int),
+                ("status", C.c_int), ("last_resid", C.c_double), ("ms_residual", C.c_double),
+                ("ms_jacobian", C.c_double), ("ms_factor", C.c_double), ("ms_gmres", C.c_double),
+                ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def lib():
+    """Load the library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.cpg_last_error.restype = C.c_char_p
+        _lib.cpg_status_string.restype = C.c_char_p
+    return _lib
+
+
+def _chk(st: int):
+    if st != 0:
+        raise CpgError(st, lib().cpg_last_error().decode())
+
+
+def declared_symbols() -> list[str]:
+    """Function names declared in include/cpolyg.h."""
+    import re
+
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(cpg_[a-z0-9_]+)\s*\(", txt)))
+
+
+def ptr(t) -> int:
+    """Device address of a torch tensor (or an int address)."""
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def _ia(a):
+    a = np.ascontiguousarray(a, np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def _da(a):
+    a = np.ascontiguousarray(a, np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+class Bsr:
+    """Device BSR matrix (``BlockMatrix``)."""
+
+    def __init__(self, rowptr, colind, bl, handle=None):
+        self.rowptr = np.ascontiguousarray(rowptr, np.int32)
+        self.colind = np.ascontiguousarray(colind, np.int32)
+        self.n = len(self.rowptr) - 1
+        self.bl = int(bl)
+        self.nnzb = len(self.colind)
+        if handle is None:
+            h = _vp()
+            _chk(lib().cpg_bsr_create(self.n, self.bl, self.nnzb, self.rowptr.ctypes.data_as(_ip),
+                                      self.colind.ctypes.data_as(_ip), C.byref(h)))
+            self.h = h
+        else:
+            self.h = handle
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.cpg_bsr_destroy(self.h)
+            self.h = None
+
+    def upload(self, blocks, stream=None):
+        b, bp = _da(blocks)
+        assert b.size == self.nnzb * self.bl * self.bl
+        _chk(lib().cpg_bsr_upload(self.h, bp, _stream(stream)))
+        _chk(lib().cpg_stream_sync(_stream(stream)))
+
+    def download(self):
+        out = np.zeros((self.nnzb, self.bl, self.bl))
+        _chk(lib().cpg_bsr_download(self.h, out.ctypes.data_as(_dp), None))
+        return out
+
+    def blocks_colmajor(self):
+        """Blocks as (nnzb, bl, bl) with [s, r, c] = entry (r, c)."""
+        return self.download().transpose(0, 2, 1)
+
+    def scale(self, s, stream=None):
+        _chk(lib().cpg_bsr_scale(self.h, C.c_double(s), _stream(stream)))
+
+    def matvec(self, x, y, stream=None):
+        _chk(lib().cpg_bsr_matvec(self.h, _vp(ptr(x)), _vp(ptr(y)), _stream(stream)))
+
+
+class Prec:
+    """Device preconditioner (``BlockJacobi`` / ``BlockILU0``)."""
+
+    def __init__(self, A: Bsr, kind=_spec.PC_DILU, precision=_spec.PREC_FP64):
+        h = _vp()
+        _chk(lib().cpg_prec_create(A.h, kind, precision, C.byref(h)))
+        self.h = h
+        self.A = A
+        self.kind = kind
+        self.precision = precision
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.cpg_prec_destroy(self.h)
+            self.h = None
+
+    def factor(self, A: Bsr | None = None, stream=None):
+        _chk(lib().cpg_prec_factor(self.h, (A or self.A).h, _stream(stream)))
+
+    def apply(self, x, stream=None):
+        _chk(lib().cpg_prec_apply(self.h, _vp(ptr(x)), _stream(stream)))
+
+    def download(self):
+        nblk = self.A.nnzb if self.kind == _spec.PC_DILU else self.A.n
+        out = np.zeros((nblk, self.A.bl, self.A.bl))
+        _chk(lib().cpg_prec_download(self.h, out.ctypes.data_as(_dp)))
+        return out
+
+    def levels(self):
+        n = C.c_int()
+        _chk(lib().cpg_prec_levels(self.h, C.byref(n)))
+        return n.value
+
+
+class Krylov:
+    def __init__(self, n, bl, restart, mode):
+        h = _vp()
+        _chk(lib().cpg_krylov_create(n, bl, restart, mode, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.cpg_krylov_destroy(self.h)
+            self.h = None
+
+    def gmres(self, A: Bsr, P: Prec | None, b, x, tol, maxit, stream=None):
+        t = C.c_double(tol)
+        it = C.c_int(maxit)
+        _chk(lib().cpg_gmres(self.h, A.h, P.h if P is not None else None, _vp(ptr(b)), _vp(ptr(x)), C.byref(t),
+                             C.byref(it), _stream(stream)))
+        return t.value, it.value
+
+
+def eq_params(s: _spec.EquationSpec) -> EqParams:
+    e = EqParams()
+    e.kind = s.kind
+    for j in range(6):
+        e.beta[j] = s.beta[j]
+    e.gamma = s.gamma
+    e.bc_kind = s.bc_kind
+    for j in range(8):
+        e.bc_par[j] = s.bc_par[j] if j < len(s.bc_par) else 0.0
+    e.periodic_trace = s.periodic_trace
+    e.jacobian = s.jacobian
+    return e
+
+
+def solver_params(s: _spec.SolverSpec) -> SolverParams:
+    return SolverParams(s.gmres_mode, s.restart, s.rtol, s.maxit, s.pc_kind, s.precision, s.newton_tol,
+                        s.newton_maxit)
+
+
+class Mesh:
+    def __init__(self, m):
+        self.mesh = m
+        self._keep = [np.ascontiguousarray(a, dt) for a, dt in (
+            (m.v, np.float64), (m.pptr, np.int32), (m.pv, np.int32), (m.p2p, np.int32), (m.bc, np.int32),
+            (m.tptr, np.int32), (m.txy, np.float64), (m.bb, np.float64))]
+        v, pptr, pv, p2p, bc, tptr, txy, bb = self._keep
+        h = _vp()
+        _chk(lib().cpg_mesh_create(m.n_cells, len(v), v.ctypes.data_as(_dp), pptr.ctypes.data_as(_ip),
+                                   pv.ctypes.data_as(_ip), p2p.ctypes.data_as(_ip), len(bc), bc.ctypes.data_as(_ip),
+                                   tptr.ctypes.data_as(_ip), txy.ctypes.data_as(_dp), bb.ctypes.data_as(_dp),
+                                   C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.cpg_mesh_destroy(self.h)
+            self.h = None
+
+
+class Equation:
+    """Device ``Equation`` (residual, Jacobian, mass matrix) for one mesh and degree."""
+
+    def __init__(self, mesh: Mesh, s: _spec.EquationSpec, deg: int):
+        self.mesh = mesh
+        self.spec = s
+        self.deg = deg
+        self.params = eq_params(s)
+        h = _vp()
+        _chk(lib().cpg_equation_create(mesh.h, C.byref(self.params), deg, C.byref(h)))
+        self.h = h
+        n, bl, nnzb = C.c_int(), C.c_int(), C.c_int()
+        _chk(lib().cpg_equation_info(self.h, C.byref(n), C.byref(bl), C.byref(nnzb)))
+        self.n, self.bl, self.nnzb = n.value, bl.value, nnzb.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.cpg_equation_destroy(self.h)
+            self.h = None
+
+    def pattern(self):
+        rp = np.zeros(self.n + 1, np.int32)
+        ci = np.zeros(self.nnzb, np.int32)
+        _chk(lib().cpg_jacobian_pattern(self.h, rp.ctypes.data_as(_ip), ci.ctypes.data_as(_ip)))
+        return rp, ci
+
+    def jacobian_matrix(self) -> Bsr:
+        h = _vp()
+        _chk(lib().cpg_jacobian_create(self.h, C.byref(h)))
+        rp, ci = self.pattern()
+        return Bsr(rp, ci, self.bl, handle=h)
+
+    def residual(self, u, t, b, stream=None):
+        _chk(lib().cpg_assemble_residual(self.h, _vp(ptr(u)), C.c_double(t), _vp(ptr(b)), _stream(stream)))
+
+    def jacobian(self, u, t, J: Bsr, scale=1.0, add_mass=False, stream=None):
+        _chk(lib().cpg_assemble_jacobian(self.h, _vp(ptr(u)), C.c_double(t), C.c_double(scale), int(add_mass), J.h,
+                                         _stream(stream)))
+
+    def mass_matvec(self, x, y, stream=None):
+        _chk(lib().cpg_mass_matvec(self.h, _vp(ptr(x)), _vp(ptr(y)), _stream(stream)))
+
+    def mass(self):
+        out = np.zeros((self.n, self.bl, self.bl))
+        _chk(lib().cpg_mass_download(self.h, out.ctypes.data_as(_dp)))
+        return out
+
+    def l2norm(self, u, stream=None):
+        out = np.zeros(self.spec.nc)
+        _chk(lib().cpg_l2norm(self.h, _vp(ptr(u)), out.ctypes.data_as(_dp), _stream(stream)))
+        return out
+
+
+class Stepper:
+    """Device ``TimeStepper`` (DIRK / backward Euler / IRK / RK4)."""
+
+    def __init__(self, eq: Equation, method: int, nstages: int, solver: _spec.SolverSpec):
+        self.eq = eq
+        self.sp = solver_params(solver)
+        h = _vp()
+        _chk(lib().cpg_stepper_create(eq.h, method, nstages, C.byref(self.sp), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.cpg_stepper_destroy(self.h)
+            self.h = None
+
+    def advance(self, u, dt, t, unp1, stream=None) -> StepStats:
+        st = StepStats()
+        _chk(lib().cpg_stepper_advance(self.h, _vp(ptr(u)), C.c_double(dt), C.c_double(t), _vp(ptr(unp1)),
+                                       C.byref(st), _stream(stream)))
+        return st
+
+    def enable_kernel_timing(self, on=True):
+        _chk(lib().cpg_stepper_enable_kernel_timing(self.h, int(on)))
+
+    def kernel_timing(self):
+        nm, na = C.c_int(), C.c_int()
+        mm, ma = C.c_double(), C.c_double()
+        _chk(lib().cpg_stepper_kernel_timing(self.h, C.byref(nm), C.byref(mm), C.byref(na), C.byref(ma)))
+        return nm.value, mm.value, na.value, ma.value

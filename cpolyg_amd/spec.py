@@ -39,8 +39,10 @@ class EquationSpec:
 
     ``periodic_trace``: 0 reproduces the reference's periodic trace
     (``src/BoundaryConditions.cpp:18``), 1 evaluates the neighbour's full series.
-    ``offdiag``: 0 reproduces the reference's off-diagonal Euler Jacobian
-    (row-cell state, ``src/Equation.cpp:84,109``), 1 uses the exact derivative.
+    ``jacobian``: 0 reproduces the reference's Euler Jacobian (off-diagonal blocks
+    built from the row cell's state, ``src/Equation.cpp:84,109``, and its x flux
+    Jacobian entry ``H - (gamma-1)|u|^2``, ``eqns/Euler.cpp:157``); 1 is the exact
+    derivative of the residual.  Both give the same converged Newton solution.
     """
     kind: int = ADVECTION
     beta: tuple = (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)
@@ -48,7 +50,7 @@ class EquationSpec:
     bc_kind: int = BC_ZERO
     bc_par: tuple = (0.0,) * 8
     periodic_trace: int = 0
-    offdiag: int = 0
+    jacobian: int = 0
 
     @property
     def nc(self) -> int:

@@ -1,0 +1,180 @@
+/*
+ * cpolyg.h -- C ABI of the MI355X implicit DG library (libcpolyg_hip.so).
+ *
+ * Plain pointers and sizes only.  Vectors passed to compute calls are DEVICE
+ * pointers (HBM); host pointers appear only in create/upload/download calls.
+ * Every call returns an int status (CPG_OK = 0); cpg_last_error() gives the
+ * message of the last failure on the calling thread.  No exception crosses the
+ * ABI.  `stream` is a hipStream_t (NULL = the default stream); calls that take
+ * a stream are asynchronous unless stated otherwise.
+ *
+ * Layouts (same as the reference):
+ *   vector   u[i*bl + c*bs + k]     (MeshFn cube, src/MeshFn.cpp:14-21)
+ *   block    bl x bl column-major   (arma::mat)
+ *   BSR      rowptr[n+1], colind[nnzb], blocks[nnzb][bl][bl]
+ *            (BlockMatrix, src/BlockMatrix.h:10-77)
+ */
+#ifndef CPOLYG_H
+#define CPOLYG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum cpg_status {
+    CPG_OK = 0,
+    CPG_ERR_ARG = 1,           /* bad argument / shape */
+    CPG_ERR_HIP = 2,           /* HIP runtime error */
+    CPG_ERR_OOM = 3,           /* device allocation failed */
+    CPG_ERR_SINGULAR = 4,      /* zero pivot in a block LU / inverse */
+    CPG_ERR_NONFINITE = 5,     /* NaN/Inf in a residual or Krylov quantity */
+    CPG_ERR_NOT_CONVERGED = 6, /* Newton or GMRES hit its iteration limit */
+    CPG_ERR_UNSUPPORTED = 7    /* size/degree combination not compiled in */
+};
+
+typedef struct cpg_bsr_s *cpg_bsr;
+typedef struct cpg_prec_s *cpg_prec;
+typedef struct cpg_krylov_s *cpg_krylov;
+typedef struct cpg_mesh_s *cpg_mesh;
+typedef struct cpg_equation_s *cpg_equation;
+typedef struct cpg_stepper_s *cpg_stepper;
+
+const char *cpg_status_string(int status);
+const char *cpg_last_error(void);
+int cpg_version(void);
+
+/* ---- device memory helpers (for callers without their own allocator) ---- */
+int cpg_device_alloc(void **ptr, unsigned long long bytes);
+int cpg_device_free(void *ptr);
+int cpg_memcpy_h2d(void *dst, const void *src, unsigned long long bytes, void *stream);
+int cpg_memcpy_d2h(void *dst, const void *src, unsigned long long bytes, void *stream);   /* synchronous */
+int cpg_memcpy_d2d(void *dst, const void *src, unsigned long long bytes, void *stream);
+int cpg_stream_sync(void *stream);
+
+/* ---- block sparse row matrix ------------------------------------------------
+ * Replaces BlockMatrix (src/BlockMatrix.h:10-77): storage, matvec
+ * (src/BlockMatrix.cpp:28-49), operator*= (:297-308).  rowptr/colind are host
+ * arrays copied at create; the block array lives in HBM. */
+int cpg_bsr_create(int n_rows, int bl, int nnzb, const int *rowptr, const int *colind, cpg_bsr *out);
+int cpg_bsr_destroy(cpg_bsr A);
+int cpg_bsr_info(cpg_bsr A, int *n_rows, int *bl, int *nnzb);
+int cpg_bsr_upload(cpg_bsr A, const double *host_blocks, void *stream);
+int cpg_bsr_download(cpg_bsr A, double *host_blocks, void *stream);   /* synchronous */
+int cpg_bsr_device_blocks(cpg_bsr A, double **dev_blocks);
+int cpg_bsr_scale(cpg_bsr A, double s, void *stream);
+/* y = A x; x and y are device vectors of n_rows*bl doubles and must not alias */
+int cpg_bsr_matvec(cpg_bsr A, const double *x, double *y, void *stream);
+
+/* ---- preconditioners --------------------------------------------------------
+ * Replace Preconditioner::solve (src/Preconditioners.h:10-15), BlockJacobi
+ * (src/Preconditioners.cpp:8-47) and BlockILU0 = block DILU
+ * (src/Preconditioners.cpp:49-158).  The DILU sweeps are level-scheduled from
+ * the matrix pattern (a colour-ordered matrix gives one level per colour).
+ * Factors are stored in `precision` (CPG_FP64, CPG_FP32, CPG_BF16 with one
+ * power-of-two scale per block); FP32/BF16 factors are applied with FP32
+ * accumulation.  x is FP64 in and out. */
+enum { CPG_PC_NONE = 0, CPG_PC_BJACOBI = 1, CPG_PC_DILU = 2 };
+enum { CPG_FP64 = 0, CPG_FP32 = 1, CPG_BF16 = 2 };
+int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out);
+int cpg_prec_destroy(cpg_prec P);
+int cpg_prec_factor(cpg_prec P, cpg_bsr A, void *stream);
+int cpg_prec_apply(cpg_prec P, double *x, void *stream);   /* x <- M^-1 x, in place */
+/* factors converted to FP64 in A's slot layout: diag = D^-1, lower = L, upper = DU
+ * (DILU) or n inverse diagonal blocks (block Jacobi); synchronous */
+int cpg_prec_download(cpg_prec P, double *host_blocks);
+int cpg_prec_levels(cpg_prec P, int *n_levels);
+
+/* ---- GMRES ------------------------------------------------------------------
+ * Replaces BlockMatrix::gmres (src/BlockMatrix.cpp:79-163) and Jacobian::solve
+ * (src/Jacobian.cpp:68-97).  mode CPG_GMRES_LEFT_REF is the reference algorithm
+ * (left preconditioning, classical Gram-Schmidt, Givens, tolerance relative to
+ * ||P^-1 b||); CPG_GMRES_FLEXIBLE is right-preconditioned flexible GMRES with
+ * the tolerance relative to ||b||.  tol and maxit are in/out exactly as in the
+ * reference; x holds the initial guess on entry.  Synchronous on return. */
+enum { CPG_GMRES_LEFT_REF = 0, CPG_GMRES_FLEXIBLE = 1 };
+int cpg_krylov_create(int n_rows, int bl, int restart, int mode, cpg_krylov *out);
+int cpg_krylov_destroy(cpg_krylov K);
+int cpg_gmres(cpg_krylov K, cpg_bsr A, cpg_prec P, const double *b, double *x, double *tol, int *maxit,
+              void *stream);
+
+/* ---- mesh and equations -----------------------------------------------------
+ * Mesh arrays are PolyMesh's public fields (src/PolyMesh.h:22-28) flattened:
+ * v[nv*2], polygon CSR pptr/pv (vertex lists), p2p (neighbour per edge, <0 is
+ * boundary id), bc[nbc*7] rows (periodic,p1,p2,a1,b1,a2,b2) for ids -1..-nbc,
+ * sub-triangles tptr/txy[ntri*6], bounding boxes bb[np*4]. */
+int cpg_mesh_create(int np, int nv, const double *v, const int *pptr, const int *pv, const int *p2p, int nbc,
+                    const int *bc, const int *tptr, const double *txy, const double *bb, cpg_mesh *out);
+int cpg_mesh_destroy(cpg_mesh M);
+
+enum { CPG_ADVECTION = 0, CPG_EULER = 1 };
+enum { CPG_BC_ZERO = 0, CPG_BC_CONSTANT = 1, CPG_BC_ISENTROPIC_VORTEX = 2, CPG_BC_REF_VORTEX = 3 };
+typedef struct {
+    int kind;               /* CPG_ADVECTION (nc=1) or CPG_EULER (nc=4) */
+    double beta[6];         /* advection: bx = b0+b1 x+b2 y, by = b3+b4 x+b5 y */
+    double gamma;           /* Euler ratio of specific heats */
+    int bc_kind;            /* Dirichlet functor for non-periodic boundary ids */
+    double bc_par[8];
+    int periodic_trace;     /* 0: reference trace (src/BoundaryConditions.cpp:18), 1: exact */
+    int jacobian;           /* Euler Jacobian form.  0: the reference's (off-diagonal blocks with the
+                               row cell's state, src/Equation.cpp:84,109; its A_x energy-row entry
+                               H-(gamma-1)|u|^2, eqns/Euler.cpp:157); 1: exact derivative */
+} cpg_eq_params;
+
+/* Equation (src/Equation.h:8-67) bound to a mesh and polynomial degree 1..4.
+ * Creation precomputes face geometry and the mass matrix (src/MassMatrix.cpp:30-94). */
+int cpg_equation_create(cpg_mesh M, const cpg_eq_params *params, int deg, cpg_equation *out);
+int cpg_equation_destroy(cpg_equation E);
+int cpg_equation_info(cpg_equation E, int *n_cells, int *bl, int *nnzb);
+/* Jacobian pattern (Jacobian::Jacobian, src/Jacobian.cpp:7-47) into host arrays */
+int cpg_jacobian_pattern(cpg_equation E, int *rowptr, int *colind);
+int cpg_jacobian_create(cpg_equation E, cpg_bsr *out);
+/* b = assemble(u, t)  (Equation::assemble, src/Equation.cpp:152-191) */
+int cpg_assemble_residual(cpg_equation E, const double *u, double t, double *b, void *stream);
+/* J = scale * jacobian(u, t) (+ mass matrix on the diagonal blocks if add_mass)
+ * (Equation::jacobian src/Equation.cpp:193-279, operator*= and J += M) */
+int cpg_assemble_jacobian(cpg_equation E, const double *u, double t, double scale, int add_mass, cpg_bsr J,
+                          void *stream);
+int cpg_mass_matvec(cpg_equation E, const double *x, double *y, void *stream);
+int cpg_mass_download(cpg_equation E, double *host_blocks);   /* n blocks bl x bl, synchronous */
+/* MeshFn::L2Norm (src/MeshFn.cpp:299-312) of u, nc values to host, synchronous */
+int cpg_l2norm(cpg_equation E, const double *u, double *out, void *stream);
+
+/* ---- time stepping ----------------------------------------------------------
+ * TimeStepper::advance (src/TimeIntegration.h:9-18) for DIRK (1 = backward
+ * Euler, 2 = SDIRK2, 3, 5 stages; src/TimeIntegration.cpp:31-151) with the
+ * Newton/GMRES loop on the device. */
+enum { CPG_STEP_DIRK = 0, CPG_STEP_IRK = 1, CPG_STEP_RK4 = 2 };
+typedef struct {
+    int gmres_mode;         /* CPG_GMRES_* */
+    int restart;
+    double rtol;
+    int maxit;
+    int pc_kind;            /* CPG_PC_* */
+    int precision;          /* factor storage CPG_FP64/FP32/BF16 */
+    double newton_tol;      /* absolute, on max_c L2Norm of the Newton residual */
+    int newton_maxit;
+} cpg_solver_params;
+
+typedef struct {
+    int newton_iters;
+    int gmres_iters;
+    int linear_solves;
+    int status;
+    double last_resid;
+    double ms_residual, ms_jacobian, ms_factor, ms_gmres, ms_total;
+} cpg_step_stats;
+
+int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver_params *params, cpg_stepper *out);
+int cpg_stepper_destroy(cpg_stepper S);
+/* unp1 = advance(u, dt, t); u and unp1 are device vectors; synchronous */
+int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, double *unp1, cpg_step_stats *stats,
+                        void *stream);
+/* per-kernel timing of BSR matvec and preconditioner apply launches inside GMRES
+ * (HIP events on the launch stream); enable before advance, read after */
+int cpg_stepper_enable_kernel_timing(cpg_stepper S, int enable);
+int cpg_stepper_kernel_timing(cpg_stepper S, int *n_matvec, double *ms_matvec, int *n_apply, double *ms_apply);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -28,7 +28,7 @@ class OMesh(C.Structure):
 class OEq(C.Structure):
     _fields_ = [("kind", C.c_int), ("beta", C.c_double * 6), ("gamma", C.c_double),
                 ("bc_kind", C.c_int), ("bc_par", C.c_double * 8), ("periodic_trace", C.c_int),
-                ("offdiag", C.c_int)]
+                ("jacobian", C.c_int)]
 
 
 class OSolver(C.Structure):
@@ -97,7 +97,7 @@ class Oracle:
         for j in range(8):
             e.bc_par[j] = spec.bc_par[j] if j < len(spec.bc_par) else 0.0
         e.periodic_trace = spec.periodic_trace
-        e.offdiag = spec.offdiag
+        e.jacobian = spec.jacobian
         return e
 
     def mass(self, deg, nc):

@@ -8,7 +8,7 @@
  *   - periodic traces use the truncated, mis-indexed Legendre series of
  *     src/BoundaryConditions.cpp:18 (periodic_trace = 0),
  *   - off-diagonal Euler Jacobian blocks use the row cell's state and the
- *     row cell's wave-speed derivative (src/Equation.cpp:84,109; offdiag = 0),
+ *     row cell's wave-speed derivative (src/Equation.cpp:84,109; jacobian = 0),
  *   - "BlockILU0" is block DILU (src/Preconditioners.cpp:49-123),
  *   - GMRES is left-preconditioned classical Gram-Schmidt with Givens
  *     rotations and a tolerance relative to ||P^-1 b|| (src/BlockMatrix.cpp:79-163).
@@ -257,15 +257,18 @@ static void euler_flux(double g, const double *U, double *Fx, double *Fy, double
     *c = sqrt(g * P / rho);
 }
 
-/* flux Jacobians, row-major 4x4 (Euler::fluxJacobian, Euler.cpp:134-165) */
-static void euler_fluxjac(double g, const double *U, double *Ax, double *Ay)
+/* flux Jacobians, row-major 4x4 (Euler::fluxJacobian, Euler.cpp:134-165).  The
+ * reference's x-Jacobian has H - (gamma-1)|u|^2 in the energy row, momentum-x column
+ * (Euler.cpp:157); the derivative is H - (gamma-1) u^2.  ref != 0 reproduces it. */
+static void euler_fluxjac(double g, const double *U, double *Ax, double *Ay, int ref)
 {
     double rho = U[0], u = U[1] / rho, v = U[2] / rho, E = U[3] / rho;
     double usq = u * u + v * v, e = E - 0.5 * usq, P = (g - 1) * rho * e, H = E + P / rho;
     double ax[16] = {0, 1, 0, 0,
                      -u * u + 0.5 * (g - 1) * usq, (3 - g) * u, -(g - 1) * v, g - 1,
                      -u * v, v, u, 0,
-                     u * (0.5 * (g - 1) * usq - H), H - (g - 1) * u * u, -(g - 1) * u * v, g * u};
+                     u * (0.5 * (g - 1) * usq - H), ref ? H - (g - 1) * usq : H - (g - 1) * u * u,
+                     -(g - 1) * u * v, g * u};
     double ay[16] = {0, 0, 1, 0,
                      -u * v, v, u, 0,
                      -v * v + 0.5 * (g - 1) * usq, -(g - 1) * u, (3 - g) * v, g - 1,
@@ -344,7 +347,7 @@ static void numflux(const o_eq *eq, const double *Um, const double *Up, double x
 
 static void fluxjac(const o_eq *eq, const double *U, double x, double y, double *Ax, double *Ay)
 {
-    if (eq->kind == O_EULER) euler_fluxjac(eq->gamma, U, Ax, Ay);
+    if (eq->kind == O_EULER) euler_fluxjac(eq->gamma, U, Ax, Ay, eq->jacobian == 0);
     else adv_beta(eq, x, y, Ax, Ay);
 }
 
@@ -355,7 +358,7 @@ static void numfluxjac(const o_eq *eq, const double *U, const double *U2, double
 {
     if (eq->kind == O_EULER) {
         double Ax[16], Ay[16], d[4], alpha;
-        euler_fluxjac(eq->gamma, U, Ax, Ay);
+        euler_fluxjac(eq->gamma, U, Ax, Ay, eq->jacobian == 0);
         euler_alpha_deriv(eq->gamma, U, U2, nx, ny, &alpha, d);
         for (int r = 0; r < 4; r++)
             for (int c = 0; c < 4; c++)
@@ -651,10 +654,10 @@ int o_jacobian(const o_mesh *m, const o_eq *eq, int deg, const double *u, double
                 outer_state(m, eq, deg, u, nb, x, y, t, Up, phip);
                 numfluxjac(eq, Um, Up, x, y, nx, ny, +1, Jd);
                 if (coupled) {
-                    if (eq->offdiag && eq->kind == O_EULER) {
+                    if (eq->jacobian && eq->kind == O_EULER) {
                         /* exact: dFhat/dU+ = 0.5 (A(U+) n - alpha I + (U- - U+) dalpha/dU+^T) */
                         double Bx[16], By[16], dd[4], alpha;
-                        euler_fluxjac(eq->gamma, Up, Bx, By);
+                        euler_fluxjac(eq->gamma, Up, Bx, By, 0);
                         euler_alpha_deriv(eq->gamma, Up, Um, nx, ny, &alpha, dd);
                         for (int r = 0; r < 4; r++)
                             for (int c = 0; c < 4; c++)

@@ -41,7 +41,9 @@ typedef struct {
     int bc_kind;            /* Dirichlet functor */
     double bc_par[8];
     int periodic_trace;     /* 0: reference trace (src/BoundaryConditions.cpp:18), 1: exact */
-    int offdiag;            /* 0: reference off-diagonal Jacobian (Equation.cpp:84,109), 1: exact */
+    int jacobian;           /* Euler Jacobian form.  0: reference (row-cell state in off-diagonal
+                               blocks, Equation.cpp:84,109; energy-row entry of A_x, Euler.cpp:157);
+                               1: exact derivative of the residual */
 } o_eq;
 
 int o_nc(const o_eq *eq);

@@ -1,0 +1,156 @@
+"""GPU parity of the block-sparse algebra: BSR matvec, block DILU and block Jacobi
+(FP64 / FP32 / BF16 factors) and GMRES, through the C ABI, against the oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+from cpolyg_amd import hip
+from cpolyg_amd.spec import (EquationSpec, PC_BJACOBI, PC_DILU, PREC_BF16, PREC_FP32, PREC_FP64,
+                             GMRES_FLEXIBLE, GMRES_LEFT_REF)
+from problems import newton_matrix, rel, vortex_mesh, vortex_state, unit_periodic_mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def _euler_system(deg=3, nx=8):
+    m = vortex_mesh(nx)
+    orc = O.Oracle(m)
+    u = vortex_state(m, deg, perturb=0.02)
+    B = newton_matrix(orc, EquationSpec.euler(), deg, u)
+    return m, orc, B
+
+
+def _adv_system(deg, nx=8):
+    m = unit_periodic_mesh(nx)
+    orc = O.Oracle(m)
+    rng = np.random.default_rng(5)
+    from cpolyg_amd.spec import bs_of
+    u = rng.standard_normal(m.n_cells * bs_of(deg))
+    B = newton_matrix(orc, EquationSpec.reference_advection(), deg, u, dt=0.05)
+    return m, orc, B
+
+
+def _dev(torch_cuda, a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(torch_cuda)
+
+
+@pytest.mark.parametrize("kind,deg", [("euler", 3), ("euler", 4), ("euler", 1), ("adv", 1), ("adv", 2), ("adv", 3)])
+def test_bsr_matvec(cuda, kind, deg):
+    m, orc, B = _euler_system(deg) if kind == "euler" else _adv_system(deg)
+    bl = B.shape[1]
+    A = hip.Bsr(orc.rowptr, orc.colind, bl)
+    A.upload(B)
+    np.testing.assert_array_equal(A.download(), B)
+    x = np.random.default_rng(1).standard_normal(m.n_cells * bl)
+    xd = _dev(cuda, x)
+    yd = _dev(cuda, np.zeros_like(x))
+    A.matvec(xd, yd)
+    y = yd.cpu().numpy()
+    yo = O.bsr_matvec(orc.rowptr, orc.colind, B, x)
+    assert rel(y, yo) < 1e-13
+
+
+@pytest.mark.parametrize("deg", [3, 4, 1])
+def test_dilu_fp64_factor_and_apply(cuda, deg):
+    m, orc, B = _euler_system(deg)
+    bl = B.shape[1]
+    A = hip.Bsr(orc.rowptr, orc.colind, bl)
+    A.upload(B)
+    P = hip.Prec(A, PC_DILU, PREC_FP64)
+    assert P.levels() == m.n_colours
+    P.factor()
+    F = P.download()
+    Fo, info = O.dilu_factor(orc.rowptr, orc.colind, B)
+    assert info == 0
+    assert rel(F, Fo) < 1e-11
+    x = np.random.default_rng(2).standard_normal(m.n_cells * bl)
+    xd = _dev(cuda, x)
+    P.apply(xd)
+    assert rel(xd.cpu().numpy(), O.dilu_apply(orc.rowptr, orc.colind, Fo, x)) < 1e-11
+
+
+@pytest.mark.parametrize("prec,tol", [(PREC_FP32, 2e-5), (PREC_BF16, 2e-2)])
+def test_dilu_reduced_precision(cuda, prec, tol):
+    m, orc, B = _euler_system(3)
+    bl = B.shape[1]
+    A = hip.Bsr(orc.rowptr, orc.colind, bl)
+    A.upload(B)
+    P = hip.Prec(A, PC_DILU, prec)
+    P.factor()
+    F = P.download()
+    Fo, _ = O.dilu_factor(orc.rowptr, orc.colind, B)
+    # stored factors are the FP64 factors rounded once (per block, relative to the block's max)
+    err = np.abs(F - Fo).reshape(len(F), -1).max(1) / np.maximum(np.abs(Fo).reshape(len(F), -1).max(1), 1e-300)
+    assert err.max() < (1e-6 if prec == PREC_FP32 else 8e-3)
+    x = np.random.default_rng(3).standard_normal(m.n_cells * bl)
+    xd = _dev(cuda, x)
+    P.apply(xd)
+    assert rel(xd.cpu().numpy(), O.dilu_apply(orc.rowptr, orc.colind, Fo, x)) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [(PREC_FP64, 1e-11), (PREC_FP32, 2e-5), (PREC_BF16, 2e-2)])
+def test_block_jacobi(cuda, prec, tol):
+    m, orc, B = _euler_system(3)
+    bl = B.shape[1]
+    A = hip.Bsr(orc.rowptr, orc.colind, bl)
+    A.upload(B)
+    P = hip.Prec(A, PC_BJACOBI, prec)
+    P.factor()
+    LU, piv, _ = O.bjacobi_factor(orc.rowptr, orc.colind, B)
+    x = np.random.default_rng(4).standard_normal(m.n_cells * bl)
+    xd = _dev(cuda, x)
+    P.apply(xd)
+    assert rel(xd.cpu().numpy(), O.bjacobi_apply(LU, piv, x)) < tol
+
+
+@pytest.mark.parametrize("mode", [GMRES_LEFT_REF, GMRES_FLEXIBLE])
+@pytest.mark.parametrize("pc", [PC_DILU, PC_BJACOBI])
+def test_gmres_matches_oracle(cuda, mode, pc):
+    m, orc, B = _euler_system(3)
+    bl = B.shape[1]
+    n = m.n_cells
+    A = hip.Bsr(orc.rowptr, orc.colind, bl)
+    A.upload(B)
+    P = hip.Prec(A, pc, PREC_FP64)
+    P.factor()
+    b = np.random.default_rng(6).standard_normal(n * bl)
+    K = hip.Krylov(n, bl, 20, mode)
+    bd = _dev(cuda, b)
+    xd = _dev(cuda, np.zeros_like(b))
+    tol, it = K.gmres(A, P, bd, xd, 1e-10, 2000)
+    xo, tolo, ito = O.gmres(orc.rowptr, orc.colind, B, b, m=20, tol=1e-10, maxit=2000, pc_kind=pc, mode=mode)
+    x = xd.cpu().numpy()
+    assert abs(it - ito) <= 1, (it, ito)
+    assert tol < 1e-10 and tolo < 1e-10
+    r = b - O.bsr_matvec(orc.rowptr, orc.colind, B, x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-8
+    assert rel(x, xo) < 1e-7
+
+
+def test_gmres_zero_rhs(cuda):
+    m, orc, B = _euler_system(1)
+    bl = B.shape[1]
+    A = hip.Bsr(orc.rowptr, orc.colind, bl)
+    A.upload(B)
+    P = hip.Prec(A, PC_DILU, PREC_FP64)
+    P.factor()
+    K = hip.Krylov(m.n_cells, bl, 10, GMRES_LEFT_REF)
+    bd = _dev(cuda, np.zeros(m.n_cells * bl))
+    xd = _dev(cuda, np.ones(m.n_cells * bl))
+    tol, it = K.gmres(A, P, bd, xd, 1e-10, 100)
+    assert it == 0 and tol == 0.0
+    assert np.all(xd.cpu().numpy() == 0.0)
+
+
+def test_singular_block_reports_status(cuda):
+    m, orc, B = _euler_system(1)
+    B = B.copy()
+    B[orc.rowptr[3]] = 0.0      # zero diagonal block of row 3
+    bl = B.shape[1]
+    A = hip.Bsr(orc.rowptr, orc.colind, bl)
+    A.upload(B)
+    P = hip.Prec(A, PC_BJACOBI, PREC_FP64)
+    with pytest.raises(hip.CpgError) as e:
+        P.factor()
+    assert e.value.status == 4

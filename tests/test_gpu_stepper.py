@@ -1,0 +1,87 @@
+"""GPU DIRK / backward-Euler / RK4 steps against the oracle's steppers."""
+import numpy as np
+import pytest
+
+import oracle as O
+from cpolyg_amd import hip, setups
+from cpolyg_amd.spec import (EquationSpec, SolverSpec, GMRES_FLEXIBLE, GMRES_LEFT_REF, PC_DILU, PC_BJACOBI,
+                             PREC_BF16, PREC_FP32, PREC_FP64, bs_of)
+from problems import rel, unit_periodic_mesh, vortex_mesh, vortex_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(cuda, a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(cuda)
+
+
+@pytest.mark.parametrize("nstages", [3, 1])
+@pytest.mark.parametrize("jacobian", [0, 1])
+def test_dirk_euler_matches_oracle(cuda, nstages, jacobian):
+    m = vortex_mesh(8)
+    deg = 2
+    spec = EquationSpec.euler(jacobian=jacobian)
+    u = vortex_state(m, deg)
+    solver = SolverSpec(gmres_mode=GMRES_LEFT_REF, restart=20, rtol=1e-10, maxit=4000, pc_kind=PC_DILU,
+                        newton_tol=5e-12, newton_maxit=40)
+    dt = 0.5
+    uo, st = O.Oracle(m).dirk_step(spec, deg, nstages, u, dt, 0.0, solver)
+    E = hip.Equation(hip.Mesh(m), spec, deg)
+    S = hip.Stepper(E, 0, nstages, solver)
+    ud, outd = _dev(cuda, u), _dev(cuda, np.zeros_like(u))
+    gs = S.advance(ud, dt, 0.0, outd)
+    assert gs.newton_iters == st.newton_iters
+    assert abs(gs.gmres_iters - st.gmres_iters) <= gs.linear_solves
+    # converged Newton solutions agree to the solver tolerances
+    assert rel(outd.cpu().numpy(), uo) < 1e-9
+
+
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32, PREC_BF16])
+def test_dirk_flexible_mixed_precision(cuda, prec):
+    m = vortex_mesh(8)
+    deg = 3
+    spec = EquationSpec.euler(periodic_trace=1, jacobian=1)
+    u = vortex_state(m, deg)
+    solver = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=30, rtol=1e-10, maxit=4000, pc_kind=PC_DILU,
+                        newton_tol=1e-11, newton_maxit=40, precision=prec)
+    dt = 0.5
+    ref = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=30, rtol=1e-10, maxit=4000, pc_kind=PC_DILU,
+                     newton_tol=1e-11, newton_maxit=40)
+    uo, st = O.Oracle(m).dirk_step(spec, deg, 3, u, dt, 0.0, ref)
+    E = hip.Equation(hip.Mesh(m), spec, deg)
+    S = hip.Stepper(E, 0, 3, solver)
+    ud, outd = _dev(cuda, u), _dev(cuda, np.zeros_like(u))
+    gs = S.advance(ud, dt, 0.0, outd)
+    assert gs.last_resid < 1e-11
+    assert rel(outd.cpu().numpy(), uo) < 1e-9
+
+
+def test_backward_euler_advection_bjacobi(cuda):
+    m = unit_periodic_mesh(8)
+    deg = 2
+    spec = EquationSpec.constant_advection(1.0, 1.0)
+    u = setups.interp(m, deg, setups.gaussian(a=50.0), 1)
+    solver = SolverSpec(gmres_mode=GMRES_LEFT_REF, restart=20, rtol=1e-12, maxit=4000, pc_kind=PC_BJACOBI,
+                        newton_tol=1e-13, newton_maxit=10)
+    uo, st = O.Oracle(m).dirk_step(spec, deg, 1, u, 0.05, 0.0, solver)
+    E = hip.Equation(hip.Mesh(m), spec, deg)
+    S = hip.Stepper(E, 0, 1, solver)
+    ud, outd = _dev(cuda, u), _dev(cuda, np.zeros_like(u))
+    gs = S.advance(ud, 0.05, 0.0, outd)
+    assert gs.newton_iters == st.newton_iters
+    assert rel(outd.cpu().numpy(), uo) < 1e-10
+
+
+def test_rk4_advection(cuda):
+    m = unit_periodic_mesh(8)
+    deg = 2
+    spec = EquationSpec.constant_advection(1.0, 1.0)
+    u = setups.interp(m, deg, setups.gaussian(), 1)
+    dt = 0.1 * m.h_min()
+    uo = O.Oracle(m).rk4_step(spec, deg, u, dt, 0.0)
+    E = hip.Equation(hip.Mesh(m), spec, deg)
+    S = hip.Stepper(E, 2, 4, SolverSpec())
+    ud, outd = _dev(cuda, u), _dev(cuda, np.zeros_like(u))
+    S.advance(ud, dt, 0.0, outd)
+    assert rel(outd.cpu().numpy(), uo) < 1e-12
