@@ -4,7 +4,7 @@ Workload (one "step" = one DIRK3 time step, i.e. three Newton solves on the devi
   isentropic vortex (strength 5, centre (5,5), free stream (1,1), gamma 1.4) on [0,10]^2,
   periodic; Voronoi mesh from a 256x256 lattice with uniform jitter 0.3h (seed 3),
   multicolour-ordered; Legendre p=3 (40x40 blocks); DIRK3 at CFL 20
-  (dt = 20 h / max(|v|+c), h = 10/256); Newton to 5e-13 on the L2 residual norm;
+  (dt = 20 h / ((2p+1) max(|v|+c)), h = 10/256); Newton to 5e-13 on the L2 residual norm;
   flexible GMRES(50) to 1e-10 with block DILU.
 The headline value is the mixed-precision path (DILU factors stored in BF16 with one
 power-of-two scale per block, FP32 accumulation, inside FP64 flexible GMRES); the FP64
@@ -105,8 +105,7 @@ def cpu_baseline(nx, deg, cfl, counts, n_target):
     mesh, u = workload(nx, deg)
     orc = O.Oracle(mesh)
     spec = EquationSpec.euler(periodic_trace=counts["trace"], jacobian=counts["jacobian"])
-    h = 10.0 / nx
-    dt = cfl * h / setups.euler_max_speed(u, deg)
+    dt = setups.dg_time_step(cfl, 10.0 / nx, deg, setups.euler_max_speed(u, deg))
     a = 0.5 * (1 + 0.435866521508459)
     t0 = time.perf_counter()
     b = orc.assemble(spec, deg, u, 0.0)
@@ -151,8 +150,7 @@ def main():
     deg = args.deg
     mesh, u0 = workload(args.nx, deg)
     n = mesh.n_cells
-    h = 10.0 / args.nx
-    dt = args.cfl * h / setups.euler_max_speed(u0, deg)
+    dt = setups.dg_time_step(args.cfl, 10.0 / args.nx, deg, setups.euler_max_speed(u0, deg))
     spec = EquationSpec.euler(periodic_trace=args.trace, jacobian=args.jacobian)
     E = hip.Equation(hip.Mesh(mesh), spec, deg)
     bl, nnzb = E.bl, E.nnzb

@@ -568,6 +568,9 @@ double eq_newton_resid(const cpg_equation_s &E, const double *kv, const double *
     return mx;
 }
 
+const double *equation_mass(const cpg_equation_s &E) { return E.mass.p; }
+int equation_bs(const cpg_equation_s &E) { return E.bs; }
+
 }  // namespace cpg
 
 namespace {

@@ -198,12 +198,13 @@ __device__ inline void numflux(const DevEq &eq, const double *Um, const double *
         euler_flux(eq.gamma, Um, Fxm, Fym, cm, um, vm);
         euler_flux(eq.gamma, Up, Fxp, Fyp, cp, up, vp);
         const double vnm = um * nx + vm * ny, vnp = up * nx + vp * ny;
-        double alpha = fabs(vnm - cm);
-        alpha = fmax(alpha, fabs(vnm));
-        alpha = fmax(alpha, fabs(vnm + cm));
-        alpha = fmax(alpha, fabs(vnp - cp));
-        alpha = fmax(alpha, fabs(vnp));
-        alpha = fmax(alpha, fabs(vnp + cp));
+        // std::max over the list (eqns/Euler.cpp:127-131): a NaN first element stays NaN,
+        // so a state with negative pressure yields a non-finite residual as in the reference
+        const double e[6] = {fabs(vnm - cm), fabs(vnm), fabs(vnm + cm), fabs(vnp - cp), fabs(vnp), fabs(vnp + cp)};
+        double alpha = e[0];
+#pragma unroll
+        for (int k = 1; k < 6; k++)
+            if (alpha < e[k]) alpha = e[k];
 #pragma unroll
         for (int c = 0; c < 4; c++)
             F[c] = 0.5 * ((Fxp[c] + Fxm[c]) * nx + (Fyp[c] + Fym[c]) * ny + alpha * (Um[c] - Up[c]));

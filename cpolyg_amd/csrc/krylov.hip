@@ -321,7 +321,9 @@ int cpg_gmres(cpg_krylov K, cpg_bsr A, cpg_prec P, const double *b, double *x, d
             CPG_LAUNCH_CHECK();
             if (left) op.pc(r);
             const double beta = op.norm(r);
-            CPG_CHECK(std::isfinite(beta), CPG_ERR_NONFINITE, "non-finite residual norm");
+            CPG_CHECK(std::isfinite(beta), CPG_ERR_NONFINITE,
+                      "non-finite residual norm at restart %d (previous cycle %d Arnoldi steps, |y_i| %g, ||b|| %g)", j,
+                      i, std::fabs(y[i]), nrm2b);
             hipLaunchKernelGGL(k_scale_by, dim3(ceil_div(N, 256)), dim3(256), 0, s, r, V, N,
                                beta > 0.0 ? 1.0 / beta : 0.0);
             CPG_LAUNCH_CHECK();
@@ -356,7 +358,11 @@ int cpg_gmres(cpg_krylov K, cpg_bsr A, cpg_prec P, const double *b, double *x, d
                 CPG_HIP(hipStreamSynchronize(s));
                 for (int k = 0; k <= i; k++) h[u0i + k] = K->hcol[k];
                 const double hnew = K->hcol[i + 1];
-                CPG_CHECK(std::isfinite(hnew), CPG_ERR_NONFINITE, "non-finite Arnoldi vector norm");
+                bool hfin = std::isfinite(hnew);
+                for (int k = 0; k <= i; k++) hfin = hfin && std::isfinite(K->hcol[k]);
+                CPG_CHECK(hfin, CPG_ERR_NONFINITE,
+                          "non-finite Hessenberg column (cycle %d, Arnoldi step %d, ||w|| %g, h_0 %g)", j, i, hnew,
+                          K->hcol[0]);
                 for (int k = 0; k < i; k++) {
                     const double tmp = cs[k] * h[uki] - sn[k] * h[uki + 1];
                     h[uki + 1] = sn[k] * h[uki] + cs[k] * h[uki + 1];
@@ -373,11 +379,17 @@ int cpg_gmres(cpg_krylov K, cpg_bsr A, cpg_prec P, const double *b, double *x, d
                 y[i] = cs[i] * y[i];
             }
             // packed upper-triangular solve (column k starts at k(k+1)/2)
+            double dmin = INFINITY;
             for (int c = i - 1; c >= 0; c--) {
+                dmin = std::fmin(dmin, std::fabs(h[c * (c + 1) / 2 + c]));
                 y[c] /= h[c * (c + 1) / 2 + c];
                 const double yc = y[c];
                 for (int rr = 0; rr < c; rr++) y[rr] -= h[c * (c + 1) / 2 + rr] * yc;
             }
+            for (int c = 0; c < i; c++)
+                CPG_CHECK(std::isfinite(y[c]), CPG_ERR_NONFINITE,
+                          "GMRES least-squares solution non-finite (cycle %d, %d Arnoldi steps, min |R_kk| %g, "
+                          "||b|| %g)", j, i, dmin, nrm2b);
             if (i > 0) {
                 CPG_HIP(hipMemcpyAsync(K->ydev.p, y.data(), i * sizeof(double), hipMemcpyHostToDevice, s));
                 hipLaunchKernelGGL(k_update, dim3(K->nblk), dim3(KWG), 0, s, left ? V : K->Z.p, N, i, K->ydev.p, x);

@@ -55,16 +55,20 @@ struct Store<double> {
     __device__ static void put(double *p, size_t e, double v, float) { p[e] = v; }
     __device__ static double get(const double *p, size_t e) { return p[e]; }
 };
+// Reduced-precision factors are only a storage format: the sweeps convert each element on
+// load and accumulate, and keep their intermediate vectors, in FP64.  FP32 accumulation
+// through the dependent colour levels of an ill-conditioned DILU costs 2-9x more Krylov
+// iterations at CFL 20, while the extra bytes of FP64 intermediates are ~1% of the traffic.
 template <>
 struct Store<float> {
-    using acc = float;
+    using acc = double;
     static constexpr int prec = CPG_FP32;
     __device__ static void put(float *p, size_t e, double v, float) { p[e] = (float)v; }
     __device__ static float get(const float *p, size_t e) { return p[e]; }
 };
 template <>
 struct Store<bf16> {
-    using acc = float;
+    using acc = double;
     static constexpr int prec = CPG_BF16;
     __device__ static void put(bf16 *p, size_t e, double v, float inv_scale) {
         p[e].v = f_to_bf16((float)(v * (double)inv_scale));
@@ -150,6 +154,7 @@ __device__ inline void store_block(int bl, T *dst, float *scale_out, const doubl
 // left half of AUG (column-major, bl rows, 2*bl columns); the right half must hold I.
 // Returns false (uniformly) if a zero pivot was met.
 __device__ bool gauss_jordan(int bl, double *AUG, double *fac, int *piv_row) {
+    __shared__ double s_invd;
     bool ok = true;
     const int t = threadIdx.x;
     const int w2 = 2 * bl;
@@ -179,10 +184,16 @@ __device__ bool gauss_jordan(int bl, double *AUG, double *fac, int *piv_row) {
             }
         }
         __syncthreads();
-        double d = AUG[p * bl + p];
-        if (d == 0.0) { ok = false; d = 1.0; }
-        const double invd = 1.0 / d;
-        for (int c = t; c < w2; c += WG) AUG[c * bl + p] *= invd;
+        // one thread reads the pivot; the barrier keeps every other thread from seeing it
+        // after the row scaling below has overwritten it with 1
+        if (t == 0) {
+            const double d = AUG[p * bl + p];
+            s_invd = d == 0.0 ? 0.0 : 1.0 / d;
+        }
+        __syncthreads();
+        const double invd = s_invd;
+        if (invd == 0.0) ok = false;
+        for (int c = t; c < w2; c += WG) AUG[c * bl + p] *= (invd == 0.0 ? 1.0 : invd);
         for (int r = t; r < bl; r += WG) fac[r] = (r == p) ? 0.0 : AUG[p * bl + r];
         __syncthreads();
         const int ncell = bl * w2;
@@ -374,10 +385,10 @@ __device__ inline void reduce_groups(int bl, A (&acc)[VW], A *red) {
     __syncthreads();
 }
 
-// forward level: y_i = x_i - sum_lower L y ; z_i = D_i^-1 y_i
-template <class T, int VW>
+// forward level: y_i = x_i - sum_lower L y ; z_i = D_i^-1 y_i   (x: FP64 or FP32 vector)
+template <class T, int VW, class XT>
 __global__ __launch_bounds__(WG) void k_dilu_forward(int bl, RowLists L, const T *__restrict__ F,
-                                                     const float *__restrict__ scale, const double *__restrict__ x,
+                                                     const float *__restrict__ scale, const XT *__restrict__ x,
                                                      typename Store<T>::acc *__restrict__ y,
                                                      typename Store<T>::acc *__restrict__ z) {
     using A = typename Store<T>::acc;
@@ -411,11 +422,11 @@ __global__ __launch_bounds__(WG) void k_dilu_forward(int bl, RowLists L, const T
 }
 
 // backward level: x_i = z_i - sum_upper DU x
-template <class T, int VW>
+template <class T, int VW, class XT>
 __global__ __launch_bounds__(WG) void k_dilu_backward(int bl, RowLists L, const T *__restrict__ F,
                                                       const float *__restrict__ scale,
                                                       const typename Store<T>::acc *__restrict__ z,
-                                                      double *__restrict__ x) {
+                                                      XT *__restrict__ x) {
     using A = typename Store<T>::acc;
     extern __shared__ unsigned char raw[];
     A *red = reinterpret_cast<A *>(raw);
@@ -431,7 +442,7 @@ __global__ __launch_bounds__(WG) void k_dilu_backward(int bl, RowLists L, const 
                          [&](int cb, int col) { return (A)x[(size_t)cb * bl + col]; }, acc);
     reduce_groups<A, VW>(bl, acc, red);
     if (threadIdx.x < bl)
-        x[(size_t)i * bl + threadIdx.x] = (double)(z[(size_t)i * bl + threadIdx.x] - red[G * bl + threadIdx.x]);
+        x[(size_t)i * bl + threadIdx.x] = (XT)(z[(size_t)i * bl + threadIdx.x] - red[G * bl + threadIdx.x]);
 }
 
 // block Jacobi apply: x_i = D_i^-1 x_i
@@ -467,6 +478,257 @@ __global__ void k_to_fp64(const T *__restrict__ F, const float *__restrict__ sca
     out[e] = v;
 }
 
+// ---- inner FP32 GMRES (variable preconditioner of the FP64 flexible GMRES) -------------
+// Vectors and the matrix copy are FP32; dot products and norms accumulate in FP64; the
+// Hessenberg matrix, Givens rotations and the small triangular solve live on the device
+// in FP64, so an inner cycle is a fixed launch sequence with no host synchronisation.
+constexpr int IWG = 256;
+constexpr int IPT = 8;
+constexpr int ICH = IWG * IPT;
+
+__global__ void k_d2f(const double *__restrict__ a, float *__restrict__ b, size_t n) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) b[e] = (float)a[e];
+}
+
+__global__ void k_f2d(const float *__restrict__ a, double *__restrict__ b, size_t n) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) b[e] = (double)a[e];
+}
+
+// FP32 BSR matvec y = A x (same row-stream scheme as the FP64 kernel, 16-byte loads)
+template <int VW>
+__global__ __launch_bounds__(256) void k_bsr_matvec_f32(int bl, const int *__restrict__ rowptr,
+                                                        const int *__restrict__ colind, const float *__restrict__ A,
+                                                        const float *__restrict__ x, float *__restrict__ y) {
+    extern __shared__ double fred[];
+    const int tpc = bl / VW;
+    const int G = 256 / tpc;
+    const int t = threadIdx.x;
+    const int g = t / tpc;
+    const int rr = (t - g * tpc) * VW;
+    const int i = blockIdx.x;
+    const int s0 = rowptr[i], s1 = rowptr[i + 1];
+    const int ncols = (s1 - s0) * bl;
+    double acc[VW];   // FP32 data, FP64 accumulation
+#pragma unroll
+    for (int v = 0; v < VW; v++) acc[v] = 0.0;
+    if (g < G) {
+        const float *base = A + (size_t)s0 * bl * bl + rr;
+#pragma unroll 4
+        for (int cg = g; cg < ncols; cg += G) {
+            const int blk = cg / bl;
+            const int col = cg - blk * bl;
+            const float xv = x[(size_t)colind[s0 + blk] * bl + col];
+            float a[VW];
+            if constexpr (VW == 4) {
+                const float4 q = *reinterpret_cast<const float4 *>(base + (size_t)cg * bl);
+                a[0] = q.x; a[1] = q.y; a[2] = q.z; a[3] = q.w;
+            } else if constexpr (VW == 2) {
+                const float2 q = *reinterpret_cast<const float2 *>(base + (size_t)cg * bl);
+                a[0] = q.x; a[1] = q.y;
+            } else {
+                a[0] = base[(size_t)cg * bl];
+            }
+#pragma unroll
+            for (int v = 0; v < VW; v++) acc[v] = fma((double)a[v], (double)xv, acc[v]);
+        }
+#pragma unroll
+        for (int v = 0; v < VW; v++) fred[g * bl + rr + v] = acc[v];
+    }
+    __syncthreads();
+    if (t < bl) {
+        double s = 0.0;
+        for (int gg = 0; gg < G; gg++) s += fred[gg * bl + t];
+        y[(size_t)i * bl + t] = (float)s;
+    }
+}
+
+// partial[b*nv + j] = sum over chunk b of Q_j . w   (FP64 accumulation)
+__global__ __launch_bounds__(IWG) void k_f32_mdot(const float *__restrict__ Q, size_t N, int nv,
+                                                  const float *__restrict__ w, double *__restrict__ partial) {
+    __shared__ double red[IWG / 64][64];
+    const size_t base = (size_t)blockIdx.x * ICH + threadIdx.x;
+    float wv[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const size_t e = base + (size_t)k * IWG;
+        wv[k] = e < N ? w[e] : 0.f;
+    }
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int j0 = 0; j0 < nv; j0 += 64) {
+        const int jn = min(64, nv - j0);
+        for (int jj = 0; jj < jn; jj++) {
+            const float *q = Q + (size_t)(j0 + jj) * N;
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < IPT; k++) {
+                const size_t e = base + (size_t)k * IWG;
+                if (e < N) s = fma((double)q[e], (double)wv[k], s);
+            }
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+            if (lane == 0) red[wid][jj] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x < jn) {
+            double s = 0.0;
+            for (int q = 0; q < IWG / 64; q++) s += red[q][threadIdx.x];
+            partial[(size_t)blockIdx.x * nv + j0 + threadIdx.x] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// out[j] = sum_b partial[b*nv + j] (sqrt if requested)
+__global__ __launch_bounds__(IWG) void k_f32_reduce(const double *__restrict__ partial, int nb, int nv,
+                                                    double *__restrict__ out, int sqrt_out) {
+    __shared__ double red[IWG / 64];
+    const int j = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += IWG) s += partial[(size_t)b * nv + j];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < IWG / 64; q++) t += red[q];
+        out[j] = sqrt_out ? sqrt(t) : t;
+    }
+}
+
+// w -= sum_j h[j] Q_j ; partial[b] = sum over chunk b of w^2
+__global__ __launch_bounds__(IWG) void k_f32_orth(const float *__restrict__ Q, size_t N, int nv,
+                                                  const double *__restrict__ h, float *__restrict__ w,
+                                                  double *__restrict__ partial) {
+    __shared__ double red[IWG / 64];
+    const size_t base = (size_t)blockIdx.x * ICH + threadIdx.x;
+    double wv[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const size_t e = base + (size_t)k * IWG;
+        wv[k] = e < N ? (double)w[e] : 0.0;
+    }
+    for (int j = 0; j < nv; j++) {
+        const double hj = h[j];
+        const float *q = Q + (size_t)j * N;
+#pragma unroll
+        for (int k = 0; k < IPT; k++) {
+            const size_t e = base + (size_t)k * IWG;
+            if (e < N) wv[k] = fma(-hj, (double)q[e], wv[k]);
+        }
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const size_t e = base + (size_t)k * IWG;
+        if (e < N) {
+            const float f = (float)wv[k];
+            w[e] = f;
+            s = fma((double)f, (double)f, s);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// q = float(x64) and partial sums of q^2
+__global__ __launch_bounds__(IWG) void k_f32_load_norm(const double *__restrict__ x, float *__restrict__ q, size_t N,
+                                                       double *__restrict__ partial) {
+    __shared__ double red[IWG / 64];
+    const size_t base = (size_t)blockIdx.x * ICH + threadIdx.x;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const size_t e = base + (size_t)k * IWG;
+        if (e < N) {
+            const float f = (float)x[e];
+            q[e] = f;
+            s = fma((double)f, (double)f, s);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// q = w / norm (0 if norm is 0)
+__global__ void k_f32_scale(const float *__restrict__ w, float *__restrict__ q, size_t N,
+                            const double *__restrict__ norm) {
+    const double nv = *norm;
+    const float inv = nv > 0.0 ? (float)(1.0 / nv) : 0.f;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < N) q[e] = w[e] * inv;
+}
+
+// g = (beta, 0, ..., 0)
+__global__ void k_inner_init(double *__restrict__ g, int k, const double *__restrict__ beta) {
+    for (int j = threadIdx.x; j <= k; j += blockDim.x) g[j] = j == 0 ? *beta : 0.0;
+}
+
+// Column i of the Hessenberg matrix (ld = k+1, column-major; H(i+1,i) = hnew): apply the
+// previous rotations, form rotation i (c = rd/dd, s = -hnew/dd as the reference) and
+// rotate the right-hand side g.
+__global__ void k_inner_givens(double *__restrict__ H, int ld, int i, const double *__restrict__ hnew,
+                               double *__restrict__ cs, double *__restrict__ sn, double *__restrict__ g) {
+    if (threadIdx.x != 0) return;
+    double *col = H + (size_t)i * ld;
+    for (int k = 0; k < i; k++) {
+        const double tmp = cs[k] * col[k] - sn[k] * col[k + 1];
+        col[k + 1] = sn[k] * col[k] + cs[k] * col[k + 1];
+        col[k] = tmp;
+    }
+    const double rd = col[i], hn = *hnew;
+    const double dd = sqrt(rd * rd + hn * hn);
+    if (dd > 0.0) {
+        cs[i] = rd / dd;
+        sn[i] = -hn / dd;
+    } else {
+        cs[i] = 1.0;
+        sn[i] = 0.0;
+    }
+    col[i] = dd;
+    g[i + 1] = sn[i] * g[i];
+    g[i] = cs[i] * g[i];
+}
+
+// y = R^-1 g for the leading k x k upper triangle R of H (zero where R_jj vanishes)
+__global__ void k_inner_trisolve(const double *__restrict__ H, int ld, int k, const double *__restrict__ g,
+                                 double *__restrict__ y) {
+    if (threadIdx.x != 0) return;
+    for (int c = k - 1; c >= 0; c--) {
+        double s = g[c];
+        for (int j = c + 1; j < k; j++) s -= H[(size_t)j * ld + c] * y[j];
+        const double d = H[(size_t)c * ld + c];
+        y[c] = d != 0.0 ? s / d : 0.0;
+    }
+}
+
+// x = sum_j y_j Q_j
+__global__ __launch_bounds__(IWG) void k_f32_combine(const float *__restrict__ Q, size_t N, int nk,
+                                                     const double *__restrict__ y, float *__restrict__ x) {
+    const size_t base = (size_t)blockIdx.x * ICH + threadIdx.x;
+    double xv[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; k++) xv[k] = 0.0;
+    for (int j = 0; j < nk; j++) {
+        const double yj = y[j];
+        const float *q = Q + (size_t)j * N;
+#pragma unroll
+        for (int k = 0; k < IPT; k++) {
+            const size_t e = base + (size_t)k * IWG;
+            if (e < N) xv[k] = fma(yj, (double)q[e], xv[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const size_t e = base + (size_t)k * IWG;
+        if (e < N) x[e] = (float)xv[k];
+    }
+}
+
 }  // namespace cpg
 
 using namespace cpg;
@@ -486,6 +748,15 @@ struct cpg_prec_s {
     DevBuf<unsigned char> ybuf, zbuf;
     DevBuf<int> status;
     size_t elem = 8;
+    // inner FP32 GMRES (kind CPG_PC_INNER_GMRES)
+    int inner_k = 20;
+    size_t N = 0;
+    int nchunk = 0;
+    const cpg_bsr_s *A = nullptr;             // matrix of the last factor (pattern for the FP32 matvec)
+    DevBuf<float> A32, Q, iw, it;              // FP32 matrix copy, basis (k+1) x N, work vectors
+    DevBuf<double> ipart, H, g, cs, sn, y, nrm;
+
+    bool dilu() const { return kind == CPG_PC_DILU || kind == CPG_PC_INNER_GMRES; }
 
     RowLists lists(bool fwd) const {
         RowLists L;
@@ -547,7 +818,7 @@ void factor_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
     const int ne = ceil_div((long long)P.bl * P.bl, WG);
 #define CPG_NE_CASE(NEV)                                                    \
     if (ne <= NEV) {                                                        \
-        if (P.kind == CPG_PC_DILU) factor_dilu_t<T, NEV>(P, A, s);          \
+        if (P.dilu()) factor_dilu_t<T, NEV>(P, A, s);                       \
         else factor_bj_t<T, NEV>(P, A, s);                                  \
         return;                                                             \
     }
@@ -556,8 +827,8 @@ void factor_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
     CPG_CHECK(false, CPG_ERR_UNSUPPORTED, "block size %d too large", P.bl);
 }
 
-template <class T, int VW>
-void apply_t(cpg_prec_s &P, double *x, hipStream_t s) {
+template <class T, int VW, class XT>
+void apply_t(cpg_prec_s &P, XT *x, hipStream_t s) {
     using A = typename Store<T>::acc;
     const int bl = P.bl;
     const int tpc = bl / VW;
@@ -565,8 +836,10 @@ void apply_t(cpg_prec_s &P, double *x, hipStream_t s) {
     const size_t lds = (size_t)(G + 2) * bl * sizeof(A);
     const float *sc = P.precision == CPG_BF16 ? P.scale.p : nullptr;
     if (P.kind == CPG_PC_BJACOBI) {
-        hipLaunchKernelGGL((k_bjacobi_apply<T, VW>), dim3(P.n), dim3(WG), lds, s, bl, Fp<T>(P), sc, x);
-        CPG_LAUNCH_CHECK();
+        if constexpr (std::is_same<XT, double>::value) {
+            hipLaunchKernelGGL((k_bjacobi_apply<T, VW>), dim3(P.n), dim3(WG), lds, s, bl, Fp<T>(P), sc, x);
+            CPG_LAUNCH_CHECK();
+        }
         return;
     }
     A *y = reinterpret_cast<A *>(P.ybuf.p);
@@ -577,7 +850,7 @@ void apply_t(cpg_prec_s &P, double *x, hipStream_t s) {
         if (!nr) continue;
         RowLists Ll = Lf;
         Ll.rows = P.frows.p + P.flev[lv];
-        hipLaunchKernelGGL((k_dilu_forward<T, VW>), dim3(nr), dim3(WG), lds, s, bl, Ll, Fp<T>(P), sc, x, y, z);
+        hipLaunchKernelGGL((k_dilu_forward<T, VW, XT>), dim3(nr), dim3(WG), lds, s, bl, Ll, Fp<T>(P), sc, x, y, z);
         CPG_LAUNCH_CHECK();
     }
     RowLists Lb = P.lists(false);
@@ -586,20 +859,87 @@ void apply_t(cpg_prec_s &P, double *x, hipStream_t s) {
         if (!nr) continue;
         RowLists Ll = Lb;
         Ll.rows = P.brows.p + P.blev[lv];
-        hipLaunchKernelGGL((k_dilu_backward<T, VW>), dim3(nr), dim3(WG), lds, s, bl, Ll, Fp<T>(P), sc, z, x);
+        hipLaunchKernelGGL((k_dilu_backward<T, VW, XT>), dim3(nr), dim3(WG), lds, s, bl, Ll, Fp<T>(P), sc, z, x);
         CPG_LAUNCH_CHECK();
     }
 }
 
-template <class T>
-void apply_dispatch(cpg_prec_s &P, double *x, hipStream_t s) {
+template <class T, class XT>
+void apply_dispatch(cpg_prec_s &P, XT *x, hipStream_t s) {
     switch (VecW<T>::pick(P.bl)) {
-    case 8: if constexpr (16 / sizeof(T) >= 8) { apply_t<T, 8>(P, x, s); return; } break;
-    case 4: if constexpr (16 / sizeof(T) >= 4) { apply_t<T, 4>(P, x, s); return; } break;
-    case 2: apply_t<T, 2>(P, x, s); return;
-    default: apply_t<T, 1>(P, x, s); return;
+    case 8: if constexpr (16 / sizeof(T) >= 8) { apply_t<T, 8, XT>(P, x, s); return; } break;
+    case 4: if constexpr (16 / sizeof(T) >= 4) { apply_t<T, 4, XT>(P, x, s); return; } break;
+    case 2: apply_t<T, 2, XT>(P, x, s); return;
+    default: apply_t<T, 1, XT>(P, x, s); return;
     }
-    apply_t<T, 1>(P, x, s);
+    apply_t<T, 1, XT>(P, x, s);
+}
+
+// DILU / block-Jacobi sweep on a vector of type XT in the stored precision
+template <class XT>
+void apply_factors(cpg_prec_s &P, XT *x, hipStream_t s) {
+    switch (P.precision) {
+    case CPG_FP64: apply_dispatch<double, XT>(P, x, s); break;
+    case CPG_FP32: apply_dispatch<float, XT>(P, x, s); break;
+    default: apply_dispatch<bf16, XT>(P, x, s); break;
+    }
+}
+
+void matvec_f32(cpg_prec_s &P, const float *x, float *y, hipStream_t s) {
+    const int bl = P.bl;
+    const int vw = bl % 4 == 0 ? 4 : bl % 2 == 0 ? 2 : 1;
+    const int G = 256 / (bl / vw);
+    const size_t lds = (size_t)G * bl * sizeof(double);
+    if (vw == 4)
+        hipLaunchKernelGGL(k_bsr_matvec_f32<4>, dim3(P.n), dim3(256), lds, s, bl, P.A->rowptr.p, P.A->colind.p, P.A32.p, x, y);
+    else if (vw == 2)
+        hipLaunchKernelGGL(k_bsr_matvec_f32<2>, dim3(P.n), dim3(256), lds, s, bl, P.A->rowptr.p, P.A->colind.p, P.A32.p, x, y);
+    else
+        hipLaunchKernelGGL(k_bsr_matvec_f32<1>, dim3(P.n), dim3(256), lds, s, bl, P.A->rowptr.p, P.A->colind.p, P.A32.p, x, y);
+    CPG_LAUNCH_CHECK();
+}
+
+// x <- inner GMRES(k) approximation of A^-1 x: FP32 Arnoldi on the FP32 copy of A,
+// right-preconditioned by the DILU factors, zero initial guess.  A fixed sequence of
+// launches on `s` with every scalar on the device.
+void apply_inner(cpg_prec_s &P, double *x, hipStream_t s) {
+    const size_t N = P.N;
+    const int k = P.inner_k, ld = k + 1, nb = P.nchunk;
+    float *Q = P.Q.p, *w = P.iw.p, *t = P.it.p;
+    const int g256 = ceil_div(N, 256);
+    hipLaunchKernelGGL(k_f32_load_norm, dim3(nb), dim3(IWG), 0, s, x, w, N, P.ipart.p);
+    CPG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_f32_reduce, dim3(1), dim3(IWG), 0, s, P.ipart.p, nb, 1, P.nrm.p, 1);
+    CPG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_f32_scale, dim3(g256), dim3(256), 0, s, w, Q, N, P.nrm.p);
+    CPG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_inner_init, dim3(1), dim3(64), 0, s, P.g.p, k, P.nrm.p);
+    CPG_LAUNCH_CHECK();
+    for (int i = 0; i < k; i++) {
+        float *qi = Q + N * i, *qn = Q + N * (i + 1);
+        CPG_HIP(hipMemcpyAsync(t, qi, N * sizeof(float), hipMemcpyDeviceToDevice, s));
+        apply_factors<float>(P, t, s);                     // t = M^-1 q_i
+        matvec_f32(P, t, w, s);                             // w = A t
+        hipLaunchKernelGGL(k_f32_mdot, dim3(nb), dim3(IWG), 0, s, Q, N, i + 1, w, P.ipart.p);
+        CPG_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_f32_reduce, dim3(i + 1), dim3(IWG), 0, s, P.ipart.p, nb, i + 1, P.H.p + (size_t)i * ld, 0);
+        CPG_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_f32_orth, dim3(nb), dim3(IWG), 0, s, Q, N, i + 1, P.H.p + (size_t)i * ld, w, P.ipart.p);
+        CPG_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_f32_reduce, dim3(1), dim3(IWG), 0, s, P.ipart.p, nb, 1, P.nrm.p, 1);
+        CPG_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_f32_scale, dim3(g256), dim3(256), 0, s, w, qn, N, P.nrm.p);
+        CPG_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_inner_givens, dim3(1), dim3(64), 0, s, P.H.p, ld, i, P.nrm.p, P.cs.p, P.sn.p, P.g.p);
+        CPG_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_inner_trisolve, dim3(1), dim3(64), 0, s, P.H.p, ld, k, P.g.p, P.y.p);
+    CPG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_f32_combine, dim3(nb), dim3(IWG), 0, s, Q, N, k, P.y.p, t);
+    CPG_LAUNCH_CHECK();
+    apply_factors<float>(P, t, s);                          // x = M^-1 Q y
+    hipLaunchKernelGGL(k_f2d, dim3(g256), dim3(256), 0, s, t, x, N);
+    CPG_LAUNCH_CHECK();
 }
 
 // level schedule from the pattern: level(i) = 1 + max level(j) over dependencies
@@ -637,7 +977,8 @@ extern "C" {
 int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
     return guard([&] {
         CPG_CHECK(A && out, CPG_ERR_ARG, "null argument");
-        CPG_CHECK(kind == CPG_PC_NONE || kind == CPG_PC_BJACOBI || kind == CPG_PC_DILU, CPG_ERR_ARG, "bad kind %d", kind);
+        CPG_CHECK(kind == CPG_PC_NONE || kind == CPG_PC_BJACOBI || kind == CPG_PC_DILU || kind == CPG_PC_INNER_GMRES,
+                  CPG_ERR_ARG, "bad kind %d", kind);
         CPG_CHECK(precision >= CPG_FP64 && precision <= CPG_BF16, CPG_ERR_ARG, "bad precision %d", precision);
         auto *P = new cpg_prec_s;
         try {
@@ -663,7 +1004,7 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
                         int st = -1;
                         for (int s2 = rp[j]; s2 < rp[j + 1]; s2++)
                             if (ci[s2] == i) { st = s2; break; }
-                        CPG_CHECK(kind != CPG_PC_DILU || st >= 0, CPG_ERR_ARG,
+                        CPG_CHECK(!(kind == CPG_PC_DILU || kind == CPG_PC_INNER_GMRES) || st >= 0, CPG_ERR_ARG,
                                   "pattern is not structurally symmetric at (%d,%d)", i, j);
                         ltrans.push_back(st);
                     } else {
@@ -689,13 +1030,23 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
             P->frows.upload(frows);
             P->brows.upload(brows);
             P->status.alloc(1);
-            if (kind == CPG_PC_DILU) {
+            if (P->dilu()) {
                 P->F.alloc((size_t)A->nnzb * bb * P->elem);
                 if (precision == CPG_BF16) P->scale.alloc(A->nnzb);
                 P->DD64.alloc((size_t)n * bb);
-                const size_t ae = precision == CPG_FP64 ? 8 : 4;
+                const size_t ae = 8;   // FP64 intermediates for every factor precision
                 P->ybuf.alloc((size_t)n * bl * ae);
                 P->zbuf.alloc((size_t)n * bl * ae);
+            }
+            if (kind == CPG_PC_INNER_GMRES) {
+                CPG_CHECK(precision != CPG_FP64, CPG_ERR_ARG, "the inner FP32 GMRES needs FP32 or BF16 factors");
+                P->N = (size_t)n * bl;
+                P->nchunk = std::max(1, ceil_div(P->N, ICH));
+                P->A32.alloc((size_t)A->nnzb * bb);
+                P->iw.alloc(P->N);
+                P->it.alloc(P->N);
+                P->nrm.alloc(2);
+                // basis, Hessenberg and rotations are sized by cpg_prec_set_inner
             } else if (kind == CPG_PC_BJACOBI) {
                 P->F.alloc((size_t)n * bb * P->elem);
                 if (precision == CPG_BF16) P->scale.alloc(n);
@@ -731,6 +1082,16 @@ int cpg_prec_factor(cpg_prec P, cpg_bsr A, void *stream) {
         case CPG_FP32: factor_t<float>(*P, *A, s); break;
         default: factor_t<bf16>(*P, *A, s); break;
         }
+        if (P->kind == CPG_PC_INNER_GMRES) {
+            if (!P->Q.p) {
+                int st2 = cpg_prec_set_inner(P, P->inner_k);
+                if (st2) throw Error{st2};
+            }
+            P->A = A;
+            hipLaunchKernelGGL(k_d2f, dim3(ceil_div(A->blocks.n, 256)), dim3(256), 0, s, A->blocks.p, P->A32.p,
+                               A->blocks.n);
+            CPG_LAUNCH_CHECK();
+        }
         int st = 0;
         CPG_HIP(hipMemcpyAsync(&st, P->status.p, sizeof(int), hipMemcpyDeviceToHost, s));
         CPG_HIP(hipStreamSynchronize(s));
@@ -743,11 +1104,28 @@ int cpg_prec_apply(cpg_prec P, double *x, void *stream) {
         CPG_CHECK(P && x, CPG_ERR_ARG, "null argument");
         if (P->kind == CPG_PC_NONE || P->n == 0) return;
         hipStream_t s = S(stream);
-        switch (P->precision) {
-        case CPG_FP64: apply_dispatch<double>(*P, x, s); break;
-        case CPG_FP32: apply_dispatch<float>(*P, x, s); break;
-        default: apply_dispatch<bf16>(*P, x, s); break;
+        if (P->kind == CPG_PC_INNER_GMRES) {
+            CPG_CHECK(P->A, CPG_ERR_ARG, "inner GMRES preconditioner applied before its first factor");
+            apply_inner(*P, x, s);
+            return;
         }
+        apply_factors<double>(*P, x, s);
+    });
+}
+
+int cpg_prec_set_inner(cpg_prec P, int iterations) {
+    return guard([&] {
+        CPG_CHECK(P && iterations > 0 && iterations <= 200, CPG_ERR_ARG, "bad inner iteration count");
+        CPG_CHECK(P->kind == CPG_PC_INNER_GMRES, CPG_ERR_ARG, "not an inner-GMRES preconditioner");
+        const int k = iterations;
+        P->inner_k = k;
+        P->Q.alloc(P->N * (k + 1));
+        P->ipart.alloc((size_t)P->nchunk * (k + 1));
+        P->H.alloc((size_t)(k + 1) * k);
+        P->g.alloc(k + 1);
+        P->cs.alloc(k);
+        P->sn.alloc(k);
+        P->y.alloc(k);
     });
 }
 
@@ -756,7 +1134,7 @@ int cpg_prec_download(cpg_prec P, double *host_blocks) {
         CPG_CHECK(P && host_blocks, CPG_ERR_ARG, "null argument");
         if (P->kind == CPG_PC_NONE) return;
         const size_t bb = (size_t)P->bl * P->bl;
-        const size_t nblk = P->kind == CPG_PC_DILU ? P->nnzb : P->n;
+        const size_t nblk = P->dilu() ? P->nnzb : P->n;
         const size_t cnt = nblk * bb;
         DevBuf<double> tmp(cnt);
         const float *sc = P->precision == CPG_BF16 ? P->scale.p : nullptr;

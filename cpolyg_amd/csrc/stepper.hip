@@ -63,8 +63,56 @@ __global__ void k_block_apply(int n, int bs, int nc, const double *__restrict__ 
     y[e] = s;
 }
 
+// Stage-coupled Newton matrix of IRK::advance (src/TimeIntegration.cpp:233-245) in the
+// stage-major layout of BlockMatrix::blockBlockMatrix (src/BlockMatrix.cpp:228-258):
+// big row (si, i) holds, for sj = 0..ns-1, the blocks of Jacobian row i scaled by
+// -dt A[si][sj], plus the mass block on (si, i, si, i).  This kernel writes the row-stage
+// si from the stage-si Jacobian J (Jacobian pattern, diagonal block first in each row).
+struct IrkCoef {
+    double c[3];
+};
+
+__global__ void k_irk_scatter(int ns, int si, int nnzb, int bl, int bs, const int *__restrict__ rowptr,
+                              const int *__restrict__ slot_row, const double *__restrict__ J, IrkCoef coef,
+                              const double *__restrict__ mass, double *__restrict__ big) {
+    const size_t bb = (size_t)bl * bl;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)ns * nnzb * bb) return;
+    const int sj = (int)(e / ((size_t)nnzb * bb));
+    const size_t rem = e - (size_t)sj * nnzb * bb;
+    const int s = (int)(rem / bb);
+    const int off = (int)(rem - (size_t)s * bb);
+    const int i = slot_row[s];
+    const int q = s - rowptr[i], len = rowptr[i + 1] - rowptr[i];
+    const size_t dst = ((size_t)si * ns * nnzb + (size_t)ns * rowptr[i] + (size_t)sj * len + q) * bb + off;
+    double v = coef.c[sj] * J[(size_t)s * bb + off];
+    if (sj == si && q == 0) {
+        const int r = off % bl, c = off / bl;
+        if (r / bs == c / bs) v += mass[(size_t)i * bs * bs + (c % bs) * bs + (r % bs)];
+    }
+    big[dst] = v;
+}
+
 void tableau(int method, int ns, double *A, double *b, double *c) {
     std::memset(A, 0, sizeof(double) * 25);
+    if (method == CPG_STEP_IRK) {   // Radau IIA (src/TimeIntegration.cpp:153-186)
+        if (ns == 2) {
+            const double a[4] = {5 / 12.0, -1 / 12.0, 3 / 4.0, 1 / 4.0};
+            std::memcpy(A, a, sizeof a);
+            b[0] = 3 / 4.0; b[1] = 1 / 4.0; c[0] = 1 / 3.0; c[1] = 1;
+            return;
+        }
+        if (ns == 3) {
+            const double s6 = std::sqrt(6.0);
+            const double a[9] = {(88 - 7 * s6) / 360.0, (296 - 169 * s6) / 1800.0, (-2 + 3 * s6) / 225.0,
+                                 (296 + 169 * s6) / 1800.0, (88 + 7 * s6) / 360.0, (-2 - 3 * s6) / 225.0,
+                                 (16 - s6) / 36.0, (16 + s6) / 36.0, 1 / 9.0};
+            std::memcpy(A, a, sizeof a);
+            b[0] = (16 - s6) / 36.0; b[1] = (16 + s6) / 36.0; b[2] = 1 / 9.0;
+            c[0] = (4 - s6) / 10.0; c[1] = (4 + s6) / 10.0; c[2] = 1;
+            return;
+        }
+    }
     if (method == CPG_STEP_DIRK) {
         if (ns == 1) { A[0] = 1; b[0] = 1; c[0] = 1; return; }
         if (ns == 2) {
@@ -119,15 +167,23 @@ struct cpg_stepper_s {
     cpg_solver_params sp{};
     double A[25], b[5], c[5];
     cpg_bsr J = nullptr;
+    cpg_bsr big = nullptr;     // IRK stage-coupled matrix
     cpg_prec P = nullptr;
     cpg_krylov K = nullptr;
-    DevBuf<double> k, cu, rhs, r, x, minv;
+    DevBuf<double> k, cu, rhs, r, x, minv, bigR, bigK;
+    DevBuf<int> slot_row;
     ~cpg_stepper_s() {
         if (P) cpg_prec_destroy(P);
         if (K) cpg_krylov_destroy(K);
         if (J) cpg_bsr_destroy(J);
+        if (big) cpg_bsr_destroy(big);
     }
 };
+
+namespace cpg {
+const double *equation_mass(const cpg_equation_s &E);
+int equation_bs(const cpg_equation_s &E);
+}  // namespace cpg
 
 namespace {
 
@@ -151,6 +207,14 @@ int check(int st) {
     return st;
 }
 
+// re-raise a failed library call with the stage / Newton iteration it happened in
+void check_ctx(int st, int stage, int it, double resid) {
+    if (!st) return;
+    std::string msg = cpg_last_error();
+    set_error("stage %d, Newton iteration %d (residual %.3e): %s", stage, it, resid, msg.c_str());
+    throw Error{st};
+}
+
 }  // namespace
 
 extern "C" {
@@ -158,7 +222,7 @@ extern "C" {
 int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver_params *params, cpg_stepper *out) {
     return guard([&] {
         CPG_CHECK(E && params && out, CPG_ERR_ARG, "null argument");
-        CPG_CHECK(method == CPG_STEP_DIRK || method == CPG_STEP_RK4, CPG_ERR_UNSUPPORTED,
+        CPG_CHECK(method == CPG_STEP_DIRK || method == CPG_STEP_RK4 || method == CPG_STEP_IRK, CPG_ERR_UNSUPPORTED,
                   "stepper method %d not available", method);
         auto *S = new cpg_stepper_s;
         try {
@@ -182,7 +246,38 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
                           "bad solver parameters");
                 check(cpg_jacobian_create(E, &S->J));
                 check(cpg_prec_create(S->J, params->pc_kind, params->precision, &S->P));
+                if (params->pc_kind == CPG_PC_INNER_GMRES) {
+                    CPG_CHECK(params->gmres_mode == CPG_GMRES_FLEXIBLE, CPG_ERR_ARG,
+                              "the inner-GMRES preconditioner varies between iterations: use flexible GMRES");
+                    check(cpg_prec_set_inner(S->P, params->inner_iters > 0 ? params->inner_iters : 20));
+                }
                 check(cpg_krylov_create(n, bl, params->restart, params->gmres_mode, &S->K));
+            } else if (method == CPG_STEP_IRK) {
+                CPG_CHECK(nstages == 2 || nstages == 3, CPG_ERR_UNSUPPORTED, "Radau IIA with %d stages", nstages);
+                CPG_CHECK(params->restart > 0 && params->maxit > 0 && params->newton_maxit > 0, CPG_ERR_ARG,
+                          "bad solver parameters");
+                const int ns = nstages;
+                check(cpg_jacobian_create(E, &S->J));
+                std::vector<int> rp(n + 1), ci(nnzb), srow(nnzb);
+                check(cpg_jacobian_pattern(E, rp.data(), ci.data()));
+                for (int i = 0; i < n; i++)
+                    for (int q = rp[i]; q < rp[i + 1]; q++) srow[q] = i;
+                std::vector<int> brow((size_t)ns * n + 1), bcol;
+                bcol.reserve((size_t)ns * ns * nnzb);
+                for (int si = 0; si < ns; si++)
+                    for (int i = 0; i < n; i++) {
+                        brow[(size_t)si * n + i] = (int)bcol.size();
+                        for (int sj = 0; sj < ns; sj++)
+                            for (int q = rp[i]; q < rp[i + 1]; q++) bcol.push_back(sj * n + ci[q]);
+                    }
+                brow[(size_t)ns * n] = (int)bcol.size();
+                check(cpg_bsr_create(ns * n, bl, (int)bcol.size(), brow.data(), bcol.data(), &S->big));
+                S->slot_row.upload(srow);
+                check(cpg_prec_create(S->big, params->pc_kind, params->precision, &S->P));
+                check(cpg_krylov_create(ns * n, bl, params->restart, params->gmres_mode, &S->K));
+                S->bigR.alloc(S->N * ns);
+                S->bigK.alloc(S->N * ns);
+                S->k.alloc(S->N * ns);
             } else {
                 // block inverse of the mass matrix (per component, bs x bs)
                 const int bs = (int)std::lround((std::sqrt(8.0 * bl + 1) - 1) / 2);  // bl = nc*bs, try nc=1
@@ -283,6 +378,70 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
             st.ms_total = ms_since(t_all);
             return;
         }
+        if (S->method == CPG_STEP_IRK) {
+            // IRK::advance (src/TimeIntegration.cpp:188-281): all stages in one Newton loop;
+            // bigK is not reset between Newton iterations, so each GMRES starts from the
+            // previous correction.
+            const int ns = S->ns;
+            const int bs = equation_bs(*S->E);
+            int nnzb = 0;
+            check(cpg_equation_info(S->E, nullptr, nullptr, &nnzb));
+            CPG_HIP(hipMemsetAsync(k, 0, N * ns * sizeof(double), s));
+            CPG_HIP(hipMemsetAsync(S->bigK.p, 0, N * ns * sizeof(double), s));
+            bool converged = false;
+            for (int it = 0; it < S->sp.newton_maxit; it++) {
+                auto t0 = clk::now();
+                double rn2 = 0.0;
+                for (int stg = 0; stg < ns; stg++) {
+                    lincomb(*S, S->cu.p, u, &S->A[stg * ns], ns, k, s);
+                    eq_residual(*S->E, S->cu.p, t + dt * S->c[stg], S->rhs.p, s);
+                    const double rs = eq_newton_resid(*S->E, k + N * stg, S->rhs.p, dt, S->bigR.p + N * stg, nullptr, s);
+                    rn2 += rs * rs;
+                }
+                const double rn = std::sqrt(rn2);
+                st.ms_residual += ms_since(t0);
+                st.last_resid = rn;
+                CPG_CHECK(std::isfinite(rn), CPG_ERR_NONFINITE, "non-finite IRK Newton residual (it %d)", it);
+                if (rn < S->sp.newton_tol) { converged = true; break; }
+                st.newton_iters++;
+                t0 = clk::now();
+                for (int stg = 0; stg < ns; stg++) {
+                    lincomb(*S, S->cu.p, u, &S->A[stg * ns], ns, k, s);
+                    eq_jacobian(*S->E, S->cu.p, t + dt * S->c[stg], 1.0, 0, S->J->blocks.p, s);
+                    IrkCoef cf{};
+                    for (int sj = 0; sj < ns; sj++) cf.c[sj] = -dt * S->A[stg * ns + sj];
+                    const size_t cnt = (size_t)ns * nnzb * S->bl * S->bl;
+                    hipLaunchKernelGGL(k_irk_scatter, dim3(ceil_div(cnt, 256)), dim3(256), 0, s, ns, stg, nnzb, S->bl,
+                                       bs, S->J->rowptr.p, S->slot_row.p, S->J->blocks.p, cf, equation_mass(*S->E),
+                                       S->big->blocks.p);
+                    CPG_LAUNCH_CHECK();
+                }
+                CPG_HIP(hipStreamSynchronize(s));
+                st.ms_jacobian += ms_since(t0);
+                t0 = clk::now();
+                check(cpg_prec_factor(S->P, S->big, s));
+                st.ms_factor += ms_since(t0);
+                t0 = clk::now();
+                double tol = S->sp.rtol;
+                int maxit = S->sp.maxit;
+                check(cpg_gmres(S->K, S->big, S->P, S->bigR.p, S->bigK.p, &tol, &maxit, s));
+                st.ms_gmres += ms_since(t0);
+                st.gmres_iters += maxit;
+                st.linear_solves++;
+                hipLaunchKernelGGL(k_axpy, dim3(ceil_div(N * ns, 256)), dim3(256), 0, s, k, -1.0, S->bigK.p, N * ns);
+                CPG_LAUNCH_CHECK();
+            }
+            lincomb(*S, unp1, u, S->b, ns, k, s);
+            CPG_HIP(hipStreamSynchronize(s));
+            st.ms_total = ms_since(t_all);
+            if (!converged) {
+                st.status = CPG_ERR_NOT_CONVERGED;
+                set_error("IRK Newton did not reach %g in %d iterations (last %g)", S->sp.newton_tol,
+                          S->sp.newton_maxit, st.last_resid);
+                throw Error{CPG_ERR_NOT_CONVERGED};
+            }
+            return;
+        }
         const int ns = S->ns;
         bool all_converged = true;
         CPG_HIP(hipMemsetAsync(k, 0, N * ns * sizeof(double), s));
@@ -290,11 +449,22 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
             double *ks = k + N * stg;
             const double ts = t + dt * S->c[stg];
             bool converged = false;
+            if (S->sp.newton_guess == 1 && stg > 0)
+                CPG_HIP(hipMemcpyAsync(ks, ks - N, N * sizeof(double), hipMemcpyDeviceToDevice, s));
             for (int it = 0; it < S->sp.newton_maxit; it++) {
                 auto t0 = clk::now();
-                lincomb(*S, S->cu.p, u, &S->A[stg * ns], stg + 1, k, s);
-                eq_residual(*S->E, S->cu.p, ts, S->rhs.p, s);
-                const double rn = eq_newton_resid(*S->E, ks, S->rhs.p, dt, S->r.p, nullptr, s);
+                double rn = 0.0;
+                for (int half = 0;; half++) {
+                    lincomb(*S, S->cu.p, u, &S->A[stg * ns], stg + 1, k, s);
+                    eq_residual(*S->E, S->cu.p, ts, S->rhs.p, s);
+                    rn = eq_newton_resid(*S->E, ks, S->rhs.p, dt, S->r.p, nullptr, s);
+                    if (std::isfinite(rn) || !S->sp.newton_backtrack || it == 0 || half >= 10) break;
+                    // the last update k_s -= x led to a non-physical state: take half of it
+                    hipLaunchKernelGGL(k_axpy, dim3(ceil_div(N, 256)), dim3(256), 0, s, ks, std::ldexp(1.0, -half - 1),
+                                       S->x.p, N);
+                    CPG_LAUNCH_CHECK();
+                    st.backtracks++;
+                }
                 st.ms_residual += ms_since(t0);
                 st.last_resid = rn;
                 CPG_CHECK(std::isfinite(rn), CPG_ERR_NONFINITE, "non-finite Newton residual (stage %d, it %d)", stg, it);
@@ -305,13 +475,13 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                 CPG_HIP(hipStreamSynchronize(s));
                 st.ms_jacobian += ms_since(t0);
                 t0 = clk::now();
-                check(cpg_prec_factor(S->P, S->J, s));
+                check_ctx(cpg_prec_factor(S->P, S->J, s), stg, it, rn);
                 st.ms_factor += ms_since(t0);
                 t0 = clk::now();
                 vec_fill(S->x.p, 0.0, N, s);
                 double tol = S->sp.rtol;
                 int maxit = S->sp.maxit;
-                check(cpg_gmres(S->K, S->J, S->P, S->r.p, S->x.p, &tol, &maxit, s));
+                check_ctx(cpg_gmres(S->K, S->J, S->P, S->r.p, S->x.p, &tol, &maxit, s), stg, it, rn);
                 st.ms_gmres += ms_since(t0);
                 st.gmres_iters += maxit;
                 st.linear_solves++;

@@ -40,14 +40,15 @@ class EqParams(C.Structure):
 class SolverParams(C.Structure):
     _fields_ = [("gmres_mode", C.c_int), ("restart", C.c_int), ("rtol", C.c_double), ("maxit", C.c_int),
                 ("pc_kind", C.c_int), ("precision", C.c_int), ("newton_tol", C.c_double),
-                ("newton_maxit", C.c_int)]
+                ("newton_maxit", C.c_int), ("inner_iters", C.c_int), ("newton_guess", C.c_int),
+                ("newton_backtrack", C.c_int)]
 
 
 class StepStats(C.Structure):
     _fields_ = [("newton_iters", C.c_int), ("gmres_iters", C.c_int), ("linear_solves", C.c_int),
                 ("status", C.c_int), ("last_resid", C.c_double), ("ms_residual", C.c_double),
                 ("ms_jacobian", C.c_double), ("ms_factor", C.c_double), ("ms_gmres", C.c_double),
-                ("ms_total", C.c_double)]
+                ("ms_total", C.c_double), ("backtracks", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -163,6 +164,9 @@ class Prec:
             _lib.cpg_prec_destroy(self.h)
             self.h = None
 
+    def set_inner(self, iterations: int):
+        _chk(lib().cpg_prec_set_inner(self.h, int(iterations)))
+
     def factor(self, A: Bsr | None = None, stream=None):
         _chk(lib().cpg_prec_factor(self.h, (A or self.A).h, _stream(stream)))
 
@@ -216,7 +220,7 @@ def eq_params(s: _spec.EquationSpec) -> EqParams:
 
 def solver_params(s: _spec.SolverSpec) -> SolverParams:
     return SolverParams(s.gmres_mode, s.restart, s.rtol, s.maxit, s.pc_kind, s.precision, s.newton_tol,
-                        s.newton_maxit)
+                        s.newton_maxit, s.inner_iters, s.newton_guess, s.newton_backtrack)
 
 
 class Mesh:

@@ -141,6 +141,45 @@ class PolyMesh:
             while c in used:
                 c += 1
             colour[i] = c
+        # empty the smallest classes where their cells fit another colour: every class
+        # is one launch per DILU sweep, so a near-empty trailing class costs two launches
+        for _ in range(2):
+            nc = int(colour.max()) + 1
+            if nc <= 2:
+                break
+            sizes = np.bincount(colour, minlength=nc)
+            small = int(np.argmin(sizes))
+            members = np.nonzero(colour == small)[0]
+            moved = colour.copy()
+            ok = True
+            for i in members:
+                used = {moved[j] for j in adj[i]}
+                cand = [c for c in range(nc) if c != small and c not in used]
+                if not cand:
+                    # free a colour c held by a single neighbour w by moving w elsewhere
+                    for c in range(nc):
+                        if c == small:
+                            continue
+                        holders = [j for j in adj[i] if moved[j] == c]
+                        if len(holders) != 1:
+                            continue
+                        w = holders[0]
+                        wused = {moved[j] for j in adj[w]}
+                        alt = [d for d in range(nc) if d not in (small, c) and d not in wused]
+                        if alt:
+                            sizes[moved[w]] -= 1
+                            moved[w] = alt[0]
+                            sizes[alt[0]] += 1
+                            cand = [c]
+                            break
+                if not cand:
+                    ok = False
+                    break
+                moved[i] = min(cand, key=lambda c: sizes[c])
+                sizes[moved[i]] += 1
+            if not ok:
+                break
+            colour = np.where(moved > small, moved - 1, moved)
         perm = np.argsort(colour, kind="stable")  # new -> old
         m = self.permute(perm)
         m.colour = colour[perm].astype(np.int32)

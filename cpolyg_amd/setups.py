@@ -65,6 +65,13 @@ def sine_density(ux=1.0, uy=0.5, p=1.0, gamma=1.4, amp=0.2):
     return f
 
 
+def dg_time_step(cfl: float, h: float, deg: int, max_speed: float) -> float:
+    """dt = CFL h / ((2p+1) max(|v|+c)): the usual DG CFL number, in units of the explicit
+    stability limit of degree p.  The reference fixes dt in each driver and has no CFL
+    convention (e.g. examples/PaperEuler.cpp:68: dt = h*cfl)."""
+    return cfl * h / ((2 * deg + 1) * max_speed)
+
+
 def euler_max_speed(u: np.ndarray, deg: int, gamma: float = 1.4) -> float:
     """max(|v| + c) over the cell means (coefficient 0 of each component)."""
     bs = bs_of(deg)

@@ -20,6 +20,7 @@ BC_REF_VORTEX = 3
 PC_NONE = 0
 PC_BJACOBI = 1
 PC_DILU = 2
+PC_INNER_GMRES = 3   # FP32 GMRES(k) with DILU inside flexible FP64 GMRES
 
 GMRES_LEFT_REF = 0
 GMRES_FLEXIBLE = 1
@@ -87,6 +88,9 @@ class SolverSpec:
     newton_tol: float = 5e-13
     newton_maxit: int = 100
     precision: int = PREC_FP64
+    inner_iters: int = 20
+    newton_guess: int = 0        # 0: k_s = 0 (reference); 1: previous stage's k
+    newton_backtrack: int = 0    # 1: halve updates that lead to a non-finite residual
 
 
 def bs_of(deg: int) -> int:

@@ -73,9 +73,16 @@ int cpg_bsr_matvec(cpg_bsr A, const double *x, double *y, void *stream);
  * Factors are stored in `precision` (CPG_FP64, CPG_FP32, CPG_BF16 with one
  * power-of-two scale per block); FP32/BF16 factors are applied with FP32
  * accumulation.  x is FP64 in and out. */
-enum { CPG_PC_NONE = 0, CPG_PC_BJACOBI = 1, CPG_PC_DILU = 2 };
+enum { CPG_PC_NONE = 0, CPG_PC_BJACOBI = 1, CPG_PC_DILU = 2, CPG_PC_INNER_GMRES = 3 };
 enum { CPG_FP64 = 0, CPG_FP32 = 1, CPG_BF16 = 2 };
+/* CPG_PC_INNER_GMRES: a variable preconditioner for CPG_GMRES_FLEXIBLE -- a fixed number
+ * of FP32 GMRES iterations (cpg_prec_set_inner, default 20) on an FP32 copy of A, right-
+ * preconditioned by the DILU factors in `precision` (FP32 or BF16); FP64 reductions,
+ * device-side Givens, no host synchronisation.  No reference counterpart: it replaces the
+ * preconditioner apply of BlockMatrix::gmres (src/BlockMatrix.cpp:122) in the mixed-
+ * precision mode.  A must stay alive between cpg_prec_factor and cpg_prec_apply. */
 int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out);
+int cpg_prec_set_inner(cpg_prec P, int iterations);
 int cpg_prec_destroy(cpg_prec P);
 int cpg_prec_factor(cpg_prec P, cpg_bsr A, void *stream);
 int cpg_prec_apply(cpg_prec P, double *x, void *stream);   /* x <- M^-1 x, in place */
@@ -153,6 +160,11 @@ typedef struct {
     int precision;          /* factor storage CPG_FP64/FP32/BF16 */
     double newton_tol;      /* absolute, on max_c L2Norm of the Newton residual */
     int newton_maxit;
+    int inner_iters;        /* CPG_PC_INNER_GMRES: FP32 iterations per outer iteration (0: 20) */
+    int newton_guess;       /* 0: each stage starts from k_s = 0 (reference, src/TimeIntegration.cpp:176);
+                               1: from the previous stage's k */
+    int newton_backtrack;   /* 0: none (reference); 1: halve a Newton update (up to 10 times)
+                               while the residual at the new iterate is non-finite */
 } cpg_solver_params;
 
 typedef struct {
@@ -162,6 +174,7 @@ typedef struct {
     int status;
     double last_resid;
     double ms_residual, ms_jacobian, ms_factor, ms_gmres, ms_total;
+    int backtracks;         /* Newton update halvings (newton_backtrack) */
 } cpg_step_stats;
 
 int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver_params *params, cpg_stepper *out);

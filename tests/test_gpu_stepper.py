@@ -85,3 +85,23 @@ def test_rk4_advection(cuda):
     ud, outd = _dev(cuda, u), _dev(cuda, np.zeros_like(u))
     S.advance(ud, dt, 0.0, outd)
     assert rel(outd.cpu().numpy(), uo) < 1e-12
+
+
+@pytest.mark.parametrize("ns", [2, 3])
+def test_irk_radau_matches_oracle(cuda, ns):
+    """IRK::advance: stage-major coupled system, DILU, GMRES warm-started from the previous
+    Newton correction (src/TimeIntegration.cpp:188-281)."""
+    m = vortex_mesh(6)
+    deg = 1
+    spec = EquationSpec.euler(jacobian=1)
+    u = vortex_state(m, deg)
+    solver = SolverSpec(gmres_mode=GMRES_LEFT_REF, restart=20, rtol=1e-11, maxit=4000, pc_kind=PC_DILU,
+                        newton_tol=1e-12, newton_maxit=30)
+    dt = 0.4
+    uo, st = O.Oracle(m).irk_step(spec, deg, ns, u, dt, 0.0, solver)
+    E = hip.Equation(hip.Mesh(m), spec, deg)
+    S = hip.Stepper(E, 1, ns, solver)
+    ud, outd = _dev(cuda, u), _dev(cuda, np.zeros_like(u))
+    gs = S.advance(ud, dt, 0.0, outd)
+    assert abs(gs.newton_iters - st.newton_iters) <= 1
+    assert rel(outd.cpu().numpy(), uo) < 1e-9
