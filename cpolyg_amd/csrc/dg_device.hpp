@@ -244,12 +244,44 @@ __device__ inline void numfluxjac(const DevEq &eq, const double *U, const double
     }
 }
 
+// Derivative of the residual's LF speed with respect to the outer state U+: the speed is
+// the ordered maximum of (|vn-c|, |vn|, |vn+c|) of U- then of U+ (NaN-skipping exactly as
+// the residual's), and only when the maximum is one of U+'s three does it depend on U+.
+__device__ inline void euler_alpha_deriv_plus(double g, const double *Um, const double *Up, double nx, double ny,
+                                              double &alpha, double *d) {
+    const double u1 = Um[1] / Um[0], v1 = Um[2] / Um[0], u2 = Up[1] / Up[0], v2 = Up[2] / Up[0];
+    const double P1 = (g - 1) * (Um[3] - 0.5 * (Um[1] * Um[1] + Um[2] * Um[2]) / Um[0]);
+    const double P2 = (g - 1) * (Up[3] - 0.5 * (Up[1] * Up[1] + Up[2] * Up[2]) / Up[0]);
+    const double c1 = sqrt(g * P1 / Um[0]), c2 = sqrt(g * P2 / Up[0]);
+    const double vn1 = u1 * nx + v1 * ny, vn2 = u2 * nx + v2 * ny;
+    const double eig[6] = {vn1 - c1, vn1, vn1 + c1, vn2 - c2, vn2, vn2 + c2};
+    int idx = 0;
+    alpha = fabs(eig[0]);
+#pragma unroll
+    for (int k = 1; k < 6; k++)
+        if (alpha < fabs(eig[k])) { alpha = fabs(eig[k]); idx = k; }
+    d[0] = d[1] = d[2] = d[3] = 0.0;
+    if (idx >= 3) {
+        const double rho = Up[0], E = Up[3] / rho;
+        d[0] = -u2 / rho * nx - v2 / rho * ny;
+        d[1] = 1 / rho * nx;
+        d[2] = 1 / rho * ny;
+        if (idx != 4) {
+            const double usq = u2 * u2 + v2 * v2;
+            const double a = sqrt(g * g - g), b = rho * sqrt(4 * E - 2 * usq);
+            const double s = idx == 3 ? -1.0 : 1.0;
+            d[0] += s * a * (usq - E) / b; d[1] -= s * a * u2 / b; d[2] -= s * a * v2 / b; d[3] += s * a / b;
+        }
+        if (eig[idx] < 0) { d[0] = -d[0]; d[1] = -d[1]; d[2] = -d[2]; d[3] = -d[3]; }
+    }
+}
+
 // exact derivative of the LF flux with respect to the outer state
 __device__ inline void euler_offdiag_exact(double g, const double *Um, const double *Up, double nx, double ny,
                                            double *J) {
     double Bx[16], By[16], d[4], alpha;
     euler_fluxjac(g, Up, Bx, By, 0);
-    euler_alpha_deriv(g, Up, Um, nx, ny, alpha, d);
+    euler_alpha_deriv_plus(g, Um, Up, nx, ny, alpha, d);
 #pragma unroll
     for (int r = 0; r < 4; r++)
 #pragma unroll

@@ -12,6 +12,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 struct cpg_equation_s;
@@ -172,6 +173,9 @@ struct cpg_stepper_s {
     cpg_krylov K = nullptr;
     DevBuf<double> k, cu, rhs, r, x, minv, bigR, bigK;
     DevBuf<int> slot_row;
+    // CPOLYG_VERBOSE=1 in the environment at creation: one stderr line per Newton iteration,
+    // the progress the reference prints (TimeIntegration.cpp:209-264)
+    bool verbose = false;
     ~cpg_stepper_s() {
         if (P) cpg_prec_destroy(P);
         if (K) cpg_krylov_destroy(K);
@@ -225,6 +229,7 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
         CPG_CHECK(method == CPG_STEP_DIRK || method == CPG_STEP_RK4 || method == CPG_STEP_IRK, CPG_ERR_UNSUPPORTED,
                   "stepper method %d not available", method);
         auto *S = new cpg_stepper_s;
+        if (const char *v = std::getenv("CPOLYG_VERBOSE")) S->verbose = v[0] != '\0' && v[0] != '0';
         try {
             S->E = E;
             S->method = method;
@@ -485,6 +490,9 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                 st.ms_gmres += ms_since(t0);
                 st.gmres_iters += maxit;
                 st.linear_solves++;
+                if (S->verbose)
+                    std::fprintf(stderr, "cpolyg: stage %d Newton iteration %d, residual norm = %.6e, GMRES iterations %d, "
+                                 "backtracks %d\n", stg, it, rn, maxit, st.backtracks);
                 hipLaunchKernelGGL(k_axpy, dim3(ceil_div(N, 256)), dim3(256), 0, s, ks, -1.0, S->x.p, N);
                 CPG_LAUNCH_CHECK();
             }

@@ -311,6 +311,44 @@ static void euler_alpha_deriv(double g, const double *U1, const double *U2, doub
     }
 }
 
+/* Derivative of the residual's LF speed w.r.t. the OUTER state U2.  The speed is the
+ * first maximal |eigenvalue| of the list (U1's three, then U2's three) -- the order in
+ * which Euler::numericalFlux (Euler.cpp:100-130) takes std::max -- so that a NaN speed of
+ * an unphysical outer state is skipped here exactly as in the residual.  Nonzero only when
+ * the maximum belongs to U2.  (Not in the reference: its off-diagonal Jacobian is the
+ * row-cell quirk; this is the derivative used by the exact Jacobian option.) */
+static void euler_alpha_deriv_outer(double g, const double *U1, const double *U2, double nx, double ny,
+                                    double *alpha, double *d)
+{
+    double u1 = U1[1] / U1[0], v1 = U1[2] / U1[0], u2 = U2[1] / U2[0], v2 = U2[2] / U2[0];
+    double P1 = (g - 1) * (U1[3] - 0.5 * (U1[1] * U1[1] + U1[2] * U1[2]) / U1[0]);
+    double c1 = sqrt(g * P1 / U1[0]);
+    double P2 = (g - 1) * (U2[3] - 0.5 * (U2[1] * U2[1] + U2[2] * U2[2]) / U2[0]);
+    double c2 = sqrt(g * P2 / U2[0]);
+    double vn1 = u1 * nx + v1 * ny, vn2 = u2 * nx + v2 * ny;
+    double eig[6] = {vn1 - c1, vn1, vn1 + c1, vn2 - c2, vn2, vn2 + c2};
+    int idx = 0;
+    double best = fabs(eig[0]);
+    for (int k = 1; k < 6; k++)
+        if (fabs(eig[k]) > best) { best = fabs(eig[k]); idx = k; }
+    *alpha = best;
+    d[0] = d[1] = d[2] = d[3] = 0;
+    if (idx >= 3) {
+        double rho = U2[0], E = U2[3] / rho;
+        d[0] = -u2 / rho * nx - v2 / rho * ny;
+        d[1] = 1 / rho * nx;
+        d[2] = 1 / rho * ny;
+        double usq = u2 * u2 + v2 * v2;
+        double a = sqrt(g * g - g), b = rho * sqrt(4 * E - 2 * usq);
+        if (idx == 3) {
+            d[0] -= a * (usq - E) / b; d[1] += a * u2 / b; d[2] += a * v2 / b; d[3] -= a / b;
+        } else if (idx == 5) {
+            d[0] += a * (usq - E) / b; d[1] -= a * u2 / b; d[2] -= a * v2 / b; d[3] += a / b;
+        }
+        if (eig[idx] < 0) { d[0] = -d[0]; d[1] = -d[1]; d[2] = -d[2]; d[3] = -d[3]; }
+    }
+}
+
 static void flux(const o_eq *eq, const double *U, double x, double y, double *Fx, double *Fy)
 {
     if (eq->kind == O_EULER) {
@@ -658,7 +696,7 @@ int o_jacobian(const o_mesh *m, const o_eq *eq, int deg, const double *u, double
                         /* exact: dFhat/dU+ = 0.5 (A(U+) n - alpha I + (U- - U+) dalpha/dU+^T) */
                         double Bx[16], By[16], dd[4], alpha;
                         euler_fluxjac(eq->gamma, Up, Bx, By, 0);
-                        euler_alpha_deriv(eq->gamma, Up, Um, nx, ny, &alpha, dd);
+                        euler_alpha_deriv_outer(eq->gamma, Um, Up, nx, ny, &alpha, dd);
                         for (int r = 0; r < 4; r++)
                             for (int c = 0; c < 4; c++)
                                 Jo[4 * r + c] = 0.5 * (Bx[4 * r + c] * nx + By[4 * r + c] * ny
