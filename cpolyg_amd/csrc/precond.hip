@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace cpg {
@@ -309,6 +310,358 @@ __global__ __launch_bounds__(WG) void k_bjacobi_factor(int bl, const int *__rest
         c[j] = e < bb ? AUG[bb + e] : 0.0;
     }
     store_block<T, NE>(bl, F + (size_t)i * bb, scale ? scale + i : nullptr, c, scratch);
+}
+
+// ---- wavefront-per-row factor: FP64 MFMA block products, in-register Gauss-Jordan -------
+//
+// One 64-lane wavefront factors one block row (L blocks and D^-1); the DU blocks follow in
+// one launch over all upper blocks.  Block products C = P Q (bl x bl, FP64) run
+// on v_mfma_f64_16x16x4f64 over ceil(bl/16)^2 output tiles, operands read from two LDS
+// staging buffers; the products are written to HBM straight from the accumulator tiles.
+// The diagonal block is inverted by Gauss-Jordan with partial pivoting with one matrix
+// row per lane: the pivot search
+// is a cross-lane argmax, the pivot row is broadcast with readlane, and rows are never
+// swapped (the row/column order is restored when the inverse is written out).
+typedef double dx4 __attribute__((ext_vector_type(4)));
+
+template <int BL>
+struct WF {
+    static constexpr int LD = BL | 1;                                   // odd: conflict-free LDS columns
+    static constexpr int TM = (BL + 15) / 16;                           // 16x16 tiles per dimension
+    static constexpr int KS = (BL + 3) / 4;                             // k-steps of the 16x16x4 MFMA
+    static constexpr int BB = BL * BL;
+    static constexpr int VEC = BL % 4 == 0 ? 4 : BL % 2 == 0 ? 2 : 1;   // consecutive elements of one column
+    static constexpr int NCH = (BB + 64 * VEC - 1) / (64 * VEC);        // copy chunks per lane
+};
+
+__device__ inline double readlane_d(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ inline double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+}
+
+// copy chunks: a block in HBM (column-major, contiguous) <-> registers <-> LDS (column stride LD)
+template <int BL>
+struct Chunks {
+    double v[WF<BL>::NCH][WF<BL>::VEC];
+
+    __device__ void load(const double *__restrict__ g) {
+        using W = WF<BL>;
+#pragma unroll
+        for (int j = 0; j < W::NCH; j++) {
+            const int e = (threadIdx.x + 64 * j) * W::VEC;
+            if (e < W::BB) {
+                if constexpr (W::VEC == 4) {
+                    const double2 a = *reinterpret_cast<const double2 *>(g + e);
+                    const double2 b = *reinterpret_cast<const double2 *>(g + e + 2);
+                    v[j][0] = a.x; v[j][1] = a.y; v[j][2] = b.x; v[j][3] = b.y;
+                } else if constexpr (W::VEC == 2) {
+                    const double2 a = *reinterpret_cast<const double2 *>(g + e);
+                    v[j][0] = a.x; v[j][1] = a.y;
+                } else {
+                    v[j][0] = g[e];
+                }
+            }
+        }
+    }
+    __device__ void to_lds(double *X) const {
+        using W = WF<BL>;
+#pragma unroll
+        for (int j = 0; j < W::NCH; j++) {
+            const int e = (threadIdx.x + 64 * j) * W::VEC;
+            if (e < W::BB) {
+                const int c = e / BL, r = e - c * BL;
+#pragma unroll
+                for (int q = 0; q < W::VEC; q++) X[c * W::LD + r + q] = v[j][q];
+            }
+        }
+    }
+    __device__ void from_lds(const double *X) {
+        using W = WF<BL>;
+#pragma unroll
+        for (int j = 0; j < W::NCH; j++) {
+            const int e = (threadIdx.x + 64 * j) * W::VEC;
+            if (e < W::BB) {
+                const int c = e / BL, r = e - c * BL;
+#pragma unroll
+                for (int q = 0; q < W::VEC; q++) v[j][q] = X[c * W::LD + r + q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < W::VEC; q++) v[j][q] = 0.0;
+            }
+        }
+    }
+    // store in precision T (BF16: one power-of-two scale for the block, written to *scale_out)
+    template <class T>
+    __device__ void store(T *__restrict__ g, float *scale_out) const {
+        using W = WF<BL>;
+        float inv = 1.0f;
+        if constexpr (Store<T>::prec == CPG_BF16) {
+            double mx = 0.0;
+#pragma unroll
+            for (int j = 0; j < W::NCH; j++)
+#pragma unroll
+                for (int q = 0; q < W::VEC; q++) mx = fmax(mx, fabs(v[j][q]));
+            const float s = pow2_scale(wave_max(mx));
+            inv = 1.0f / s;
+            if (threadIdx.x == 0) *scale_out = s;
+        }
+#pragma unroll
+        for (int j = 0; j < W::NCH; j++) {
+            const int e = (threadIdx.x + 64 * j) * W::VEC;
+            if (e < W::BB) {
+#pragma unroll
+                for (int q = 0; q < W::VEC; q++) Store<T>::put(g, e + q, v[j][q], inv);
+            }
+        }
+    }
+};
+
+// acc (+)= (NEG ? -P : P) * Q with P, Q in LDS (column stride LD)
+template <int BL, bool NEG>
+__device__ inline void mma_lds(const double *P, const double *Q, dx4 (&acc)[WF<BL>::TM][WF<BL>::TM]) {
+    using W = WF<BL>;
+    const int lr = threadIdx.x & 15, lk = threadIdx.x >> 4;
+#pragma unroll
+    for (int ks = 0; ks < W::KS; ks++) {
+        const int k = 4 * ks + lk;
+        double a[W::TM], b[W::TM];
+#pragma unroll
+        for (int t = 0; t < W::TM; t++) {
+            const int m = 16 * t + lr;
+            const bool in = k < BL && m < BL;
+            const double pa = P[in ? k * W::LD + m : 0];
+            const double qb = Q[in ? m * W::LD + k : 0];
+            a[t] = in ? (NEG ? -pa : pa) : 0.0;
+            b[t] = in ? qb : 0.0;
+        }
+#pragma unroll
+        for (int tm = 0; tm < W::TM; tm++)
+#pragma unroll
+            for (int tn = 0; tn < W::TM; tn++)
+                acc[tm][tn] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[tm], b[tn], acc[tm][tn], 0, 0, 0);
+    }
+}
+
+template <int BL>
+__device__ inline void tiles_zero(dx4 (&t)[WF<BL>::TM][WF<BL>::TM]) {
+#pragma unroll
+    for (int a = 0; a < WF<BL>::TM; a++)
+#pragma unroll
+        for (int b = 0; b < WF<BL>::TM; b++) t[a][b] = dx4{0.0, 0.0, 0.0, 0.0};
+}
+
+// tile element (tm, tn, i) of lane l sits at row 16 tm + l / 16 + 4 i, column 16 tn + l % 16
+// (the v_mfma_f64_16x16x4f64 result layout)
+template <int BL>
+__device__ inline void tiles_load(dx4 (&t)[WF<BL>::TM][WF<BL>::TM], const double *__restrict__ g) {
+    using W = WF<BL>;
+    const int r0 = threadIdx.x >> 4, c0 = threadIdx.x & 15;
+#pragma unroll
+    for (int tm = 0; tm < W::TM; tm++)
+#pragma unroll
+        for (int tn = 0; tn < W::TM; tn++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int row = 16 * tm + r0 + 4 * i, col = 16 * tn + c0;
+                t[tm][tn][i] = (row < BL && col < BL) ? g[col * BL + row] : 0.0;
+            }
+}
+
+template <int BL>
+__device__ inline void tiles_to_lds(double *X, const dx4 (&t)[WF<BL>::TM][WF<BL>::TM]) {
+    using W = WF<BL>;
+    const int r0 = threadIdx.x >> 4, c0 = threadIdx.x & 15;
+#pragma unroll
+    for (int tm = 0; tm < W::TM; tm++)
+#pragma unroll
+        for (int tn = 0; tn < W::TM; tn++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int row = 16 * tm + r0 + 4 * i, col = 16 * tn + c0;
+                if (row < BL && col < BL) X[col * W::LD + row] = t[tm][tn][i];
+            }
+}
+
+template <class T, int BL>
+__device__ inline void tiles_store(T *__restrict__ g, float *scale_out, const dx4 (&t)[WF<BL>::TM][WF<BL>::TM]) {
+    using W = WF<BL>;
+    const int r0 = threadIdx.x >> 4, c0 = threadIdx.x & 15;
+    float inv = 1.0f;
+    if constexpr (Store<T>::prec == CPG_BF16) {
+        double mx = 0.0;
+#pragma unroll
+        for (int tm = 0; tm < W::TM; tm++)
+#pragma unroll
+            for (int tn = 0; tn < W::TM; tn++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int row = 16 * tm + r0 + 4 * i, col = 16 * tn + c0;
+                    if (row < BL && col < BL) mx = fmax(mx, fabs(t[tm][tn][i]));
+                }
+        const float s = pow2_scale(wave_max(mx));
+        inv = 1.0f / s;
+        if (threadIdx.x == 0) *scale_out = s;
+    }
+#pragma unroll
+    for (int tm = 0; tm < W::TM; tm++)
+#pragma unroll
+        for (int tn = 0; tn < W::TM; tn++) {
+            const int col = 16 * tn + c0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int row = 16 * tm + r0 + 4 * i;
+                if (row < BL && col < BL) Store<T>::put(g, (size_t)col * BL + row, t[tm][tn][i], inv);
+            }
+        }
+}
+
+// In-place Gauss-Jordan inverse of the bl x bl block in X (column stride LD), one row per
+// lane, partial pivoting on the first maximal |pivot| among rows not yet used.  Rows are
+// never swapped and the pivot row is not rescaled during the elimination: a row keeps its
+// stored values and a scale 1/pivot from the step it was chosen, so every lane applies
+// the same update w -= (w_p / pivot) * w_pivotrow (zero for the pivot lane itself) and the
+// scales are applied when the inverse is written back to X.  Returns false (uniformly)
+// on a zero pivot.
+template <int BL>
+__device__ inline bool wave_gauss_jordan(double *X) {
+    using W = WF<BL>;
+    const int lane = threadIdx.x;
+    const bool row_lane = lane < BL;
+    double w[BL];
+#pragma unroll
+    for (int c = 0; c < BL; c++) w[c] = row_lane ? X[c * W::LD + lane] : 0.0;
+    bool used = !row_lane;
+    int my_step = 0;         // step at which this lane's row was the pivot row
+    int my_rho = 0;          // lane p: pivot row of step p
+    double my_scale = 1.0;   // 1 / pivot of that step
+    bool ok = true;
+    // w is rotated by one column per step: at step p, w[0] holds column p and the finished
+    // column goes to the end, so after bl steps the columns are back in their order and no
+    // register is ever indexed by the loop variable
+#pragma unroll 1
+    for (int p = 0; p < BL; p++) {
+        const double v = w[0];
+        // first maximal |W[r][p]| over unused rows r; a NaN entry ranks below every number but
+        // above the used rows (and the lanes past bl), so the pivot is always an unused row
+        double best = used ? -1.0 : (v == v ? fabs(v) : -0.5);
+        int bi = lane;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double ob = __shfl_xor(best, off);
+            const int oi = __shfl_xor(bi, off);
+            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        const int rho = __builtin_amdgcn_readfirstlane(bi);
+        const double piv = readlane_d(v, rho);
+        ok = ok && piv != 0.0;
+        const double inv = piv != 0.0 ? 1.0 / piv : 1.0;
+        const bool me = lane == rho;
+        if (me) { used = true; my_step = p; my_scale = inv; }
+        if (lane == p) my_rho = rho;
+        const double f = me ? 0.0 : v * inv;
+#pragma unroll
+        for (int c = 1; c < BL; c++) w[c - 1] = fma(-f, readlane_d(w[c], rho), w[c]);
+        w[BL - 1] = me ? 1.0 : -f;
+    }
+    __syncthreads();
+    // A^-1[step(r)][rho_q] = scale(r) W[r][q]
+#pragma unroll
+    for (int q = 0; q < BL; q++) {
+        const int col = __builtin_amdgcn_readlane(my_rho, q);
+        if (row_lane) X[col * W::LD + my_step] = w[q] * my_scale;
+    }
+    __syncthreads();
+    return ok;
+}
+
+template <class T, int BL>
+__global__ __launch_bounds__(64) void k_dilu_factor_wf(RowLists L, const double *__restrict__ A,
+                                                       T *__restrict__ F, float *__restrict__ scale,
+                                                       double *__restrict__ DD64, int *__restrict__ status) {
+    using W = WF<BL>;
+    __shared__ double X[BL * W::LD];
+    __shared__ double Y[BL * W::LD];
+    const int i = L.rows[blockIdx.x];
+    const int d = L.diag[i];
+    constexpr size_t bb = (size_t)W::BB;
+    dx4 acc[W::TM][W::TM];
+    tiles_load<BL>(acc, A + (size_t)d * bb);
+    Chunks<BL> ca, cb;
+    for (int q = L.lptr[i]; q < L.lptr[i + 1]; q++) {
+        const int s = L.lslot[q], k = L.lcol[q], st = L.ltrans[q];
+        ca.load(A + (size_t)s * bb);
+        cb.load(DD64 + (size_t)k * bb);
+        __syncthreads();
+        ca.to_lds(X);
+        cb.to_lds(Y);
+        __syncthreads();
+        ca.load(A + (size_t)st * bb);   // A_ki, in flight during the first product
+        dx4 t[W::TM][W::TM];
+        tiles_zero<BL>(t);
+        mma_lds<BL, false>(X, Y, t);     // L_ik = A_ik D_k^-1
+        tiles_store<T, BL>(F + (size_t)s * bb, scale ? scale + s : nullptr, t);
+        __syncthreads();
+        tiles_to_lds<BL>(X, t);
+        ca.to_lds(Y);
+        __syncthreads();
+        mma_lds<BL, true>(X, Y, acc);    // D_i -= L_ik A_ki
+    }
+    __syncthreads();
+    tiles_to_lds<BL>(X, acc);
+    __syncthreads();
+    const bool ok = wave_gauss_jordan<BL>(X);
+    if (!ok && threadIdx.x == 0) atomicOr(status, 1);
+    ca.from_lds(X);
+    ca.template store<double>(DD64 + (size_t)i * bb, nullptr);
+    ca.template store<T>(F + (size_t)d * bb, scale ? scale + d : nullptr);
+}
+
+// DU_ij = D_i^-1 A_ij for every upper block, one wavefront per block, after all levels
+template <class T, int BL>
+__global__ __launch_bounds__(64) void k_dilu_upper_wf(const int *__restrict__ urow, const int *__restrict__ uslot,
+                                                      const double *__restrict__ A, const double *__restrict__ DD64,
+                                                      T *__restrict__ F, float *__restrict__ scale) {
+    using W = WF<BL>;
+    __shared__ double X[BL * W::LD];
+    __shared__ double Y[BL * W::LD];
+    const int i = urow[blockIdx.x], s = uslot[blockIdx.x];
+    constexpr size_t bb = (size_t)W::BB;
+    Chunks<BL> ca, cb;
+    ca.load(DD64 + (size_t)i * bb);
+    cb.load(A + (size_t)s * bb);
+    ca.to_lds(X);
+    cb.to_lds(Y);
+    __syncthreads();
+    dx4 t[W::TM][W::TM];
+    tiles_zero<BL>(t);
+    mma_lds<BL, false>(X, Y, t);
+    tiles_store<T, BL>(F + (size_t)s * bb, scale ? scale + s : nullptr, t);
+}
+
+template <class T, int BL>
+__global__ __launch_bounds__(64) void k_bjacobi_factor_wf(const int *__restrict__ diag, const double *__restrict__ A,
+                                                          T *__restrict__ F, float *__restrict__ scale,
+                                                          int *__restrict__ status) {
+    using W = WF<BL>;
+    __shared__ double X[BL * W::LD];
+    const int i = blockIdx.x;
+    const int d = diag[i];
+    constexpr size_t bb = (size_t)W::BB;
+    Chunks<BL> c;
+    c.load(A + (size_t)d * bb);
+    c.to_lds(X);
+    __syncthreads();
+    const bool ok = wave_gauss_jordan<BL>(X);
+    if (!ok && threadIdx.x == 0) atomicOr(status, 1);
+    c.from_lds(X);
+    c.template store<T>(F + (size_t)i * bb, scale ? scale + i : nullptr);
 }
 
 // ---- apply kernels -----------------------------------------------------------------
@@ -739,6 +1092,8 @@ struct cpg_prec_s {
     int n = 0, bl = 0, nnzb = 0;
     // pattern-derived lists
     DevBuf<int> diag, lptr, lslot, lcol, ltrans, uptr, uslot, ucol;
+    DevBuf<int> urow;             // row of each upper slot (flat, in uslot order)
+    int nup = 0;
     DevBuf<int> frows, brows;
     std::vector<int> flev, blev;  // level pointers into frows / brows
     // storage
@@ -813,8 +1168,55 @@ void factor_bj_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
     CPG_LAUNCH_CHECK();
 }
 
+template <class T, int BL>
+void factor_wf_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
+    float *sc = P.precision == CPG_BF16 ? P.scale.p : nullptr;
+    if (!P.dilu()) {
+        hipLaunchKernelGGL((k_bjacobi_factor_wf<T, BL>), dim3(P.n), dim3(64), 0, s, P.diag.p, A.blocks.p, Fp<T>(P), sc,
+                           P.status.p);
+        CPG_LAUNCH_CHECK();
+        return;
+    }
+    RowLists L = P.lists(true);
+    for (size_t lv = 0; lv + 1 < P.flev.size(); lv++) {
+        const int nr = P.flev[lv + 1] - P.flev[lv];
+        if (!nr) continue;
+        RowLists Ll = L;
+        Ll.rows = P.frows.p + P.flev[lv];
+        hipLaunchKernelGGL((k_dilu_factor_wf<T, BL>), dim3(nr), dim3(64), 0, s, Ll, A.blocks.p, Fp<T>(P), sc,
+                           P.DD64.p, P.status.p);
+        CPG_LAUNCH_CHECK();
+    }
+    if (P.nup > 0) {
+        hipLaunchKernelGGL((k_dilu_upper_wf<T, BL>), dim3(P.nup), dim3(64), 0, s, P.urow.p, P.uslot.p, A.blocks.p,
+                           P.DD64.p, Fp<T>(P), sc);
+        CPG_LAUNCH_CHECK();
+    }
+}
+
+// Block sizes of the DG systems (4 bs for Euler, bs for scalar advection, bs = (p+1)(p+2)/2)
+// take the wavefront MFMA kernels; CPOLYG_FACTOR_LDS=1 selects the workgroup LDS kernels.
+bool use_lds_factor() {
+    static const bool v = [] {
+        const char *e = std::getenv("CPOLYG_FACTOR_LDS");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
+
 template <class T>
 void factor_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
+    if (!use_lds_factor()) {
+        switch (P.bl) {
+        case 10: factor_wf_t<T, 10>(P, A, s); return;
+        case 12: factor_wf_t<T, 12>(P, A, s); return;
+        case 15: factor_wf_t<T, 15>(P, A, s); return;
+        case 24: factor_wf_t<T, 24>(P, A, s); return;
+        case 40: factor_wf_t<T, 40>(P, A, s); return;
+        case 60: factor_wf_t<T, 60>(P, A, s); return;
+        default: break;
+        }
+    }
     const int ne = ceil_div((long long)P.bl * P.bl, WG);
 #define CPG_NE_CASE(NEV)                                                    \
     if (ne <= NEV) {                                                        \
@@ -992,7 +1394,7 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
             const size_t bb = (size_t)bl * bl;
             const auto &rp = A->h_rowptr;
             const auto &ci = A->h_colind;
-            std::vector<int> diag(n, -1), lptr(n + 1, 0), uptr(n + 1, 0), lslot, lcol, ltrans, uslot, ucol;
+            std::vector<int> diag(n, -1), lptr(n + 1, 0), uptr(n + 1, 0), lslot, lcol, ltrans, uslot, ucol, urow;
             for (int i = 0; i < n; i++) {
                 for (int s = rp[i]; s < rp[i + 1]; s++) {
                     const int j = ci[s];
@@ -1010,6 +1412,7 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
                     } else {
                         uslot.push_back(s);
                         ucol.push_back(j);
+                        urow.push_back(i);
                     }
                 }
                 CPG_CHECK(kind == CPG_PC_NONE || diag[i] >= 0, CPG_ERR_ARG, "row %d has no diagonal block", i);
@@ -1024,6 +1427,8 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
             P->ltrans.upload(ltrans);
             P->uslot.upload(uslot);
             P->ucol.upload(ucol);
+            P->urow.upload(urow);
+            P->nup = (int)urow.size();
             std::vector<int> frows, brows;
             levels(n, lptr, lcol, true, P->flev, frows);
             levels(n, uptr, ucol, false, P->blev, brows);
