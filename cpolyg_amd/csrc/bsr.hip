@@ -1,14 +1,14 @@
 // Block sparse row matrix in HBM and its matvec.
 //
 // Replaces BlockMatrix storage and BlockMatrix::matvec (src/BlockMatrix.cpp:10-49),
-// which does one BLAS dgemv per block on a heap copy of x.  Here one workgroup
-// owns one block row: the row's blocks are contiguous (nnzb_row * bl * bl
-// doubles) and column-major, so the row is read as a stream of bl-long columns.
-// A group of bl/VW lanes covers one column with VW-wide loads; G = 256/(bl/VW)
-// groups walk the row's columns with stride G; the G partial vectors are summed
-// through LDS.  x is gathered one double per lane (bl/VW lanes share it).
+// which does one BLAS dgemv per block on a heap copy of x.  A row's blocks are
+// contiguous (nnzb_row * bl * bl doubles, column-major), so the matvec is the row-stream
+// kernel of rowstream.hpp with the BSR arrays as its stream: one workgroup per block row,
+// x gathered once into LDS, 16-byte column loads, fixed-order LDS reduction.
 #include "cpg_internal.hpp"
+#include "rowstream.hpp"
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstring>
 
@@ -23,74 +23,43 @@ void set_error(const char *fmt, ...) {
     va_end(ap);
 }
 
-// ------------------------------------------------------------------------------------
-template <int BL, int VW>
-__global__ __launch_bounds__(256) void k_bsr_matvec(int bl_rt, const int *__restrict__ rowptr,
-                                                    const int *__restrict__ colind,
-                                                    const double *__restrict__ A, const double *__restrict__ x,
-                                                    double *__restrict__ y) {
-    extern __shared__ double red[];
-    const int bl = BL > 0 ? BL : bl_rt;
-    const int tpc = bl / VW;
-    const int G = 256 / tpc;
-    const int t = threadIdx.x;
-    const int g = t / tpc;
-    const int rr = (t - g * tpc) * VW;
-    const int i = blockIdx.x;
-    const int s0 = rowptr[i], s1 = rowptr[i + 1];
-    const int ncols = (s1 - s0) * bl;
-    double acc[VW];
-#pragma unroll
-    for (int v = 0; v < VW; v++) acc[v] = 0.0;
-    if (g < G) {
-        const double *base = A + (size_t)s0 * bl * bl + rr;
-#pragma unroll 4
-        for (int cg = g; cg < ncols; cg += G) {
-            const int blk = cg / bl;
-            const int col = cg - blk * bl;
-            const double xv = x[(size_t)colind[s0 + blk] * bl + col];
-            if constexpr (VW == 2) {
-                const double2 a = *reinterpret_cast<const double2 *>(base + (size_t)cg * bl);
-                acc[0] = fma(a.x, xv, acc[0]);
-                acc[1] = fma(a.y, xv, acc[1]);
-            } else {
-                acc[0] = fma(base[(size_t)cg * bl], xv, acc[0]);
-            }
-        }
-#pragma unroll
-        for (int v = 0; v < VW; v++) red[g * bl + rr + v] = acc[v];
-    }
-    __syncthreads();
-    if (t < bl) {
-        double s = 0.0;
-        for (int gg = 0; gg < G; gg++) s += red[gg * bl + t];
-        y[(size_t)i * bl + t] = s;
-    }
+// ---- row-stream launch shape --------------------------------------------------------
+// Waves per row of the compiled row-stream kernels, per storage element size (8, 4, 2) and
+// block size.  Defaults come from timing sweeps on MI355X (cpg_stream_tune re-measures them).
+namespace {
+struct WprEntry {
+    int elem, bl, wpr;
+};
+std::vector<WprEntry> &wpr_table() {
+    static std::vector<WprEntry> t;
+    return t;
+}
+}  // namespace
+
+int stream_wpr(int elem_bytes, int bl) {
+    for (const auto &e : wpr_table())
+        if (e.elem == elem_bytes && e.bl == bl) return e.wpr;
+    return 4;
 }
 
-template <int BL, int VW>
-static void launch_matvec(const cpg_bsr_s &A, const double *x, double *y, hipStream_t s) {
-    const int bl = A.bl;
-    const int tpc = bl / VW;
-    const int G = 256 / tpc;
-    size_t lds = (size_t)G * bl * sizeof(double);
-    hipLaunchKernelGGL((k_bsr_matvec<BL, VW>), dim3(A.n), dim3(256), lds, s, bl, A.rowptr.p, A.colind.p,
-                       A.blocks.p, x, y);
-    CPG_LAUNCH_CHECK();
+void set_stream_wpr(int elem_bytes, int bl, int wpr) {
+    for (auto &e : wpr_table())
+        if (e.elem == elem_bytes && e.bl == bl) {
+            e.wpr = wpr;
+            return;
+        }
+    wpr_table().push_back({elem_bytes, bl, wpr});
 }
 
 void bsr_matvec(const cpg_bsr_s &A, const double *x, double *y, hipStream_t s) {
     if (A.n == 0) return;
-    switch (A.bl) {
-    case 40: launch_matvec<40, 2>(A, x, y, s); break;
-    case 60: launch_matvec<60, 2>(A, x, y, s); break;
-    case 120: launch_matvec<120, 2>(A, x, y, s); break;
-    case 10: launch_matvec<10, 2>(A, x, y, s); break;
-    default:
-        CPG_CHECK(A.bl <= 256, CPG_ERR_UNSUPPORTED, "block size %d > 256", A.bl);
-        if (A.bl % 2 == 0) launch_matvec<0, 2>(A, x, y, s);
-        else launch_matvec<0, 1>(A, x, y, s);
-    }
+    StreamArgs<double> a;
+    a.S = RowStream{nullptr, A.rowptr.p, A.colind.p};
+    a.items = A.n;
+    a.nbmax = A.nbmax;
+    a.src1 = x;
+    a.out = y;
+    launch_rowstream<double, double>(A.bl, a, A.blocks.p, nullptr, s);
 }
 
 // ------------------------------------------------------------------------------------
@@ -182,6 +151,7 @@ int cpg_bsr_create(int n_rows, int bl, int nnzb, const int *rowptr, const int *c
             A->nnzb = nnzb;
             A->h_rowptr.assign(rowptr, rowptr + n_rows + 1);
             A->h_colind.assign(colind, colind + nnzb);
+            for (int i = 0; i < n_rows; i++) A->nbmax = std::max(A->nbmax, rowptr[i + 1] - rowptr[i]);
             A->rowptr.upload(A->h_rowptr);
             A->colind.upload(A->h_colind);
             A->blocks.alloc((size_t)nnzb * bl * bl);
@@ -245,6 +215,83 @@ int cpg_bsr_matvec(cpg_bsr A, const double *x, double *y, void *stream) {
         CPG_CHECK(A && x && y, CPG_ERR_ARG, "null argument");
         CPG_CHECK(x != y, CPG_ERR_ARG, "x and y alias");
         bsr_matvec(*A, x, y, S(stream));
+    });
+}
+
+static int elem_of(int precision) { return precision == CPG_FP64 ? 8 : precision == CPG_FP32 ? 4 : 2; }
+
+int cpg_stream_set_wpr(int precision, int bl, int wpr) {
+    return guard([&] {
+        CPG_CHECK(precision >= CPG_FP64 && precision <= CPG_BF16, CPG_ERR_ARG, "bad precision %d", precision);
+        CPG_CHECK(wpr == 1 || wpr == 2 || wpr == 4, CPG_ERR_ARG, "waves per row must be 1, 2 or 4 (got %d)", wpr);
+        CPG_CHECK(bl > 0, CPG_ERR_ARG, "bad block size %d", bl);
+        set_stream_wpr(elem_of(precision), bl, wpr);
+    });
+}
+
+int cpg_stream_get_wpr(int precision, int bl, int *wpr) {
+    return guard([&] {
+        CPG_CHECK(wpr && precision >= CPG_FP64 && precision <= CPG_BF16, CPG_ERR_ARG, "bad argument");
+        *wpr = stream_wpr(elem_of(precision), bl);
+    });
+}
+
+// Times the row stream of A's pattern (a matvec with blocks stored in `precision`) for each
+// launch shape and keeps the fastest.  The block values do not change the timing, so the
+// reduced-precision streams run over zero-filled blocks.
+int cpg_stream_autotune(cpg_bsr A, int precision, int reps, int *best_wpr, float *ms, void *stream) {
+    return guard([&] {
+        CPG_CHECK(A && precision >= CPG_FP64 && precision <= CPG_BF16 && reps > 0, CPG_ERR_ARG, "bad argument");
+        if (A->n == 0) return;
+        hipStream_t s = S(stream);
+        const int elem = elem_of(precision);
+        const size_t N = (size_t)A->n * A->bl;
+        DevBuf<double> x(N), y(N);
+        CPG_HIP(hipMemsetAsync(x.p, 0, N * sizeof(double), s));
+        DevBuf<unsigned char> F;
+        DevBuf<float> sc;
+        if (precision != CPG_FP64) {
+            F.alloc(A->blocks.n * elem);
+            CPG_HIP(hipMemsetAsync(F.p, 0, F.n, s));
+            if (precision == CPG_BF16) {
+                sc.alloc(A->nnzb);
+                CPG_HIP(hipMemsetAsync(sc.p, 0, sc.n * sizeof(float), s));
+            }
+        }
+        StreamArgs<double> a;
+        a.S = RowStream{nullptr, A->rowptr.p, A->colind.p};
+        a.items = A->n;
+        a.nbmax = A->nbmax;
+        a.src1 = x.p;
+        a.out = y.p;
+        auto run = [&](int wpr) {
+            if (precision == CPG_FP64) launch_rowstream<double, double>(A->bl, a, A->blocks.p, nullptr, s, wpr);
+            else if (precision == CPG_FP32)
+                launch_rowstream<float, double>(A->bl, a, reinterpret_cast<const float *>(F.p), nullptr, s, wpr);
+            else launch_rowstream<bf16, double>(A->bl, a, reinterpret_cast<const bf16 *>(F.p), sc.p, s, wpr);
+        };
+        hipEvent_t e0, e1;
+        CPG_HIP(hipEventCreate(&e0));
+        CPG_HIP(hipEventCreate(&e1));
+        const int cand[3] = {1, 2, 4};
+        float best = 1e30f;
+        int bw = 4;
+        for (int c = 0; c < 3; c++) {
+            run(cand[c]);
+            CPG_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; r++) run(cand[c]);
+            CPG_HIP(hipEventRecord(e1, s));
+            CPG_HIP(hipEventSynchronize(e1));
+            float t = 0.0f;
+            CPG_HIP(hipEventElapsedTime(&t, e0, e1));
+            t /= (float)reps;
+            if (ms) ms[c] = t;
+            if (t < best) { best = t; bw = cand[c]; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        set_stream_wpr(elem, A->bl, bw);
+        if (best_wpr) *best_wpr = bw;
     });
 }
 

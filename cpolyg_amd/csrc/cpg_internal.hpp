@@ -100,6 +100,7 @@ inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 // ---- handle types ------------------------------------------------------------------
 struct cpg_bsr_s {
     int n = 0, bl = 0, nnzb = 0;
+    int nbmax = 1;   // longest block row
     std::vector<int> h_rowptr, h_colind;
     cpg::DevBuf<int> rowptr, colind;
     cpg::DevBuf<double> blocks;
@@ -109,4 +110,7 @@ namespace cpg {
 // shared kernels-launchers used across translation units
 void bsr_matvec(const cpg_bsr_s &A, const double *x, double *y, hipStream_t s);
 void vec_fill(double *x, double v, size_t n, hipStream_t s);
+// waves per block row of the row-stream kernels (rowstream.hpp), per element size and block size
+int stream_wpr(int elem_bytes, int bl);
+void set_stream_wpr(int elem_bytes, int bl, int wpr);
 }  // namespace cpg

@@ -13,10 +13,17 @@
 // The factor runs in FP64 and rounds once when storing to the requested precision;
 // an FP64 copy of D^-1 is kept for the L products of later levels.
 //
-// Apply (one workgroup per row, each level a launch): forward y_i = x_i - sum L_ik y_k,
-// z_i = D_i^-1 y_i (same kernel), backward x_i = z_i - sum DU_ij x_j.  Blocks are
-// streamed column by column exactly like the BSR matvec.
+// Storage: the factor blocks are packed in the order the sweeps read them (rowstream.hpp):
+// the forward stream holds, level by level and row by row, the L blocks of each row; the
+// backward stream, level by level (from the last), each row's D^-1 followed by its DU
+// blocks.  pos[slot] maps a slot of A to its packed block.
+// Apply (row-stream launches, one per level and direction):
+//   forward  y_i = x_i - sum_k L_ik y_k,
+//   backward x_i = D_i^-1 y_i - sum_j DU_ij x_j,
+// which is (I + DU)^-1 D^-1 (I + L)^-1 x with the D^-1 product moved into the backward
+// sweep, so every row of either sweep is one independent stream of blocks.
 #include "cpg_internal.hpp"
+#include "rowstream.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -26,9 +33,6 @@
 namespace cpg {
 
 // ---- storage types -----------------------------------------------------------------
-struct bf16 {
-    uint16_t v;
-};
 
 __host__ __device__ inline float bf16_to_f(uint16_t h) {
     uint32_t u = (uint32_t)h << 16;
@@ -209,6 +213,7 @@ __device__ bool gauss_jordan(int bl, double *AUG, double *fac, int *piv_row) {
 
 // ---- factor kernels ----------------------------------------------------------------
 struct RowLists {
+    const int *pos;       // packed block of each slot of A
     const int *rows;      // rows of this level
     const int *diag;      // diagonal slot per row
     const int *lptr, *lslot, *lcol, *ltrans;  // lower slots, their columns, transposed slot
@@ -245,7 +250,7 @@ __global__ __launch_bounds__(WG) void k_dilu_factor(int bl, RowLists L, const do
         load_block<NE>(bl, Q, DD64 + (size_t)k * bb);
         __syncthreads();
         gemm_lds<NE>(bl, P, Q, c);  // L_ik = A_ik D_k^-1
-        store_block<T, NE>(bl, F + (size_t)s * bb, scale ? scale + s : nullptr, c, scratch);
+        store_block<T, NE>(bl, F + (size_t)L.pos[s] * bb, scale ? scale + L.pos[s] : nullptr, c, scratch);
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < NE; j++) {
@@ -272,14 +277,14 @@ __global__ __launch_bounds__(WG) void k_dilu_factor(int bl, RowLists L, const do
         c[j] = e < bb ? Dinv[e] : 0.0;
         if (e < bb) DD64[(size_t)i * bb + e] = c[j];
     }
-    store_block<T, NE>(bl, F + (size_t)d * bb, scale ? scale + d : nullptr, c, scratch);
+    store_block<T, NE>(bl, F + (size_t)L.pos[d] * bb, scale ? scale + L.pos[d] : nullptr, c, scratch);
     for (int q = L.uptr[i]; q < L.uptr[i + 1]; q++) {
         const int s = L.uslot[q];
         __syncthreads();
         load_block<NE>(bl, Q, A + (size_t)s * bb);
         __syncthreads();
         gemm_lds<NE>(bl, Dinv, Q, c);  // DU_ij = D_i^-1 A_ij
-        store_block<T, NE>(bl, F + (size_t)s * bb, scale ? scale + s : nullptr, c, scratch);
+        store_block<T, NE>(bl, F + (size_t)L.pos[s] * bb, scale ? scale + L.pos[s] : nullptr, c, scratch);
     }
 }
 
@@ -606,7 +611,7 @@ __global__ __launch_bounds__(64) void k_dilu_factor_wf(RowLists L, const double 
         dx4 t[W::TM][W::TM];
         tiles_zero<BL>(t);
         mma_lds<BL, false>(X, Y, t);     // L_ik = A_ik D_k^-1
-        tiles_store<T, BL>(F + (size_t)s * bb, scale ? scale + s : nullptr, t);
+        tiles_store<T, BL>(F + (size_t)L.pos[s] * bb, scale ? scale + L.pos[s] : nullptr, t);
         __syncthreads();
         tiles_to_lds<BL>(X, t);
         ca.to_lds(Y);
@@ -620,14 +625,15 @@ __global__ __launch_bounds__(64) void k_dilu_factor_wf(RowLists L, const double 
     if (!ok && threadIdx.x == 0) atomicOr(status, 1);
     ca.from_lds(X);
     ca.template store<double>(DD64 + (size_t)i * bb, nullptr);
-    ca.template store<T>(F + (size_t)d * bb, scale ? scale + d : nullptr);
+    ca.template store<T>(F + (size_t)L.pos[d] * bb, scale ? scale + L.pos[d] : nullptr);
 }
 
 // DU_ij = D_i^-1 A_ij for every upper block, one wavefront per block, after all levels
 template <class T, int BL>
 __global__ __launch_bounds__(64) void k_dilu_upper_wf(const int *__restrict__ urow, const int *__restrict__ uslot,
-                                                      const double *__restrict__ A, const double *__restrict__ DD64,
-                                                      T *__restrict__ F, float *__restrict__ scale) {
+                                                      const int *__restrict__ pos, const double *__restrict__ A,
+                                                      const double *__restrict__ DD64, T *__restrict__ F,
+                                                      float *__restrict__ scale) {
     using W = WF<BL>;
     __shared__ double X[BL * W::LD];
     __shared__ double Y[BL * W::LD];
@@ -642,7 +648,7 @@ __global__ __launch_bounds__(64) void k_dilu_upper_wf(const int *__restrict__ ur
     dx4 t[W::TM][W::TM];
     tiles_zero<BL>(t);
     mma_lds<BL, false>(X, Y, t);
-    tiles_store<T, BL>(F + (size_t)s * bb, scale ? scale + s : nullptr, t);
+    tiles_store<T, BL>(F + (size_t)pos[s] * bb, scale ? scale + pos[s] : nullptr, t);
 }
 
 template <class T, int BL>
@@ -664,170 +670,16 @@ __global__ __launch_bounds__(64) void k_bjacobi_factor_wf(const int *__restrict_
     c.template store<T>(F + (size_t)i * bb, scale ? scale + i : nullptr);
 }
 
-// ---- apply kernels -----------------------------------------------------------------
+// factors to FP64 in A's slot order (for download): out block s = packed block pos[s]
 template <class T>
-struct VecW {
-    // elements per lane-load: 16 bytes when bl allows it
-    static int pick(int bl) {
-        const int maxw = 16 / (int)sizeof(T);
-        for (int w = maxw; w > 1; w >>= 1)
-            if (bl % w == 0) return w;
-        return 1;
-    }
-};
-
-template <class T, int VW>
-__device__ inline void load_vec(const T *p, typename Store<T>::acc (&a)[VW]) {
-    using A = typename Store<T>::acc;
-    if constexpr (sizeof(T) * VW == 16) {
-        const uint4 raw = *reinterpret_cast<const uint4 *>(p);
-        const T *e = reinterpret_cast<const T *>(&raw);
-#pragma unroll
-        for (int v = 0; v < VW; v++) a[v] = (A)Store<T>::get(e, v);
-    } else if constexpr (sizeof(T) * VW == 8) {
-        const uint2 raw = *reinterpret_cast<const uint2 *>(p);
-        const T *e = reinterpret_cast<const T *>(&raw);
-#pragma unroll
-        for (int v = 0; v < VW; v++) a[v] = (A)Store<T>::get(e, v);
-    } else {
-#pragma unroll
-        for (int v = 0; v < VW; v++) a[v] = (A)Store<T>::get(p, v);
-    }
-}
-
-// acc(rows rr..rr+VW) += sum over the listed blocks of blk * vec(col block)
-template <class T, int VW, class SL, class CL, class XV>
-__device__ inline void stream_blocks(int bl, int nblk, SL slot_of, CL col_of, const T *F,
-                                     const float *scale, XV xval, typename Store<T>::acc (&acc)[VW]) {
-    using A = typename Store<T>::acc;
-    const int tpc = bl / VW;
-    const int G = WG / tpc;
-    const int g = threadIdx.x / tpc;
-    const int rr = (threadIdx.x - g * tpc) * VW;
-    if (g >= G) return;
-    const int ncols = nblk * bl;
-#pragma unroll 4
-    for (int cg = g; cg < ncols; cg += G) {
-        const int blk = cg / bl;
-        const int col = cg - blk * bl;
-        const int s = slot_of(blk);
-        A xv = xval(col_of(blk), col);
-        if constexpr (Store<T>::prec == CPG_BF16) xv *= scale[s];
-        A a[VW];
-        load_vec<T, VW>(F + (size_t)s * bl * bl + (size_t)col * bl + rr, a);
-#pragma unroll
-        for (int v = 0; v < VW; v++) acc[v] = fma(a[v], xv, acc[v]);
-    }
-}
-
-template <class A, int VW>
-__device__ inline void reduce_groups(int bl, A (&acc)[VW], A *red) {
-    const int tpc = bl / VW;
-    const int G = WG / tpc;
-    const int g = threadIdx.x / tpc;
-    const int rr = (threadIdx.x - g * tpc) * VW;
-    if (g < G)
-#pragma unroll
-        for (int v = 0; v < VW; v++) red[g * bl + rr + v] = acc[v];
-    __syncthreads();
-    if (threadIdx.x < bl) {
-        A s = 0;
-        for (int gg = 0; gg < G; gg++) s += red[gg * bl + threadIdx.x];
-        red[G * bl + threadIdx.x] = s;  // result row
-    }
-    __syncthreads();
-}
-
-// forward level: y_i = x_i - sum_lower L y ; z_i = D_i^-1 y_i   (x: FP64 or FP32 vector)
-template <class T, int VW, class XT>
-__global__ __launch_bounds__(WG) void k_dilu_forward(int bl, RowLists L, const T *__restrict__ F,
-                                                     const float *__restrict__ scale, const XT *__restrict__ x,
-                                                     typename Store<T>::acc *__restrict__ y,
-                                                     typename Store<T>::acc *__restrict__ z) {
-    using A = typename Store<T>::acc;
-    extern __shared__ unsigned char raw[];
-    A *red = reinterpret_cast<A *>(raw);
-    const int tpc = bl / VW;
-    const int G = WG / tpc;
-    A *yi = red + (G + 1) * bl;
-    const int i = L.rows[blockIdx.x];
-    A acc[VW];
-#pragma unroll
-    for (int v = 0; v < VW; v++) acc[v] = 0;
-    const int l0 = L.lptr[i];
-    stream_blocks<T, VW>(bl, L.lptr[i + 1] - l0, [&](int b) { return L.lslot[l0 + b]; },
-                         [&](int b) { return L.lcol[l0 + b]; }, F, scale,
-                         [&](int cb, int col) { return y[(size_t)cb * bl + col]; }, acc);
-    reduce_groups<A, VW>(bl, acc, red);
-    if (threadIdx.x < bl) {
-        const A v = (A)x[(size_t)i * bl + threadIdx.x] - red[G * bl + threadIdx.x];
-        yi[threadIdx.x] = v;
-        y[(size_t)i * bl + threadIdx.x] = v;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int v = 0; v < VW; v++) acc[v] = 0;
-    const int d = L.diag[i];
-    stream_blocks<T, VW>(bl, 1, [&](int) { return d; }, [&](int) { return i; }, F, scale,
-                         [&](int, int col) { return yi[col]; }, acc);
-    reduce_groups<A, VW>(bl, acc, red);
-    if (threadIdx.x < bl) z[(size_t)i * bl + threadIdx.x] = red[G * bl + threadIdx.x];
-}
-
-// backward level: x_i = z_i - sum_upper DU x
-template <class T, int VW, class XT>
-__global__ __launch_bounds__(WG) void k_dilu_backward(int bl, RowLists L, const T *__restrict__ F,
-                                                      const float *__restrict__ scale,
-                                                      const typename Store<T>::acc *__restrict__ z,
-                                                      XT *__restrict__ x) {
-    using A = typename Store<T>::acc;
-    extern __shared__ unsigned char raw[];
-    A *red = reinterpret_cast<A *>(raw);
-    const int tpc = bl / VW;
-    const int G = WG / tpc;
-    const int i = L.rows[blockIdx.x];
-    A acc[VW];
-#pragma unroll
-    for (int v = 0; v < VW; v++) acc[v] = 0;
-    const int u0 = L.uptr[i];
-    stream_blocks<T, VW>(bl, L.uptr[i + 1] - u0, [&](int b) { return L.uslot[u0 + b]; },
-                         [&](int b) { return L.ucol[u0 + b]; }, F, scale,
-                         [&](int cb, int col) { return (A)x[(size_t)cb * bl + col]; }, acc);
-    reduce_groups<A, VW>(bl, acc, red);
-    if (threadIdx.x < bl)
-        x[(size_t)i * bl + threadIdx.x] = (XT)(z[(size_t)i * bl + threadIdx.x] - red[G * bl + threadIdx.x]);
-}
-
-// block Jacobi apply: x_i = D_i^-1 x_i
-template <class T, int VW>
-__global__ __launch_bounds__(WG) void k_bjacobi_apply(int bl, const T *__restrict__ F, const float *__restrict__ scale,
-                                                      double *__restrict__ x) {
-    using A = typename Store<T>::acc;
-    extern __shared__ unsigned char raw[];
-    A *red = reinterpret_cast<A *>(raw);
-    const int tpc = bl / VW;
-    const int G = WG / tpc;
-    A *xi = red + (G + 1) * bl;
-    const int i = blockIdx.x;
-    if (threadIdx.x < bl) xi[threadIdx.x] = (A)x[(size_t)i * bl + threadIdx.x];
-    __syncthreads();
-    A acc[VW];
-#pragma unroll
-    for (int v = 0; v < VW; v++) acc[v] = 0;
-    stream_blocks<T, VW>(bl, 1, [&](int) { return i; }, [&](int) { return i; }, F, scale,
-                         [&](int, int col) { return xi[col]; }, acc);
-    reduce_groups<A, VW>(bl, acc, red);
-    if (threadIdx.x < bl) x[(size_t)i * bl + threadIdx.x] = (double)red[G * bl + threadIdx.x];
-}
-
-// factors to FP64 (for download)
-template <class T>
-__global__ void k_to_fp64(const T *__restrict__ F, const float *__restrict__ scale, size_t bb, size_t n,
-                          double *__restrict__ out) {
+__global__ void k_to_fp64(const T *__restrict__ F, const float *__restrict__ scale, const int *__restrict__ pos,
+                          size_t bb, size_t n, double *__restrict__ out) {
     size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
-    double v = (double)Store<T>::get(F, e);
-    if (scale) v *= (double)scale[e / bb];
+    const size_t s = e / bb;
+    const size_t p = pos ? (size_t)pos[s] : s;
+    double v = (double)Store<T>::get(F, p * bb + (e - s * bb));
+    if (scale) v *= (double)scale[p];
     out[e] = v;
 }
 
@@ -847,54 +699,6 @@ __global__ void k_d2f(const double *__restrict__ a, float *__restrict__ b, size_
 __global__ void k_f2d(const float *__restrict__ a, double *__restrict__ b, size_t n) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < n) b[e] = (double)a[e];
-}
-
-// FP32 BSR matvec y = A x (same row-stream scheme as the FP64 kernel, 16-byte loads)
-template <int VW>
-__global__ __launch_bounds__(256) void k_bsr_matvec_f32(int bl, const int *__restrict__ rowptr,
-                                                        const int *__restrict__ colind, const float *__restrict__ A,
-                                                        const float *__restrict__ x, float *__restrict__ y) {
-    extern __shared__ double fred[];
-    const int tpc = bl / VW;
-    const int G = 256 / tpc;
-    const int t = threadIdx.x;
-    const int g = t / tpc;
-    const int rr = (t - g * tpc) * VW;
-    const int i = blockIdx.x;
-    const int s0 = rowptr[i], s1 = rowptr[i + 1];
-    const int ncols = (s1 - s0) * bl;
-    double acc[VW];   // FP32 data, FP64 accumulation
-#pragma unroll
-    for (int v = 0; v < VW; v++) acc[v] = 0.0;
-    if (g < G) {
-        const float *base = A + (size_t)s0 * bl * bl + rr;
-#pragma unroll 4
-        for (int cg = g; cg < ncols; cg += G) {
-            const int blk = cg / bl;
-            const int col = cg - blk * bl;
-            const float xv = x[(size_t)colind[s0 + blk] * bl + col];
-            float a[VW];
-            if constexpr (VW == 4) {
-                const float4 q = *reinterpret_cast<const float4 *>(base + (size_t)cg * bl);
-                a[0] = q.x; a[1] = q.y; a[2] = q.z; a[3] = q.w;
-            } else if constexpr (VW == 2) {
-                const float2 q = *reinterpret_cast<const float2 *>(base + (size_t)cg * bl);
-                a[0] = q.x; a[1] = q.y;
-            } else {
-                a[0] = base[(size_t)cg * bl];
-            }
-#pragma unroll
-            for (int v = 0; v < VW; v++) acc[v] = fma((double)a[v], (double)xv, acc[v]);
-        }
-#pragma unroll
-        for (int v = 0; v < VW; v++) fred[g * bl + rr + v] = acc[v];
-    }
-    __syncthreads();
-    if (t < bl) {
-        double s = 0.0;
-        for (int gg = 0; gg < G; gg++) s += fred[gg * bl + t];
-        y[(size_t)i * bl + t] = (float)s;
-    }
 }
 
 // partial[b*nv + j] = sum over chunk b of Q_j . w   (FP64 accumulation)
@@ -1096,11 +900,16 @@ struct cpg_prec_s {
     int nup = 0;
     DevBuf<int> frows, brows;
     std::vector<int> flev, blev;  // level pointers into frows / brows
+    // packed factor streams: pos[slot] = packed block; forward stream (L blocks) and backward
+    // stream (D^-1, DU blocks) pointers per position in frows / brows; scol = source of each
+    // packed block (column block j >= 0, or -1-i for D_i^-1, which multiplies y_i)
+    DevBuf<int> pos, fptr, bptr, scol;
+    int nbmax_f = 1, nbmax_b = 1;
     // storage
     DevBuf<unsigned char> F;      // factor blocks in `precision`
     DevBuf<float> scale;          // per-block scale (BF16)
     DevBuf<double> DD64;          // FP64 D^-1 (DILU)
-    DevBuf<unsigned char> ybuf, zbuf;
+    DevBuf<double> ybuf;          // forward-sweep result y
     DevBuf<int> status;
     size_t elem = 8;
     // inner FP32 GMRES (kind CPG_PC_INNER_GMRES)
@@ -1115,6 +924,7 @@ struct cpg_prec_s {
 
     RowLists lists(bool fwd) const {
         RowLists L;
+        L.pos = pos.p;
         L.rows = fwd ? frows.p : brows.p;
         L.diag = diag.p;
         L.lptr = lptr.p; L.lslot = lslot.p; L.lcol = lcol.p; L.ltrans = ltrans.p;
@@ -1188,8 +998,8 @@ void factor_wf_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
         CPG_LAUNCH_CHECK();
     }
     if (P.nup > 0) {
-        hipLaunchKernelGGL((k_dilu_upper_wf<T, BL>), dim3(P.nup), dim3(64), 0, s, P.urow.p, P.uslot.p, A.blocks.p,
-                           P.DD64.p, Fp<T>(P), sc);
+        hipLaunchKernelGGL((k_dilu_upper_wf<T, BL>), dim3(P.nup), dim3(64), 0, s, P.urow.p, P.uslot.p, P.pos.p,
+                           A.blocks.p, P.DD64.p, Fp<T>(P), sc);
         CPG_LAUNCH_CHECK();
     }
 }
@@ -1229,52 +1039,55 @@ void factor_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
     CPG_CHECK(false, CPG_ERR_UNSUPPORTED, "block size %d too large", P.bl);
 }
 
-template <class T, int VW, class XT>
+template <class T, class XT>
 void apply_t(cpg_prec_s &P, XT *x, hipStream_t s) {
-    using A = typename Store<T>::acc;
     const int bl = P.bl;
-    const int tpc = bl / VW;
-    const int G = WG / tpc;
-    const size_t lds = (size_t)(G + 2) * bl * sizeof(A);
     const float *sc = P.precision == CPG_BF16 ? P.scale.p : nullptr;
+    const T *F = Fp<T>(P);
     if (P.kind == CPG_PC_BJACOBI) {
-        if constexpr (std::is_same<XT, double>::value) {
-            hipLaunchKernelGGL((k_bjacobi_apply<T, VW>), dim3(P.n), dim3(WG), lds, s, bl, Fp<T>(P), sc, x);
-            CPG_LAUNCH_CHECK();
-        }
+        StreamArgs<XT> a;
+        a.S = RowStream{nullptr, nullptr, nullptr};
+        a.items = P.n;
+        a.src1 = x;
+        a.out = x;
+        launch_rowstream<T, XT>(bl, a, F, sc, s);
         return;
     }
-    A *y = reinterpret_cast<A *>(P.ybuf.p);
-    A *z = reinterpret_cast<A *>(P.zbuf.p);
-    RowLists Lf = P.lists(true);
+    double *y = P.ybuf.p;
     for (size_t lv = 0; lv + 1 < P.flev.size(); lv++) {
-        const int nr = P.flev[lv + 1] - P.flev[lv];
+        const int r0 = P.flev[lv], nr = P.flev[lv + 1] - r0;
         if (!nr) continue;
-        RowLists Ll = Lf;
-        Ll.rows = P.frows.p + P.flev[lv];
-        hipLaunchKernelGGL((k_dilu_forward<T, VW, XT>), dim3(nr), dim3(WG), lds, s, bl, Ll, Fp<T>(P), sc, x, y, z);
-        CPG_LAUNCH_CHECK();
+        // y_i = x_i - sum_k L_ik y_k
+        StreamArgs<double> a;
+        a.S = RowStream{P.frows.p + r0, P.fptr.p + r0, P.scol.p};
+        a.items = nr;
+        a.nbmax = P.nbmax_f;
+        a.src1 = y;
+        a.c1 = -1.0;
+        if constexpr (std::is_same<XT, double>::value) a.base64 = x;
+        else a.base32 = x;
+        a.out = y;
+        launch_rowstream<T, double>(bl, a, F, sc, s);
     }
-    RowLists Lb = P.lists(false);
     for (size_t lv = 0; lv + 1 < P.blev.size(); lv++) {
-        const int nr = P.blev[lv + 1] - P.blev[lv];
+        const int r0 = P.blev[lv], nr = P.blev[lv + 1] - r0;
         if (!nr) continue;
-        RowLists Ll = Lb;
-        Ll.rows = P.brows.p + P.blev[lv];
-        hipLaunchKernelGGL((k_dilu_backward<T, VW, XT>), dim3(nr), dim3(WG), lds, s, bl, Ll, Fp<T>(P), sc, z, x);
-        CPG_LAUNCH_CHECK();
+        // x_i = D_i^-1 y_i - sum_j DU_ij x_j
+        StreamArgs<XT> a;
+        a.S = RowStream{P.brows.p + r0, P.bptr.p + r0, P.scol.p};
+        a.items = nr;
+        a.nbmax = P.nbmax_b;
+        a.src0 = y;
+        a.src1 = x;
+        a.c1 = -1.0;
+        a.out = x;
+        launch_rowstream<T, XT>(bl, a, F, sc, s);
     }
 }
 
 template <class T, class XT>
 void apply_dispatch(cpg_prec_s &P, XT *x, hipStream_t s) {
-    switch (VecW<T>::pick(P.bl)) {
-    case 8: if constexpr (16 / sizeof(T) >= 8) { apply_t<T, 8, XT>(P, x, s); return; } break;
-    case 4: if constexpr (16 / sizeof(T) >= 4) { apply_t<T, 4, XT>(P, x, s); return; } break;
-    case 2: apply_t<T, 2, XT>(P, x, s); return;
-    default: apply_t<T, 1, XT>(P, x, s); return;
-    }
-    apply_t<T, 1, XT>(P, x, s);
+    apply_t<T, XT>(P, x, s);
 }
 
 // DILU / block-Jacobi sweep on a vector of type XT in the stored precision
@@ -1288,17 +1101,13 @@ void apply_factors(cpg_prec_s &P, XT *x, hipStream_t s) {
 }
 
 void matvec_f32(cpg_prec_s &P, const float *x, float *y, hipStream_t s) {
-    const int bl = P.bl;
-    const int vw = bl % 4 == 0 ? 4 : bl % 2 == 0 ? 2 : 1;
-    const int G = 256 / (bl / vw);
-    const size_t lds = (size_t)G * bl * sizeof(double);
-    if (vw == 4)
-        hipLaunchKernelGGL(k_bsr_matvec_f32<4>, dim3(P.n), dim3(256), lds, s, bl, P.A->rowptr.p, P.A->colind.p, P.A32.p, x, y);
-    else if (vw == 2)
-        hipLaunchKernelGGL(k_bsr_matvec_f32<2>, dim3(P.n), dim3(256), lds, s, bl, P.A->rowptr.p, P.A->colind.p, P.A32.p, x, y);
-    else
-        hipLaunchKernelGGL(k_bsr_matvec_f32<1>, dim3(P.n), dim3(256), lds, s, bl, P.A->rowptr.p, P.A->colind.p, P.A32.p, x, y);
-    CPG_LAUNCH_CHECK();
+    StreamArgs<float> a;
+    a.S = RowStream{nullptr, P.A->rowptr.p, P.A->colind.p};
+    a.items = P.n;
+    a.nbmax = P.A->nbmax;
+    a.src1 = x;
+    a.out = y;
+    launch_rowstream<float, float>(P.bl, a, P.A32.p, nullptr, s);
 }
 
 // x <- inner GMRES(k) approximation of A^-1 x: FP32 Arnoldi on the FP32 copy of A,
@@ -1434,14 +1243,46 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
             levels(n, uptr, ucol, false, P->blev, brows);
             P->frows.upload(frows);
             P->brows.upload(brows);
+            if (P->dilu()) {
+                // packed streams in sweep order
+                std::vector<int> pos(A->nnzb, -1), fptr(n + 1, 0), bptr(n + 1, 0), scol;
+                int cur = 0;
+                for (int p = 0; p < n; p++) {
+                    const int i = frows[p];
+                    fptr[p] = cur;
+                    for (int q = lptr[i]; q < lptr[i + 1]; q++) {
+                        pos[lslot[q]] = cur++;
+                        scol.push_back(lcol[q]);
+                    }
+                    P->nbmax_f = std::max(P->nbmax_f, lptr[i + 1] - lptr[i]);
+                }
+                fptr[n] = cur;
+                for (int p = 0; p < n; p++) {
+                    const int i = brows[p];
+                    bptr[p] = cur;
+                    pos[diag[i]] = cur++;
+                    scol.push_back(-1 - i);
+                    for (int q = uptr[i]; q < uptr[i + 1]; q++) {
+                        pos[uslot[q]] = cur++;
+                        scol.push_back(ucol[q]);
+                    }
+                    P->nbmax_b = std::max(P->nbmax_b, 1 + uptr[i + 1] - uptr[i]);
+                }
+                bptr[n] = cur;
+                // every slot is the diagonal, a lower or an upper block of its row
+                for (int q = 0; q < A->nnzb; q++)
+                    CPG_CHECK(pos[q] >= 0, CPG_ERR_ARG, "slot %d is a second diagonal block of its row", q);
+                P->pos.upload(pos);
+                P->fptr.upload(fptr);
+                P->bptr.upload(bptr);
+                P->scol.upload(scol);
+            }
             P->status.alloc(1);
             if (P->dilu()) {
                 P->F.alloc((size_t)A->nnzb * bb * P->elem);
                 if (precision == CPG_BF16) P->scale.alloc(A->nnzb);
                 P->DD64.alloc((size_t)n * bb);
-                const size_t ae = 8;   // FP64 intermediates for every factor precision
-                P->ybuf.alloc((size_t)n * bl * ae);
-                P->zbuf.alloc((size_t)n * bl * ae);
+                P->ybuf.alloc((size_t)n * bl);   // FP64 intermediate for every factor precision
             }
             if (kind == CPG_PC_INNER_GMRES) {
                 CPG_CHECK(precision != CPG_FP64, CPG_ERR_ARG, "the inner FP32 GMRES needs FP32 or BF16 factors");
@@ -1543,11 +1384,12 @@ int cpg_prec_download(cpg_prec P, double *host_blocks) {
         const size_t cnt = nblk * bb;
         DevBuf<double> tmp(cnt);
         const float *sc = P->precision == CPG_BF16 ? P->scale.p : nullptr;
+        const int *pos = P->dilu() ? P->pos.p : nullptr;
         const int g = ceil_div(cnt, 256);
         switch (P->precision) {
-        case CPG_FP64: hipLaunchKernelGGL(k_to_fp64<double>, dim3(g), dim3(256), 0, nullptr, Fp<double>(*P), sc, bb, cnt, tmp.p); break;
-        case CPG_FP32: hipLaunchKernelGGL(k_to_fp64<float>, dim3(g), dim3(256), 0, nullptr, Fp<float>(*P), sc, bb, cnt, tmp.p); break;
-        default: hipLaunchKernelGGL(k_to_fp64<bf16>, dim3(g), dim3(256), 0, nullptr, Fp<bf16>(*P), sc, bb, cnt, tmp.p); break;
+        case CPG_FP64: hipLaunchKernelGGL(k_to_fp64<double>, dim3(g), dim3(256), 0, nullptr, Fp<double>(*P), sc, pos, bb, cnt, tmp.p); break;
+        case CPG_FP32: hipLaunchKernelGGL(k_to_fp64<float>, dim3(g), dim3(256), 0, nullptr, Fp<float>(*P), sc, pos, bb, cnt, tmp.p); break;
+        default: hipLaunchKernelGGL(k_to_fp64<bf16>, dim3(g), dim3(256), 0, nullptr, Fp<bf16>(*P), sc, pos, bb, cnt, tmp.p); break;
         }
         CPG_LAUNCH_CHECK();
         CPG_HIP(hipMemcpy(host_blocks, tmp.p, cnt * sizeof(double), hipMemcpyDeviceToHost));

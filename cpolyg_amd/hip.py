@@ -147,6 +147,25 @@ class Bsr:
     def matvec(self, x, y, stream=None):
         _chk(lib().cpg_bsr_matvec(self.h, _vp(ptr(x)), _vp(ptr(y)), _stream(stream)))
 
+    def autotune(self, precision=_spec.PREC_FP64, reps=5, stream=None):
+        """Time this matrix's row stream stored in `precision` for 1, 2 and 4 wavefronts per
+        block row and keep the fastest shape for (precision, bl); returns (best, [ms1, ms2, ms4])."""
+        best = C.c_int()
+        ms = (C.c_float * 3)()
+        _chk(lib().cpg_stream_autotune(self.h, precision, reps, C.byref(best), ms, _stream(stream)))
+        return best.value, list(ms)
+
+
+def set_stream_wpr(precision, bl, wpr):
+    """Wavefronts per block row of the row-stream kernels for (storage precision, block size)."""
+    _chk(lib().cpg_stream_set_wpr(precision, bl, wpr))
+
+
+def get_stream_wpr(precision, bl):
+    v = C.c_int()
+    _chk(lib().cpg_stream_get_wpr(precision, bl, C.byref(v)))
+    return v.value
+
 
 class Prec:
     """Device preconditioner (``BlockJacobi`` / ``BlockILU0``)."""

@@ -65,14 +65,25 @@ int cpg_bsr_scale(cpg_bsr A, double s, void *stream);
 /* y = A x; x and y are device vectors of n_rows*bl doubles and must not alias */
 int cpg_bsr_matvec(cpg_bsr A, const double *x, double *y, void *stream);
 
+/* ---- launch shapes of the block-row streams ----------------------------------
+ * The matvec, DILU sweeps and block-Jacobi apply run one workgroup of 1, 2 or 4
+ * wavefronts per block row.  No reference counterpart (the reference calls BLAS per
+ * block).  cpg_stream_autotune times A's row stream in `precision` for each shape (reps
+ * launches each; ms[3] receives the average ms for 1, 2, 4 waves; synchronous) and keeps
+ * the fastest for (precision, bl) in a process-wide table that later launches read. */
+int cpg_stream_set_wpr(int precision, int bl, int wpr);
+int cpg_stream_get_wpr(int precision, int bl, int *wpr);
+int cpg_stream_autotune(cpg_bsr A, int precision, int reps, int *best_wpr, float *ms, void *stream);
+
 /* ---- preconditioners --------------------------------------------------------
  * Replace Preconditioner::solve (src/Preconditioners.h:10-15), BlockJacobi
  * (src/Preconditioners.cpp:8-47) and BlockILU0 = block DILU
  * (src/Preconditioners.cpp:49-158).  The DILU sweeps are level-scheduled from
  * the matrix pattern (a colour-ordered matrix gives one level per colour).
  * Factors are stored in `precision` (CPG_FP64, CPG_FP32, CPG_BF16 with one
- * power-of-two scale per block); FP32/BF16 factors are applied with FP32
- * accumulation.  x is FP64 in and out. */
+ * power-of-two scale per block), packed in the order the sweeps read them; FP32
+ * factors are applied with FP64 accumulation, BF16 factors with FP32 per-lane
+ * partial sums and an FP64 total.  x is FP64 in and out. */
 enum { CPG_PC_NONE = 0, CPG_PC_BJACOBI = 1, CPG_PC_DILU = 2, CPG_PC_INNER_GMRES = 3 };
 enum { CPG_FP64 = 0, CPG_FP32 = 1, CPG_BF16 = 2 };
 /* CPG_PC_INNER_GMRES: a variable preconditioner for CPG_GMRES_FLEXIBLE -- a fixed number
