@@ -176,162 +176,213 @@ __global__ __launch_bounds__(64) void k_residual(DevEq eq, Geo g, const double *
 }
 
 // ---------------------------------------------------------------------------------------
-// Jacobian: one 256-thread workgroup per cell.  Point data (basis, scaled gradients,
-// flux Jacobians) is staged in LDS in chunks; each thread owns NE entries of the
-// diagonal block and accumulates them over all points, and computes its entries of
-// each off-diagonal block edge by edge.
-constexpr int JWG = 256;
-constexpr int JCH = 64;
+// Jacobian: one workgroup of NC wavefronts per cell; wavefront c2 owns the trial-component
+// column group c2 (columns c2*bs .. c2*bs+bs-1) of every block of the cell's row.
+//
+// Every block is a sum over quadrature points of rank-one products, i.e. a GEMM with the
+// points as the contraction index:
+//   volume   J_ii[(c1,k), l] += sum_q (Gx_k Ax[c1][c2] + Gy_k Ay[c1][c2])(q) * phi_l(q)
+//   edge     J_ii[(c1,k), l] -= sum_q (W psi_k Jd[c1][c2])(q) * phi_l(q)
+//            J_i,nb[(c1,k), l] = -scale sum_q (W psi_k Jo[c1][c2])(q) * phin_l(q)
+// (Gx = W d_x psi, Gy = W d_y psi).  The products run on v_mfma_f64_16x16x4f64: the row
+// index (c1,k) spans ceil(bl/16) 16-row tiles, the trial index l one 16-column tile, and each
+// MFMA consumes four points.  Point data (basis values, scaled gradients, flux Jacobians) is
+// evaluated once per point by one thread and staged in LDS in chunks of JQC points; the MFMA
+// operands are formed from it on the fly (A = test factor x flux-Jacobian entry, B = trial
+// basis value).  Results go from the accumulator tiles straight to the BSR slots.
+typedef double jdx4 __attribute__((ext_vector_type(4)));
+constexpr int JQC = 64;   // points per LDS chunk
 
 template <int DEG, int NC>
 struct JacLayout {
     static constexpr int BS = Deg<DEG>::BS, BL = NC * BS, NQL = Deg<DEG>::NQL;
     static constexpr int NC2 = NC * NC;
-    static constexpr int PV = 3 * BS + 2 * NC2;   // phi, gx, gy, Ax, Ay
-    static constexpr int NE = (BL * BL + JWG - 1) / JWG;
-    static constexpr int LDS_VOL = JCH * PV;
-    static constexpr int LDS_EDGE = NQL * PV;       // psiW, phi, phin, Jd, Jo
-    static constexpr int LDS = (LDS_VOL > LDS_EDGE ? LDS_VOL : LDS_EDGE) + BL;
+    static constexpr int MT = (BL + 15) / 16;          // 16-row tiles of a block
+    static constexpr int NQLP = 4 * ((NQL + 3) / 4);   // edge points padded to whole MFMA k-steps
+    static constexpr int EPC = JQC / NQLP;             // edges per chunk
+    static_assert(BS <= 16, "trial basis must fit one 16-column tile");
+    // volume point record: Gx[BS] Gy[BS] phi[16] Ax[NC2] Ay[NC2]
+    static constexpr int V_GX = 0, V_GY = BS, V_PHI = 2 * BS, V_AX = 2 * BS + 16, V_AY = V_AX + NC2;
+    static constexpr int VREC = V_AY + NC2;
+    // edge point record: W psi[BS] phi[16] phin[16] Jd[NC2] Jo[NC2]
+    static constexpr int E_WPHI = 0, E_PHI = BS, E_PHIN = BS + 16, E_JD = BS + 32, E_JO = E_JD + NC2;
+    static constexpr int EREC = E_JO + NC2;
+    static constexpr int REC = (VREC > EREC ? VREC : EREC) | 1;   // odd stride: fewer LDS bank conflicts
 };
 
 template <int DEG, int NC>
-__global__ __launch_bounds__(JWG) void k_jacobian(DevEq eq, Geo g, const double *__restrict__ u, double t,
-                                                  double scale, int add_mass, const double *__restrict__ mass,
-                                                  const int *__restrict__ rowptr, double *__restrict__ J) {
+__device__ inline void jac_tiles_store(double *__restrict__ blk, const jdx4 (&acc)[JacLayout<DEG, NC>::MT],
+                                       double scale, const double *__restrict__ massi, int c2) {
     using L = JacLayout<DEG, NC>;
-    constexpr int BS = L::BS, BL = L::BL, NQL = L::NQL, NC2 = L::NC2, PV = L::PV, NE = L::NE;
-    __shared__ double lds[L::LDS];
-    double *sU = lds + (L::LDS - BL);
+    const int lane = threadIdx.x & 63;
+    const int n = lane & 15, r0 = lane >> 4;
+    if (n >= L::BS) return;
+    double *col = blk + (size_t)(c2 * L::BS + n) * L::BL;
+#pragma unroll
+    for (int t = 0; t < L::MT; t++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int row = 16 * t + r0 + 4 * e;
+            if (row < L::BL) {
+                double v = scale * acc[t][e];
+                if (massi && row / L::BS == c2) v += massi[n * L::BS + (row - c2 * L::BS)];
+                col[row] = v;
+            }
+        }
+}
+
+template <int DEG, int NC>
+__global__ __launch_bounds__(64 * NC, 2) void k_jacobian(DevEq eq, Geo g, const double *__restrict__ u, double t,
+                                                      double scale, int add_mass, const double *__restrict__ mass,
+                                                      const int *__restrict__ rowptr, double *__restrict__ J) {
+    using L = JacLayout<DEG, NC>;
+    constexpr int BS = L::BS, BL = L::BL, NQL = L::NQL, NC2 = L::NC2, MT = L::MT, REC = L::REC;
+    constexpr int NT = 64 * NC;
+    constexpr size_t BB = (size_t)BL * BL;
+    __shared__ double rec[JQC * REC];
+    __shared__ double sU[BL];
     const int i = blockIdx.x;
-    const int tid = threadIdx.x;
-    for (int r = tid; r < BL; r += JWG) sU[r] = u[(size_t)i * BL + r];
+    const int tid = threadIdx.x, lane = tid & 63, c2 = tid >> 6;
+    for (int r = tid; r < BL; r += NT) sU[r] = u[(size_t)i * BL + r];
     const double4 bb = g.bb[i];
     const double sx = 2.0 / bb.z, sy = 2.0 / bb.w;
-
-    int ek[NE], el[NE], ecc[NE];
-    bool ev[NE];
-    double acc[NE];
+    // this lane's rows of the A operand: row m = 16 t + lane % 16 = (c1, k)
+    int tk[MT], tc[MT];
+    bool tv[MT];
 #pragma unroll
-    for (int j = 0; j < NE; j++) {
-        const int e = tid + JWG * j;
-        ev[j] = e < BL * BL;
-        const int r = ev[j] ? e % BL : 0, col = ev[j] ? e / BL : 0;
-        const int c1 = r / BS, c2 = col / BS;
-        ek[j] = r - c1 * BS;
-        el[j] = col - c2 * BS;
-        ecc[j] = c1 * NC + c2;
-        acc[j] = 0.0;
+    for (int tt = 0; tt < MT; tt++) {
+        const int m = 16 * tt + (lane & 15);
+        tv[tt] = m < BL;
+        tc[tt] = tv[tt] ? m / BS : 0;
+        tk[tt] = tv[tt] ? m - tc[tt] * BS : 0;
     }
+    const int qsub = lane >> 4, ncol = lane & 15;
+    jdx4 accD[MT];
+#pragma unroll
+    for (int tt = 0; tt < MT; tt++) accD[tt] = jdx4{0.0, 0.0, 0.0, 0.0};
     __syncthreads();
 
     // ---- volume terms
     const int v0 = g.vptr[i], v1 = g.vptr[i + 1];
-    for (int q0 = v0; q0 < v1; q0 += JCH) {
-        const int nq = min(JCH, v1 - q0);
-        if (tid < nq) {
-            const double4 P = g.vpts[q0 + tid];
-            double *d = lds + tid * PV;
-            double phi[BS], dxi[BS], deta[BS], Um[NC], Ax[NC2], Ay[NC2];
-            const double xi = 2.0 * (P.x - bb.x) / bb.z - 1.0, eta = 2.0 * (P.y - bb.y) / bb.w - 1.0;
-            basis<DEG>(xi, eta, phi, dxi, deta);
-            eval_state<BS, NC>(sU, phi, Um);
-            fluxjac<NC>(eq, Um, P.x, P.y, Ax, Ay);
+    for (int q0 = v0; q0 < v1; q0 += JQC) {
+        const int nq = min(JQC, v1 - q0);
+        for (int p = tid; p < JQC; p += NT) {
+            double *d = rec + p * REC;
+            if (p < nq) {
+                // results go to the LDS record as they are produced (short live ranges)
+                const double4 P = g.vpts[q0 + p];
+                double phi[BS], Um[NC];
+                {
+                    double dxi[BS], deta[BS];
+                    const double xi = 2.0 * (P.x - bb.x) / bb.z - 1.0, eta = 2.0 * (P.y - bb.y) / bb.w - 1.0;
+                    basis<DEG>(xi, eta, phi, dxi, deta);
 #pragma unroll
-            for (int k = 0; k < BS; k++) {
-                d[k] = phi[k];
-                d[BS + k] = P.z * dxi[k] * sx;
-                d[2 * BS + k] = P.z * deta[k] * sy;
-            }
+                    for (int k = 0; k < BS; k++) {
+                        d[L::V_GX + k] = P.z * dxi[k] * sx;
+                        d[L::V_GY + k] = P.z * deta[k] * sy;
+                    }
+                }
 #pragma unroll
-            for (int c = 0; c < NC2; c++) {
-                d[3 * BS + c] = Ax[c];
-                d[3 * BS + NC2 + c] = Ay[c];
+                for (int k = 0; k < 16; k++) d[L::V_PHI + k] = k < BS ? phi[k] : 0.0;
+                eval_state<BS, NC>(sU, phi, Um);
+                fluxjac<NC>(eq, Um, P.x, P.y, d + L::V_AX, d + L::V_AY);
+            } else {
+#pragma unroll
+                for (int k = 0; k < L::VREC; k++) d[k] = 0.0;
             }
         }
         __syncthreads();
+        const int nks = (nq + 3) >> 2;
+        for (int ks = 0; ks < nks; ks++) {
+            const double *d = rec + (4 * ks + qsub) * REC;
+            const double b = d[L::V_PHI + ncol];
 #pragma unroll
-        for (int j = 0; j < NE; j++) {
-            if (!ev[j]) continue;
-            double s = 0.0;
-            const int k = ek[j], l = el[j], cc = ecc[j];
-            for (int q = 0; q < nq; q++) {
-                const double *d = lds + q * PV;
-                s = fma(d[l], d[3 * BS + cc] * d[BS + k] + d[3 * BS + NC2 + cc] * d[2 * BS + k], s);
+            for (int tt = 0; tt < MT; tt++) {
+                const int cc = tc[tt] * NC + c2;
+                const double a = tv[tt] ? fma(d[L::V_GX + tk[tt]], d[L::V_AX + cc], d[L::V_GY + tk[tt]] * d[L::V_AY + cc])
+                                        : 0.0;
+                accD[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, accD[tt], 0, 0, 0);
             }
-            acc[j] += s;
         }
         __syncthreads();
     }
 
     // ---- edge terms
     const double *lr = line_rule<NQL>();
-    const size_t bb2 = (size_t)BL * BL;
-    for (int e = g.eptr[i]; e < g.eptr[i + 1]; e++) {
-        const Edge E = g.edges[e];
-        if (tid < NQL) {
-            const int q = tid;
-            double *d = lds + q * PV;  // psiW[BS] phi[BS] phin[BS] Jd[NC2] Jo[NC2]
-            const double s = 0.5 * (lr[2 * q] + 1.0);
-            const double x = E.x1 + s * (E.x2 - E.x1), y = E.y1 + s * (E.y2 - E.y1);
-            const double W = 0.5 * E.len * lr[2 * q + 1];
-            const double xi = 2.0 * (x - bb.x) / bb.z - 1.0, eta = 2.0 * (y - bb.y) / bb.w - 1.0;
-            double phi[BS], phin[BS], Um[NC], Up[NC], Jd[NC2], Jo[NC2];
-            basis<DEG>(xi, eta, phi, nullptr, nullptr);
-            eval_state<BS, NC>(sU, phi, Um);
-            outer_state<DEG, NC>(eq, g, E, u, x, y, t, Up, phin);
-            numfluxjac<NC>(eq, Um, Up, x, y, E.nx, E.ny, +1, Jd);
-            if (E.kind != 2) {
-                if constexpr (NC == 4) {
-                    if (eq.jacobian) euler_offdiag_exact(eq.gamma, Um, Up, E.nx, E.ny, Jo);
-                    else numfluxjac<NC>(eq, Um, Up, x, y, E.nx, E.ny, -1, Jo);
+    const int e0 = g.eptr[i], ne = g.eptr[i + 1] - e0;
+    for (int eb = 0; eb < ne; eb += L::EPC) {
+        const int nec = min(L::EPC, ne - eb);
+        for (int p = tid; p < nec * L::NQLP; p += NT) {
+            const int el = p / L::NQLP, q = p - el * L::NQLP;
+            double *d = rec + p * REC;
+            if (q < NQL) {
+                const Edge E = g.edges[e0 + eb + el];
+                const double s = 0.5 * (lr[2 * q] + 1.0);
+                const double x = E.x1 + s * (E.x2 - E.x1), y = E.y1 + s * (E.y2 - E.y1);
+                const double W = 0.5 * E.len * lr[2 * q + 1];
+                const double xi = 2.0 * (x - bb.x) / bb.z - 1.0, eta = 2.0 * (y - bb.y) / bb.w - 1.0;
+                double Um[NC], Up[NC];
+                {
+                    double phi[BS];
+                    basis<DEG>(xi, eta, phi, nullptr, nullptr);
+#pragma unroll
+                    for (int k = 0; k < BS; k++) d[L::E_WPHI + k] = W * phi[k];
+#pragma unroll
+                    for (int k = 0; k < 16; k++) d[L::E_PHI + k] = k < BS ? phi[k] : 0.0;
+                    eval_state<BS, NC>(sU, phi, Um);
+                }
+                {
+                    double phin[BS];
+                    outer_state<DEG, NC>(eq, g, E, u, x, y, t, Up, phin);
+#pragma unroll
+                    for (int k = 0; k < 16; k++) d[L::E_PHIN + k] = k < BS ? phin[k] : 0.0;
+                }
+                numfluxjac<NC>(eq, Um, Up, x, y, E.nx, E.ny, +1, d + L::E_JD);
+                double *Jo = d + L::E_JO;
+                if (E.kind != 2) {
+                    if constexpr (NC == 4) {
+                        if (eq.jacobian) euler_offdiag_exact(eq.gamma, Um, Up, E.nx, E.ny, Jo);
+                        else numfluxjac<NC>(eq, Um, Up, x, y, E.nx, E.ny, -1, Jo);
+                    } else {
+                        numfluxjac<NC>(eq, Um, Up, x, y, E.nx, E.ny, -1, Jo);
+                    }
                 } else {
-                    numfluxjac<NC>(eq, Um, Up, x, y, E.nx, E.ny, -1, Jo);
+#pragma unroll
+                    for (int c = 0; c < NC2; c++) Jo[c] = 0.0;
                 }
             } else {
 #pragma unroll
-                for (int c = 0; c < NC2; c++) Jo[c] = 0.0;
-            }
-#pragma unroll
-            for (int k = 0; k < BS; k++) {
-                d[k] = W * phi[k];
-                d[BS + k] = phi[k];
-                d[2 * BS + k] = phin[k];
-            }
-#pragma unroll
-            for (int c = 0; c < NC2; c++) {
-                d[3 * BS + c] = Jd[c];
-                d[3 * BS + NC2 + c] = Jo[c];
+                for (int k = 0; k < L::EREC; k++) d[k] = 0.0;
             }
         }
         __syncthreads();
-        double *O = E.slot >= 0 ? J + (size_t)E.slot * bb2 : nullptr;
+        for (int el = 0; el < nec; el++) {
+            const int slot = g.edges[e0 + eb + el].slot;
+            jdx4 accO[MT];
 #pragma unroll
-        for (int j = 0; j < NE; j++) {
-            if (!ev[j]) continue;
-            const int k = ek[j], l = el[j], cc = ecc[j];
-            double sd = 0.0, so = 0.0;
+            for (int tt = 0; tt < MT; tt++) accO[tt] = jdx4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int q = 0; q < NQL; q++) {
-                const double *d = lds + q * PV;
-                const double pw = d[k];
-                sd = fma(pw * d[BS + l], d[3 * BS + cc], sd);
-                so = fma(pw * d[2 * BS + l], d[3 * BS + NC2 + cc], so);
+            for (int ks = 0; ks < L::NQLP / 4; ks++) {
+                const double *d = rec + (el * L::NQLP + 4 * ks + qsub) * REC;
+                const double bd = d[L::E_PHI + ncol];
+                const double bo = d[L::E_PHIN + ncol];
+#pragma unroll
+                for (int tt = 0; tt < MT; tt++) {
+                    const int cc = tc[tt] * NC + c2;
+                    const double w = tv[tt] ? d[L::E_WPHI + tk[tt]] : 0.0;
+                    accD[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-w * d[L::E_JD + cc], bd, accD[tt], 0, 0, 0);
+                    if (slot >= 0)
+                        accO[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(w * d[L::E_JO + cc], bo, accO[tt], 0, 0, 0);
+                }
             }
-            acc[j] -= sd;
-            if (O) O[tid + JWG * j] = -scale * so;
+            if (slot >= 0) jac_tiles_store<DEG, NC>(J + (size_t)slot * BB, accO, -scale, nullptr, c2);
         }
         __syncthreads();
     }
 
-    // ---- diagonal block
-    double *D = J + (size_t)rowptr[i] * bb2;
-#pragma unroll
-    for (int j = 0; j < NE; j++) {
-        if (!ev[j]) continue;
-        double v = scale * acc[j];
-        if (add_mass && (ecc[j] / NC == ecc[j] % NC)) v += mass[(size_t)i * BS * BS + el[j] * BS + ek[j]];
-        D[tid + JWG * j] = v;
-    }
+    // ---- diagonal block: scale * J_ii (+ M_i on the diagonal component blocks)
+    jac_tiles_store<DEG, NC>(J + (size_t)rowptr[i] * BB, accD, scale,
+                             add_mass ? mass + (size_t)i * BS * BS : nullptr, c2);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -476,7 +527,7 @@ static void launch_residual(const cpg_equation_s &E, const double *u, double t, 
 template <int DEG, int NC>
 static void launch_jacobian(const cpg_equation_s &E, const double *u, double t, double scale, int add_mass,
                             double *J, hipStream_t s) {
-    hipLaunchKernelGGL((k_jacobian<DEG, NC>), dim3(E.n), dim3(JWG), 0, s, E.eq, E.geo(), u, t, scale, add_mass,
+    hipLaunchKernelGGL((k_jacobian<DEG, NC>), dim3(E.n), dim3(64 * NC), 0, s, E.eq, E.geo(), u, t, scale, add_mass,
                        E.mass.p, E.d_rowptr.p, J);
     CPG_LAUNCH_CHECK();
 }
