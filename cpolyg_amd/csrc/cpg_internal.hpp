@@ -113,4 +113,6 @@ void vec_fill(double *x, double v, size_t n, hipStream_t s);
 // waves per block row of the row-stream kernels (rowstream.hpp), per element size and block size
 int stream_wpr(int elem_bytes, int bl);
 void set_stream_wpr(int elem_bytes, int bl, int wpr);
+// out = M^-1 in for a factored preconditioner (in and out distinct)
+void prec_apply_to(cpg_prec P, const double *in, double *out, hipStream_t s);
 }  // namespace cpg

@@ -223,6 +223,16 @@ struct KOps {
         if (st) throw Error{st};
         K.mark(1, false, s);
     }
+    // out = M^-1 in (a copy when there is no preconditioner)
+    void pc_to(const double *in, double *out) {
+        if (!P) {
+            CPG_HIP(hipMemcpyAsync(out, in, K.N * sizeof(double), hipMemcpyDeviceToDevice, s));
+            return;
+        }
+        K.mark(1, true, s);
+        prec_apply_to(P, in, out, s);
+        K.mark(1, false, s);
+    }
     // returns ||x|| (synchronous)
     double norm(const double *x) {
         hipLaunchKernelGGL(k_mdot, dim3(K.nblk), dim3(KWG), 0, s, x, K.N, 1, x, K.partial.p);
@@ -339,8 +349,7 @@ int cpg_gmres(cpg_krylov K, cpg_bsr A, cpg_prec P, const double *b, double *x, d
                     op.pc(vi1);
                 } else {
                     double *zi = K->Z.p + N * i;
-                    CPG_HIP(hipMemcpyAsync(zi, vi, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-                    op.pc(zi);
+                    op.pc_to(vi, zi);
                     op.matvec(zi, vi1);
                 }
                 // classical Gram-Schmidt: h = V^T w ; w -= V h ; hnew = ||w||

@@ -917,7 +917,11 @@ struct cpg_prec_s {
     size_t N = 0;
     int nchunk = 0;
     const cpg_bsr_s *A = nullptr;             // matrix of the last factor (pattern for the FP32 matvec)
-    DevBuf<float> A32, Q, iw, it;              // FP32 matrix copy, basis (k+1) x N, work vectors
+    // FP32 copy of A (a BF16 copy is too coarse for this solve: with per-block scales its
+    // ~2^-9 relative error, amplified by the condition number of M - dt a J at large CFL,
+    // leaves the inner solve a poor approximation of A^-1 and multiplies the outer
+    // iterations), basis (k+1) x N, work vectors
+    DevBuf<float> A32, Q, iw, it;
     DevBuf<double> ipart, H, g, cs, sn, y, nrm;
 
     bool dilu() const { return kind == CPG_PC_DILU || kind == CPG_PC_INNER_GMRES; }
@@ -1039,8 +1043,9 @@ void factor_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
     CPG_CHECK(false, CPG_ERR_UNSUPPORTED, "block size %d too large", P.bl);
 }
 
+// out = M^-1 in (in may equal out)
 template <class T, class XT>
-void apply_t(cpg_prec_s &P, XT *x, hipStream_t s) {
+void apply_t(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s) {
     const int bl = P.bl;
     const float *sc = P.precision == CPG_BF16 ? P.scale.p : nullptr;
     const T *F = Fp<T>(P);
@@ -1048,8 +1053,8 @@ void apply_t(cpg_prec_s &P, XT *x, hipStream_t s) {
         StreamArgs<XT> a;
         a.S = RowStream{nullptr, nullptr, nullptr};
         a.items = P.n;
-        a.src1 = x;
-        a.out = x;
+        a.src1 = in;
+        a.out = out;
         launch_rowstream<T, XT>(bl, a, F, sc, s);
         return;
     }
@@ -1064,8 +1069,8 @@ void apply_t(cpg_prec_s &P, XT *x, hipStream_t s) {
         a.nbmax = P.nbmax_f;
         a.src1 = y;
         a.c1 = -1.0;
-        if constexpr (std::is_same<XT, double>::value) a.base64 = x;
-        else a.base32 = x;
+        if constexpr (std::is_same<XT, double>::value) a.base64 = in;
+        else a.base32 = in;
         a.out = y;
         launch_rowstream<T, double>(bl, a, F, sc, s);
     }
@@ -1078,25 +1083,20 @@ void apply_t(cpg_prec_s &P, XT *x, hipStream_t s) {
         a.items = nr;
         a.nbmax = P.nbmax_b;
         a.src0 = y;
-        a.src1 = x;
+        a.src1 = out;
         a.c1 = -1.0;
-        a.out = x;
+        a.out = out;
         launch_rowstream<T, XT>(bl, a, F, sc, s);
     }
 }
 
-template <class T, class XT>
-void apply_dispatch(cpg_prec_s &P, XT *x, hipStream_t s) {
-    apply_t<T, XT>(P, x, s);
-}
-
-// DILU / block-Jacobi sweep on a vector of type XT in the stored precision
+// DILU / block-Jacobi sweep on vectors of type XT with the factors in their stored precision
 template <class XT>
-void apply_factors(cpg_prec_s &P, XT *x, hipStream_t s) {
+void apply_factors(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s) {
     switch (P.precision) {
-    case CPG_FP64: apply_dispatch<double, XT>(P, x, s); break;
-    case CPG_FP32: apply_dispatch<float, XT>(P, x, s); break;
-    default: apply_dispatch<bf16, XT>(P, x, s); break;
+    case CPG_FP64: apply_t<double, XT>(P, in, out, s); break;
+    case CPG_FP32: apply_t<float, XT>(P, in, out, s); break;
+    default: apply_t<bf16, XT>(P, in, out, s); break;
     }
 }
 
@@ -1110,10 +1110,9 @@ void matvec_f32(cpg_prec_s &P, const float *x, float *y, hipStream_t s) {
     launch_rowstream<float, float>(P.bl, a, P.A32.p, nullptr, s);
 }
 
-// x <- inner GMRES(k) approximation of A^-1 x: FP32 Arnoldi on the FP32 copy of A,
-// right-preconditioned by the DILU factors, zero initial guess.  A fixed sequence of
-// launches on `s` with every scalar on the device.
-void apply_inner(cpg_prec_s &P, double *x, hipStream_t s) {
+// out <- inner GMRES(k) approximation of A^-1 in: FP32 Arnoldi on the FP32 copy of A, right-preconditioned by the DILU factors, zero initial guess.  A fixed sequence
+// of launches on `s` with every scalar on the device.
+void apply_inner(cpg_prec_s &P, const double *x, double *xout, hipStream_t s) {
     const size_t N = P.N;
     const int k = P.inner_k, ld = k + 1, nb = P.nchunk;
     float *Q = P.Q.p, *w = P.iw.p, *t = P.it.p;
@@ -1128,8 +1127,7 @@ void apply_inner(cpg_prec_s &P, double *x, hipStream_t s) {
     CPG_LAUNCH_CHECK();
     for (int i = 0; i < k; i++) {
         float *qi = Q + N * i, *qn = Q + N * (i + 1);
-        CPG_HIP(hipMemcpyAsync(t, qi, N * sizeof(float), hipMemcpyDeviceToDevice, s));
-        apply_factors<float>(P, t, s);                     // t = M^-1 q_i
+        apply_factors<float>(P, qi, t, s);                 // t = M^-1 q_i
         matvec_f32(P, t, w, s);                             // w = A t
         hipLaunchKernelGGL(k_f32_mdot, dim3(nb), dim3(IWG), 0, s, Q, N, i + 1, w, P.ipart.p);
         CPG_LAUNCH_CHECK();
@@ -1148,8 +1146,8 @@ void apply_inner(cpg_prec_s &P, double *x, hipStream_t s) {
     CPG_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_f32_combine, dim3(nb), dim3(IWG), 0, s, Q, N, k, P.y.p, t);
     CPG_LAUNCH_CHECK();
-    apply_factors<float>(P, t, s);                          // x = M^-1 Q y
-    hipLaunchKernelGGL(k_f2d, dim3(g256), dim3(256), 0, s, t, x, N);
+    apply_factors<float>(P, t, t, s);                       // x = M^-1 Q y
+    hipLaunchKernelGGL(k_f2d, dim3(g256), dim3(256), 0, s, t, xout, N);
     CPG_LAUNCH_CHECK();
 }
 
@@ -1352,12 +1350,32 @@ int cpg_prec_apply(cpg_prec P, double *x, void *stream) {
         hipStream_t s = S(stream);
         if (P->kind == CPG_PC_INNER_GMRES) {
             CPG_CHECK(P->A, CPG_ERR_ARG, "inner GMRES preconditioner applied before its first factor");
-            apply_inner(*P, x, s);
+            apply_inner(*P, x, x, s);
             return;
         }
-        apply_factors<double>(*P, x, s);
+        apply_factors<double>(*P, x, x, s);
     });
 }
+
+}  // extern "C"
+
+namespace cpg {
+// out = M^-1 in, in and out distinct (used by flexible GMRES for z_i = M^-1 v_i)
+void prec_apply_to(cpg_prec P, const double *in, double *out, hipStream_t s) {
+    if (P->kind == CPG_PC_NONE || P->n == 0) {
+        CPG_HIP(hipMemcpyAsync(out, in, (size_t)P->n * P->bl * sizeof(double), hipMemcpyDeviceToDevice, s));
+        return;
+    }
+    if (P->kind == CPG_PC_INNER_GMRES) {
+        CPG_CHECK(P->A, CPG_ERR_ARG, "inner GMRES preconditioner applied before its first factor");
+        apply_inner(*P, in, out, s);
+        return;
+    }
+    apply_factors<double>(*P, in, out, s);
+}
+}  // namespace cpg
+
+extern "C" {
 
 int cpg_prec_set_inner(cpg_prec P, int iterations) {
     return guard([&] {
