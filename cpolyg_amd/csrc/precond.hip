@@ -319,14 +319,15 @@ __global__ __launch_bounds__(WG) void k_bjacobi_factor(int bl, const int *__rest
 
 // ---- wavefront-per-row factor: FP64 MFMA block products, in-register Gauss-Jordan -------
 //
-// One 64-lane wavefront factors one block row (L blocks and D^-1); the DU blocks follow in
-// one launch over all upper blocks.  Block products C = P Q (bl x bl, FP64) run
-// on v_mfma_f64_16x16x4f64 over ceil(bl/16)^2 output tiles, operands read from two LDS
-// staging buffers; the products are written to HBM straight from the accumulator tiles.
-// The diagonal block is inverted by Gauss-Jordan with partial pivoting with one matrix
-// row per lane: the pivot search
-// is a cross-lane argmax, the pivot row is broadcast with readlane, and rows are never
-// swapped (the row/column order is restored when the inverse is written out).
+// Per level, one 64-lane wavefront per block row forms the L blocks and the Schur
+// complement, a second launch inverts the complements, and the DU blocks follow in one
+// launch over all upper blocks.  Block products C = P Q (bl x bl, FP64) run on
+// v_mfma_f64_16x16x4f64 over ceil(bl/16)^2 output tiles with operands loaded from L2
+// straight into the MFMA registers (or from one LDS buffer holding the previous product);
+// results pass through LDS so that they are written to HBM coalesced.  The diagonal block
+// is inverted by Gauss-Jordan with partial pivoting with one matrix row per lane: the pivot
+// search is a cross-lane argmax, the pivot row is broadcast with readlane, and rows are
+// never swapped (the row/column order is restored when the inverse is written out).
 typedef double dx4 __attribute__((ext_vector_type(4)));
 
 template <int BL>
@@ -349,6 +350,30 @@ __device__ inline double readlane_d(double v, int lane) {
 __device__ inline double wave_max(double v) {
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
     return v;
+}
+
+// one DPP step of a 64-bit max: v = max(v, v of the lane selected by CTRL) on the rows in RM
+template <int CTRL, int RM>
+__device__ inline unsigned long long dpp_max_u64(unsigned long long v) {
+    const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);
+    const int mlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, RM, 0xF, false);
+    const int mhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, RM, 0xF, false);
+    const unsigned long long m = ((unsigned long long)(unsigned)mhi << 32) | (unsigned)mlo;
+    return m > v ? m : v;
+}
+
+// maximum over the 64 lanes (uniform result): quad swaps and half-row/row mirrors reduce each
+// row of 16 lanes, row_bcast:15 and row_bcast:31 carry the row maxima into lane 63
+__device__ inline unsigned long long wave_max_u64(unsigned long long v) {
+    v = dpp_max_u64<0xB1, 0xF>(v);    // quad_perm [1,0,3,2]
+    v = dpp_max_u64<0x4E, 0xF>(v);    // quad_perm [2,3,0,1]
+    v = dpp_max_u64<0x141, 0xF>(v);   // row_half_mirror
+    v = dpp_max_u64<0x140, 0xF>(v);   // row_mirror
+    v = dpp_max_u64<0x142, 0xA>(v);   // row_bcast:15 into rows 1 and 3
+    v = dpp_max_u64<0x143, 0xC>(v);   // row_bcast:31 into rows 2 and 3
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
 }
 
 // copy chunks: a block in HBM (column-major, contiguous) <-> registers <-> LDS (column stride LD)
@@ -428,32 +453,6 @@ struct Chunks {
     }
 };
 
-// acc (+)= (NEG ? -P : P) * Q with P, Q in LDS (column stride LD)
-template <int BL, bool NEG>
-__device__ inline void mma_lds(const double *P, const double *Q, dx4 (&acc)[WF<BL>::TM][WF<BL>::TM]) {
-    using W = WF<BL>;
-    const int lr = threadIdx.x & 15, lk = threadIdx.x >> 4;
-#pragma unroll
-    for (int ks = 0; ks < W::KS; ks++) {
-        const int k = 4 * ks + lk;
-        double a[W::TM], b[W::TM];
-#pragma unroll
-        for (int t = 0; t < W::TM; t++) {
-            const int m = 16 * t + lr;
-            const bool in = k < BL && m < BL;
-            const double pa = P[in ? k * W::LD + m : 0];
-            const double qb = Q[in ? m * W::LD + k : 0];
-            a[t] = in ? (NEG ? -pa : pa) : 0.0;
-            b[t] = in ? qb : 0.0;
-        }
-#pragma unroll
-        for (int tm = 0; tm < W::TM; tm++)
-#pragma unroll
-            for (int tn = 0; tn < W::TM; tn++)
-                acc[tm][tn] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[tm], b[tn], acc[tm][tn], 0, 0, 0);
-    }
-}
-
 template <int BL>
 __device__ inline void tiles_zero(dx4 (&t)[WF<BL>::TM][WF<BL>::TM]) {
 #pragma unroll
@@ -494,58 +493,20 @@ __device__ inline void tiles_to_lds(double *X, const dx4 (&t)[WF<BL>::TM][WF<BL>
             }
 }
 
-template <class T, int BL>
-__device__ inline void tiles_store(T *__restrict__ g, float *scale_out, const dx4 (&t)[WF<BL>::TM][WF<BL>::TM]) {
-    using W = WF<BL>;
-    const int r0 = threadIdx.x >> 4, c0 = threadIdx.x & 15;
-    float inv = 1.0f;
-    if constexpr (Store<T>::prec == CPG_BF16) {
-        double mx = 0.0;
-#pragma unroll
-        for (int tm = 0; tm < W::TM; tm++)
-#pragma unroll
-            for (int tn = 0; tn < W::TM; tn++)
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int row = 16 * tm + r0 + 4 * i, col = 16 * tn + c0;
-                    if (row < BL && col < BL) mx = fmax(mx, fabs(t[tm][tn][i]));
-                }
-        const float s = pow2_scale(wave_max(mx));
-        inv = 1.0f / s;
-        if (threadIdx.x == 0) *scale_out = s;
-    }
-#pragma unroll
-    for (int tm = 0; tm < W::TM; tm++)
-#pragma unroll
-        for (int tn = 0; tn < W::TM; tn++) {
-            const int col = 16 * tn + c0;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int row = 16 * tm + r0 + 4 * i;
-                if (row < BL && col < BL) Store<T>::put(g, (size_t)col * BL + row, t[tm][tn][i], inv);
-            }
-        }
-}
-
-// In-place Gauss-Jordan inverse of the bl x bl block in X (column stride LD), one row per
-// lane, partial pivoting on the first maximal |pivot| among rows not yet used.  Rows are
-// never swapped and the pivot row is not rescaled during the elimination: a row keeps its
-// stored values and a scale 1/pivot from the step it was chosen, so every lane applies
-// the same update w -= (w_p / pivot) * w_pivotrow (zero for the pivot lane itself) and the
-// scales are applied when the inverse is written back to X.  Returns false (uniformly)
-// on a zero pivot.
+// Gauss-Jordan inverse of a bl x bl block held one row per lane (w[c] = entry (lane, c);
+// lanes past bl hold zeros), partial pivoting on the first maximal |pivot| among rows not
+// yet used.  Rows are never swapped and the pivot row is not rescaled during the
+// elimination: a row keeps its stored values and a scale 1/pivot from the step it was
+// chosen, so every lane applies the same update w -= (w_p / pivot) * w_pivotrow (zero for
+// the pivot lane itself).  On return the inverse is A^-1[my_step][rho_q] = my_scale * w[q],
+// with rho_q = my_rho of lane q.  Returns false (uniformly) on a zero pivot.
 template <int BL>
-__device__ inline bool wave_gauss_jordan(double *X) {
-    using W = WF<BL>;
-    const int lane = threadIdx.x;
-    const bool row_lane = lane < BL;
-    double w[BL];
-#pragma unroll
-    for (int c = 0; c < BL; c++) w[c] = row_lane ? X[c * W::LD + lane] : 0.0;
-    bool used = !row_lane;
-    int my_step = 0;         // step at which this lane's row was the pivot row
-    int my_rho = 0;          // lane p: pivot row of step p
-    double my_scale = 1.0;   // 1 / pivot of that step
+__device__ inline bool wave_gj_core(double (&w)[BL], int &my_step, int &my_rho, double &my_scale) {
+    const int lane = threadIdx.x & 63;
+    bool used = lane >= BL;
+    my_step = 0;     // step at which this lane's row was the pivot row
+    my_rho = 0;      // lane p: pivot row of step p
+    my_scale = 1.0;  // 1 / pivot of that step
     bool ok = true;
     // w is rotated by one column per step: at step p, w[0] holds column p and the finished
     // column goes to the end, so after bl steps the columns are back in their order and no
@@ -555,15 +516,14 @@ __device__ inline bool wave_gauss_jordan(double *X) {
         const double v = w[0];
         // first maximal |W[r][p]| over unused rows r; a NaN entry ranks below every number but
         // above the used rows (and the lanes past bl), so the pivot is always an unused row
-        double best = used ? -1.0 : (v == v ? fabs(v) : -0.5);
-        int bi = lane;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const double ob = __shfl_xor(best, off);
-            const int oi = __shfl_xor(bi, off);
-            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        // key: used rows < NaN < numbers; among numbers larger |v| first (|v| compared to 46
+        // mantissa bits), then the lower lane
+        unsigned long long key = (unsigned long long)(63 - lane);
+        if (!used) {
+            if (v != v) key += 64;
+            else key |= ((unsigned long long)__double_as_longlong(fabs(v)) & ~63ull) + 128;
         }
-        const int rho = __builtin_amdgcn_readfirstlane(bi);
+        const int rho = 63 - (int)(wave_max_u64(key) & 63);
         const double piv = readlane_d(v, rho);
         ok = ok && piv != 0.0;
         const double inv = piv != 0.0 ? 1.0 / piv : 1.0;
@@ -575,8 +535,22 @@ __device__ inline bool wave_gauss_jordan(double *X) {
         for (int c = 1; c < BL; c++) w[c - 1] = fma(-f, readlane_d(w[c], rho), w[c]);
         w[BL - 1] = me ? 1.0 : -f;
     }
+    return ok;
+}
+
+// In-place Gauss-Jordan inverse of the bl x bl block in X (column stride LD)
+template <int BL>
+__device__ inline bool wave_gauss_jordan(double *X) {
+    using W = WF<BL>;
+    const int lane = threadIdx.x;
+    const bool row_lane = lane < BL;
+    double w[BL];
+#pragma unroll
+    for (int c = 0; c < BL; c++) w[c] = row_lane ? X[c * W::LD + lane] : 0.0;
+    int my_step, my_rho;
+    double my_scale;
+    const bool ok = wave_gj_core<BL>(w, my_step, my_rho, my_scale);
     __syncthreads();
-    // A^-1[step(r)][rho_q] = scale(r) W[r][q]
 #pragma unroll
     for (int q = 0; q < BL; q++) {
         const int col = __builtin_amdgcn_readlane(my_rho, q);
@@ -586,69 +560,244 @@ __device__ inline bool wave_gauss_jordan(double *X) {
     return ok;
 }
 
-template <class T, int BL>
-__global__ __launch_bounds__(64) void k_dilu_factor_wf(RowLists L, const double *__restrict__ A,
-                                                       T *__restrict__ F, float *__restrict__ scale,
-                                                       double *__restrict__ DD64, int *__restrict__ status) {
+// ---- DILU factor, one level: two kernels per level, then one launch for all DU blocks.
+// Schur kernel (one workgroup of four wavefronts per row i of the level):
+//   L_ik = A_ik D_k^-1 (stored in the factor precision),  S_i = A_ii - sum_k L_ik A_ki,
+// with S_i written in FP64 to DD64[i].  Inverse kernel: D_i^-1 = S_i^-1 by the in-register
+// Gauss-Jordan, written over DD64[i] (FP64, used by the later levels and the DU blocks) and
+// to the factor array.  Splitting keeps each kernel's registers low enough for several
+// wavefronts per SIMD, so the latency-bound elimination of one row overlaps the MFMA work
+// and memory traffic of others.
+// workgroup of 4 wavefronts: the ceil(bl/16)^2 output tiles of a block product are dealt
+// round-robin to the waves; blocks are staged in LDS by all 256 threads with 16-byte loads
+constexpr int FWG = 256;
+
+template <int BL>
+struct FT {
+    static constexpr int NT = WF<BL>::TM * WF<BL>::TM;   // output tiles
+    static constexpr int NJ = (NT + 3) / 4;                // tiles per wave
+    static constexpr int V = BL % 2 == 0 ? 2 : 1;          // doubles per load
+};
+
+// X (column stride LD) <- the bl x bl block at g, all threads of the workgroup
+template <int BL>
+__device__ inline void wg_block_to_lds(double *X, const double *__restrict__ g) {
     using W = WF<BL>;
+    constexpr int V = FT<BL>::V;
+    for (int e = threadIdx.x * V; e < W::BB; e += FWG * V) {
+        const int c = e / BL, r = e - c * BL;
+        if constexpr (V == 2) {
+            const double2 a = *reinterpret_cast<const double2 *>(g + e);
+            X[c * W::LD + r] = a.x;
+            X[c * W::LD + r + 1] = a.y;
+        } else {
+            X[c * W::LD + r] = g[e];
+        }
+    }
+}
+
+// the block in X to global memory in precision T, all threads (BF16: one power-of-two scale
+// for the block from a workgroup max, written to *scale_out)
+template <class T, int BL>
+__device__ inline void wg_lds_block_store(const double *X, T *__restrict__ g, float *scale_out, double *red) {
+    using W = WF<BL>;
+    float inv = 1.0f;
+    if constexpr (Store<T>::prec == CPG_BF16) {
+        double mx = 0.0;
+        for (int e = threadIdx.x; e < W::BB; e += FWG) {
+            const int c = e / BL, r = e - c * BL;
+            mx = fmax(mx, fabs(X[c * W::LD + r]));
+        }
+        mx = wave_max(mx);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+        __syncthreads();
+        mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        const float sc = pow2_scale(mx);
+        inv = 1.0f / sc;
+        if (threadIdx.x == 0) *scale_out = sc;
+    }
+    for (int e = threadIdx.x; e < W::BB; e += FWG) {
+        const int c = e / BL, r = e - c * BL;
+        Store<T>::put(g, e, X[c * W::LD + r], inv);
+    }
+}
+
+// acc (+)= (NEG ? -P : P) Q for output tile (tm, tn); P, Q in LDS (column stride LD)
+template <int BL, bool NEG>
+__device__ inline void mma_tile_lds(const double *P, const double *Q, int tm, int tn, dx4 &acc) {
+    using W = WF<BL>;
+    const int lr = threadIdx.x & 15, lk = (threadIdx.x & 63) >> 4;
+    const int m = 16 * tm + lr, n = 16 * tn + lr;
+#pragma unroll
+    for (int ks = 0; ks < W::KS; ks++) {
+        const int k = 4 * ks + lk;
+        const bool ka = k < BL && m < BL, kb = k < BL && n < BL;
+        const double pa = P[ka ? k * W::LD + m : 0];
+        const double a = ka ? (NEG ? -pa : pa) : 0.0;
+        const double b = kb ? Q[n * W::LD + k] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+}
+
+// tile (tm, tn) of this wave's accumulator to X (column stride LD)
+template <int BL>
+__device__ inline void tile_to_lds(double *X, int tm, int tn, const dx4 &t) {
+    using W = WF<BL>;
+    const int l = threadIdx.x & 63;
+    const int r0 = l >> 4, c0 = l & 15;
+    const int col = 16 * tn + c0;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int row = 16 * tm + r0 + 4 * e;
+        if (row < BL && col < BL) X[col * W::LD + row] = t[e];
+    }
+}
+
+template <int BL>
+__device__ inline void tile_load(dx4 &t, int tm, int tn, const double *__restrict__ g) {
+    const int l = threadIdx.x & 63;
+    const int r0 = l >> 4, c0 = l & 15;
+    const int col = 16 * tn + c0;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int row = 16 * tm + r0 + 4 * e;
+        t[e] = (row < BL && col < BL) ? g[col * BL + row] : 0.0;
+    }
+}
+
+template <class T, int BL>
+__global__ __launch_bounds__(FWG) void k_dilu_schur(RowLists L, const double *__restrict__ A, T *__restrict__ F,
+                                                    float *__restrict__ scale, double *__restrict__ DD64) {
+    using W = WF<BL>;
+    using U = FT<BL>;
     __shared__ double X[BL * W::LD];
     __shared__ double Y[BL * W::LD];
+    __shared__ double red[4];
+    const int wv = threadIdx.x >> 6;
     const int i = L.rows[blockIdx.x];
     const int d = L.diag[i];
     constexpr size_t bb = (size_t)W::BB;
-    dx4 acc[W::TM][W::TM];
-    tiles_load<BL>(acc, A + (size_t)d * bb);
-    Chunks<BL> ca, cb;
+    dx4 acc[U::NJ];
+#pragma unroll
+    for (int j = 0; j < U::NJ; j++) {
+        const int tt = wv + 4 * j;
+        if (tt < U::NT) tile_load<BL>(acc[j], tt / W::TM, tt % W::TM, A + (size_t)d * bb);
+        else acc[j] = dx4{0.0, 0.0, 0.0, 0.0};
+    }
     for (int q = L.lptr[i]; q < L.lptr[i + 1]; q++) {
         const int s = L.lslot[q], k = L.lcol[q], st = L.ltrans[q];
-        ca.load(A + (size_t)s * bb);
-        cb.load(DD64 + (size_t)k * bb);
+        wg_block_to_lds<BL>(X, A + (size_t)s * bb);
+        wg_block_to_lds<BL>(Y, DD64 + (size_t)k * bb);
         __syncthreads();
-        ca.to_lds(X);
-        cb.to_lds(Y);
+        dx4 t[U::NJ];
+#pragma unroll
+        for (int j = 0; j < U::NJ; j++) {
+            t[j] = dx4{0.0, 0.0, 0.0, 0.0};
+            const int tt = wv + 4 * j;
+            if (tt < U::NT) mma_tile_lds<BL, false>(X, Y, tt / W::TM, tt % W::TM, t[j]);   // L_ik = A_ik D_k^-1
+        }
         __syncthreads();
-        ca.load(A + (size_t)st * bb);   // A_ki, in flight during the first product
-        dx4 t[W::TM][W::TM];
-        tiles_zero<BL>(t);
-        mma_lds<BL, false>(X, Y, t);     // L_ik = A_ik D_k^-1
-        tiles_store<T, BL>(F + (size_t)L.pos[s] * bb, scale ? scale + L.pos[s] : nullptr, t);
+#pragma unroll
+        for (int j = 0; j < U::NJ; j++) {
+            const int tt = wv + 4 * j;
+            if (tt < U::NT) tile_to_lds<BL>(X, tt / W::TM, tt % W::TM, t[j]);
+        }
+        wg_block_to_lds<BL>(Y, A + (size_t)st * bb);
         __syncthreads();
-        tiles_to_lds<BL>(X, t);
-        ca.to_lds(Y);
+        wg_lds_block_store<T, BL>(X, F + (size_t)L.pos[s] * bb, scale ? scale + L.pos[s] : nullptr, red);
+#pragma unroll
+        for (int j = 0; j < U::NJ; j++) {
+            const int tt = wv + 4 * j;
+            if (tt < U::NT) mma_tile_lds<BL, true>(X, Y, tt / W::TM, tt % W::TM, acc[j]);   // S_i -= L_ik A_ki
+        }
         __syncthreads();
-        mma_lds<BL, true>(X, Y, acc);    // D_i -= L_ik A_ki
+    }
+#pragma unroll
+    for (int j = 0; j < U::NJ; j++) {
+        const int tt = wv + 4 * j;
+        if (tt < U::NT) tile_to_lds<BL>(X, tt / W::TM, tt % W::TM, acc[j]);
     }
     __syncthreads();
-    tiles_to_lds<BL>(X, acc);
-    __syncthreads();
-    const bool ok = wave_gauss_jordan<BL>(X);
-    if (!ok && threadIdx.x == 0) atomicOr(status, 1);
-    ca.from_lds(X);
-    ca.template store<double>(DD64 + (size_t)i * bb, nullptr);
-    ca.template store<T>(F + (size_t)L.pos[d] * bb, scale ? scale + L.pos[d] : nullptr);
+    wg_lds_block_store<double, BL>(X, DD64 + (size_t)i * bb, nullptr, red);
 }
 
-// DU_ij = D_i^-1 A_ij for every upper block, one wavefront per block, after all levels
 template <class T, int BL>
-__global__ __launch_bounds__(64) void k_dilu_upper_wf(const int *__restrict__ urow, const int *__restrict__ uslot,
-                                                      const int *__restrict__ pos, const double *__restrict__ A,
-                                                      const double *__restrict__ DD64, T *__restrict__ F,
-                                                      float *__restrict__ scale) {
+__global__ __launch_bounds__(64) void k_dilu_inv_wf(const int *__restrict__ rows, const int *__restrict__ diag,
+                                                    const int *__restrict__ pos, double *__restrict__ DD64,
+                                                    T *__restrict__ F, float *__restrict__ scale,
+                                                    int *__restrict__ status) {
+    constexpr size_t bb = (size_t)BL * BL;
+    const int lane = threadIdx.x;
+    const int i = rows[blockIdx.x];
+    const int pd = pos[diag[i]];
+    double *S = DD64 + (size_t)i * bb;
+    double w[BL];
+#pragma unroll
+    for (int c = 0; c < BL; c++) w[c] = lane < BL ? S[c * BL + lane] : 0.0;
+    int my_step, my_rho;
+    double my_scale;
+    const bool ok = wave_gj_core<BL>(w, my_step, my_rho, my_scale);
+    if (!ok && lane == 0) atomicOr(status, 1);
+    T *Fd = F + (size_t)pd * bb;
+    float inv = 1.0f;
+    if constexpr (Store<T>::prec == CPG_BF16) {
+        double mx = 0.0;
+#pragma unroll
+        for (int q = 0; q < BL; q++) mx = fmax(mx, lane < BL ? fabs(w[q] * my_scale) : 0.0);
+        const float sc = pow2_scale(wave_max(mx));
+        inv = 1.0f / sc;
+        if (lane == 0) scale[pd] = sc;
+    }
+    // D^-1[my_step][rho_q] = my_scale * w[q]; every lane has read its row of S before any write
+#pragma unroll
+    for (int q = 0; q < BL; q++) {
+        const int col = __builtin_amdgcn_readlane(my_rho, q);
+        if (lane < BL) {
+            const double v = w[q] * my_scale;
+            const size_t e = (size_t)col * BL + my_step;
+            S[e] = v;
+            Store<T>::put(Fd, e, v, inv);
+        }
+    }
+}
+
+// DU_ij = D_i^-1 A_ij for the upper blocks of row urows[w], one workgroup per row, after all
+// levels (D_i^-1 staged once per row)
+template <class T, int BL>
+__global__ __launch_bounds__(FWG) void k_dilu_upper(const int *__restrict__ urows, const int *__restrict__ uptr,
+                                                    const int *__restrict__ uslot, const int *__restrict__ pos,
+                                                    const double *__restrict__ A, const double *__restrict__ DD64,
+                                                    T *__restrict__ F, float *__restrict__ scale) {
     using W = WF<BL>;
+    using U = FT<BL>;
     __shared__ double X[BL * W::LD];
     __shared__ double Y[BL * W::LD];
-    const int i = urow[blockIdx.x], s = uslot[blockIdx.x];
+    __shared__ double red[4];
+    const int wv = threadIdx.x >> 6;
+    const int i = urows[blockIdx.x];
     constexpr size_t bb = (size_t)W::BB;
-    Chunks<BL> ca, cb;
-    ca.load(DD64 + (size_t)i * bb);
-    cb.load(A + (size_t)s * bb);
-    ca.to_lds(X);
-    cb.to_lds(Y);
-    __syncthreads();
-    dx4 t[W::TM][W::TM];
-    tiles_zero<BL>(t);
-    mma_lds<BL, false>(X, Y, t);
-    tiles_store<T, BL>(F + (size_t)pos[s] * bb, scale ? scale + pos[s] : nullptr, t);
+    wg_block_to_lds<BL>(X, DD64 + (size_t)i * bb);
+    for (int q = uptr[i]; q < uptr[i + 1]; q++) {
+        const int s = uslot[q];
+        wg_block_to_lds<BL>(Y, A + (size_t)s * bb);
+        __syncthreads();
+        dx4 t[U::NJ];
+#pragma unroll
+        for (int j = 0; j < U::NJ; j++) {
+            t[j] = dx4{0.0, 0.0, 0.0, 0.0};
+            const int tt = wv + 4 * j;
+            if (tt < U::NT) mma_tile_lds<BL, false>(X, Y, tt / W::TM, tt % W::TM, t[j]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < U::NJ; j++) {
+            const int tt = wv + 4 * j;
+            if (tt < U::NT) tile_to_lds<BL>(Y, tt / W::TM, tt % W::TM, t[j]);
+        }
+        __syncthreads();
+        wg_lds_block_store<T, BL>(Y, F + (size_t)pos[s] * bb, scale ? scale + pos[s] : nullptr, red);
+        __syncthreads();
+    }
 }
 
 template <class T, int BL>
@@ -897,7 +1046,8 @@ struct cpg_prec_s {
     // pattern-derived lists
     DevBuf<int> diag, lptr, lslot, lcol, ltrans, uptr, uslot, ucol;
     DevBuf<int> urow;             // row of each upper slot (flat, in uslot order)
-    int nup = 0;
+    DevBuf<int> urows;            // rows with at least one upper block
+    int nup = 0, nurows = 0;
     DevBuf<int> frows, brows;
     std::vector<int> flev, blev;  // level pointers into frows / brows
     // packed factor streams: pos[slot] = packed block; forward stream (L blocks) and backward
@@ -997,13 +1147,16 @@ void factor_wf_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
         if (!nr) continue;
         RowLists Ll = L;
         Ll.rows = P.frows.p + P.flev[lv];
-        hipLaunchKernelGGL((k_dilu_factor_wf<T, BL>), dim3(nr), dim3(64), 0, s, Ll, A.blocks.p, Fp<T>(P), sc,
-                           P.DD64.p, P.status.p);
+        hipLaunchKernelGGL((k_dilu_schur<T, BL>), dim3(nr), dim3(FWG), 0, s, Ll, A.blocks.p, Fp<T>(P), sc,
+                           P.DD64.p);
+        CPG_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_dilu_inv_wf<T, BL>), dim3(nr), dim3(64), 0, s, Ll.rows, P.diag.p, P.pos.p, P.DD64.p,
+                           Fp<T>(P), sc, P.status.p);
         CPG_LAUNCH_CHECK();
     }
     if (P.nup > 0) {
-        hipLaunchKernelGGL((k_dilu_upper_wf<T, BL>), dim3(P.nup), dim3(64), 0, s, P.urow.p, P.uslot.p, P.pos.p,
-                           A.blocks.p, P.DD64.p, Fp<T>(P), sc);
+        hipLaunchKernelGGL((k_dilu_upper<T, BL>), dim3(P.nurows), dim3(FWG), 0, s, P.urows.p, P.uptr.p, P.uslot.p,
+                           P.pos.p, A.blocks.p, P.DD64.p, Fp<T>(P), sc);
         CPG_LAUNCH_CHECK();
     }
 }
@@ -1236,6 +1389,13 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
             P->ucol.upload(ucol);
             P->urow.upload(urow);
             P->nup = (int)urow.size();
+            {
+                std::vector<int> ur;
+                for (int i = 0; i < n; i++)
+                    if (uptr[i + 1] > uptr[i]) ur.push_back(i);
+                P->nurows = (int)ur.size();
+                if (!ur.empty()) P->urows.upload(ur);
+            }
             std::vector<int> frows, brows;
             levels(n, lptr, lcol, true, P->flev, frows);
             levels(n, uptr, ucol, false, P->blev, brows);
