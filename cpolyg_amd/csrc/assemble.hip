@@ -233,7 +233,8 @@ __device__ inline void jac_tiles_store(double *__restrict__ blk, const jdx4 (&ac
 
 template <int DEG, int NC>
 __global__ __launch_bounds__(64 * NC, 2) void k_jacobian(DevEq eq, Geo g, const double *__restrict__ u, double t,
-                                                      double scale, int add_mass, const double *__restrict__ mass,
+                                                      double scale, const double *__restrict__ cfac, int add_mass,
+                                                      const double *__restrict__ mass,
                                                       const int *__restrict__ rowptr, double *__restrict__ J) {
     using L = JacLayout<DEG, NC>;
     constexpr int BS = L::BS, BL = L::BL, NQL = L::NQL, NC2 = L::NC2, MT = L::MT, REC = L::REC;
@@ -243,6 +244,7 @@ __global__ __launch_bounds__(64 * NC, 2) void k_jacobian(DevEq eq, Geo g, const 
     __shared__ double sU[BL];
     const int i = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, c2 = tid >> 6;
+    if (cfac) scale *= cfac[i];   // per-cell factor (batched solves: each instance has its own dt)
     for (int r = tid; r < BL; r += NT) sU[r] = u[(size_t)i * BL + r];
     const double4 bb = g.bb[i];
     const double sx = 2.0 / bb.z, sy = 2.0 / bb.w;
@@ -434,14 +436,15 @@ __global__ void k_mass_matvec(int n, const double *__restrict__ M, const double 
 // r_i = M_i k_i - dt b_i (or r = x when k is null); normc[i*NC + c] = r_c^T M_i r_c
 template <int BS, int NC>
 __global__ void k_newton_resid(int n, const double *__restrict__ M, const double *__restrict__ kv,
-                               const double *__restrict__ b, double dt, double *__restrict__ r,
-                               double *__restrict__ normc) {
+                               const double *__restrict__ b, double dt, const double *__restrict__ cfac,
+                               double *__restrict__ r, double *__restrict__ normc) {
     constexpr int BL = NC * BS;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // (cell, component)
     if (e >= (size_t)n * NC) return;
     const size_t i = e / NC;
     const int c = (int)(e - i * NC);
     const double *Mi = M + i * BS * BS;
+    if (cfac) dt *= cfac[i];
     double rv[BS];
     if (kv) {
         const double *ki = kv + i * BL + c * BS;
@@ -525,10 +528,10 @@ static void launch_residual(const cpg_equation_s &E, const double *u, double t, 
 }
 
 template <int DEG, int NC>
-static void launch_jacobian(const cpg_equation_s &E, const double *u, double t, double scale, int add_mass,
-                            double *J, hipStream_t s) {
-    hipLaunchKernelGGL((k_jacobian<DEG, NC>), dim3(E.n), dim3(64 * NC), 0, s, E.eq, E.geo(), u, t, scale, add_mass,
-                       E.mass.p, E.d_rowptr.p, J);
+static void launch_jacobian(const cpg_equation_s &E, const double *u, double t, double scale, const double *cfac,
+                            int add_mass, double *J, hipStream_t s) {
+    hipLaunchKernelGGL((k_jacobian<DEG, NC>), dim3(E.n), dim3(64 * NC), 0, s, E.eq, E.geo(), u, t, scale, cfac,
+                       add_mass, E.mass.p, E.d_rowptr.p, J);
     CPG_LAUNCH_CHECK();
 }
 
@@ -559,7 +562,12 @@ void eq_residual(const cpg_equation_s &E, const double *u, double t, double *b, 
 
 void eq_jacobian(const cpg_equation_s &E, const double *u, double t, double scale, int add_mass, double *J,
                  hipStream_t s) {
-    CPG_DISPATCH(E, launch_jacobian, E, u, t, scale, add_mass, J, s);
+    CPG_DISPATCH(E, launch_jacobian, E, u, t, scale, nullptr, add_mass, J, s);
+}
+
+void eq_jacobian_cells(const cpg_equation_s &E, const double *u, double t, double scale, const double *cfac,
+                       int add_mass, double *J, hipStream_t s) {
+    CPG_DISPATCH(E, launch_jacobian, E, u, t, scale, cfac, add_mass, J, s);
 }
 
 template <int BS, int NC>
@@ -570,11 +578,11 @@ static void launch_mass_matvec(const cpg_equation_s &E, const double *x, double 
 }
 
 template <int BS, int NC>
-static void launch_newton_resid(const cpg_equation_s &E, const double *kv, const double *b, double dt, double *r,
-                                hipStream_t s) {
+static void launch_newton_resid(const cpg_equation_s &E, const double *kv, const double *b, double dt,
+                                const double *cfac, double *r, hipStream_t s) {
     const size_t cnt = (size_t)E.n * NC;
     hipLaunchKernelGGL((k_newton_resid<BS, NC>), dim3(ceil_div(cnt, 128)), dim3(128), 0, s, E.n, E.mass.p, kv, b,
-                       dt, r, E.normc.p);
+                       dt, cfac, r, E.normc.p);
     CPG_LAUNCH_CHECK();
 }
 
@@ -602,7 +610,7 @@ void eq_mass_matvec(const cpg_equation_s &E, const double *x, double *y, hipStre
 // with k null, the norm of b itself.  out_nc (optional) receives the per-component norms.
 double eq_newton_resid(const cpg_equation_s &E, const double *kv, const double *b, double dt, double *r,
                        double *out_nc, hipStream_t s) {
-    CPG_DISPATCH_BS(E, launch_newton_resid, E, kv, b, dt, r, s);
+    CPG_DISPATCH_BS(E, launch_newton_resid, E, kv, b, dt, nullptr, r, s);
     hipLaunchKernelGGL(k_colsum_partial, dim3(E.npartial, E.nc), dim3(256), 0, s, E.normc.p, E.n, E.nc, E.partial.p);
     CPG_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(64), 0, s, E.partial.p, E.npartial, E.nc, E.sums.p);
@@ -618,6 +626,15 @@ double eq_newton_resid(const cpg_equation_s &E, const double *kv, const double *
     }
     return mx;
 }
+
+// r = M k - dt cfac_i b per cell; the per-cell, per-component norms r_c^T M_i r_c stay on
+// the device (eq_normc) for segmented reductions
+void eq_newton_resid_cells(const cpg_equation_s &E, const double *kv, const double *b, double dt, const double *cfac,
+                           double *r, hipStream_t s) {
+    CPG_DISPATCH_BS(E, launch_newton_resid, E, kv, b, dt, cfac, r, s);
+}
+
+const double *eq_normc(const cpg_equation_s &E) { return E.normc.p; }
 
 const double *equation_mass(const cpg_equation_s &E) { return E.mass.p; }
 int equation_bs(const cpg_equation_s &E) { return E.bs; }

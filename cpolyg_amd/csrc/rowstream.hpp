@@ -13,11 +13,13 @@
 //      BF16 per-block power-of-two scale is folded in here, so the stream loop does none of
 //      this work;
 //   2. stream: a group of TPC = bl/VW lanes covers one column with 16-byte loads (VW
-//      elements of one column per lane), the G = WPR * 64/TPC groups walk the row's columns
-//      with stride G, so each wavefront load is a contiguous ~1 KB segment of the stream;
-//      every lane owns VW fixed rows and keeps their partial sums in registers;
-//   3. reduce: the G partial vectors are summed through LDS in a fixed order (deterministic,
-//      no atomics) and added to base_i.
+//      elements of one column per lane); the row's columns are dealt to G = 4 * 64/TPC
+//      logical groups (column cg to group cg mod G), each wavefront load is a contiguous
+//      ~1 KB segment of the stream, and every lane owns VW fixed rows of 4/WPR logical groups
+//      and keeps their partial sums in registers;
+//   3. reduce: the G partial vectors are summed through LDS in a fixed order and added to
+//      base_i.  No atomics, and the logical groups do not depend on WPR, so the result is
+//      bitwise the same for every launch shape the autotuner may pick.
 // Storage types: FP64, FP32 (FP64 accumulation) and BF16 (FP32 accumulation of the
 // per-lane partial sums, FP64 final sum).
 #pragma once
@@ -112,7 +114,7 @@ template <class T, int BL, int WPR>
 inline size_t rowstream_lds(int nbmax) {
     using A = typename StreamAcc<T>::type;
     constexpr int TPC = BL / stream_vw<T, BL>();
-    constexpr int G = (64 / TPC) * WPR;
+    constexpr int G = (64 / TPC) * 4;
     return (size_t)(G + (nbmax > 0 ? nbmax : 1)) * BL * sizeof(A);
 }
 
@@ -130,7 +132,8 @@ __global__ __launch_bounds__(64 * WPR) void k_rowstream(RowStream S, const T *__
     constexpr int TPC = BL / VW;
     static_assert(TPC <= 64, "a column must fit one wavefront");
     constexpr int GW = 64 / TPC;
-    constexpr int G = GW * WPR;
+    constexpr int G = GW * 4;       // logical groups (independent of WPR)
+    constexpr int LPW = 4 / WPR;    // logical groups per lane
     constexpr int NT = 64 * WPR;
     constexpr size_t BB = (size_t)BL * BL;
     extern __shared__ double lds_d[];
@@ -152,23 +155,45 @@ __global__ __launch_bounds__(64 * WPR) void k_rowstream(RowStream S, const T *__
     __syncthreads();
     const int lane = tid & 63, wv = tid >> 6;
     const int gl = lane / TPC;
-    const int g = gl + GW * wv;
     const int rr = (lane - gl * TPC) * VW;
     if (gl < GW) {
-        A acc[VW];
+        // logical group of slot l of this lane: gl + GW (wv + WPR l)
+        A acc[LPW][VW];
 #pragma unroll
-        for (int v = 0; v < VW; v++) acc[v] = 0;
+        for (int l = 0; l < LPW; l++)
+#pragma unroll
+            for (int v = 0; v < VW; v++) acc[l][v] = 0;
         const T *p = F + (size_t)b0 * BB + rr;
+        const int g0 = gl + GW * wv;
+        if constexpr (LPW == 1) {
 #pragma unroll 4
-        for (int cg = g; cg < ncol; cg += G) {
-            A a[VW];
-            stream_load<T, VW, A>(p + (size_t)cg * BL, a);
-            const A xv = xs[cg];
+            for (int cg = g0; cg < ncol; cg += G) {
+                A a[VW];
+                stream_load<T, VW, A>(p + (size_t)cg * BL, a);
+                const A xv = xs[cg];
 #pragma unroll
-            for (int v = 0; v < VW; v++) acc[v] = fma(a[v], xv, acc[v]);
+                for (int v = 0; v < VW; v++) acc[0][v] = fma(a[v], xv, acc[0][v]);
+            }
+        } else {
+#pragma unroll 2
+            for (int base = 0; base < ncol; base += G) {
+#pragma unroll
+                for (int l = 0; l < LPW; l++) {
+                    const int cg = base + g0 + GW * WPR * l;
+                    if (cg < ncol) {
+                        A a[VW];
+                        stream_load<T, VW, A>(p + (size_t)cg * BL, a);
+                        const A xv = xs[cg];
+#pragma unroll
+                        for (int v = 0; v < VW; v++) acc[l][v] = fma(a[v], xv, acc[l][v]);
+                    }
+                }
+            }
         }
 #pragma unroll
-        for (int v = 0; v < VW; v++) red[g * BL + rr + v] = acc[v];
+        for (int l = 0; l < LPW; l++)
+#pragma unroll
+            for (int v = 0; v < VW; v++) red[(g0 + GW * WPR * l) * BL + rr + v] = acc[l][v];
     }
     __syncthreads();
     for (int t = tid; t < BL; t += NT) {

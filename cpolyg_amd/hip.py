@@ -344,3 +344,47 @@ class Stepper:
         mm, ma = C.c_double(), C.c_double()
         _chk(lib().cpg_stepper_kernel_timing(self.h, C.byref(nm), C.byref(mm), C.byref(na), C.byref(ma)))
         return nm.value, mm.value, na.value, ma.value
+
+
+class Batch:
+    """Batched implicit DIRK steps over many independent instances (``cpg_batch``).
+
+    ``classes`` is a list of ``(Equation, cell_ptr)``: each equation is built on a union mesh
+    (:func:`cpolyg_amd.mesh.union_mesh`) whose parts are the instances of that class."""
+
+    def __init__(self, classes, nstages: int, solver: _spec.SolverSpec, use_graph: bool = True):
+        self.classes = classes
+        self.sp = solver_params(solver)
+        n = len(classes)
+        eqs = (C.c_void_p * n)(*[e.h.value if isinstance(e.h, C.c_void_p) else e.h for e, _ in classes])
+        self._cptr = [np.ascontiguousarray(cp, np.int32) for _, cp in classes]
+        ninst = (C.c_int * n)(*[len(cp) - 1 for cp in self._cptr])
+        cps = (_ip * n)(*[cp.ctypes.data_as(_ip) for cp in self._cptr])
+        h = _vp()
+        _chk(lib().cpg_batch_create(n, eqs, ninst, cps, int(nstages), C.byref(self.sp), int(use_graph), C.byref(h)))
+        self.h = h
+        self.ninst = sum(len(cp) - 1 for cp in self._cptr)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.cpg_batch_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _chk(lib().cpg_batch_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"instances": a.value, "chunks": b.value, "graph_captured": bool(c.value)}
+
+    def advance(self, us, dts, t, outs, stream=None):
+        """One step of every instance; ``us``/``outs`` are per-class device vectors, ``dts``
+        one dt per instance.  Returns (StepStats, per-instance Arnoldi steps)."""
+        n = len(self.classes)
+        uarr = (C.c_void_p * n)(*[ptr(u) for u in us])
+        oarr = (C.c_void_p * n)(*[ptr(o) for o in outs])
+        dt = np.ascontiguousarray(dts, np.float64)
+        assert len(dt) == self.ninst
+        its = np.zeros(self.ninst, np.int32)
+        st = StepStats()
+        _chk(lib().cpg_batch_advance(self.h, uarr, dt.ctypes.data_as(_dp), C.c_double(t), oarr, C.byref(st),
+                                     its.ctypes.data_as(_ip), _stream(stream)))
+        return st, its

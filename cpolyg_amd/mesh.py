@@ -47,6 +47,7 @@ __all__ = [
     "quad_rectangle",
     "jittered_lattice_seeds",
     "uniform_seeds",
+    "union_mesh",
 ]
 
 
@@ -250,6 +251,50 @@ class PolyMesh:
         if "colour" in z:
             m.colour, m.colour_ptr = z["colour"], z["colour_ptr"]
         return m
+
+
+def union_mesh(meshes: list[PolyMesh]) -> tuple[PolyMesh, np.ndarray]:
+    """Disjoint union of meshes: one mesh whose connected components are the given meshes,
+    cells numbered mesh by mesh.  Returns the union and ``cell_ptr`` (mesh ``q`` owns cells
+    ``cell_ptr[q] .. cell_ptr[q+1]-1``).  Boundary ids of mesh ``q`` are shifted past those of
+    the meshes before it and periodic rows are renumbered, so every cell keeps exactly the
+    neighbours and boundary data it had.  Each part keeps its own cell order (e.g. colour-major),
+    so the level schedule of a block DILU on the union is the union of the parts' schedules.
+    The batched solver assembles and factors one such union per polynomial degree."""
+    if not meshes:
+        raise ValueError("empty mesh list")
+    vs, pptrs, pvs, p2ps, bcs, tptrs, txys, bbs = [], [], [], [], [], [], [], []
+    cell_ptr = np.zeros(len(meshes) + 1, np.int32)
+    nv = nc = nb = nt = 0
+    for q, m in enumerate(meshes):
+        vs.append(m.v)
+        pptrs.append(m.pptr[:-1] + len(np.concatenate(pvs)) if pvs else m.pptr[:-1])
+        pvs.append(m.pv + nv)
+        p2p = m.p2p.astype(np.int64).copy()
+        p2p[p2p >= 0] += nc
+        p2p[p2p < 0] -= nb
+        p2ps.append(p2p)
+        bc = m.bc.astype(np.int64).copy()
+        if len(bc):
+            per = bc[:, 0] != 0
+            bc[per, 1:3] += nc
+            bc[per, 3:7] += nv
+        bcs.append(bc)
+        tptrs.append(m.tptr[:-1] + nt)
+        txys.append(m.txy)
+        bbs.append(m.bb)
+        nv += len(m.v)
+        nc += m.n_cells
+        nb += len(m.bc)
+        nt += len(m.txy)
+        cell_ptr[q + 1] = nc
+    pv = np.concatenate(pvs).astype(np.int32)
+    pptr = np.concatenate(pptrs + [np.asarray([len(pv)])]).astype(np.int32)
+    tptr = np.concatenate(tptrs + [np.asarray([nt])]).astype(np.int32)
+    u = PolyMesh(np.concatenate(vs), pptr, pv, np.concatenate(p2ps).astype(np.int32),
+                 np.concatenate(bcs).astype(np.int32).reshape(-1, 7), tptr, np.concatenate(txys),
+                 np.concatenate(bbs), max(m.width for m in meshes), max(m.height for m in meshes))
+    return u, cell_ptr
 
 
 # --------------------------------------------------------------------------------------

@@ -82,3 +82,38 @@ def euler_max_speed(u: np.ndarray, deg: int, gamma: float = 1.4) -> float:
     P = (gamma - 1) * (U[:, 3] - 0.5 * rho * (vx * vx + vy * vy))
     c = np.sqrt(gamma * P / rho)
     return float(np.max(np.hypot(vx, vy) + c))
+
+
+def batch_advection(count: int = 64, seed: int = 4, nmin: int = 500, nmax: int = 20000, degs=(1, 2, 3, 4),
+                    cfl: float = 5.0, width: float = 100.0):
+    """Independent advection problems for a batched implicit step: instance q has a Voronoi
+    mesh of the unit square (Dirichlet boundary, zero inflow data) from uniform random seeds,
+    its cell count log-uniform in [nmin, nmax], a degree drawn from ``degs``, a Gaussian bump
+    ``exp(-width |x - c|^2)`` with centre c uniform in [0.3, 0.7]^2, and the rotating field
+    ``(-(y - 1/2), x - 1/2)``; dt_q = dg_time_step(cfl, 1/sqrt(n_q), p_q, max |beta|).
+
+    Returns a list of classes ``(deg, union mesh, cell_ptr, u0, instance ids)`` (one per degree,
+    instances in id order inside a class) and the per-instance ``(n, deg, dt)`` in class order."""
+    from .mesh import box_voronoi, uniform_seeds, union_mesh
+
+    rng = np.random.default_rng(seed)
+    inst = []
+    for q in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        p = int(degs[int(rng.integers(0, len(degs)))])
+        cx, cy = rng.uniform(0.3, 0.7, 2)
+        inst.append((q, n, p, float(cx), float(cy)))
+    vmax = math.sqrt(0.5)   # max |(-(y-1/2), x-1/2)| on the unit square
+    classes, meta = [], []
+    for p in sorted({i[2] for i in inst}):
+        members = [i for i in inst if i[2] == p]
+        meshes, us, ids = [], [], []
+        for q, n, _, cx, cy in members:
+            m = box_voronoi(uniform_seeds(n, 1.0, 1.0, 1000 * seed + q), 1.0, 1.0)
+            meshes.append(m)
+            us.append(interp(m, p, gaussian(cx, cy, width), 1))
+            ids.append(q)
+            meta.append((m.n_cells, p, dg_time_step(cfl, 1.0 / math.sqrt(m.n_cells), p, vmax)))
+        um, cptr = union_mesh(meshes)
+        classes.append((p, um, cptr, np.concatenate(us), ids))
+    return classes, meta

@@ -198,6 +198,32 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
 int cpg_stepper_enable_kernel_timing(cpg_stepper S, int enable);
 int cpg_stepper_kernel_timing(cpg_stepper S, int *n_matvec, double *ms_matvec, int *n_apply, double *ms_apply);
 
+/* ---- batched implicit steps --------------------------------------------------
+ * Many independent implicit solves advanced together.  No reference counterpart: the
+ * reference advances one mesh at a time (DIRK::advance, src/TimeIntegration.cpp:93-151,
+ * BackwardEuler::advance :31-66).  Instances are grouped into classes; class c is one
+ * equation over a union mesh whose connected components are its ninst[c] instances
+ * (instance q of class c owns cells cell_ptr[c][q] .. cell_ptr[c][q+1]-1).  Instances are
+ * numbered class by class.  Each stage's Newton systems of all instances are solved by one
+ * batched flexible GMRES(params->restart): matvec, DILU factor and sweeps cover whole
+ * classes, inner products are segmented per instance, Hessenberg/Givens/least squares run
+ * on the device per instance (an instance that converges stops while the others go on),
+ * and with use_graph each restart cycle is captured once into a hipGraph and replayed.
+ * The row-stream launch shapes of every class's block size are autotuned at creation.
+ * DIRK with nstages 1 (backward Euler) or 2 (SDIRK2) and the other tableaux of
+ * cpg_stepper_create; each instance has its own dt (dt[q]); the stage time passed to the
+ * residual is t + c_s dt of a class's first instance, so the equations must have steady
+ * coefficients and time-independent Dirichlet data.  u[c], unp1[c]: device vectors of
+ * class c.  Synchronous.  inst_gmres_iters (optional, ninst entries): Arnoldi steps per
+ * instance in this step. */
+typedef struct cpg_batch_s *cpg_batch;
+int cpg_batch_create(int nclass, const cpg_equation *eqs, const int *ninst, const int *const *cell_ptr, int nstages,
+                     const cpg_solver_params *params, int use_graph, cpg_batch *out);
+int cpg_batch_destroy(cpg_batch B);
+int cpg_batch_info(cpg_batch B, int *ninst, int *nchunk, int *graph_captured);
+int cpg_batch_advance(cpg_batch B, const double *const *u, const double *dt, double t, double *const *unp1,
+                      cpg_step_stats *stats, int *inst_gmres_iters, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
