@@ -48,6 +48,11 @@ def parse():
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-nx", type=int, default=128, help="lattice size of the CPU-baseline sample")
+    p.add_argument("--workload", choices=["dirk3-euler", "batch"], default="dirk3-euler",
+                   help="batch: 64 independent implicit advection solves advanced as one batch (prints its own line)")
+    p.add_argument("--batch-stages", type=int, default=2, help="batch: 1 backward Euler, 2 SDIRK2")
+    p.add_argument("--batch-count", type=int, default=64)
+    p.add_argument("--no-sequential", action="store_true", help="batch: skip the one-stepper-per-instance timing")
     return p.parse_args()
 
 
@@ -139,8 +144,90 @@ def cpu_baseline(nx, deg, cfl, counts, n_target):
             "s_per_step": step}
 
 
+def bench_batch(args):
+    """Batched implicit advection: 64 instances (Voronoi meshes of the unit square with
+    log-uniform 500..20,000 cells, degrees 1..4, Gaussian bumps, rotating field, Dirichlet
+    inflow), CFL 5 each, SDIRK2 (or backward Euler), FP32 block DILU in FP64 flexible
+    GMRES(30) to 1e-10, one batch per step with hipGraph-captured restart cycles.  Also times
+    the same instances stepped one stepper at a time (same solver) for comparison."""
+    import numpy as np
+    import torch
+
+    from cpolyg_amd import hip, setups
+    from cpolyg_amd.spec import EquationSpec, GMRES_FLEXIBLE, PC_DILU, PREC_FP32, SolverSpec
+
+    t0 = time.perf_counter()
+    classes, meta = setups.batch_advection(args.batch_count, seed=4)
+    t_setup = time.perf_counter() - t0
+    spec = EquationSpec.rotating_advection()
+    solver = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=30, rtol=1e-10, maxit=6000, pc_kind=PC_DILU,
+                        precision=PREC_FP32, newton_tol=1e-12, newton_maxit=10)
+    eqs = [(hip.Equation(hip.Mesh(um), spec, p), cp) for p, um, cp, _, _, _ in classes]
+    t0 = time.perf_counter()
+    B = hip.Batch(eqs, args.batch_stages, solver, use_graph=True)
+    t_create = time.perf_counter() - t0
+    us = [torch.from_numpy(u0).cuda() for _, _, _, u0, _, _ in classes]
+    outs = [torch.zeros_like(u) for u in us]
+    dts = [d for _, _, d in meta]
+    t = 0.0
+    for _ in range(args.warmup):
+        B.advance(us, dts, t, outs)
+        us, outs = outs, us
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = []
+    st = None
+    for _ in range(args.steps):
+        st, it = B.advance(us, dts, t, outs)
+        its.append(it)
+        us, outs = outs, us
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    its = np.asarray(its)
+    cells = sum(n for n, _, _ in meta)
+    dofs = sum(n * (p + 1) * (p + 2) // 2 for n, p, _ in meta)
+    out = {"metric": "batched implicit steps: 64 advection solves, instance-steps/s", "value": len(meta) / wall,
+           "unit": "instance-steps/s", "ms_per_batch_step": wall * 1e3, "steps": args.steps, "warmup": args.warmup,
+           "config": {"workload": f"{len(meta)} rotating-field advection solves, Voronoi unit squares, "
+                                  f"{min(n for n, _, _ in meta)}..{max(n for n, _, _ in meta)} cells, p 1..4, "
+                                  f"{'SDIRK2' if args.batch_stages == 2 else 'backward Euler'} at CFL 5, FP32 DILU in "
+                                  f"FP64 FGMRES(30) rtol 1e-10, hipGraph restart cycles",
+                      "instances": len(meta), "cells": cells, "dofs": dofs,
+                      "classes": [{"deg": p, "instances": len(cp) - 1, "cells": int(um.n_cells)}
+                                  for p, um, cp, _, _, _ in classes]},
+           "gmres_steps_per_instance_step": {"min": int(its.min()), "max": int(its.max()), "mean": float(its.mean())},
+           "newton_iters_per_step": st.newton_iters, "graph": B.info(), "setup_s": t_setup, "create_s": t_create}
+    if not args.no_sequential:
+        # the same instances one stepper at a time (same solver, same kernels, one launch per operation
+        # per instance)
+        seq = 0.0
+        for c, (p, um, cp, u0, ids, meshes) in enumerate(classes):
+            bs = (p + 1) * (p + 2) // 2
+            for k, m in enumerate(meshes):
+                q = sum(len(cc[2]) - 1 for cc in classes[:c]) + k
+                E = hip.Equation(hip.Mesh(m), spec, p)
+                S = hip.Stepper(E, 0, args.batch_stages, solver)
+                ui = torch.from_numpy(u0[cp[k] * bs:cp[k + 1] * bs]).cuda()
+                uo = torch.zeros_like(ui)
+                S.advance(ui, dts[q], 0.0, uo)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                S.advance(ui, dts[q], 0.0, uo)
+                torch.cuda.synchronize()
+                seq += time.perf_counter() - t1
+        out["sequential_ms_per_step"] = seq * 1e3
+        out["batch_speedup_vs_sequential"] = seq / wall
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
+    if args.workload == "batch":
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        bench_batch(args)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is described in "
               f"DESIGN.md and not measured.", file=sys.stderr)

@@ -13,8 +13,9 @@
 //     live on the device, one thread per instance, so a restart cycle has no host
 //     synchronisation; an instance that converges stops updating its Krylov space and
 //     solution while the others continue;
-//   * a restart cycle (residual, m Arnoldi steps, update) is captured once into a hipGraph
-//     and replayed; the host reads the convergence flags once per cycle.
+//   * a restart cycle is captured once into hipGraphs (start; blocks of 5 Arnoldi steps;
+//     finish) and replayed; between blocks the host reads the instances' activity flags and
+//     ends the cycle early when all have converged.
 // Each instance has its own dt (its own CFL step): the Newton matrix M - dt_k a J and the
 // Newton residual M k - dt_k b take a per-cell dt.
 #include "cpg_internal.hpp"
@@ -44,10 +45,12 @@ struct Chunks {
 };
 
 // partial[k * ld + j] = sum over chunk k of V_j . w  (j < nv); V_j = V + j * N
-__global__ __launch_bounds__(BWG) void k_bmdot(Chunks C, const double *__restrict__ V, size_t N, int nv,
-                                               const double *__restrict__ w, double *__restrict__ partial, int ld) {
+__global__ __launch_bounds__(BWG) void k_bmdot(Chunks C, const int *__restrict__ active, const double *__restrict__ V,
+                                               size_t N, int nv, const double *__restrict__ w,
+                                               double *__restrict__ partial, int ld) {
     __shared__ double red[BWG / 64][64];
     const int k = blockIdx.x;
+    if (active && !active[C.inst[k]]) return;   // converged instance: its partial sums are not read
     const int e0 = C.c0[k], e1 = C.c1[k];
     double wv[BPT];
 #pragma unroll
@@ -90,11 +93,12 @@ __global__ __launch_bounds__(64) void k_binst_sum(const int *__restrict__ ckptr,
 }
 
 // w -= sum_{j<nv} h[inst][j] V_j over each chunk; partial[k] = sum over chunk k of w^2
-__global__ __launch_bounds__(BWG) void k_borth(Chunks C, const double *__restrict__ V, size_t N, int nv,
-                                               const double *__restrict__ H, int ldh, double *__restrict__ w,
-                                               double *__restrict__ partial) {
+__global__ __launch_bounds__(BWG) void k_borth(Chunks C, const int *__restrict__ active, const double *__restrict__ V,
+                                               size_t N, int nv, const double *__restrict__ H, int ldh,
+                                               double *__restrict__ w, double *__restrict__ partial) {
     __shared__ double red[BWG / 64];
     const int k = blockIdx.x;
+    if (active && !active[C.inst[k]]) return;
     const int e0 = C.c0[k], e1 = C.c1[k];
     const double *h = H + (size_t)C.inst[k] * ldh;
     double wv[BPT];
@@ -153,9 +157,10 @@ __global__ __launch_bounds__(BWG) void k_bnorm2(Chunks C, const double *__restri
 }
 
 // v = w / norm[inst] per chunk (0 when the norm is 0)
-__global__ __launch_bounds__(BWG) void k_bscale(Chunks C, const double *__restrict__ w, double *__restrict__ v,
-                                                const double *__restrict__ norm, int ldn) {
+__global__ __launch_bounds__(BWG) void k_bscale(Chunks C, const int *__restrict__ active, const double *__restrict__ w,
+                                                double *__restrict__ v, const double *__restrict__ norm, int ldn) {
     const int k = blockIdx.x;
+    if (active && !active[C.inst[k]]) return;
     const int e0 = C.c0[k], e1 = C.c1[k];
     const double nv = norm[(size_t)C.inst[k] * ldn];
     const double inv = nv > 0.0 ? 1.0 / nv : 0.0;
@@ -386,16 +391,19 @@ struct cpg_batch_s {
     std::vector<double> h_resid, h_nres;
     hipStream_t s = nullptr;
     bool use_graph = true;
-    hipGraphExec_t cycle = nullptr;
-    hipGraph_t graph = nullptr;
+    int kblock = 5;                         // Arnoldi steps per captured block
+    hipGraphExec_t g_start = nullptr, g_finish = nullptr;
+    std::vector<hipGraphExec_t> g_steps;
 
     Chunks chunks() const { return Chunks{ck0.p, ck1.p, ckinst.p}; }
     BState state() {
         return BState{H.p, hcol.p, cs.p, sn.p, g.p, beta.p, thr.p, active.p, kdim.p, conv.p, iters.p, resid.p, m};
     }
     ~cpg_batch_s() {
-        if (cycle) (void)hipGraphExecDestroy(cycle);
-        if (graph) (void)hipGraphDestroy(graph);
+        if (g_start) (void)hipGraphExecDestroy(g_start);
+        if (g_finish) (void)hipGraphExecDestroy(g_finish);
+        for (auto g : g_steps)
+            if (g) (void)hipGraphExecDestroy(g);
         for (auto &C : cls) {
             if (C.P) cpg_prec_destroy(C.P);
             if (C.J) cpg_bsr_destroy(C.J);
@@ -424,48 +432,95 @@ void batch_matvec(cpg_batch_s &B, const double *x, double *y) {
     for (auto &C : B.cls) bsr_matvec(*C.J, x + C.off, y + C.off, B.s);
 }
 
-// one restart cycle of the batched flexible GMRES for A x = r (x updated in place)
-void gmres_cycle(cpg_batch_s &B) {
+// A restart cycle of the batched flexible GMRES for A x = r (x updated in place), in three
+// parts so that the host can stop a cycle early once every instance has converged:
+// start (residual, beta, V_0), Arnoldi steps i0..i1-1, finish (least squares, x update).
+void gmres_start(cpg_batch_s &B) {
+    const size_t N = B.N;
+    BState S = B.state();
+    double *w = B.w.p;
+    // w = r - A x ; beta = ||w|| per instance ; V_0 = w / beta
+    batch_matvec(B, B.x.p, w);
+    hipLaunchKernelGGL(k_bsub_from, dim3(ceil_div(N, 256)), dim3(256), 0, B.s, B.r.p, w, N);
+    CPG_LAUNCH_CHECK();
+    seg_norm(B, w, B.beta.p);
+    hipLaunchKernelGGL(k_bcycle_init, dim3(ceil_div(B.ninst, 64)), dim3(64), 0, B.s, B.ninst, S);
+    CPG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bscale, dim3(B.nchunk), dim3(BWG), 0, B.s, B.chunks(), nullptr, w, B.V.p, B.beta.p, 1);
+    CPG_LAUNCH_CHECK();
+}
+
+void gmres_steps(cpg_batch_s &B, int i0, int i1) {
     const size_t N = B.N;
     const int m = B.m;
     BState S = B.state();
-    const int gi = ceil_div(B.ninst, 64);
-    const int gN = ceil_div(N, 256);
     double *V = B.V.p, *Z = B.Z.p, *w = B.w.p;
-    // w = r - A x ; beta = ||w|| per instance ; V_0 = w / beta
-    batch_matvec(B, B.x.p, w);
-    hipLaunchKernelGGL(k_bsub_from, dim3(gN), dim3(256), 0, B.s, B.r.p, w, N);
-    CPG_LAUNCH_CHECK();
-    seg_norm(B, w, B.beta.p);
-    hipLaunchKernelGGL(k_bcycle_init, dim3(gi), dim3(64), 0, B.s, B.ninst, S);
-    CPG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_bscale, dim3(B.nchunk), dim3(BWG), 0, B.s, B.chunks(), w, V, B.beta.p, 1);
-    CPG_LAUNCH_CHECK();
-    for (int i = 0; i < m; i++) {
+    const int *act = B.active.p;
+    for (int i = i0; i < i1; i++) {
         double *vi = V + N * i, *vi1 = V + N * (i + 1), *zi = Z + N * i;
         for (auto &C : B.cls) prec_apply_to(C.P, vi + C.off, zi + C.off, B.s);   // z_i = M^-1 v_i
         batch_matvec(B, zi, w);                                                // w = A z_i
         // classical Gram-Schmidt: h = V^T w ; w -= V h ; ||w||  (segmented per instance)
-        hipLaunchKernelGGL(k_bmdot, dim3(B.nchunk), dim3(BWG), 0, B.s, B.chunks(), V, N, i + 1, w, B.partial.p, m + 1);
+        hipLaunchKernelGGL(k_bmdot, dim3(B.nchunk), dim3(BWG), 0, B.s, B.chunks(), act, V, N, i + 1, w, B.partial.p,
+                           m + 1);
         CPG_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_binst_sum, dim3(B.ninst, i + 1), dim3(64), 0, B.s, B.ckptr.p, B.partial.p, m + 1, B.hcol.p,
                            m + 1, 0);
         CPG_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_borth, dim3(B.nchunk), dim3(BWG), 0, B.s, B.chunks(), V, N, i + 1, B.hcol.p, m + 1, w,
+        hipLaunchKernelGGL(k_borth, dim3(B.nchunk), dim3(BWG), 0, B.s, B.chunks(), act, V, N, i + 1, B.hcol.p, m + 1, w,
                            B.partial.p);
         CPG_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_binst_sum, dim3(B.ninst, 1), dim3(64), 0, B.s, B.ckptr.p, B.partial.p, 1,
                            B.hcol.p + i + 1, m + 1, 1);
         CPG_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_bscale, dim3(B.nchunk), dim3(BWG), 0, B.s, B.chunks(), w, vi1, B.hcol.p + i + 1, m + 1);
+        hipLaunchKernelGGL(k_bscale, dim3(B.nchunk), dim3(BWG), 0, B.s, B.chunks(), act, w, vi1, B.hcol.p + i + 1,
+                           m + 1);
         CPG_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_bgivens, dim3(gi), dim3(64), 0, B.s, B.ninst, i, S);
+        hipLaunchKernelGGL(k_bgivens, dim3(ceil_div(B.ninst, 64)), dim3(64), 0, B.s, B.ninst, i, S);
         CPG_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_bsolve, dim3(gi), dim3(64), 0, B.s, B.ninst, S, B.ybuf.p);
+}
+
+void gmres_finish(cpg_batch_s &B) {
+    BState S = B.state();
+    hipLaunchKernelGGL(k_bsolve, dim3(ceil_div(B.ninst, 64)), dim3(64), 0, B.s, B.ninst, S, B.ybuf.p);
     CPG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_bupdate, dim3(B.nchunk), dim3(BWG), 0, B.s, B.chunks(), Z, N, m, B.ybuf.p, B.kdim.p, B.x.p);
+    hipLaunchKernelGGL(k_bupdate, dim3(B.nchunk), dim3(BWG), 0, B.s, B.chunks(), B.Z.p, B.N, B.m, B.ybuf.p, B.kdim.p,
+                       B.x.p);
     CPG_LAUNCH_CHECK();
+}
+
+// run `body` on B.s, from a graph captured at its first use when graphs are on
+template <class Fn>
+void run_part(cpg_batch_s &B, hipGraphExec_t &exec, Fn &&body) {
+    if (!B.use_graph) {
+        body();
+        return;
+    }
+    if (!exec) {
+        CPG_HIP(hipStreamBeginCapture(B.s, hipStreamCaptureModeThreadLocal));
+        hipGraph_t g = nullptr;
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(B.s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        CPG_HIP(hipStreamEndCapture(B.s, &g));
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        CPG_HIP(e);
+    }
+    CPG_HIP(hipGraphLaunch(exec, B.s));
+}
+
+bool any_active(cpg_batch_s &B) {
+    CPG_HIP(hipMemcpyAsync(B.h_conv.data(), B.active.p, B.ninst * sizeof(int), hipMemcpyDeviceToHost, B.s));
+    CPG_HIP(hipStreamSynchronize(B.s));
+    for (int q = 0; q < B.ninst; q++)
+        if (B.h_conv[q]) return true;
+    return false;
 }
 
 // solve A x = r for every instance (x = 0 start, FGMRES tolerance rtol * ||r_q||)
@@ -487,25 +542,16 @@ void batch_solve(cpg_batch_s &B, cpg_step_stats &st) {
     CPG_LAUNCH_CHECK();
     const int maxcyc = std::max(1, (B.sp.maxit + B.m - 1) / B.m);
     int cyc = 0;
+    const int nblk = (B.m + B.kblock - 1) / B.kblock;
+    if (B.g_steps.empty()) B.g_steps.assign(nblk, nullptr);
     for (; cyc < maxcyc; cyc++) {
-        if (B.use_graph) {
-            if (!B.cycle) {
-                CPG_HIP(hipStreamBeginCapture(B.s, hipStreamCaptureModeThreadLocal));
-                try {
-                    gmres_cycle(B);
-                } catch (...) {
-                    hipGraph_t gdead = nullptr;
-                    (void)hipStreamEndCapture(B.s, &gdead);
-                    if (gdead) (void)hipGraphDestroy(gdead);
-                    throw;
-                }
-                CPG_HIP(hipStreamEndCapture(B.s, &B.graph));
-                CPG_HIP(hipGraphInstantiate(&B.cycle, B.graph, nullptr, nullptr, 0));
-            }
-            CPG_HIP(hipGraphLaunch(B.cycle, B.s));
-        } else {
-            gmres_cycle(B);
+        run_part(B, B.g_start, [&] { gmres_start(B); });
+        // Arnoldi steps in blocks of kblock, stopping once every instance has converged
+        for (int bk = 0; bk < nblk && any_active(B); bk++) {
+            const int i0 = bk * B.kblock, i1 = std::min(B.m, i0 + B.kblock);
+            run_part(B, B.g_steps[bk], [&] { gmres_steps(B, i0, i1); });
         }
+        run_part(B, B.g_finish, [&] { gmres_finish(B); });
         CPG_HIP(hipMemcpyAsync(B.h_conv.data(), B.conv.p, B.ninst * sizeof(int), hipMemcpyDeviceToHost, B.s));
         CPG_HIP(hipStreamSynchronize(B.s));
         bool all = true;
@@ -641,7 +687,7 @@ int cpg_batch_info(cpg_batch B, int *ninst, int *nchunk, int *graph_captured) {
         CPG_CHECK(B, CPG_ERR_ARG, "null batch");
         if (ninst) *ninst = B->ninst;
         if (nchunk) *nchunk = B->nchunk;
-        if (graph_captured) *graph_captured = B->cycle != nullptr;
+        if (graph_captured) *graph_captured = B->g_start != nullptr;
     });
 }
 

@@ -92,8 +92,9 @@ def batch_advection(count: int = 64, seed: int = 4, nmin: int = 500, nmax: int =
     ``exp(-width |x - c|^2)`` with centre c uniform in [0.3, 0.7]^2, and the rotating field
     ``(-(y - 1/2), x - 1/2)``; dt_q = dg_time_step(cfl, 1/sqrt(n_q), p_q, max |beta|).
 
-    Returns a list of classes ``(deg, union mesh, cell_ptr, u0, instance ids)`` (one per degree,
-    instances in id order inside a class) and the per-instance ``(n, deg, dt)`` in class order."""
+    Returns a list of classes ``(deg, union mesh, cell_ptr, u0, instance ids, instance meshes)``
+    (one per degree, instances in id order inside a class) and the per-instance ``(n, deg, dt)``
+    in class order."""
     from .mesh import box_voronoi, uniform_seeds, union_mesh
 
     rng = np.random.default_rng(seed)
@@ -115,5 +116,5 @@ def batch_advection(count: int = 64, seed: int = 4, nmin: int = 500, nmax: int =
             ids.append(q)
             meta.append((m.n_cells, p, dg_time_step(cfl, 1.0 / math.sqrt(m.n_cells), p, vmax)))
         um, cptr = union_mesh(meshes)
-        classes.append((p, um, cptr, np.concatenate(us), ids))
+        classes.append((p, um, cptr, np.concatenate(us), ids, meshes))
     return classes, meta

@@ -208,7 +208,9 @@ int cpg_stepper_kernel_timing(cpg_stepper S, int *n_matvec, double *ms_matvec, i
  * batched flexible GMRES(params->restart): matvec, DILU factor and sweeps cover whole
  * classes, inner products are segmented per instance, Hessenberg/Givens/least squares run
  * on the device per instance (an instance that converges stops while the others go on),
- * and with use_graph each restart cycle is captured once into a hipGraph and replayed.
+ * and with use_graph a restart cycle is captured once into hipGraphs (start, blocks of 5
+ * Arnoldi steps, finish) and replayed, the host ending a cycle early once every instance
+ * has converged.
  * The row-stream launch shapes of every class's block size are autotuned at creation.
  * DIRK with nstages 1 (backward Euler) or 2 (SDIRK2) and the other tableaux of
  * cpg_stepper_create; each instance has its own dt (dt[q]); the stage time passed to the

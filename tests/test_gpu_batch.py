@@ -23,9 +23,9 @@ def _small_batch(count=6, seed=7):
 def _run(cuda, classes, meta, nstages, solver, use_graph, steps=1):
     import torch
     spec = EquationSpec.rotating_advection()
-    eqs = [(hip.Equation(hip.Mesh(um), spec, p), cp) for p, um, cp, _, _ in classes]
+    eqs = [(hip.Equation(hip.Mesh(um), spec, p), cp) for p, um, cp, _, _, _ in classes]
     B = hip.Batch(eqs, nstages, solver, use_graph=use_graph)
-    us = [torch.from_numpy(u0).to(cuda) for _, _, _, u0, _ in classes]
+    us = [torch.from_numpy(u0).to(cuda) for _, _, _, u0, _, _ in classes]
     outs = [torch.zeros_like(u) for u in us]
     dts = [d for _, _, d in meta]
     its_all = []
@@ -45,13 +45,11 @@ def test_batch_matches_oracle_per_instance(cuda, nstages):
     ref = SolverSpec(gmres_mode=GMRES_LEFT_REF, restart=40, rtol=1e-13, maxit=4000, pc_kind=PC_DILU,
                      newton_tol=1e-13, newton_maxit=10)
     q = 0
-    for c, (p, um, cp, u0, ids) in enumerate(classes):
+    for c, (p, um, cp, u0, ids, meshes) in enumerate(classes):
         bs = (p + 1) * (p + 2) // 2
         for k in range(len(cp) - 1):
-            from cpolyg_amd.mesh import box_voronoi, uniform_seeds
             lo, hi = cp[k] * bs, cp[k + 1] * bs
-            # the instance's own mesh, rebuilt exactly as the batch built it
-            m = box_voronoi(uniform_seeds(cp[k + 1] - cp[k], 1.0, 1.0, 1000 * 7 + ids[k]), 1.0, 1.0)
+            m = meshes[k]   # the instance's own mesh
             uo, _ = O.Oracle(m).dirk_step(spec, p, nstages, u0[lo:hi], meta[q][2], 0.0, ref)
             assert rel(got[c][lo:hi], uo) < 1e-8, (c, k, rel(got[c][lo:hi], uo))
             assert its[0][q] > 0
@@ -78,13 +76,12 @@ def test_batch_uneven_convergence_fp64(cuda):
     spec = EquationSpec.rotating_advection()
     ref = SolverSpec(gmres_mode=GMRES_LEFT_REF, restart=40, rtol=1e-13, maxit=4000, pc_kind=PC_DILU,
                      newton_tol=1e-13, newton_maxit=10)
-    from cpolyg_amd.mesh import box_voronoi, uniform_seeds
     q = 0
-    for c, (p, um, cp, u0, ids) in enumerate(classes):
+    for c, (p, um, cp, u0, ids, meshes) in enumerate(classes):
         bs = (p + 1) * (p + 2) // 2
         for k in range(len(cp) - 1):
             lo, hi = cp[k] * bs, cp[k + 1] * bs
-            m = box_voronoi(uniform_seeds(cp[k + 1] - cp[k], 1.0, 1.0, 1000 * 3 + ids[k]), 1.0, 1.0)
+            m = meshes[k]
             uo, _ = O.Oracle(m).dirk_step(spec, p, 1, u0[lo:hi], meta[q][2], 0.0, ref)
             assert rel(got[c][lo:hi], uo) < 1e-8
             q += 1

@@ -101,8 +101,9 @@ def test_union_mesh_is_disjoint_union():
 def test_batch_workload_shapes():
     from cpolyg_amd import setups
     classes, meta = setups.batch_advection(count=9, seed=5, nmin=20, nmax=80)
-    assert sum(len(cp) - 1 for _, _, cp, _, _ in classes) == 9 == len(meta)
-    for p, um, cp, u0, ids in classes:
+    assert sum(len(cp) - 1 for _, _, cp, _, _, _ in classes) == 9 == len(meta)
+    for p, um, cp, u0, ids, meshes in classes:
+        assert [mm.n_cells for mm in meshes] == list(np.diff(cp))
         assert um.n_cells == cp[-1] and len(u0) == um.n_cells * (p + 1) * (p + 2) // 2
         assert ids == sorted(ids)
     assert all(20 <= n <= 80 and dt > 0 for n, _, dt in meta)
