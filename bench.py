@@ -8,10 +8,10 @@ Workload (one "step" = one DIRK3 time step, i.e. three Newton solves on the devi
   flexible GMRES(50) to 1e-10 with block DILU; exact periodic traces and exact Euler
   off-diagonal Jacobian blocks (the reference's truncated periodic trace is a parity option:
   on this workload it drives the third DIRK stage of the fourth step to non-physical states).
-The headline value is the mixed-precision path (by default DILU factors stored and applied in
-FP32 with FP64 accumulation, inside FP64 flexible GMRES); the FP64 path (FP64 factors, same
-solver) is timed in the same run and reported beside it, with the difference between the two
-solutions.
+The headline value is the mixed-precision path (by default DILU factors stored in BF16 with one
+power-of-two scale per block and applied with FP32 partial sums, inside FP64 flexible GMRES); the
+FP64 path (FP64 factors, same solver) is timed in the same run and reported beside it, with the
+difference between the two solutions.
 
 Prints ONE JSON line.  Only one GPU is used: --gpus above 1 exits with a message.
 """
@@ -28,6 +28,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "DIRK steps/s, Euler p=3 65k cells: mixed-prec vs FP64 on GPU at 1e-10 resid"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per BSR-matvec launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
+# kernel (bench.py --workload pmc), FETCH scaled x2 and WRITE x1 by a 1 GiB copy of known size;
+# keyed by (cells, degree).  profiles/bsr_matvec_euler_p3_65k_pmc_traffic.txt has the passes.
+MATVEC_TRAFFIC = {(65536, 3): 6.243e9}
 
 
 def parse():
@@ -39,7 +43,7 @@ def parse():
     p.add_argument("--deg", type=int, default=3)
     p.add_argument("--cfl", type=float, default=20.0)
     p.add_argument("--restart", type=int, default=50)
-    p.add_argument("--mixed", choices=["fp32", "bf16", "inner-fp32", "inner-bf16"], default="fp32",
+    p.add_argument("--mixed", choices=["fp32", "bf16", "inner-fp32", "inner-bf16"], default="bf16",
                    help="fp32/bf16: DILU factors stored in that precision inside FP64 FGMRES; inner-*: FP32 "
                         "GMRES(--inner) with those factors as the FGMRES preconditioner")
     p.add_argument("--inner", type=int, default=20)
@@ -48,8 +52,9 @@ def parse():
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-nx", type=int, default=128, help="lattice size of the CPU-baseline sample")
-    p.add_argument("--workload", choices=["dirk3-euler", "batch"], default="dirk3-euler",
-                   help="batch: 64 independent implicit advection solves advanced as one batch (prints its own line)")
+    p.add_argument("--workload", choices=["dirk3-euler", "batch", "pmc"], default="dirk3-euler",
+                   help="batch: 64 independent implicit advection solves advanced as one batch (prints its own "
+                        "line); pmc: the short kernel sequence profiled with rocprofv3 --pmc for HBM traffic")
     p.add_argument("--batch-stages", type=int, default=2, help="batch: 1 backward Euler, 2 SDIRK2")
     p.add_argument("--batch-count", type=int, default=64)
     p.add_argument("--no-sequential", action="store_true", help="batch: skip the one-stepper-per-instance timing")
@@ -220,8 +225,42 @@ def bench_batch(args):
     print(json.dumps(out))
 
 
+def pmc_workload(args):
+    """Kernel sequence of the HBM-traffic profile: a 1 GiB device copy of known size (calibrates
+    FETCH_SIZE / WRITE_SIZE), three FP64 BSR matvecs of the headline Newton matrix, one BF16 DILU
+    apply.  Meant to run under rocprofv3 --pmc; prints nothing but a completion line."""
+    import torch
+
+    from cpolyg_amd import hip, setups
+    from cpolyg_amd.spec import EquationSpec, PC_DILU, PREC_BF16
+
+    mesh, u0 = workload(args.nx, args.deg)
+    E = hip.Equation(hip.Mesh(mesh), EquationSpec.euler(periodic_trace=args.trace, jacobian=args.jacobian), args.deg)
+    J = E.jacobian_matrix()
+    u = torch.from_numpy(u0).cuda()
+    dt = setups.dg_time_step(args.cfl, 10.0 / args.nx, args.deg, setups.euler_max_speed(u0, args.deg))
+    E.jacobian(u, 0.0, J, scale=-dt * 0.5 * (1 + 0.435866521508459), add_mass=True)
+    P = hip.Prec(J, PC_DILU, PREC_BF16)
+    P.factor(J)
+    x = torch.randn(E.n * E.bl, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    a = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        J.matvec(x, y)
+    P.apply(x)
+    torch.cuda.synchronize()
+    print("pmc workload done", flush=True)
+
+
 def main():
     args = parse()
+    if args.workload == "pmc":
+        pmc_workload(args)
+        return
     if args.workload == "batch":
         if args.gpus != 1:
             print("bench.py runs on one GPU only", file=sys.stderr)
@@ -302,7 +341,8 @@ def main():
                    "dt": dt, "cfl": args.cfl, "mixed": args.mixed, "jacobian": "exact" if args.jacobian else "reference",
                    "periodic_trace": "exact" if args.trace else "reference", "colours": mesh.n_colours},
         "roofline": {"bound": "hbm", "kernel": "bsr_matvec (FP64, 40x40 blocks)", "achieved": gbs,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": MATVEC_TRAFFIC.get((n, deg)),
                      "algorithmic_bytes": b_mv, "avg_ms": mv_ms, "launches": nm},
         "dilu_apply": ({"precision": args.mixed, "algorithmic_bytes": b_ap, "avg_ms": ap_ms,
                         "achieved_gbs": b_ap / (ap_ms * 1e-3) / 1e9 if ap_ms > 0 else None, "launches": na}

@@ -1175,6 +1175,8 @@ template <class T>
 void factor_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
     if (!use_lds_factor()) {
         switch (P.bl) {
+        case 3: factor_wf_t<T, 3>(P, A, s); return;
+        case 6: factor_wf_t<T, 6>(P, A, s); return;
         case 10: factor_wf_t<T, 10>(P, A, s); return;
         case 12: factor_wf_t<T, 12>(P, A, s); return;
         case 15: factor_wf_t<T, 15>(P, A, s); return;
