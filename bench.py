@@ -32,6 +32,8 @@ HBM_PEAK_GBS = 8000.0
 # kernel (bench.py --workload pmc), FETCH scaled x2 and WRITE x1 by a 1 GiB copy of known size;
 # keyed by (cells, degree).  profiles/bsr_matvec_euler_p3_65k_pmc_traffic.txt has the passes.
 MATVEC_TRAFFIC = {(65536, 3): 6.243e9}
+# the same for one DILU apply (all 12 launches), keyed by (cells, degree, factor precision)
+APPLY_TRAFFIC = {(65536, 3, "bf16"): 1.752e9}
 
 
 def parse():
@@ -48,8 +50,13 @@ def parse():
                         "GMRES(--inner) with those factors as the FGMRES preconditioner")
     p.add_argument("--inner", type=int, default=20)
     p.add_argument("--jacobian", type=int, default=1, help="1: exact off-diagonal Jacobian, 0: reference form")
+    p.add_argument("--operator", choices=["matrix-free", "assembled"], default="matrix-free",
+                   help="GMRES operator: the Newton matrix applied at the quadrature points, or the BSR matvec of "
+                        "the assembled matrix")
     p.add_argument("--trace", type=int, default=1, help="periodic trace: 1 exact, 0 reference (truncated)")
     p.add_argument("--no-fp64", action="store_true")
+    p.add_argument("--no-compare", dest="compare_operators", action="store_false",
+                   help="skip the two runs with the other GMRES operator")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-nx", type=int, default=128, help="lattice size of the CPU-baseline sample")
     p.add_argument("--workload", choices=["dirk3-euler", "batch", "pmc"], default="dirk3-euler",
@@ -78,7 +85,10 @@ def algorithmic_bytes(n, nnzb, bl, elem, scales):
     return matvec, apply
 
 
-def run_path(E, u0, dt, steps, warmup, solver, timing):
+def run_path(E, u0, dt, steps, warmup, solver):
+    """Warmup, then `steps` timed DIRK3 steps (the production path: captured GMRES cycles),
+    then one more step with HIP events around every operator and preconditioner application
+    (eager launches; not part of the timed region's wall time)."""
     import torch
     from cpolyg_amd import hip
 
@@ -91,8 +101,6 @@ def run_path(E, u0, dt, steps, warmup, solver, timing):
         S.advance(u, dt, t, nxt, stream)
         u, nxt = nxt, u
         t += dt
-    if timing:
-        S.enable_kernel_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     stats = []
@@ -103,8 +111,49 @@ def run_path(E, u0, dt, steps, warmup, solver, timing):
         t += dt
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kt = S.kernel_timing() if timing else None
-    return u.cpu().numpy(), wall, stats, kt
+    S.enable_kernel_timing(True)
+    S.advance(u, dt, t, nxt, stream)
+    kt = S.kernel_timing()
+    S.enable_kernel_timing(False)
+    torch.cuda.synchronize()
+    return u.cpu().numpy(), wall, stats, kt, u, t
+
+
+def standalone_rates(E, u, t, scale, reps=20):
+    """Per-launch HIP-event timing on torch's current stream (the stream the library launches on)
+    of the BSR matvec, the matrix-free Newton apply and the DILU apply in each factor precision,
+    on the Newton matrix M + scale J(u, t)."""
+    import torch
+    from cpolyg_amd import hip
+    from cpolyg_amd.spec import PC_DILU, PREC_BF16, PREC_FP32, PREC_FP64
+
+    stream = torch.cuda.current_stream()
+    J = E.jacobian_matrix()
+    E.jacobian(u, t, J, scale=scale, add_mass=True, stream=stream)
+    x = torch.randn(E.n * E.bl, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+
+    def per_launch(fn, pre=None):
+        fn()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for e0, e1 in ev:
+            if pre:
+                pre()
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+        torch.cuda.synchronize()
+        return sum(e0.elapsed_time(e1) for e0, e1 in ev) / reps
+
+    out = {"bsr_matvec_ms": per_launch(lambda: J.matvec(x, y, stream)),
+           "newton_apply_ms": per_launch(lambda: E.newton_apply(u, t, scale, x, y, stream))}
+    for name, prec in (("bf16", PREC_BF16), ("fp32", PREC_FP32), ("fp64", PREC_FP64)):
+        P = hip.Prec(J, PC_DILU, prec)
+        P.factor(J, stream)
+        out[f"dilu_apply_{name}_ms"] = per_launch(lambda: P.apply(y, stream), lambda: y.copy_(x))
+        del P
+    return out
 
 
 def cpu_baseline(nx, deg, cfl, counts, n_target):
@@ -275,8 +324,8 @@ def main():
     import torch
 
     from cpolyg_amd import hip, setups
-    from cpolyg_amd.spec import (EquationSpec, GMRES_FLEXIBLE, PC_DILU, PC_INNER_GMRES, PREC_BF16, PREC_FP32,
-                                 PREC_FP64, SolverSpec)
+    from cpolyg_amd.spec import (EquationSpec, GMRES_FLEXIBLE, OP_ASSEMBLED, OP_MATRIX_FREE, PC_DILU, PC_INNER_GMRES,
+                                 PREC_BF16, PREC_FP32, PREC_FP64, SolverSpec)
 
     deg = args.deg
     mesh, u0 = workload(args.nx, deg)
@@ -286,42 +335,71 @@ def main():
     E = hip.Equation(hip.Mesh(mesh), spec, deg)
     bl, nnzb = E.bl, E.nnzb
 
-    def solver(prec, pc=PC_DILU):
+    def solver(prec, pc=PC_DILU, op=None):
         # each stage's Newton iteration starts from the previous stage's increment and halves an
         # update that would leave the physical states (the reference starts from zero; at CFL 20 its
         # first update on the third stage can overshoot to negative pressure)
         return SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=args.restart, rtol=1e-10, maxit=20000,
                           pc_kind=pc, newton_tol=5e-13, newton_maxit=100, precision=prec, inner_iters=args.inner,
-                          newton_guess=1, newton_backtrack=1)
+                          newton_guess=1, newton_backtrack=1,
+                          newton_operator=(OP_MATRIX_FREE if args.operator == "matrix-free" else OP_ASSEMBLED)
+                          if op is None else op)
 
     mixed_prec = PREC_BF16 if args.mixed.endswith("bf16") else PREC_FP32
     mixed_pc = PC_INNER_GMRES if args.mixed.startswith("inner") else PC_DILU
-    u_mix, wall_mix, st_mix, kt = run_path(E, u0, dt, args.steps, args.warmup, solver(mixed_prec, mixed_pc), True)
+    steps = args.steps
+    a_ss = 0.5 * (1 + 0.435866521508459)
+    u_mix, wall_mix, st_mix, kt, u_last, t_last = run_path(E, u0, dt, steps, args.warmup, solver(mixed_prec, mixed_pc))
     res = {}
     if not args.no_fp64:
-        u64, wall64, st64, kt64 = run_path(E, u0, dt, args.steps, args.warmup, solver(PREC_FP64), True)
-        res["fp64_steps_per_s"] = args.steps / wall64
-        res["fp64_ms_per_step"] = 1e3 * wall64 / args.steps
+        u64, wall64, st64, kt64, _, _ = run_path(E, u0, dt, steps, args.warmup, solver(PREC_FP64))
+        res["fp64_steps_per_s"] = steps / wall64
+        res["fp64_ms_per_step"] = 1e3 * wall64 / steps
         res["speedup_mixed_vs_fp64"] = wall64 / wall_mix
         res["mixed_vs_fp64_max_rel_diff"] = float(np.max(np.abs(u_mix - u64)) / np.max(np.abs(u64)))
-        res["fp64_gmres_iters_per_step"] = sum(s["gmres_iters"] for s in st64) / args.steps
-        res["fp64_newton_iters_per_step"] = sum(s["newton_iters"] for s in st64) / args.steps
-        nm64, msm64, na64, msa64 = kt64
-        b_ap64 = algorithmic_bytes(n, nnzb, bl, 8, False)[1]
-        res["fp64_dilu_apply"] = {"algorithmic_bytes": b_ap64, "avg_ms": msa64 / max(na64, 1),
-                                  "achieved_gbs": b_ap64 / (msa64 / max(na64, 1) * 1e-3) / 1e9 if na64 else None}
-    steps = args.steps
-    nm, msm, na, msa = kt
+        res["fp64_gmres_iters_per_step"] = sum(s["gmres_iters"] for s in st64) / steps
+        res["fp64_newton_iters_per_step"] = sum(s["newton_iters"] for s in st64) / steps
+        res["fp64_phase_ms_per_step"] = {k: sum(s[k] for s in st64) / steps for k in
+                                         ("ms_residual", "ms_jacobian", "ms_factor", "ms_gmres")}
+    if args.compare_operators:
+        # the same two paths with the other GMRES operator
+        other = OP_ASSEMBLED if args.operator == "matrix-free" else OP_MATRIX_FREE
+        cmp_ = {"operator": "assembled" if other == OP_ASSEMBLED else "matrix-free"}
+        ua, wa, _, _, _, _ = run_path(E, u0, dt, steps, args.warmup, solver(mixed_prec, mixed_pc, other))
+        cmp_["mixed_ms_per_step"] = 1e3 * wa / steps
+        cmp_["mixed_max_rel_diff"] = float(np.max(np.abs(ua - u_mix)) / np.max(np.abs(u_mix)))
+        if not args.no_fp64:
+            _, wa64, _, _, _, _ = run_path(E, u0, dt, steps, args.warmup, solver(PREC_FP64, PC_DILU, other))
+            cmp_["fp64_ms_per_step"] = 1e3 * wa64 / steps
+            cmp_["speedup_mixed_vs_fp64"] = wa64 / wa
+        res["other_operator"] = cmp_
+    # per-launch rates on the last Newton matrix of the mixed run (stage-3 scale)
+    rates = standalone_rates(E, u_last, t_last, -dt * a_ss)
     elem = {PREC_BF16: 2, PREC_FP32: 4}[mixed_prec]
     b_mv, b_ap = algorithmic_bytes(n, nnzb, bl, elem, mixed_prec == PREC_BF16)
-    if mixed_pc != PC_DILU:
-        res["inner_solve"] = {"iterations": args.inner, "factors": args.mixed, "avg_ms": msa / max(na, 1),
-                              "launches": na}
-    mv_ms = msm / max(nm, 1)
+    b_ap32 = algorithmic_bytes(n, nnzb, bl, 4, False)[1]
+    b_ap64 = algorithmic_bytes(n, nnzb, bl, 8, False)[1]
+    nm, msm, na, msa = kt
+    op_ms = msm / max(nm, 1)
     ap_ms = msa / max(na, 1)
-    gbs = b_mv / (mv_ms * 1e-3) / 1e9
+    if mixed_pc != PC_DILU:
+        res["inner_solve"] = {"iterations": args.inner, "factors": args.mixed, "avg_ms": ap_ms, "launches": na}
+    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9
+    mv_gbs = gbs(b_mv, rates["bsr_matvec_ms"])
     newton = sum(s["newton_iters"] for s in st_mix) / steps
     gm = sum(s["gmres_iters"] for s in st_mix) / steps
+    if mixed_pc == PC_DILU:
+        # dominant kernel of the mixed step: the DILU apply (12 row-stream launches of 6 colour
+        # levels, timed as one unit inside GMRES)
+        ap_gbs = gbs(b_ap, ap_ms)
+        roof = {"bound": "hbm", "kernel": f"dilu_apply ({args.mixed} factors, 40x40 blocks, 6 levels x 2 sweeps)",
+                "achieved": ap_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ap_gbs / HBM_PEAK_GBS,
+                "traffic": APPLY_TRAFFIC.get((n, deg, args.mixed)), "algorithmic_bytes": b_ap, "avg_ms": ap_ms,
+                "launches": na}
+    else:
+        roof = {"bound": "hbm", "kernel": "bsr_matvec (FP64, 40x40 blocks)", "achieved": mv_gbs,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": mv_gbs / HBM_PEAK_GBS,
+                "traffic": MATVEC_TRAFFIC.get((n, deg)), "algorithmic_bytes": b_mv, "avg_ms": rates["bsr_matvec_ms"]}
     out = {
         "metric": METRIC,
         "value": steps / wall_mix,
@@ -337,16 +415,21 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"Euler isentropic vortex, periodic Voronoi {n} cells ({args.nx}^2 jittered "
                                f"lattice), p={deg} ({bl}x{bl} blocks), DIRK3 CFL {args.cfl:g}, FGMRES({args.restart}) "
-                               f"rtol 1e-10, block DILU", "cells": n, "deg": deg, "block": bl, "nnzb": nnzb,
-                   "dt": dt, "cfl": args.cfl, "mixed": args.mixed, "jacobian": "exact" if args.jacobian else "reference",
+                               f"rtol 1e-10, block DILU, {args.operator} Newton operator", "cells": n, "deg": deg,
+                   "block": bl, "nnzb": nnzb, "dt": dt, "cfl": args.cfl, "mixed": args.mixed,
+                   "operator": args.operator, "jacobian": "exact" if args.jacobian else "reference",
                    "periodic_trace": "exact" if args.trace else "reference", "colours": mesh.n_colours},
-        "roofline": {"bound": "hbm", "kernel": "bsr_matvec (FP64, 40x40 blocks)", "achieved": gbs,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                     "traffic": MATVEC_TRAFFIC.get((n, deg)),
-                     "algorithmic_bytes": b_mv, "avg_ms": mv_ms, "launches": nm},
-        "dilu_apply": ({"precision": args.mixed, "algorithmic_bytes": b_ap, "avg_ms": ap_ms,
-                        "achieved_gbs": b_ap / (ap_ms * 1e-3) / 1e9 if ap_ms > 0 else None, "launches": na}
-                       if mixed_pc == PC_DILU else None),
+        "roofline": roof,
+        "in_solver_ms": {"operator_avg_ms": op_ms, "operator_launches": nm, "operator": args.operator,
+                         "prec_apply_avg_ms": ap_ms, "prec_applies": na},
+        "bsr_matvec": {"algorithmic_bytes": b_mv, "avg_ms": rates["bsr_matvec_ms"], "achieved_gbs": mv_gbs,
+                       "frac": mv_gbs / HBM_PEAK_GBS, "traffic": MATVEC_TRAFFIC.get((n, deg))},
+        "newton_apply_avg_ms": rates["newton_apply_ms"],
+        "dilu_apply_standalone": {p: {"avg_ms": rates[f"dilu_apply_{p}_ms"], "algorithmic_bytes": b,
+                                      "achieved_gbs": gbs(b, rates[f"dilu_apply_{p}_ms"]),
+                                      "frac": gbs(b, rates[f"dilu_apply_{p}_ms"]) / HBM_PEAK_GBS}
+                                  for p, b in (("bf16", algorithmic_bytes(n, nnzb, bl, 2, True)[1]),
+                                               ("fp32", b_ap32), ("fp64", b_ap64))},
         "newton_iters_per_step": newton,
         "gmres_iters_per_step": gm,
         "phase_ms_per_step": {k: sum(s[k] for s in st_mix) / steps for k in

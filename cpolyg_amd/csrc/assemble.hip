@@ -82,7 +82,7 @@ struct Edge {
     int nb;                 // neighbour cell (interior or periodic partner), -1 Dirichlet
     int kind;               // 0 interior, 1 periodic, 2 Dirichlet
     int slot;               // BSR slot of the off-diagonal block, -1 if none
-    int pad;
+    int bslot;              // Dirichlet edges: index into the boundary-state table, else -1
 };
 
 struct Geo {
@@ -91,15 +91,38 @@ struct Geo {
     const double4 *vpts;    // x, y, W
     const int *eptr;        // edges per cell
     const Edge *edges;
+    const double *bcst;     // Dirichlet states at the edge points: [bslot][q][c] (k_bc_states)
 };
+
+// Dirichlet states of every boundary edge point at time t (BoundaryConditions::boundaryValue,
+// src/BoundaryConditions.cpp:11-13).  Kept out of the cell kernels: the Dirichlet functors
+// (exp, pow, atan2) would otherwise hold libm constants and parameter quotients in registers
+// across their edge loops.
+template <int DEG, int NC>
+__global__ void k_bc_states(DevEq eq, const Edge *__restrict__ edges, const int *__restrict__ bedges, int nbe,
+                            double t, double *__restrict__ bcst) {
+    constexpr int NQL = Deg<DEG>::NQL;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nbe * NQL) return;
+    const int b = e / NQL, q = e - b * NQL;
+    const Edge E = edges[bedges[b]];
+    const double *lr = line_rule<NQL>();
+    const double s = 0.5 * (lr[2 * q] + 1.0);
+    const double x = E.x1 + s * (E.x2 - E.x1), y = E.y1 + s * (E.y2 - E.y1);
+    double U[NC];
+    eval_bc<NC>(eq, x, y, t, U);
+#pragma unroll
+    for (int c = 0; c < NC; c++) bcst[(size_t)(E.bslot * NQL + q) * NC + c] = U[c];
+}
 
 // ---------------------------------------------------------------------------------------
 template <int DEG, int NC>
 __device__ inline void outer_state(const DevEq &eq, const Geo &g, const Edge &E, const double *__restrict__ u,
-                                   double x, double y, double t, double *Up, double *phin) {
-    constexpr int BS = Deg<DEG>::BS, BL = NC * BS;
+                                   double x, double y, int q, double *Up, double *phin) {
+    constexpr int BS = Deg<DEG>::BS, BL = NC * BS, NQL = Deg<DEG>::NQL;
     if (E.kind == 2) {
-        eval_bc<NC>(eq, x, y, t, Up);
+#pragma unroll
+        for (int c = 0; c < NC; c++) Up[c] = g.bcst[(size_t)(E.bslot * NQL + q) * NC + c];
 #pragma unroll
         for (int k = 0; k < BS; k++) phin[k] = 0.0;
         return;
@@ -118,61 +141,218 @@ __device__ inline void outer_state(const DevEq &eq, const Geo &g, const Edge &E,
     eval_state<BS, NC>(u + (size_t)E.nb * BL, phin, Up);
 }
 
-// one wavefront per cell; lanes take quadrature points, partial sums reduced by shuffles
-template <int DEG, int NC>
-__global__ __launch_bounds__(64) void k_residual(DevEq eq, Geo g, const double *__restrict__ u, double t,
-                                                 double *__restrict__ b) {
-    constexpr int BS = Deg<DEG>::BS, BL = NC * BS, NQL = Deg<DEG>::NQL;
-    __shared__ double sU[BL];
-    __shared__ double sOut[BL];
-    const int i = blockIdx.x;
-    const int lane = threadIdx.x;
-    for (int r = lane; r < BL; r += 64) sU[r] = u[(size_t)i * BL + r];
-    __syncthreads();
+// ---------------------------------------------------------------------------------------
+// Cell operators: the residual b(u) (LIN = false, Equation::assemble, src/Equation.cpp:152-191)
+// and the Newton operator applied without assembling it (LIN = true):
+//   y = M v + scale * J(u) v,
+// the product the BSR matvec forms with the matrix k_jacobian writes (scale * J + M;
+// src/Jacobian.cpp:99-110 with src/BlockMatrix.cpp:28-49).  J(u) v is summed over the same
+// quadrature points from the same per-point flux Jacobians that k_jacobian contracts
+// (volume: A_x, A_y of fluxjac; edges: numfluxjac(+1) for the cell's own trace, and
+// euler_offdiag_exact or numfluxjac(-1) for the neighbour's), applied to the traces of v
+// instead of to basis functions, so the two agree to rounding in either Jacobian form.
+//
+// One wavefront per cell.  Lanes first take quadrature points: each evaluates the basis,
+// the traces of u (and v) and the pointwise flux (or its derivative along v), and writes an
+// entry (test-function values D[k], flux values G[c]) per point to an LDS record; a volume
+// point gives two entries (d_xi phi_k with W sx F_x,c, then d_eta phi_k with W sy F_y,c, one
+// record fill each), an edge point one (phi_k with -W Fhat_c).  Then the record is contracted:
+// lane (jg, k) keeps NC accumulators out[c][k] over the entries j = jg mod JG (one LDS read of
+// D and NC/2 16-byte reads of G per entry), and the JG partial sums of every output are added
+// in a fixed order at the end.  The neighbours' coefficients of an edge chunk are staged in LDS
+// with coalesced loads.
+template <int DEG, int NC, bool LIN>
+struct CellOpLayout {
+    static constexpr int BS = Deg<DEG>::BS, BL = NC * BS, NQL = Deg<DEG>::NQL;
+    static constexpr int EPC = 64 / NQL;              // edges per chunk
+    static constexpr int NE = 64;                     // record entries
+    static constexpr int REC = NE * (BS + NC);        // D[NE][BS], then G[NE][NC]
+    static constexpr int JG = 64 / BS;                // entry groups of the contraction
+    static constexpr int SW = (LIN ? 2 : 1) * BL;     // staged doubles per edge: u_nb (, v_nb)
+    static constexpr int STG = EPC * SW;
+    static constexpr int SH0 = REC > STG ? REC : STG;
+    static constexpr int SH = SH0 > JG * BL ? SH0 : JG * BL;
+};
+
+// acc[c] += sum over entries j = jg, jg + JG, ... < nent of D[j][k] G[j][c]
+template <int BS, int NC, int JG>
+__device__ inline void cellop_contract(const double *D, const double *G, int jg, int k, int nent, double (&acc)[NC]) {
+    for (int j = jg; j < nent; j += JG) {
+        const double dk = D[j * BS + k];
+        const double *gj = G + j * NC;
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[c] = fma(dk, gj[c], acc[c]);
+    }
+}
+
+template <int DEG, int NC, bool LIN>
+__global__ __launch_bounds__(64, 2) void k_cellop(DevEq eq, Geo g, const double *__restrict__ u,
+                                                  const double *__restrict__ v, double scale,
+                                                  const double *__restrict__ scale_dev, const int *__restrict__ active,
+                                                  const double *__restrict__ mass, double *__restrict__ out) {
+    using L = CellOpLayout<DEG, NC, LIN>;
+    if (active && !*active) return;   // a converged Krylov cycle replayed from a graph
+    if (scale_dev) scale = *scale_dev;
+    constexpr int BS = L::BS, BL = L::BL, NQL = L::NQL, SW = L::SW, JG = L::JG;
+    // own coefficients: u alone, or u and v interleaved (sU[2r] = u_r, sU[2r+1] = v_r)
+    __shared__ __align__(16) double sU[LIN ? 2 * BL : BL];
+    __shared__ __align__(16) double sh[L::SH];
+    double *D = sh;
+    double *G = sh + L::NE * BS;
+    const int i = blockIdx.x, lane = threadIdx.x;
+    for (int r = lane; r < BL; r += 64) {
+        if constexpr (LIN) {
+            sU[2 * r] = u[(size_t)i * BL + r];
+            sU[2 * r + 1] = v[(size_t)i * BL + r];
+        } else {
+            sU[r] = u[(size_t)i * BL + r];
+        }
+    }
     const double4 bb = g.bb[i];
     const double sx = 2.0 / bb.z, sy = 2.0 / bb.w;
-    double acc[BL];
+    const int jg = lane / BS, ck = lane - jg * BS;   // contraction role (lanes < JG*BS)
+    const bool cl = jg < JG;
+    double acc[NC];
 #pragma unroll
-    for (int r = 0; r < BL; r++) acc[r] = 0.0;
-    double phi[BS], dxi[BS], deta[BS], Um[NC], Fx[NC], Fy[NC];
-    for (int q = g.vptr[i] + lane; q < g.vptr[i + 1]; q += 64) {
-        const double4 P = g.vpts[q];
-        const double xi = 2.0 * (P.x - bb.x) / bb.z - 1.0, eta = 2.0 * (P.y - bb.y) / bb.w - 1.0;
-        basis<DEG>(xi, eta, phi, dxi, deta);
-        eval_state<BS, NC>(sU, phi, Um);
-        flux<NC>(eq, Um, P.x, P.y, Fx, Fy);
+    for (int c = 0; c < NC; c++) acc[c] = 0.0;
+    __syncthreads();
+
+    // ---- volume points, 64 per chunk
+    const int v0 = g.vptr[i], v1 = g.vptr[i + 1];
+    for (int q0 = v0; q0 < v1; q0 += 64) {
+        const int nq = min(64, v1 - q0);
+        double deta[BS], Gy[NC];
+        __syncthreads();   // readers of the previous record are done
+        if (lane < nq) {
+            const double4 P = g.vpts[q0 + lane];
+            const double xi = 2.0 * (P.x - bb.x) / bb.z - 1.0, eta = 2.0 * (P.y - bb.y) / bb.w - 1.0;
+            double phi[BS], dxi[BS], Um[NC], Fx[NC];
+            basis<DEG>(xi, eta, phi, dxi, deta);
 #pragma unroll
-        for (int c = 0; c < NC; c++)
+            for (int k = 0; k < BS; k++) D[lane * BS + k] = dxi[k];
+            if constexpr (LIN) {
+                double Vm[NC];
+                eval_state2<BS, NC>(sU, phi, Um, Vm);
+                if constexpr (NC == 4) {
+                    euler_jvp(eq.gamma, Um, Vm, eq.jacobian == 0, Fx, Gy);
+                } else {
+                    double bx, by;
+                    adv_beta(eq, P.x, P.y, bx, by);
+                    Fx[0] = bx * Vm[0];
+                    Gy[0] = by * Vm[0];
+                }
+            } else {
+                eval_state<BS, NC>(sU, phi, Um);
+                flux<NC>(eq, Um, P.x, P.y, Fx, Gy);
+            }
+            const double wx = P.z * sx, wy = P.z * sy;
 #pragma unroll
-            for (int k = 0; k < BS; k++) acc[c * BS + k] += P.z * (Fx[c] * dxi[k] * sx + Fy[c] * deta[k] * sy);
+            for (int c = 0; c < NC; c++) {
+                G[lane * NC + c] = wx * Fx[c];
+                Gy[c] *= wy;
+            }
+        }
+        __syncthreads();
+        if (cl) cellop_contract<BS, NC, JG>(D, G, jg, ck, nq, acc);
+        __syncthreads();
+        if (lane < nq) {
+#pragma unroll
+            for (int k = 0; k < BS; k++) D[lane * BS + k] = deta[k];
+#pragma unroll
+            for (int c = 0; c < NC; c++) G[lane * NC + c] = Gy[c];
+        }
+        __syncthreads();
+        if (cl) cellop_contract<BS, NC, JG>(D, G, jg, ck, nq, acc);
     }
+
+    // ---- edge points, EPC edges per chunk
     const double *lr = line_rule<NQL>();
     const int e0 = g.eptr[i], ne = g.eptr[i + 1] - e0;
-    double phin[BS], Up[NC], F[NC];
-    for (int idx = lane; idx < ne * NQL; idx += 64) {
-        const int e = idx / NQL, q = idx - e * NQL;
-        const Edge E = g.edges[e0 + e];
-        const double s = 0.5 * (lr[2 * q] + 1.0);
-        const double x = E.x1 + s * (E.x2 - E.x1), y = E.y1 + s * (E.y2 - E.y1);
-        const double W = 0.5 * E.len * lr[2 * q + 1];
-        const double xi = 2.0 * (x - bb.x) / bb.z - 1.0, eta = 2.0 * (y - bb.y) / bb.w - 1.0;
-        basis<DEG>(xi, eta, phi, nullptr, nullptr);
-        eval_state<BS, NC>(sU, phi, Um);
-        outer_state<DEG, NC>(eq, g, E, u, x, y, t, Up, phin);
-        numflux<NC>(eq, Um, Up, x, y, E.nx, E.ny, F);
+    for (int eb = 0; eb < ne; eb += L::EPC) {
+        const int nec = min(L::EPC, ne - eb);
+        __syncthreads();   // readers of the previous record are done before the staging overwrites it
+        for (int idx = lane; idx < nec * SW; idx += 64) {
+            const int el = idx / SW, r = idx - el * SW;
+            const int nb = g.edges[e0 + eb + el].nb;   // -1 on a Dirichlet edge
+            double val = 0.0;
+            if (nb >= 0) {
+                // LIN: u_nb and v_nb interleaved like sU
+                if constexpr (LIN) val = (r & 1) ? v[(size_t)nb * BL + (r >> 1)] : u[(size_t)nb * BL + (r >> 1)];
+                else val = u[(size_t)nb * BL + r];
+            }
+            sh[idx] = val;
+        }
+        __syncthreads();
+        const int np = nec * NQL;
+        double phi[BS], Ge[NC];
+        if (lane < np) {
+            const int el = lane / NQL, q = lane - el * NQL;
+            const Edge E = g.edges[e0 + eb + el];
+            const double s = 0.5 * (lr[2 * q] + 1.0);
+            const double x = E.x1 + s * (E.x2 - E.x1), y = E.y1 + s * (E.y2 - E.y1);
+            const double W = 0.5 * E.len * lr[2 * q + 1];
+            const double xi = 2.0 * (x - bb.x) / bb.z - 1.0, eta = 2.0 * (y - bb.y) / bb.w - 1.0;
+            basis<DEG>(xi, eta, phi, nullptr, nullptr);
+            double Um[NC], Up[NC], Vm[NC], Vp[NC], phin[BS];
+            if constexpr (LIN) eval_state2<BS, NC>(sU, phi, Um, Vm);
+            else eval_state<BS, NC>(sU, phi, Um);
+            const double *nbc = sh + el * SW;
+            if (E.kind == 2) {
 #pragma unroll
-        for (int c = 0; c < NC; c++)
+                for (int c = 0; c < NC; c++) {
+                    Up[c] = g.bcst[(size_t)(E.bslot * NQL + q) * NC + c];
+                    Vp[c] = 0.0;
+                }
+            } else {
+                const double4 b = g.bb[E.nb];
+                const double xo = E.kind == 0 ? x : x + E.tx, yo = E.kind == 0 ? y : y + E.ty;
+                const double xin = 2.0 * (xo - b.x) / b.z - 1.0, etan = 2.0 * (yo - b.y) / b.w - 1.0;
+                if (E.kind == 0) basis<DEG>(xin, etan, phin, nullptr, nullptr);
+                else trace_basis<DEG>(eq.periodic_trace, xin, etan, phin);
+                if constexpr (LIN) eval_state2<BS, NC>(nbc, phin, Up, Vp);
+                else eval_state<BS, NC>(nbc, phin, Up);
+            }
+            double F[NC];
+            if constexpr (LIN) {
+                numflux_jvp<NC>(eq, Um, Up, Vm, Vp, x, y, E.nx, E.ny, F);
+            } else {
+                numflux<NC>(eq, Um, Up, x, y, E.nx, E.ny, F);
+            }
 #pragma unroll
-            for (int k = 0; k < BS; k++) acc[c * BS + k] -= W * phi[k] * F[c];
+            for (int c = 0; c < NC; c++) Ge[c] = -W * F[c];
+        }
+        __syncthreads();   // staged coefficients are read before the record overwrites them
+        if (lane < np) {
+#pragma unroll
+            for (int k = 0; k < BS; k++) D[lane * BS + k] = phi[k];
+#pragma unroll
+            for (int c = 0; c < NC; c++) G[lane * NC + c] = Ge[c];
+        }
+        __syncthreads();
+        if (cl) cellop_contract<BS, NC, JG>(D, G, jg, ck, np, acc);
     }
+
+    // ---- sum the JG partial sums of every output in a fixed order
+    __syncthreads();
+    if (cl) {
 #pragma unroll
-    for (int r = 0; r < BL; r++) {
-        double v = acc[r];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0) sOut[r] = v;
+        for (int c = 0; c < NC; c++) sh[jg * BL + c * BS + ck] = acc[c];
     }
     __syncthreads();
-    for (int r = lane; r < BL; r += 64) b[(size_t)i * BL + r] = sOut[r];
+    if (lane < BL) {
+        double r = 0.0;
+#pragma unroll
+        for (int q = 0; q < JG; q++) r += sh[q * BL + lane];
+        if constexpr (LIN) {
+            const int oc = lane / BS, ok = lane - oc * BS;
+            const double *Mi = mass + (size_t)i * BS * BS;
+            double m = 0.0;
+#pragma unroll
+            for (int l = 0; l < BS; l++) m = fma(Mi[l * BS + ok], sU[2 * (oc * BS + l) + 1], m);
+            r = fma(scale, r, m);
+        }
+        out[(size_t)i * BL + lane] = r;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -335,7 +515,7 @@ __global__ __launch_bounds__(64 * NC, 2) void k_jacobian(DevEq eq, Geo g, const 
                 }
                 {
                     double phin[BS];
-                    outer_state<DEG, NC>(eq, g, E, u, x, y, t, Up, phin);
+                    outer_state<DEG, NC>(eq, g, E, u, x, y, q, Up, phin);
 #pragma unroll
                     for (int k = 0; k < 16; k++) d[L::E_PHIN + k] = k < BS ? phin[k] : 0.0;
                 }
@@ -512,24 +692,56 @@ struct cpg_equation_s {
     DevBuf<double4> d_bb, d_vpts;
     DevBuf<int> d_vptr, d_eptr;
     DevBuf<Edge> d_edges;
+    DevBuf<int> d_bedges;       // Dirichlet edges (indices into d_edges)
+    DevBuf<double> bcst;        // their states at the edge points (k_bc_states)
+    int nbe = 0;
     DevBuf<double> mass;        // n x bs x bs
     DevBuf<double> normc, partial, sums;
     int npartial = 0;
-    Geo geo() const { return Geo{d_bb.p, d_vptr.p, d_vpts.p, d_eptr.p, d_edges.p}; }
+    Geo geo() const { return Geo{d_bb.p, d_vptr.p, d_vpts.p, d_eptr.p, d_edges.p, bcst.p}; }
 };
 
 namespace cpg {
 
 // Launch helpers shared with the stepper
+// Dirichlet states at time t for the kernels that follow on the same stream
+template <int DEG, int NC>
+static void launch_bc_states(const cpg_equation_s &E, double t, hipStream_t s) {
+    if (!E.nbe) return;
+    const int cnt = E.nbe * Deg<DEG>::NQL;
+    hipLaunchKernelGGL((k_bc_states<DEG, NC>), dim3(ceil_div(cnt, 256)), dim3(256), 0, s, E.eq, E.d_edges.p,
+                       E.d_bedges.p, E.nbe, t, E.bcst.p);
+    CPG_LAUNCH_CHECK();
+}
+
 template <int DEG, int NC>
 static void launch_residual(const cpg_equation_s &E, const double *u, double t, double *b, hipStream_t s) {
-    hipLaunchKernelGGL((k_residual<DEG, NC>), dim3(E.n), dim3(64), 0, s, E.eq, E.geo(), u, t, b);
+    launch_bc_states<DEG, NC>(E, t, s);
+    hipLaunchKernelGGL((k_cellop<DEG, NC, false>), dim3(E.n), dim3(64), 0, s, E.eq, E.geo(), u, nullptr, 0.0,
+                       nullptr, nullptr, nullptr, b);
     CPG_LAUNCH_CHECK();
+}
+
+// y = M v + scale J(u) v with the Dirichlet states already at the right time; scale_dev (if
+// non-null) overrides scale from device memory, so a captured launch can be replayed for
+// another Newton matrix
+template <int DEG, int NC>
+static void launch_newton_apply(const cpg_equation_s &E, const double *u, double scale, const double *scale_dev,
+                                const double *v, double *y, const int *active, hipStream_t s) {
+    hipLaunchKernelGGL((k_cellop<DEG, NC, true>), dim3(E.n), dim3(64), 0, s, E.eq, E.geo(), u, v, scale, scale_dev,
+                       active, E.mass.p, y);
+    CPG_LAUNCH_CHECK();
+}
+
+template <int DEG, int NC>
+static void launch_bc_only(const cpg_equation_s &E, double t, hipStream_t s) {
+    launch_bc_states<DEG, NC>(E, t, s);
 }
 
 template <int DEG, int NC>
 static void launch_jacobian(const cpg_equation_s &E, const double *u, double t, double scale, const double *cfac,
                             int add_mass, double *J, hipStream_t s) {
+    launch_bc_states<DEG, NC>(E, t, s);
     hipLaunchKernelGGL((k_jacobian<DEG, NC>), dim3(E.n), dim3(64 * NC), 0, s, E.eq, E.geo(), u, t, scale, cfac,
                        add_mass, E.mass.p, E.d_rowptr.p, J);
     CPG_LAUNCH_CHECK();
@@ -558,6 +770,20 @@ static void launch_jacobian(const cpg_equation_s &E, const double *u, double t, 
 
 void eq_residual(const cpg_equation_s &E, const double *u, double t, double *b, hipStream_t s) {
     CPG_DISPATCH(E, launch_residual, E, u, t, b, s);
+}
+
+// y = M v + scale J(u, t) v without assembling J
+void eq_newton_apply(const cpg_equation_s &E, const double *u, double t, double scale, const double *v, double *y,
+                     hipStream_t s) {
+    CPG_DISPATCH(E, launch_bc_only, E, t, s);
+    CPG_DISPATCH(E, launch_newton_apply, E, u, scale, nullptr, v, y, nullptr, s);
+}
+
+// the same with the scale in device memory and the Dirichlet states of the last residual,
+// Jacobian or Newton apply of E (the Krylov operator of a Newton step)
+void eq_newton_apply_dev(const cpg_equation_s &E, const double *u, const double *scale_dev, const double *v,
+                         double *y, const int *active, hipStream_t s) {
+    CPG_DISPATCH(E, launch_newton_apply, E, u, 0.0, scale_dev, v, y, active, s);
 }
 
 void eq_jacobian(const cpg_equation_s &E, const double *u, double t, double scale, int add_mass, double *J,
@@ -733,7 +959,7 @@ int cpg_equation_create(cpg_mesh M, const cpg_eq_params *p, int deg, cpg_equatio
             // Jacobian pattern (src/Jacobian.cpp:7-47) and edge records
             E->rowptr.assign(n + 1, 0);
             std::vector<Edge> edges;
-            std::vector<int> eptr(n + 1, 0);
+            std::vector<int> eptr(n + 1, 0), bedges;
             for (int i = 0; i < n; i++) {
                 E->rowptr[i] = (int)E->colind.size();
                 E->colind.push_back(i);
@@ -755,6 +981,7 @@ int cpg_equation_create(cpg_mesh M, const cpg_eq_params *p, int deg, cpg_equatio
                     Ed.ny = sg * dx / len;
                     Ed.len = std::sqrt((Ed.x2 - Ed.x1) * (Ed.x2 - Ed.x1) + (Ed.y2 - Ed.y1) * (Ed.y2 - Ed.y1));
                     Ed.slot = -1;
+                    Ed.bslot = -1;
                     if (nb >= 0) {
                         Ed.kind = 0;
                         Ed.nb = nb;
@@ -768,6 +995,8 @@ int cpg_equation_create(cpg_mesh M, const cpg_eq_params *p, int deg, cpg_equatio
                         } else {
                             Ed.kind = 2;
                             Ed.nb = -1;
+                            Ed.bslot = E->nbe++;
+                            bedges.push_back((int)edges.size());
                         }
                     }
                     if (Ed.kind != 2) {
@@ -812,6 +1041,11 @@ int cpg_equation_create(cpg_mesh M, const cpg_eq_params *p, int deg, cpg_equatio
             E->d_vpts.upload(vpts);
             E->d_eptr.upload(eptr);
             E->d_edges.upload(edges);
+            if (E->nbe) {
+                const int nql = 2 * deg <= 2 ? 2 : 2 * deg <= 4 ? 4 : 2 * deg <= 6 ? 6 : 10;
+                E->d_bedges.upload(bedges);
+                E->bcst.alloc((size_t)E->nbe * nql * E->nc);
+            }
             E->mass.alloc((size_t)n * E->bs * E->bs);
             switch (deg) {
             case 1: launch_mass<1>(*E); break;
@@ -876,6 +1110,15 @@ int cpg_assemble_jacobian(cpg_equation E, const double *u, double t, double scal
         CPG_CHECK(J->n == E->n && J->bl == E->bl && J->nnzb == E->nnzb && J->h_colind == E->colind, CPG_ERR_ARG,
                   "matrix does not have the Jacobian pattern");
         eq_jacobian(*E, u, t, scale, add_mass, J->blocks.p, S(stream));
+    });
+}
+
+int cpg_newton_apply(cpg_equation E, const double *u, double t, double scale, const double *v, double *y,
+                     void *stream) {
+    return guard([&] {
+        CPG_CHECK(E && u && v && y, CPG_ERR_ARG, "null argument");
+        CPG_CHECK(v != y && u != y, CPG_ERR_ARG, "y aliases u or v");
+        eq_newton_apply(*E, u, t, scale, v, y, S(stream));
     });
 }
 

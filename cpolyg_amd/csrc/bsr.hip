@@ -51,10 +51,10 @@ void set_stream_wpr(int elem_bytes, int bl, int wpr) {
     wpr_table().push_back({elem_bytes, bl, wpr});
 }
 
-void bsr_matvec(const cpg_bsr_s &A, const double *x, double *y, hipStream_t s) {
+void bsr_matvec(const cpg_bsr_s &A, const double *x, double *y, hipStream_t s, const int *active) {
     if (A.n == 0) return;
     StreamArgs<double> a;
-    a.S = RowStream{nullptr, A.rowptr.p, A.colind.p};
+    a.S = RowStream{nullptr, A.rowptr.p, A.colind.p, active};
     a.items = A.n;
     a.nbmax = A.nbmax;
     a.src1 = x;

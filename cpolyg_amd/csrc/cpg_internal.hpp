@@ -108,11 +108,24 @@ struct cpg_bsr_s {
 
 namespace cpg {
 // shared kernels-launchers used across translation units
-void bsr_matvec(const cpg_bsr_s &A, const double *x, double *y, hipStream_t s);
+// `active` (optional device flag): the launches do nothing when it reads 0
+void bsr_matvec(const cpg_bsr_s &A, const double *x, double *y, hipStream_t s, const int *active = nullptr);
 void vec_fill(double *x, double v, size_t n, hipStream_t s);
 // waves per block row of the row-stream kernels (rowstream.hpp), per element size and block size
 int stream_wpr(int elem_bytes, int bl);
 void set_stream_wpr(int elem_bytes, int bl, int wpr);
-// out = M^-1 in for a factored preconditioner (in and out distinct)
-void prec_apply_to(cpg_prec P, const double *in, double *out, hipStream_t s);
+// out = M^-1 in for a factored preconditioner (in and out distinct); `active` as for bsr_matvec
+// (DILU and block-Jacobi sweeps only)
+void prec_apply_to(cpg_prec P, const double *in, double *out, hipStream_t s, const int *active = nullptr);
+
+// y = A x for the Krylov solvers; `active` as for bsr_matvec
+struct LinOp {
+    void (*fn)(void *ctx, const double *x, double *y, const int *active, hipStream_t s);
+    void *ctx;
+};
+// the BSR matvec as a LinOp (ctx = const cpg_bsr_s *)
+void bsr_linop(void *ctx, const double *x, double *y, const int *active, hipStream_t s);
+// GMRES with the driver of K's mode (krylov.hip); x holds the initial guess unless x_zero
+void gmres_solve(cpg_krylov K, const LinOp &A, cpg_prec P, const double *b, double *x, bool x_zero, double *tol,
+                 int *maxit, hipStream_t s);
 }  // namespace cpg

@@ -112,6 +112,26 @@ __device__ inline void eval_state(const double *U, const double *phi, double *ou
     }
 }
 
+// traces of two coefficient vectors stored interleaved, UV[2r] = u_r, UV[2r+1] = v_r
+// (one 16-byte read per coefficient pair)
+template <int BS, int NC>
+__device__ inline void eval_state2(const double *UV, const double *phi, double *outU, double *outV) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 *p = reinterpret_cast<const d2 *>(UV);
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        double su = 0.0, sv = 0.0;
+#pragma unroll
+        for (int k = 0; k < BS; k++) {
+            const d2 w = p[c * BS + k];
+            su += w.x * phi[k];
+            sv += w.y * phi[k];
+        }
+        outU[c] = su;
+        outV[c] = sv;
+    }
+}
+
 // ---- Euler ---------------------------------------------------------------------------
 __device__ inline void euler_flux(double g, const double *U, double *Fx, double *Fy, double &c, double &u, double &v) {
     const double rho = U[0], ru = U[1], rv = U[2], rE = U[3];
@@ -287,6 +307,112 @@ __device__ inline void euler_offdiag_exact(double g, const double *Um, const dou
 #pragma unroll
         for (int c = 0; c < 4; c++)
             J[4 * r + c] = 0.5 * (Bx[4 * r + c] * nx + By[4 * r + c] * ny - (r == c ? alpha : 0.0) + (Um[r] - Up[r]) * d[c]);
+}
+
+// Euler state primitives for the flux derivatives: one reciprocal of the density
+struct EulerPrim {
+    double ir, u, v, E, k, H;   // 1/rho, velocity, specific energy, (g-1)|u|^2/2, enthalpy
+};
+__device__ inline EulerPrim euler_prim(double g, const double *U) {
+    EulerPrim p;
+    p.ir = 1.0 / U[0];
+    p.u = U[1] * p.ir;
+    p.v = U[2] * p.ir;
+    p.E = U[3] * p.ir;
+    const double usq = p.u * p.u + p.v * p.v;
+    p.k = 0.5 * (g - 1) * usq;
+    p.H = p.E + (g - 1) * (p.E - 0.5 * usq);
+    return p;
+}
+
+// (nx A_x(U) + ny A_y(U)) V from the primitives (entries of euler_fluxjac; ref as there)
+__device__ inline void euler_an_v(double g, const EulerPrim &p, const double *V, int ref, double nx, double ny,
+                                  double *f) {
+    const double gm = g - 1, u = p.u, v = p.v, uv = u * v;
+    const double r2 = -uv * V[0] + v * V[1] + u * V[2];   // row 2 of A_x = row 1 of A_y
+    const double ax1 = (p.k - u * u) * V[0] + (3 - g) * u * V[1] - gm * v * V[2] + gm * V[3];
+    const double ay2 = (p.k - v * v) * V[0] - gm * u * V[1] + (3 - g) * v * V[2] + gm * V[3];
+    const double h13 = ref ? p.H - 2 * p.k : p.H - gm * u * u;
+    const double ax3 = u * (p.k - p.H) * V[0] + h13 * V[1] - gm * uv * V[2] + g * u * V[3];
+    const double ay3 = v * (p.k - p.H) * V[0] - gm * uv * V[1] + (p.H - gm * v * v) * V[2] + g * v * V[3];
+    f[0] = nx * V[1] + ny * V[2];
+    f[1] = nx * ax1 + ny * r2;
+    f[2] = nx * r2 + ny * ay2;
+    f[3] = nx * ax3 + ny * ay3;
+}
+
+// A_x(U) V and A_y(U) V, the products with the matrices of euler_fluxjac (ref as there)
+__device__ inline void euler_jvp(double g, const double *U, const double *V, int ref, double *fx, double *fy) {
+    const EulerPrim p = euler_prim(g, U);
+    const double gm = g - 1, u = p.u, v = p.v, uv = u * v;
+    const double r2 = -uv * V[0] + v * V[1] + u * V[2];
+    fx[0] = V[1];
+    fx[1] = (p.k - u * u) * V[0] + (3 - g) * u * V[1] - gm * v * V[2] + gm * V[3];
+    fx[2] = r2;
+    fx[3] = u * (p.k - p.H) * V[0] + (ref ? p.H - 2 * p.k : p.H - gm * u * u) * V[1] - gm * uv * V[2] + g * u * V[3];
+    fy[0] = V[2];
+    fy[1] = r2;
+    fy[2] = (p.k - v * v) * V[0] - gm * u * V[1] + (3 - g) * v * V[2] + gm * V[3];
+    fy[3] = v * (p.k - p.H) * V[0] - gm * uv * V[1] + (p.H - gm * v * v) * V[2] + g * v * V[3];
+}
+
+// Derivative of the numerical flux along (Vm, Vp): Jd Vm + Jo Vp with the matrices k_jacobian
+// uses (Jd = numfluxjac(+1); Jo = euler_offdiag_exact for the exact Jacobian form, numfluxjac(-1)
+// for the reference form).  Vp = 0 on Dirichlet edges.  The LF speed is the first maximum of
+// |vn-c|, |vn|, |vn+c| of Um, then of Up, and its derivative is taken with respect to the state
+// that owns it (Euler::alphaDerivative).
+template <int NC>
+__device__ inline void numflux_jvp(const DevEq &eq, const double *Um, const double *Up, const double *Vm,
+                                   const double *Vp, double x, double y, double nx, double ny, double *f) {
+    if constexpr (NC == 4) {
+        const double g = eq.gamma;
+        const EulerPrim pm = euler_prim(g, Um), pp = euler_prim(g, Up);
+        const double c1 = sqrt(g * (g - 1) * (pm.E - 0.5 * (pm.u * pm.u + pm.v * pm.v)));
+        const double c2 = sqrt(g * (g - 1) * (pp.E - 0.5 * (pp.u * pp.u + pp.v * pp.v)));
+        const double vn1 = pm.u * nx + pm.v * ny, vn2 = pp.u * nx + pp.v * ny;
+        const double eig[6] = {vn1 - c1, vn1, vn1 + c1, vn2 - c2, vn2, vn2 + c2};
+        int idx = 0;
+        double e = eig[0], alpha = fabs(e);
+#pragma unroll
+        for (int k = 1; k < 6; k++)
+            if (alpha < fabs(eig[k])) { alpha = fabs(eig[k]); idx = k; e = eig[k]; }
+        const int side = idx >= 3, j = idx - 3 * side;
+        // d alpha / d U of the owning state
+        const double ir = side ? pp.ir : pm.ir, uu = side ? pp.u : pm.u, vv = side ? pp.v : pm.v;
+        const double E = side ? pp.E : pm.E;
+        double d[4] = {-(uu * nx + vv * ny) * ir, nx * ir, ny * ir, 0.0};
+        if (j != 1) {
+            const double usq = uu * uu + vv * vv;
+            const double s = (j == 0 ? -1.0 : 1.0) * sqrt(g * g - g) * ir / sqrt(4 * E - 2 * usq);
+            d[0] += s * (usq - E); d[1] -= s * uu; d[2] -= s * vv; d[3] += s;
+        }
+        if (e < 0) { d[0] = -d[0]; d[1] = -d[1]; d[2] = -d[2]; d[3] = -d[3]; }
+        double fm[4], fp[4], dv = 0.0;
+        if (eq.jacobian) {
+            euler_an_v(g, pm, Vm, 0, nx, ny, fm);
+            euler_an_v(g, pp, Vp, 0, nx, ny, fp);
+#pragma unroll
+            for (int c = 0; c < 4; c++) dv = fma(d[c], side ? Vp[c] : Vm[c], dv);
+        } else {
+            double Vs[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) Vs[c] = Vm[c] + Vp[c];
+            euler_an_v(g, pm, Vs, 1, nx, ny, fm);
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                fp[c] = 0.0;
+                dv = fma(d[c], Vs[c], dv);
+            }
+            if (side) dv = 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) f[c] = 0.5 * (fm[c] + fp[c] + alpha * (Vm[c] - Vp[c]) + (Um[c] - Up[c]) * dv);
+    } else {
+        double bx, by;
+        adv_beta(eq, x, y, bx, by);
+        const double bn = bx * nx + by * ny;
+        f[0] = bn > 0 ? bn * Vm[0] : bn * Vp[0];
+    }
 }
 
 // Dirichlet data

@@ -1200,13 +1200,13 @@ void factor_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
 
 // out = M^-1 in (in may equal out)
 template <class T, class XT>
-void apply_t(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s) {
+void apply_t(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s, const int *active) {
     const int bl = P.bl;
     const float *sc = P.precision == CPG_BF16 ? P.scale.p : nullptr;
     const T *F = Fp<T>(P);
     if (P.kind == CPG_PC_BJACOBI) {
         StreamArgs<XT> a;
-        a.S = RowStream{nullptr, nullptr, nullptr};
+        a.S = RowStream{nullptr, nullptr, nullptr, active};
         a.items = P.n;
         a.src1 = in;
         a.out = out;
@@ -1219,7 +1219,7 @@ void apply_t(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s) {
         if (!nr) continue;
         // y_i = x_i - sum_k L_ik y_k
         StreamArgs<double> a;
-        a.S = RowStream{P.frows.p + r0, P.fptr.p + r0, P.scol.p};
+        a.S = RowStream{P.frows.p + r0, P.fptr.p + r0, P.scol.p, active};
         a.items = nr;
         a.nbmax = P.nbmax_f;
         a.src1 = y;
@@ -1234,7 +1234,7 @@ void apply_t(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s) {
         if (!nr) continue;
         // x_i = D_i^-1 y_i - sum_j DU_ij x_j
         StreamArgs<XT> a;
-        a.S = RowStream{P.brows.p + r0, P.bptr.p + r0, P.scol.p};
+        a.S = RowStream{P.brows.p + r0, P.bptr.p + r0, P.scol.p, active};
         a.items = nr;
         a.nbmax = P.nbmax_b;
         a.src0 = y;
@@ -1247,11 +1247,11 @@ void apply_t(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s) {
 
 // DILU / block-Jacobi sweep on vectors of type XT with the factors in their stored precision
 template <class XT>
-void apply_factors(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s) {
+void apply_factors(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s, const int *active = nullptr) {
     switch (P.precision) {
-    case CPG_FP64: apply_t<double, XT>(P, in, out, s); break;
-    case CPG_FP32: apply_t<float, XT>(P, in, out, s); break;
-    default: apply_t<bf16, XT>(P, in, out, s); break;
+    case CPG_FP64: apply_t<double, XT>(P, in, out, s, active); break;
+    case CPG_FP32: apply_t<float, XT>(P, in, out, s, active); break;
+    default: apply_t<bf16, XT>(P, in, out, s, active); break;
     }
 }
 
@@ -1523,7 +1523,7 @@ int cpg_prec_apply(cpg_prec P, double *x, void *stream) {
 
 namespace cpg {
 // out = M^-1 in, in and out distinct (used by flexible GMRES for z_i = M^-1 v_i)
-void prec_apply_to(cpg_prec P, const double *in, double *out, hipStream_t s) {
+void prec_apply_to(cpg_prec P, const double *in, double *out, hipStream_t s, const int *active) {
     if (P->kind == CPG_PC_NONE || P->n == 0) {
         CPG_HIP(hipMemcpyAsync(out, in, (size_t)P->n * P->bl * sizeof(double), hipMemcpyDeviceToDevice, s));
         return;
@@ -1533,7 +1533,7 @@ void prec_apply_to(cpg_prec P, const double *in, double *out, hipStream_t s) {
         apply_inner(*P, in, out, s);
         return;
     }
-    apply_factors<double>(*P, in, out, s);
+    apply_factors<double>(*P, in, out, s, active);
 }
 }  // namespace cpg
 

@@ -39,6 +39,7 @@ struct RowStream {
     const int *rows;
     const int *bptr;
     const int *bcol;
+    const int *active = nullptr;   // device flag: when it reads 0 the launch does nothing
 };
 
 template <class T>
@@ -139,6 +140,7 @@ __global__ __launch_bounds__(64 * WPR) void k_rowstream(RowStream S, const T *__
     extern __shared__ double lds_d[];
     A *red = reinterpret_cast<A *>(lds_d);
     A *xs = red + G * BL;
+    if (S.active && !*S.active) return;
     const int w = blockIdx.x;
     const int tid = threadIdx.x;
     const int i = S.rows ? S.rows[w] : w;
@@ -215,6 +217,7 @@ __global__ __launch_bounds__(256) void k_rowstream_any(int bl, RowStream S, cons
                                                        const float *__restrict__ base32, XT *out) {
     extern __shared__ double lds_d[];
     double *xs = lds_d;
+    if (S.active && !*S.active) return;
     const int w = blockIdx.x;
     const int tid = threadIdx.x;
     const int i = S.rows ? S.rows[w] : w;

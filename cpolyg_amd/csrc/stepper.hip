@@ -26,6 +26,22 @@ double eq_newton_resid(const cpg_equation_s &E, const double *kv, const double *
                        double *out_nc, hipStream_t s);
 void krylov_timing(cpg_krylov K, bool on);
 void krylov_timing_read(cpg_krylov K, int *nm, double *msm, int *na, double *msa);
+void eq_newton_apply_dev(const cpg_equation_s &E, const double *u, const double *scale_dev, const double *v,
+                         double *y, const int *active, hipStream_t s);
+
+// the Newton matrix M - dt a J(cu) applied at the quadrature points; cu and the scale stay in
+// device memory, so captured Krylov cycles are replayed for every stage and Newton iteration
+struct NewtonOp {
+    const cpg_equation_s *E;
+    const double *u;
+    const double *scale;
+};
+void newton_linop(void *ctx, const double *x, double *y, const int *active, hipStream_t s) {
+    const auto *o = static_cast<const NewtonOp *>(ctx);
+    eq_newton_apply_dev(*o->E, o->u, o->scale, x, y, active, s);
+}
+
+__global__ void k_set_scalar(double *p, double v) { *p = v; }
 
 struct LinComb {
     const double *v[6];
@@ -173,6 +189,8 @@ struct cpg_stepper_s {
     cpg_krylov K = nullptr;
     DevBuf<double> k, cu, rhs, r, x, minv, bigR, bigK;
     DevBuf<int> slot_row;
+    DevBuf<double> opscale;    // -dt a_ss of the matrix-free Newton operator
+    cpg::NewtonOp nop{};
     // CPOLYG_VERBOSE=1 in the environment at creation: one stderr line per Newton iteration,
     // the progress the reference prints (TimeIntegration.cpp:209-264)
     bool verbose = false;
@@ -257,6 +275,10 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
                     check(cpg_prec_set_inner(S->P, params->inner_iters > 0 ? params->inner_iters : 20));
                 }
                 check(cpg_krylov_create(n, bl, params->restart, params->gmres_mode, &S->K));
+                CPG_CHECK(params->newton_operator == 0 || params->newton_operator == 1, CPG_ERR_ARG,
+                          "bad newton_operator %d", params->newton_operator);
+                S->opscale.alloc(1);
+                S->nop = cpg::NewtonOp{E, S->cu.p, S->opscale.p};
             } else if (method == CPG_STEP_IRK) {
                 CPG_CHECK(nstages == 2 || nstages == 3, CPG_ERR_UNSUPPORTED, "Radau IIA with %d stages", nstages);
                 CPG_CHECK(params->restart > 0 && params->maxit > 0 && params->newton_maxit > 0, CPG_ERR_ARG,
@@ -483,10 +505,19 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                 check_ctx(cpg_prec_factor(S->P, S->J, s), stg, it, rn);
                 st.ms_factor += ms_since(t0);
                 t0 = clk::now();
-                vec_fill(S->x.p, 0.0, N, s);
                 double tol = S->sp.rtol;
                 int maxit = S->sp.maxit;
-                check_ctx(cpg_gmres(S->K, S->J, S->P, S->r.p, S->x.p, &tol, &maxit, s), stg, it, rn);
+                // GMRES from x = 0 (Jacobian::solve, src/Jacobian.cpp:72) on the assembled matrix or
+                // on the same matrix applied at the quadrature points (the Dirichlet states the
+                // Jacobian assembly just computed are those of time ts)
+                LinOp op{bsr_linop, S->J};
+                if (S->sp.newton_operator == 1) {
+                    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, S->opscale.p, -dt * S->A[stg * ns + stg]);
+                    CPG_LAUNCH_CHECK();
+                    op = LinOp{newton_linop, &S->nop};
+                }
+                check_ctx(guard([&] { gmres_solve(S->K, op, S->P, S->r.p, S->x.p, true, &tol, &maxit, s); }), stg, it,
+                          rn);
                 st.ms_gmres += ms_since(t0);
                 st.gmres_iters += maxit;
                 st.linear_solves++;

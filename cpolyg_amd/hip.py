@@ -41,7 +41,7 @@ class SolverParams(C.Structure):
     _fields_ = [("gmres_mode", C.c_int), ("restart", C.c_int), ("rtol", C.c_double), ("maxit", C.c_int),
                 ("pc_kind", C.c_int), ("precision", C.c_int), ("newton_tol", C.c_double),
                 ("newton_maxit", C.c_int), ("inner_iters", C.c_int), ("newton_guess", C.c_int),
-                ("newton_backtrack", C.c_int)]
+                ("newton_backtrack", C.c_int), ("newton_operator", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -239,7 +239,7 @@ def eq_params(s: _spec.EquationSpec) -> EqParams:
 
 def solver_params(s: _spec.SolverSpec) -> SolverParams:
     return SolverParams(s.gmres_mode, s.restart, s.rtol, s.maxit, s.pc_kind, s.precision, s.newton_tol,
-                        s.newton_maxit, s.inner_iters, s.newton_guess, s.newton_backtrack)
+                        s.newton_maxit, s.inner_iters, s.newton_guess, s.newton_backtrack, s.newton_operator)
 
 
 class Mesh:
@@ -300,6 +300,11 @@ class Equation:
     def jacobian(self, u, t, J: Bsr, scale=1.0, add_mass=False, stream=None):
         _chk(lib().cpg_assemble_jacobian(self.h, _vp(ptr(u)), C.c_double(t), C.c_double(scale), int(add_mass), J.h,
                                          _stream(stream)))
+
+    def newton_apply(self, u, t, scale, v, y, stream=None):
+        """y = M v + scale J(u, t) v without assembling J (``cpg_newton_apply``)."""
+        _chk(lib().cpg_newton_apply(self.h, _vp(ptr(u)), C.c_double(t), C.c_double(scale), _vp(ptr(v)),
+                                    _vp(ptr(y)), _stream(stream)))
 
     def mass_matvec(self, x, y, stream=None):
         _chk(lib().cpg_mass_matvec(self.h, _vp(ptr(x)), _vp(ptr(y)), _stream(stream)))

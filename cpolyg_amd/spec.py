@@ -29,6 +29,9 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16 = 2
 
+OP_ASSEMBLED = 0     # GMRES operator: BSR matvec of the assembled Newton matrix (reference)
+OP_MATRIX_FREE = 1   # the same matrix applied at the quadrature points (cpg_newton_apply)
+
 
 @dataclasses.dataclass
 class EquationSpec:
@@ -91,6 +94,7 @@ class SolverSpec:
     inner_iters: int = 20
     newton_guess: int = 0        # 0: k_s = 0 (reference); 1: previous stage's k
     newton_backtrack: int = 0    # 1: halve updates that lead to a non-finite residual
+    newton_operator: int = OP_ASSEMBLED
 
 
 def bs_of(deg: int) -> int:

@@ -152,6 +152,14 @@ int cpg_assemble_residual(cpg_equation E, const double *u, double t, double *b, 
  * (Equation::jacobian src/Equation.cpp:193-279, operator*= and J += M) */
 int cpg_assemble_jacobian(cpg_equation E, const double *u, double t, double scale, int add_mass, cpg_bsr J,
                           void *stream);
+/* y = M v + scale * J(u, t) v: the product of v with the Newton matrix that
+ * cpg_assemble_jacobian(E, u, t, scale, add_mass = 1, ...) writes (Jacobian::operator*= and
+ * J += M, src/Jacobian.cpp:99-110, then BlockMatrix::matvec, src/BlockMatrix.cpp:28-49, as in
+ * DIRK::advance, src/TimeIntegration.cpp:127-140), formed at the quadrature points without
+ * assembling the matrix.  The per-point flux Jacobians are those of the assembled form selected
+ * by the equation's `jacobian` option, so the two agree to rounding.  y must not alias u or v. */
+int cpg_newton_apply(cpg_equation E, const double *u, double t, double scale, const double *v, double *y,
+                     void *stream);
 int cpg_mass_matvec(cpg_equation E, const double *x, double *y, void *stream);
 int cpg_mass_download(cpg_equation E, double *host_blocks);   /* n blocks bl x bl, synchronous */
 /* MeshFn::L2Norm (src/MeshFn.cpp:299-312) of u, nc values to host, synchronous */
@@ -176,6 +184,10 @@ typedef struct {
                                1: from the previous stage's k */
     int newton_backtrack;   /* 0: none (reference); 1: halve a Newton update (up to 10 times)
                                while the residual at the new iterate is non-finite */
+    int newton_operator;    /* operator of the GMRES solves (DIRK/backward Euler): 0 the BSR matvec of the
+                               assembled Newton matrix (reference, src/TimeIntegration.cpp:127-140); 1 the
+                               same matrix applied at the quadrature points (cpg_newton_apply), which agrees
+                               with it to rounding; the assembled matrix still feeds the preconditioner */
 } cpg_solver_params;
 
 typedef struct {

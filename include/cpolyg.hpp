@@ -265,6 +265,14 @@ class Equation {
         check(cpg_stream_sync(nullptr));
         return J;
     }
+    // (M + scale J(f, t)) v without assembling J: the matvec of the Newton matrix
+    // `B = jacobian(f, t); B *= scale; B += M` (src/TimeIntegration.cpp:127-129)
+    MeshFn newtonApply(const MeshFn &f, double t, double scale, const MeshFn &v) const {
+        MeshFn y(np_, deg_, nc_);
+        check(cpg_newton_apply(handle(), f.data(), t, scale, v.data(), y.data(), nullptr));
+        check(cpg_stream_sync(nullptr));
+        return y;
+    }
 
   protected:
     int deg_, nc_, np_ = 0;

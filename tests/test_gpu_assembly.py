@@ -89,6 +89,19 @@ def test_residual_and_jacobian(cuda, kind, deg, kw):
     # L2 norm
     nrm = E.l2norm(ud)
     assert rel(nrm, orc.l2norm(deg, spec.nc, u)) < 1e-12
+    # the Newton matrix applied at the quadrature points against the oracle's assembled
+    # matrix times v and against the BSR matvec of the GPU-assembled matrix
+    v = np.random.default_rng(7 + deg).standard_normal(u.shape)
+    vd = _dev(cuda, v)
+    yd = _dev(cuda, np.zeros_like(u))
+    E.newton_apply(ud, t, sc, vd, yd)
+    bl = E.bl
+    yo = np.einsum("scr,sc->sr", Bo, v.reshape(-1, bl)[orc.colind])
+    yo = np.add.reduceat(yo, orc.rowptr[:-1], axis=0).ravel()
+    assert rel(yd.cpu().numpy(), yo) < 1e-12
+    zd = _dev(cuda, np.zeros_like(u))
+    J.matvec(vd, zd)
+    assert rel(yd.cpu().numpy(), zd.cpu().numpy()) < 1e-12
 
 
 def test_mass_matvec(cuda):

@@ -4,8 +4,8 @@ import pytest
 
 import oracle as O
 from cpolyg_amd import hip, setups
-from cpolyg_amd.spec import (EquationSpec, SolverSpec, GMRES_FLEXIBLE, GMRES_LEFT_REF, PC_DILU, PC_BJACOBI,
-                             PREC_BF16, PREC_FP32, PREC_FP64, bs_of)
+from cpolyg_amd.spec import (EquationSpec, SolverSpec, GMRES_FLEXIBLE, GMRES_LEFT_REF, OP_ASSEMBLED, OP_MATRIX_FREE,
+                             PC_DILU, PC_BJACOBI, PREC_BF16, PREC_FP32, PREC_FP64, bs_of)
 from problems import rel, unit_periodic_mesh, vortex_mesh, vortex_state
 
 pytestmark = pytest.mark.gpu
@@ -38,13 +38,14 @@ def test_dirk_euler_matches_oracle(cuda, nstages, jacobian):
 
 
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32, PREC_BF16])
-def test_dirk_flexible_mixed_precision(cuda, prec):
+@pytest.mark.parametrize("op", [OP_ASSEMBLED, OP_MATRIX_FREE])
+def test_dirk_flexible_mixed_precision(cuda, prec, op):
     m = vortex_mesh(8)
     deg = 3
     spec = EquationSpec.euler(periodic_trace=1, jacobian=1)
     u = vortex_state(m, deg)
     solver = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=30, rtol=1e-10, maxit=4000, pc_kind=PC_DILU,
-                        newton_tol=1e-11, newton_maxit=40, precision=prec)
+                        newton_tol=1e-11, newton_maxit=40, precision=prec, newton_operator=op)
     dt = 0.5
     ref = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=30, rtol=1e-10, maxit=4000, pc_kind=PC_DILU,
                      newton_tol=1e-11, newton_maxit=40)
