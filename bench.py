@@ -54,6 +54,9 @@ def parse():
                    help="GMRES operator: the Newton matrix applied at the quadrature points, or the BSR matvec of "
                         "the assembled matrix")
     p.add_argument("--trace", type=int, default=1, help="periodic trace: 1 exact, 0 reference (truncated)")
+    p.add_argument("--pc-refresh", type=int, default=0, choices=[0, 1, 2],
+                   help="0: Newton matrix and preconditioner at every Newton iteration (reference); 1: once per stage; "
+                        "2: once per step (matrix-free operator only)")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-compare", dest="compare_operators", action="store_false",
                    help="skip the two runs with the other GMRES operator")
@@ -341,7 +344,7 @@ def main():
         # first update on the third stage can overshoot to negative pressure)
         return SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=args.restart, rtol=1e-10, maxit=20000,
                           pc_kind=pc, newton_tol=5e-13, newton_maxit=100, precision=prec, inner_iters=args.inner,
-                          newton_guess=1, newton_backtrack=1,
+                          newton_guess=1, newton_backtrack=1, pc_refresh=args.pc_refresh,
                           newton_operator=(OP_MATRIX_FREE if args.operator == "matrix-free" else OP_ASSEMBLED)
                           if op is None else op)
 

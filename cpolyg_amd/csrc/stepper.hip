@@ -190,6 +190,7 @@ struct cpg_stepper_s {
     DevBuf<double> k, cu, rhs, r, x, minv, bigR, bigK;
     DevBuf<int> slot_row;
     DevBuf<double> opscale;    // -dt a_ss of the matrix-free Newton operator
+    bool factored = false;     // the preconditioner holds a factor of this step (pc_refresh 2)
     cpg::NewtonOp nop{};
     // CPOLYG_VERBOSE=1 in the environment at creation: one stderr line per Newton iteration,
     // the progress the reference prints (TimeIntegration.cpp:209-264)
@@ -277,6 +278,8 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
                 check(cpg_krylov_create(n, bl, params->restart, params->gmres_mode, &S->K));
                 CPG_CHECK(params->newton_operator == 0 || params->newton_operator == 1, CPG_ERR_ARG,
                           "bad newton_operator %d", params->newton_operator);
+                CPG_CHECK(params->pc_refresh >= 0 && params->pc_refresh <= 2, CPG_ERR_ARG, "bad pc_refresh %d",
+                          params->pc_refresh);
                 S->opscale.alloc(1);
                 S->nop = cpg::NewtonOp{E, S->cu.p, S->opscale.p};
             } else if (method == CPG_STEP_IRK) {
@@ -471,6 +474,7 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
         }
         const int ns = S->ns;
         bool all_converged = true;
+        S->factored = false;
         CPG_HIP(hipMemsetAsync(k, 0, N * ns * sizeof(double), s));
         for (int stg = 0; stg < ns; stg++) {
             double *ks = k + N * stg;
@@ -497,13 +501,21 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                 CPG_CHECK(std::isfinite(rn), CPG_ERR_NONFINITE, "non-finite Newton residual (stage %d, it %d)", stg, it);
                 if (rn < S->sp.newton_tol) { converged = true; break; }
                 st.newton_iters++;
-                t0 = clk::now();
-                eq_jacobian(*S->E, S->cu.p, ts, -dt * S->A[stg * ns + stg], 1, S->J->blocks.p, s);
-                CPG_HIP(hipStreamSynchronize(s));
-                st.ms_jacobian += ms_since(t0);
-                t0 = clk::now();
-                check_ctx(cpg_prec_factor(S->P, S->J, s), stg, it, rn);
-                st.ms_factor += ms_since(t0);
+                // with the matrix-free operator the matrix only feeds the preconditioner: pc_refresh 1
+                // refreshes it at the first Newton iteration of a stage, 2 at that of the step
+                bool refresh = true;
+                if (S->sp.newton_operator == 1 && S->sp.pc_refresh == 1) refresh = it == 0;
+                if (S->sp.newton_operator == 1 && S->sp.pc_refresh == 2) refresh = it == 0 && (stg == 0 || !S->factored);
+                if (refresh) S->factored = true;
+                if (refresh) {
+                    t0 = clk::now();
+                    eq_jacobian(*S->E, S->cu.p, ts, -dt * S->A[stg * ns + stg], 1, S->J->blocks.p, s);
+                    CPG_HIP(hipStreamSynchronize(s));
+                    st.ms_jacobian += ms_since(t0);
+                    t0 = clk::now();
+                    check_ctx(cpg_prec_factor(S->P, S->J, s), stg, it, rn);
+                    st.ms_factor += ms_since(t0);
+                }
                 t0 = clk::now();
                 double tol = S->sp.rtol;
                 int maxit = S->sp.maxit;

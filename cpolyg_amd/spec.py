@@ -95,6 +95,7 @@ class SolverSpec:
     newton_guess: int = 0        # 0: k_s = 0 (reference); 1: previous stage's k
     newton_backtrack: int = 0    # 1: halve updates that lead to a non-finite residual
     newton_operator: int = OP_ASSEMBLED
+    pc_refresh: int = 0          # 1: with OP_MATRIX_FREE, refresh the preconditioner once per stage
 
 
 def bs_of(deg: int) -> int:

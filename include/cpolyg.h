@@ -188,6 +188,11 @@ typedef struct {
                                assembled Newton matrix (reference, src/TimeIntegration.cpp:127-140); 1 the
                                same matrix applied at the quadrature points (cpg_newton_apply), which agrees
                                with it to rounding; the assembled matrix still feeds the preconditioner */
+    int pc_refresh;         /* DIRK/backward Euler: 0 assemble the Newton matrix and factor the preconditioner
+                               at every Newton iteration (reference, src/TimeIntegration.cpp:127-136); with
+                               newton_operator = 1 only: 1 at the first Newton iteration of each stage, 2 at
+                               the first Newton iteration of the step (the Newton iteration itself is
+                               unchanged: only the preconditioner lags) */
 } cpg_solver_params;
 
 typedef struct {

@@ -58,6 +58,26 @@ def test_dirk_flexible_mixed_precision(cuda, prec, op):
     assert rel(outd.cpu().numpy(), uo) < 1e-9
 
 
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_BF16])
+def test_dirk_stage_lagged_preconditioner(cuda, prec):
+    """Matrix-free operator with the preconditioner refreshed once per stage: the Newton
+    iteration is unchanged, so the step matches the oracle's (which refreshes every iteration)."""
+    m = vortex_mesh(8)
+    deg = 3
+    spec = EquationSpec.euler(periodic_trace=1, jacobian=1)
+    u = vortex_state(m, deg)
+    kw = dict(gmres_mode=GMRES_FLEXIBLE, restart=30, rtol=1e-10, maxit=4000, pc_kind=PC_DILU, newton_tol=1e-11,
+              newton_maxit=40)
+    uo, st = O.Oracle(m).dirk_step(spec, deg, 3, u, 0.5, 0.0, SolverSpec(**kw))
+    S = hip.Stepper(hip.Equation(hip.Mesh(m), spec, deg), 0, 3,
+                    SolverSpec(**kw, precision=prec, newton_operator=OP_MATRIX_FREE, pc_refresh=1))
+    ud, outd = _dev(cuda, u), _dev(cuda, np.zeros_like(u))
+    gs = S.advance(ud, 0.5, 0.0, outd)
+    assert gs.newton_iters == st.newton_iters
+    assert gs.last_resid < 1e-11
+    assert rel(outd.cpu().numpy(), uo) < 1e-9
+
+
 def test_backward_euler_advection_bjacobi(cuda):
     m = unit_periodic_mesh(8)
     deg = 2
