@@ -152,15 +152,19 @@ __device__ inline void outer_state(const DevEq &eq, const Geo &g, const Edge &E,
 // euler_offdiag_exact or numfluxjac(-1) for the neighbour's), applied to the traces of v
 // instead of to basis functions, so the two agree to rounding in either Jacobian form.
 //
-// One wavefront per cell.  Lanes first take quadrature points: each evaluates the basis,
-// the traces of u (and v) and the pointwise flux (or its derivative along v), and writes an
-// entry (test-function values D[k], flux values G[c]) per point to an LDS record; a volume
-// point gives two entries (d_xi phi_k with W sx F_x,c, then d_eta phi_k with W sy F_y,c, one
-// record fill each), an edge point one (phi_k with -W Fhat_c).  Then the record is contracted:
-// lane (jg, k) keeps NC accumulators out[c][k] over the entries j = jg mod JG (one LDS read of
-// D and NC/2 16-byte reads of G per entry), and the JG partial sums of every output are added
-// in a fixed order at the end.  The neighbours' coefficients of an edge chunk are staged in LDS
-// with coalesced loads.
+// One wavefront per cell.  Lanes first take quadrature points: each evaluates the basis, the
+// trace of its input and the pointwise flux (residual) or flux derivative (operator), and
+// writes an entry (test-function values D[k], flux values G[c]) per point to an LDS record; a
+// volume point gives two entries (d_xi phi_k with W sx F_x,c, then d_eta phi_k with W sy F_y,c,
+// one record fill each), an edge point one (phi_k with -W Fhat_c).  Then the record is
+// contracted: lane (jg, k) keeps NC accumulators out[c][k] over the entries j = jg mod JG (one
+// LDS read of D and NC/2 16-byte reads of G per entry), and the JG partial sums of every output
+// are added in a fixed order at the end.  The neighbours' coefficients of an edge chunk are
+// staged in LDS with coalesced loads.
+//
+// The operator (LIN) reads what depends on u alone from the point data k_linprep stored for
+// this u (Euler; advection needs none), so its inner loops hold only traces of v: no divisions,
+// square roots or traces of u per GMRES iteration.
 template <int DEG, int NC, bool LIN>
 struct CellOpLayout {
     static constexpr int BS = Deg<DEG>::BS, BL = NC * BS, NQL = Deg<DEG>::NQL;
@@ -168,7 +172,7 @@ struct CellOpLayout {
     static constexpr int NE = 64;                     // record entries
     static constexpr int REC = NE * (BS + NC);        // D[NE][BS], then G[NE][NC]
     static constexpr int JG = 64 / BS;                // entry groups of the contraction
-    static constexpr int SW = (LIN ? 2 : 1) * BL;     // staged doubles per edge: u_nb (, v_nb)
+    static constexpr int SW = BL;                     // staged doubles per edge (the input's coefficients)
     static constexpr int STG = EPC * SW;
     static constexpr int SH0 = REC > STG ? REC : STG;
     static constexpr int SH = SH0 > JG * BL ? SH0 : JG * BL;
@@ -185,29 +189,31 @@ __device__ inline void cellop_contract(const double *D, const double *G, int jg,
     }
 }
 
+// Point data of the Euler Newton operator: nv volume points (LPV fields each) and nep edge
+// points (LPE fields each), stored field-major so that a wavefront's loads are contiguous
+struct LinData {
+    const double *vol;   // vol[f * nv + q]
+    const double *edg;   // edg[f * nep + e * NQL + q] for edge e (global edge index) and point q
+    int nv, nep;
+};
+
+// out = b(x) (residual, LIN = false; x = u) or out = M x + scale J(u) x (LIN = true; x = v, the
+// u-dependent part from `ld`)
 template <int DEG, int NC, bool LIN>
-__global__ __launch_bounds__(64, 2) void k_cellop(DevEq eq, Geo g, const double *__restrict__ u,
-                                                  const double *__restrict__ v, double scale,
-                                                  const double *__restrict__ scale_dev, const int *__restrict__ active,
-                                                  const double *__restrict__ mass, double *__restrict__ out) {
+__global__ __launch_bounds__(64, DEG <= 3 ? 3 : 2) void k_cellop(DevEq eq, Geo g, const double *__restrict__ x, LinData ld,
+                                                  double scale, const double *__restrict__ scale_dev,
+                                                  const int *__restrict__ active, const double *__restrict__ mass,
+                                                  double *__restrict__ out) {
     using L = CellOpLayout<DEG, NC, LIN>;
     if (active && !*active) return;   // a converged Krylov cycle replayed from a graph
     if (scale_dev) scale = *scale_dev;
     constexpr int BS = L::BS, BL = L::BL, NQL = L::NQL, SW = L::SW, JG = L::JG;
-    // own coefficients: u alone, or u and v interleaved (sU[2r] = u_r, sU[2r+1] = v_r)
-    __shared__ __align__(16) double sU[LIN ? 2 * BL : BL];
+    __shared__ __align__(16) double sX[BL];   // the cell's coefficients of x
     __shared__ __align__(16) double sh[L::SH];
     double *D = sh;
     double *G = sh + L::NE * BS;
     const int i = blockIdx.x, lane = threadIdx.x;
-    for (int r = lane; r < BL; r += 64) {
-        if constexpr (LIN) {
-            sU[2 * r] = u[(size_t)i * BL + r];
-            sU[2 * r + 1] = v[(size_t)i * BL + r];
-        } else {
-            sU[r] = u[(size_t)i * BL + r];
-        }
-    }
+    for (int r = lane; r < BL; r += 64) sX[r] = x[(size_t)i * BL + r];
     const double4 bb = g.bb[i];
     const double sx = 2.0 / bb.z, sy = 2.0 / bb.w;
     const int jg = lane / BS, ck = lane - jg * BS;   // contraction role (lanes < JG*BS)
@@ -226,24 +232,25 @@ __global__ __launch_bounds__(64, 2) void k_cellop(DevEq eq, Geo g, const double 
         if (lane < nq) {
             const double4 P = g.vpts[q0 + lane];
             const double xi = 2.0 * (P.x - bb.x) / bb.z - 1.0, eta = 2.0 * (P.y - bb.y) / bb.w - 1.0;
-            double phi[BS], dxi[BS], Um[NC], Fx[NC];
+            double phi[BS], dxi[BS], X[NC], Fx[NC];
             basis<DEG>(xi, eta, phi, dxi, deta);
 #pragma unroll
             for (int k = 0; k < BS; k++) D[lane * BS + k] = dxi[k];
+            eval_state<BS, NC>(sX, phi, X);
             if constexpr (LIN) {
-                double Vm[NC];
-                eval_state2<BS, NC>(sU, phi, Um, Vm);
                 if constexpr (NC == 4) {
-                    euler_jvp(eq.gamma, Um, Vm, eq.jacobian == 0, Fx, Gy);
+                    double p[LPV];
+#pragma unroll
+                    for (int f = 0; f < LPV; f++) p[f] = ld.vol[(size_t)f * ld.nv + q0 + lane];
+                    euler_jvp_p(eq.gamma, p, X, eq.jacobian == 0, Fx, Gy);
                 } else {
                     double bx, by;
                     adv_beta(eq, P.x, P.y, bx, by);
-                    Fx[0] = bx * Vm[0];
-                    Gy[0] = by * Vm[0];
+                    Fx[0] = bx * X[0];
+                    Gy[0] = by * X[0];
                 }
             } else {
-                eval_state<BS, NC>(sU, phi, Um);
-                flux<NC>(eq, Um, P.x, P.y, Fx, Gy);
+                flux<NC>(eq, X, P.x, P.y, Fx, Gy);
             }
             const double wx = P.z * sx, wy = P.z * sy;
 #pragma unroll
@@ -274,13 +281,7 @@ __global__ __launch_bounds__(64, 2) void k_cellop(DevEq eq, Geo g, const double 
         for (int idx = lane; idx < nec * SW; idx += 64) {
             const int el = idx / SW, r = idx - el * SW;
             const int nb = g.edges[e0 + eb + el].nb;   // -1 on a Dirichlet edge
-            double val = 0.0;
-            if (nb >= 0) {
-                // LIN: u_nb and v_nb interleaved like sU
-                if constexpr (LIN) val = (r & 1) ? v[(size_t)nb * BL + (r >> 1)] : u[(size_t)nb * BL + (r >> 1)];
-                else val = u[(size_t)nb * BL + r];
-            }
-            sh[idx] = val;
+            sh[idx] = nb >= 0 ? x[(size_t)nb * BL + r] : 0.0;
         }
         __syncthreads();
         const int np = nec * NQL;
@@ -289,34 +290,38 @@ __global__ __launch_bounds__(64, 2) void k_cellop(DevEq eq, Geo g, const double 
             const int el = lane / NQL, q = lane - el * NQL;
             const Edge E = g.edges[e0 + eb + el];
             const double s = 0.5 * (lr[2 * q] + 1.0);
-            const double x = E.x1 + s * (E.x2 - E.x1), y = E.y1 + s * (E.y2 - E.y1);
+            const double px = E.x1 + s * (E.x2 - E.x1), py = E.y1 + s * (E.y2 - E.y1);
             const double W = 0.5 * E.len * lr[2 * q + 1];
-            const double xi = 2.0 * (x - bb.x) / bb.z - 1.0, eta = 2.0 * (y - bb.y) / bb.w - 1.0;
+            const double xi = 2.0 * (px - bb.x) / bb.z - 1.0, eta = 2.0 * (py - bb.y) / bb.w - 1.0;
             basis<DEG>(xi, eta, phi, nullptr, nullptr);
-            double Um[NC], Up[NC], Vm[NC], Vp[NC], phin[BS];
-            if constexpr (LIN) eval_state2<BS, NC>(sU, phi, Um, Vm);
-            else eval_state<BS, NC>(sU, phi, Um);
-            const double *nbc = sh + el * SW;
+            double Xm[NC], Xp[NC];
+            eval_state<BS, NC>(sX, phi, Xm);
             if (E.kind == 2) {
+                // Dirichlet: the outer state is data (k_bc_states), independent of u
 #pragma unroll
-                for (int c = 0; c < NC; c++) {
-                    Up[c] = g.bcst[(size_t)(E.bslot * NQL + q) * NC + c];
-                    Vp[c] = 0.0;
-                }
+                for (int c = 0; c < NC; c++) Xp[c] = LIN ? 0.0 : g.bcst[(size_t)(E.bslot * NQL + q) * NC + c];
             } else {
+                double phin[BS];
                 const double4 b = g.bb[E.nb];
-                const double xo = E.kind == 0 ? x : x + E.tx, yo = E.kind == 0 ? y : y + E.ty;
+                const double xo = E.kind == 0 ? px : px + E.tx, yo = E.kind == 0 ? py : py + E.ty;
                 const double xin = 2.0 * (xo - b.x) / b.z - 1.0, etan = 2.0 * (yo - b.y) / b.w - 1.0;
                 if (E.kind == 0) basis<DEG>(xin, etan, phin, nullptr, nullptr);
                 else trace_basis<DEG>(eq.periodic_trace, xin, etan, phin);
-                if constexpr (LIN) eval_state2<BS, NC>(nbc, phin, Up, Vp);
-                else eval_state<BS, NC>(nbc, phin, Up);
+                eval_state<BS, NC>(sh + el * SW, phin, Xp);
             }
             double F[NC];
             if constexpr (LIN) {
-                numflux_jvp<NC>(eq, Um, Up, Vm, Vp, x, y, E.nx, E.ny, F);
+                if constexpr (NC == 4) {
+                    double rec[LPE];
+                    const size_t ep = (size_t)(e0 + eb + el) * NQL + q;
+#pragma unroll
+                    for (int f = 0; f < LPE; f++) rec[f] = ld.edg[(size_t)f * ld.nep + ep];
+                    euler_numflux_jvp_p(eq, rec, Xm, Xp, E.nx, E.ny, F);
+                } else {
+                    F[0] = adv_numflux_jvp(eq, Xm[0], Xp[0], px, py, E.nx, E.ny);
+                }
             } else {
-                numflux<NC>(eq, Um, Up, x, y, E.nx, E.ny, F);
+                numflux<NC>(eq, Xm, Xp, px, py, E.nx, E.ny, F);
             }
 #pragma unroll
             for (int c = 0; c < NC; c++) Ge[c] = -W * F[c];
@@ -348,10 +353,74 @@ __global__ __launch_bounds__(64, 2) void k_cellop(DevEq eq, Geo g, const double 
             const double *Mi = mass + (size_t)i * BS * BS;
             double m = 0.0;
 #pragma unroll
-            for (int l = 0; l < BS; l++) m = fma(Mi[l * BS + ok], sU[2 * (oc * BS + l) + 1], m);
+            for (int l = 0; l < BS; l++) m = fma(Mi[l * BS + ok], sX[oc * BS + l], m);
             r = fma(scale, r, m);
         }
         out[(size_t)i * BL + lane] = r;
+    }
+}
+
+// Point data of the Euler Newton operator for the state u (the LinData layout): one
+// wavefront per cell, lanes over quadrature points, u's traces from the cell's and the
+// neighbours' coefficients (staged in LDS) or the Dirichlet states
+template <int DEG>
+__global__ __launch_bounds__(64) void k_linprep(DevEq eq, Geo g, const double *__restrict__ u, double *__restrict__ vol,
+                                                double *__restrict__ edg, int nv, int nep) {
+    constexpr int NC = 4, BS = Deg<DEG>::BS, BL = NC * BS, NQL = Deg<DEG>::NQL, EPC = 64 / NQL;
+    __shared__ double sU[BL];
+    __shared__ double stg[EPC * BL];
+    const int i = blockIdx.x, lane = threadIdx.x;
+    for (int r = lane; r < BL; r += 64) sU[r] = u[(size_t)i * BL + r];
+    const double4 bb = g.bb[i];
+    __syncthreads();
+    const int v0 = g.vptr[i], v1 = g.vptr[i + 1];
+    for (int q = v0 + lane; q < v1; q += 64) {
+        const double4 P = g.vpts[q];
+        const double xi = 2.0 * (P.x - bb.x) / bb.z - 1.0, eta = 2.0 * (P.y - bb.y) / bb.w - 1.0;
+        double phi[BS], U[NC], p[LPV];
+        basis<DEG>(xi, eta, phi, nullptr, nullptr);
+        eval_state<BS, NC>(sU, phi, U);
+        euler_prim4(eq.gamma, U, p);
+#pragma unroll
+        for (int f = 0; f < LPV; f++) vol[(size_t)f * nv + q] = p[f];
+    }
+    const double *lr = line_rule<NQL>();
+    const int e0 = g.eptr[i], ne = g.eptr[i + 1] - e0;
+    for (int eb = 0; eb < ne; eb += EPC) {
+        const int nec = min(EPC, ne - eb);
+        __syncthreads();
+        for (int idx = lane; idx < nec * BL; idx += 64) {
+            const int el = idx / BL, r = idx - el * BL;
+            const int nb = g.edges[e0 + eb + el].nb;
+            stg[idx] = nb >= 0 ? u[(size_t)nb * BL + r] : 0.0;
+        }
+        __syncthreads();
+        if (lane < nec * NQL) {
+            const int el = lane / NQL, q = lane - el * NQL;
+            const Edge E = g.edges[e0 + eb + el];
+            const double s = 0.5 * (lr[2 * q] + 1.0);
+            const double px = E.x1 + s * (E.x2 - E.x1), py = E.y1 + s * (E.y2 - E.y1);
+            const double xi = 2.0 * (px - bb.x) / bb.z - 1.0, eta = 2.0 * (py - bb.y) / bb.w - 1.0;
+            double phi[BS], Um[NC], Up[NC], rec[LPE];
+            basis<DEG>(xi, eta, phi, nullptr, nullptr);
+            eval_state<BS, NC>(sU, phi, Um);
+            if (E.kind == 2) {
+#pragma unroll
+                for (int c = 0; c < NC; c++) Up[c] = g.bcst[(size_t)(E.bslot * NQL + q) * NC + c];
+            } else {
+                const double4 b = g.bb[E.nb];
+                const double xo = E.kind == 0 ? px : px + E.tx, yo = E.kind == 0 ? py : py + E.ty;
+                const double xin = 2.0 * (xo - b.x) / b.z - 1.0, etan = 2.0 * (yo - b.y) / b.w - 1.0;
+                double phin[BS];
+                if (E.kind == 0) basis<DEG>(xin, etan, phin, nullptr, nullptr);
+                else trace_basis<DEG>(eq.periodic_trace, xin, etan, phin);
+                eval_state<BS, NC>(stg + el * BL, phin, Up);
+            }
+            euler_edge_prep(eq.gamma, Um, Up, E.nx, E.ny, rec);
+            const size_t ep = (size_t)(e0 + eb + el) * NQL + q;
+#pragma unroll
+            for (int f = 0; f < LPE; f++) edg[(size_t)f * nep + ep] = rec[f];
+        }
     }
 }
 
@@ -695,6 +764,9 @@ struct cpg_equation_s {
     DevBuf<int> d_bedges;       // Dirichlet edges (indices into d_edges)
     DevBuf<double> bcst;        // their states at the edge points (k_bc_states)
     int nbe = 0;
+    int nvpts = 0, nedges = 0;  // volume quadrature points, polygon edges
+    DevBuf<double> lin_vol, lin_edg;   // point data of the Newton operator (k_linprep; Euler)
+    bool prepared = false;
     DevBuf<double> mass;        // n x bs x bs
     DevBuf<double> normc, partial, sums;
     int npartial = 0;
@@ -717,18 +789,34 @@ static void launch_bc_states(const cpg_equation_s &E, double t, hipStream_t s) {
 template <int DEG, int NC>
 static void launch_residual(const cpg_equation_s &E, const double *u, double t, double *b, hipStream_t s) {
     launch_bc_states<DEG, NC>(E, t, s);
-    hipLaunchKernelGGL((k_cellop<DEG, NC, false>), dim3(E.n), dim3(64), 0, s, E.eq, E.geo(), u, nullptr, 0.0,
+    hipLaunchKernelGGL((k_cellop<DEG, NC, false>), dim3(E.n), dim3(64), 0, s, E.eq, E.geo(), u, LinData{}, 0.0,
                        nullptr, nullptr, nullptr, b);
     CPG_LAUNCH_CHECK();
 }
 
-// y = M v + scale J(u) v with the Dirichlet states already at the right time; scale_dev (if
-// non-null) overrides scale from device memory, so a captured launch can be replayed for
-// another Newton matrix
+// point data of the Newton operator for u (Euler; advection has none), with the Dirichlet
+// states already at the right time
 template <int DEG, int NC>
-static void launch_newton_apply(const cpg_equation_s &E, const double *u, double scale, const double *scale_dev,
-                                const double *v, double *y, const int *active, hipStream_t s) {
-    hipLaunchKernelGGL((k_cellop<DEG, NC, true>), dim3(E.n), dim3(64), 0, s, E.eq, E.geo(), u, v, scale, scale_dev,
+static void launch_newton_prepare(cpg_equation_s &E, const double *u, hipStream_t s) {
+    if constexpr (NC == 4) {
+        const size_t nep = (size_t)E.nedges * Deg<DEG>::NQL;
+        if (!E.lin_vol.p) {
+            E.lin_vol.alloc((size_t)LPV * E.nvpts);
+            E.lin_edg.alloc((size_t)LPE * nep);
+        }
+        hipLaunchKernelGGL((k_linprep<DEG>), dim3(E.n), dim3(64), 0, s, E.eq, E.geo(), u, E.lin_vol.p, E.lin_edg.p,
+                           E.nvpts, (int)nep);
+        CPG_LAUNCH_CHECK();
+    }
+}
+
+// y = M v + scale J(u) v for the u of the last prepare; scale_dev (if non-null) overrides scale
+// from device memory, so a captured launch can be replayed for another Newton matrix
+template <int DEG, int NC>
+static void launch_newton_apply(const cpg_equation_s &E, double scale, const double *scale_dev, const double *v,
+                                double *y, const int *active, hipStream_t s) {
+    const LinData ld{E.lin_vol.p, E.lin_edg.p, E.nvpts, E.nedges * Deg<DEG>::NQL};
+    hipLaunchKernelGGL((k_cellop<DEG, NC, true>), dim3(E.n), dim3(64), 0, s, E.eq, E.geo(), v, ld, scale, scale_dev,
                        active, E.mass.p, y);
     CPG_LAUNCH_CHECK();
 }
@@ -772,18 +860,25 @@ void eq_residual(const cpg_equation_s &E, const double *u, double t, double *b, 
     CPG_DISPATCH(E, launch_residual, E, u, t, b, s);
 }
 
-// y = M v + scale J(u, t) v without assembling J
-void eq_newton_apply(const cpg_equation_s &E, const double *u, double t, double scale, const double *v, double *y,
-                     hipStream_t s) {
+// the u-dependent point data of the Newton operator at (u, t)
+void eq_newton_prepare(cpg_equation_s &E, const double *u, double t, hipStream_t s) {
     CPG_DISPATCH(E, launch_bc_only, E, t, s);
-    CPG_DISPATCH(E, launch_newton_apply, E, u, scale, nullptr, v, y, nullptr, s);
+    CPG_DISPATCH(E, launch_newton_prepare, E, u, s);
 }
 
-// the same with the scale in device memory and the Dirichlet states of the last residual,
-// Jacobian or Newton apply of E (the Krylov operator of a Newton step)
-void eq_newton_apply_dev(const cpg_equation_s &E, const double *u, const double *scale_dev, const double *v,
-                         double *y, const int *active, hipStream_t s) {
-    CPG_DISPATCH(E, launch_newton_apply, E, u, 0.0, scale_dev, v, y, active, s);
+// y = M v + scale J(u, t) v without assembling J
+void eq_newton_apply(cpg_equation_s &E, const double *u, double t, double scale, const double *v, double *y,
+                     hipStream_t s) {
+    eq_newton_prepare(E, u, t, s);
+    E.prepared = true;
+    CPG_DISPATCH(E, launch_newton_apply, E, scale, nullptr, v, y, nullptr, s);
+}
+
+// the same for the (u, t) of the last eq_newton_prepare, with the scale in device memory (the
+// Krylov operator of a Newton step)
+void eq_newton_apply_dev(const cpg_equation_s &E, const double *scale_dev, const double *v, double *y,
+                         const int *active, hipStream_t s, double scale) {
+    CPG_DISPATCH(E, launch_newton_apply, E, scale, scale_dev, v, y, active, s);
 }
 
 void eq_jacobian(const cpg_equation_s &E, const double *u, double t, double scale, int add_mass, double *J,
@@ -1041,6 +1136,8 @@ int cpg_equation_create(cpg_mesh M, const cpg_eq_params *p, int deg, cpg_equatio
             E->d_vpts.upload(vpts);
             E->d_eptr.upload(eptr);
             E->d_edges.upload(edges);
+            E->nvpts = (int)vpts.size();
+            E->nedges = (int)edges.size();
             if (E->nbe) {
                 const int nql = 2 * deg <= 2 ? 2 : 2 * deg <= 4 ? 4 : 2 * deg <= 6 ? 6 : 10;
                 E->d_bedges.upload(bedges);
@@ -1119,6 +1216,22 @@ int cpg_newton_apply(cpg_equation E, const double *u, double t, double scale, co
         CPG_CHECK(E && u && v && y, CPG_ERR_ARG, "null argument");
         CPG_CHECK(v != y && u != y, CPG_ERR_ARG, "y aliases u or v");
         eq_newton_apply(*E, u, t, scale, v, y, S(stream));
+    });
+}
+
+int cpg_newton_prepare(cpg_equation E, const double *u, double t, void *stream) {
+    return guard([&] {
+        CPG_CHECK(E && u, CPG_ERR_ARG, "null argument");
+        eq_newton_prepare(*E, u, t, S(stream));
+        E->prepared = true;
+    });
+}
+
+int cpg_newton_apply_prepared(cpg_equation E, double scale, const double *v, double *y, void *stream) {
+    return guard([&] {
+        CPG_CHECK(E && v && y && v != y, CPG_ERR_ARG, "bad argument");
+        CPG_CHECK(E->prepared || E->nc != 4, CPG_ERR_ARG, "cpg_newton_apply_prepared before cpg_newton_prepare");
+        eq_newton_apply_dev(*E, nullptr, v, y, nullptr, S(stream), scale);
     });
 }
 

@@ -325,94 +325,122 @@ __device__ inline EulerPrim euler_prim(double g, const double *U) {
     return p;
 }
 
-// (nx A_x(U) + ny A_y(U)) V from the primitives (entries of euler_fluxjac; ref as there)
-__device__ inline void euler_an_v(double g, const EulerPrim &p, const double *V, int ref, double nx, double ny,
-                                  double *f) {
-    const double gm = g - 1, u = p.u, v = p.v, uv = u * v;
-    const double r2 = -uv * V[0] + v * V[1] + u * V[2];   // row 2 of A_x = row 1 of A_y
-    const double ax1 = (p.k - u * u) * V[0] + (3 - g) * u * V[1] - gm * v * V[2] + gm * V[3];
-    const double ay2 = (p.k - v * v) * V[0] - gm * u * V[1] + (3 - g) * v * V[2] + gm * V[3];
-    const double h13 = ref ? p.H - 2 * p.k : p.H - gm * u * u;
-    const double ax3 = u * (p.k - p.H) * V[0] + h13 * V[1] - gm * uv * V[2] + g * u * V[3];
-    const double ay3 = v * (p.k - p.H) * V[0] - gm * uv * V[1] + (p.H - gm * v * v) * V[2] + g * v * V[3];
-    f[0] = nx * V[1] + ny * V[2];
-    f[1] = nx * ax1 + ny * r2;
-    f[2] = nx * r2 + ny * ay2;
-    f[3] = nx * ax3 + ny * ay3;
+// ---- state-dependent data of the Euler Newton operator at the quadrature points ---------
+// The Newton matrix M - dt a J(u) is applied many times for one u (every GMRES iteration of
+// a Newton step), so what depends on u alone is evaluated once per quadrature point
+// (k_linprep) and the operator kernel combines it with the traces of v:
+//   volume point: the primitives p = (u, v, k, H) of U, with k = (gamma-1)|u|^2/2 and H the
+//                 enthalpy (the entries of euler_fluxjac are polynomials in them);
+//   edge point:   p(U-), p(U+), U- - U+, the LF speed alpha, and alpha's derivative with respect
+//                 to U- and to U+ (one of the two is zero: alpha belongs to one state).
+constexpr int LPV = 4;    // doubles per volume point
+constexpr int LPE = 21;   // doubles per edge point: pm[4] pp[4] dU[4] alpha dm[4] dp[4]
+
+__device__ inline void euler_prim4(double g, const double *U, double *p) {
+    const EulerPrim q = euler_prim(g, U);
+    p[0] = q.u;
+    p[1] = q.v;
+    p[2] = q.k;
+    p[3] = q.H;
 }
 
-// A_x(U) V and A_y(U) V, the products with the matrices of euler_fluxjac (ref as there)
-__device__ inline void euler_jvp(double g, const double *U, const double *V, int ref, double *fx, double *fy) {
-    const EulerPrim p = euler_prim(g, U);
-    const double gm = g - 1, u = p.u, v = p.v, uv = u * v;
+// A_x(U) V and A_y(U) V from p = (u, v, k, H): the products with the matrices of
+// euler_fluxjac (ref: the reference's energy-row entry H - (gamma-1)|u|^2 of A_x)
+__device__ inline void euler_jvp_p(double g, const double *p, const double *V, int ref, double *fx, double *fy) {
+    const double gm = g - 1, u = p[0], v = p[1], k = p[2], H = p[3], uv = u * v;
     const double r2 = -uv * V[0] + v * V[1] + u * V[2];
     fx[0] = V[1];
-    fx[1] = (p.k - u * u) * V[0] + (3 - g) * u * V[1] - gm * v * V[2] + gm * V[3];
+    fx[1] = (k - u * u) * V[0] + (3 - g) * u * V[1] - gm * v * V[2] + gm * V[3];
     fx[2] = r2;
-    fx[3] = u * (p.k - p.H) * V[0] + (ref ? p.H - 2 * p.k : p.H - gm * u * u) * V[1] - gm * uv * V[2] + g * u * V[3];
+    fx[3] = u * (k - H) * V[0] + (ref ? H - 2 * k : H - gm * u * u) * V[1] - gm * uv * V[2] + g * u * V[3];
     fy[0] = V[2];
     fy[1] = r2;
-    fy[2] = (p.k - v * v) * V[0] - gm * u * V[1] + (3 - g) * v * V[2] + gm * V[3];
-    fy[3] = v * (p.k - p.H) * V[0] - gm * uv * V[1] + (p.H - gm * v * v) * V[2] + g * v * V[3];
+    fy[2] = (k - v * v) * V[0] - gm * u * V[1] + (3 - g) * v * V[2] + gm * V[3];
+    fy[3] = v * (k - H) * V[0] - gm * uv * V[1] + (H - gm * v * v) * V[2] + g * v * V[3];
 }
 
-// Derivative of the numerical flux along (Vm, Vp): Jd Vm + Jo Vp with the matrices k_jacobian
-// uses (Jd = numfluxjac(+1); Jo = euler_offdiag_exact for the exact Jacobian form, numfluxjac(-1)
-// for the reference form).  Vp = 0 on Dirichlet edges.  The LF speed is the first maximum of
-// |vn-c|, |vn|, |vn+c| of Um, then of Up, and its derivative is taken with respect to the state
-// that owns it (Euler::alphaDerivative).
-template <int NC>
-__device__ inline void numflux_jvp(const DevEq &eq, const double *Um, const double *Up, const double *Vm,
-                                   const double *Vp, double x, double y, double nx, double ny, double *f) {
-    if constexpr (NC == 4) {
-        const double g = eq.gamma;
-        const EulerPrim pm = euler_prim(g, Um), pp = euler_prim(g, Up);
-        const double c1 = sqrt(g * (g - 1) * (pm.E - 0.5 * (pm.u * pm.u + pm.v * pm.v)));
-        const double c2 = sqrt(g * (g - 1) * (pp.E - 0.5 * (pp.u * pp.u + pp.v * pp.v)));
-        const double vn1 = pm.u * nx + pm.v * ny, vn2 = pp.u * nx + pp.v * ny;
-        const double eig[6] = {vn1 - c1, vn1, vn1 + c1, vn2 - c2, vn2, vn2 + c2};
-        int idx = 0;
-        double e = eig[0], alpha = fabs(e);
+// (nx A_x + ny A_y) V from p
+__device__ inline void euler_an_v_p(double g, const double *p, const double *V, int ref, double nx, double ny,
+                                    double *f) {
+    double fx[4], fy[4];
+    euler_jvp_p(g, p, V, ref, fx, fy);
 #pragma unroll
-        for (int k = 1; k < 6; k++)
-            if (alpha < fabs(eig[k])) { alpha = fabs(eig[k]); idx = k; e = eig[k]; }
-        const int side = idx >= 3, j = idx - 3 * side;
-        // d alpha / d U of the owning state
-        const double ir = side ? pp.ir : pm.ir, uu = side ? pp.u : pm.u, vv = side ? pp.v : pm.v;
-        const double E = side ? pp.E : pm.E;
-        double d[4] = {-(uu * nx + vv * ny) * ir, nx * ir, ny * ir, 0.0};
-        if (j != 1) {
-            const double usq = uu * uu + vv * vv;
-            const double s = (j == 0 ? -1.0 : 1.0) * sqrt(g * g - g) * ir / sqrt(4 * E - 2 * usq);
-            d[0] += s * (usq - E); d[1] -= s * uu; d[2] -= s * vv; d[3] += s;
-        }
-        if (e < 0) { d[0] = -d[0]; d[1] = -d[1]; d[2] = -d[2]; d[3] = -d[3]; }
-        double fm[4], fp[4], dv = 0.0;
-        if (eq.jacobian) {
-            euler_an_v(g, pm, Vm, 0, nx, ny, fm);
-            euler_an_v(g, pp, Vp, 0, nx, ny, fp);
+    for (int c = 0; c < 4; c++) f[c] = nx * fx[c] + ny * fy[c];
+}
+
+// The u-dependent edge-point data: LF speed alpha = first maximum of |vn-c|, |vn|, |vn+c| of
+// Um, then of Up (the tie rule of euler_alpha_deriv / euler_alpha_deriv_plus), and d alpha
+// with respect to the state that owns it (Euler::alphaDerivative), written to rec[LPE]
+__device__ inline void euler_edge_prep(double g, const double *Um, const double *Up, double nx, double ny,
+                                       double *rec) {
+    const EulerPrim pm = euler_prim(g, Um), pp = euler_prim(g, Up);
+    const double c1 = sqrt(g * (g - 1) * (pm.E - 0.5 * (pm.u * pm.u + pm.v * pm.v)));
+    const double c2 = sqrt(g * (g - 1) * (pp.E - 0.5 * (pp.u * pp.u + pp.v * pp.v)));
+    const double vn1 = pm.u * nx + pm.v * ny, vn2 = pp.u * nx + pp.v * ny;
+    const double eig[6] = {vn1 - c1, vn1, vn1 + c1, vn2 - c2, vn2, vn2 + c2};
+    int idx = 0;
+    double e = eig[0], alpha = fabs(e);
 #pragma unroll
-            for (int c = 0; c < 4; c++) dv = fma(d[c], side ? Vp[c] : Vm[c], dv);
-        } else {
-            double Vs[4];
-#pragma unroll
-            for (int c = 0; c < 4; c++) Vs[c] = Vm[c] + Vp[c];
-            euler_an_v(g, pm, Vs, 1, nx, ny, fm);
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                fp[c] = 0.0;
-                dv = fma(d[c], Vs[c], dv);
-            }
-            if (side) dv = 0.0;
-        }
-#pragma unroll
-        for (int c = 0; c < 4; c++) f[c] = 0.5 * (fm[c] + fp[c] + alpha * (Vm[c] - Vp[c]) + (Um[c] - Up[c]) * dv);
-    } else {
-        double bx, by;
-        adv_beta(eq, x, y, bx, by);
-        const double bn = bx * nx + by * ny;
-        f[0] = bn > 0 ? bn * Vm[0] : bn * Vp[0];
+    for (int k = 1; k < 6; k++)
+        if (alpha < fabs(eig[k])) { alpha = fabs(eig[k]); idx = k; e = eig[k]; }
+    const int side = idx >= 3, j = idx - 3 * side;
+    const double ir = side ? pp.ir : pm.ir, uu = side ? pp.u : pm.u, vv = side ? pp.v : pm.v;
+    const double E = side ? pp.E : pm.E;
+    double d[4] = {-(uu * nx + vv * ny) * ir, nx * ir, ny * ir, 0.0};
+    if (j != 1) {
+        const double usq = uu * uu + vv * vv;
+        const double sc = (j == 0 ? -1.0 : 1.0) * sqrt(g * g - g) * ir / sqrt(4 * E - 2 * usq);
+        d[0] += sc * (usq - E); d[1] -= sc * uu; d[2] -= sc * vv; d[3] += sc;
     }
+    if (e < 0) { d[0] = -d[0]; d[1] = -d[1]; d[2] = -d[2]; d[3] = -d[3]; }
+    rec[0] = pm.u; rec[1] = pm.v; rec[2] = pm.k; rec[3] = pm.H;
+    rec[4] = pp.u; rec[5] = pp.v; rec[6] = pp.k; rec[7] = pp.H;
+#pragma unroll
+    for (int c = 0; c < 4; c++) rec[8 + c] = Um[c] - Up[c];
+    rec[12] = alpha;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        rec[13 + c] = side ? 0.0 : d[c];
+        rec[17 + c] = side ? d[c] : 0.0;
+    }
+}
+
+// Derivative of the Euler numerical flux along (Vm, Vp) from the edge-point data: Jd Vm + Jo Vp
+// with the matrices k_jacobian uses (Jd = numfluxjac(+1); Jo = euler_offdiag_exact for the exact
+// Jacobian form, numfluxjac(-1), i.e. A(U-) and alpha' with respect to U-, for the reference
+// form).  Vp = 0 on Dirichlet edges.
+__device__ inline void euler_numflux_jvp_p(const DevEq &eq, const double *rec, const double *Vm, const double *Vp,
+                                           double nx, double ny, double *f) {
+    const double g = eq.gamma, alpha = rec[12];
+    const double *dU = rec + 8, *dm = rec + 13, *dp = rec + 17;
+    double fm[4], fp[4], dv = 0.0;
+    if (eq.jacobian) {
+        euler_an_v_p(g, rec, Vm, 0, nx, ny, fm);
+        euler_an_v_p(g, rec + 4, Vp, 0, nx, ny, fp);
+#pragma unroll
+        for (int c = 0; c < 4; c++) dv = fma(dm[c], Vm[c], fma(dp[c], Vp[c], dv));
+    } else {
+        double Vs[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) Vs[c] = Vm[c] + Vp[c];
+        euler_an_v_p(g, rec, Vs, 1, nx, ny, fm);
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            fp[c] = 0.0;
+            dv = fma(dm[c], Vs[c], dv);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) f[c] = 0.5 * (fm[c] + fp[c] + alpha * (Vm[c] - Vp[c]) + dU[c] * dv);
+}
+
+// advection: upwind flux derivative (numfluxjac for nc = 1)
+__device__ inline double adv_numflux_jvp(const DevEq &eq, double Vm, double Vp, double x, double y, double nx,
+                                         double ny) {
+    double bx, by;
+    adv_beta(eq, x, y, bx, by);
+    const double bn = bx * nx + by * ny;
+    return bn > 0 ? bn * Vm : bn * Vp;
 }
 
 // Dirichlet data

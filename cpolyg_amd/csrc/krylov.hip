@@ -25,6 +25,7 @@
 #include "cpg_internal.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace cpg {
@@ -32,7 +33,7 @@ namespace cpg {
 constexpr int KWG = 256;
 constexpr int KPT = 8;  // elements per thread in the vector kernels
 constexpr int KCH = KWG * KPT;
-constexpr int KBLK = 5;  // Arnoldi steps per captured block (flexible driver)
+constexpr int KBLK = 2;  // Arnoldi steps per captured block (flexible driver)
 
 // partial[b*nv + j] = sum over chunk b of V[j] . w
 __global__ __launch_bounds__(KWG) void k_mdot(const double *__restrict__ V, size_t N, int nv,
@@ -287,6 +288,7 @@ struct cpg_krylov_s {
     } sig;
     hipGraphExec_t g_start0 = nullptr, g_start = nullptr, g_finish = nullptr;
     std::vector<hipGraphExec_t> g_blocks;
+    bool use_graph = true;   // CPOLYG_GMRES_GRAPH=0 at creation: launch the flexible cycles eagerly
     // kernel timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -561,7 +563,7 @@ void fg_finish(cpg_krylov_s &K, KOps &op, double *x) {
 // run `body` on s from a graph captured at its first use (eagerly when timing)
 template <class Fn>
 void run_part(cpg_krylov_s &K, hipStream_t s, hipGraphExec_t &exec, Fn &&body) {
-    if (K.timing) {
+    if (K.timing || !K.use_graph) {
         body();
         return;
     }
@@ -677,6 +679,7 @@ int cpg_krylov_create(int n_rows, int bl, int restart, int mode, cpg_krylov *out
             K->bl = bl;
             K->m = restart;
             K->mode = mode;
+            if (const char *g = std::getenv("CPOLYG_GMRES_GRAPH")) K->use_graph = !(g[0] == '0');
             K->N = (size_t)n_rows * bl;
             K->nblk = std::max(1, ceil_div(K->N, KCH));
             K->V.alloc(K->N * (restart + 1));

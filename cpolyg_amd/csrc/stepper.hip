@@ -26,19 +26,20 @@ double eq_newton_resid(const cpg_equation_s &E, const double *kv, const double *
                        double *out_nc, hipStream_t s);
 void krylov_timing(cpg_krylov K, bool on);
 void krylov_timing_read(cpg_krylov K, int *nm, double *msm, int *na, double *msa);
-void eq_newton_apply_dev(const cpg_equation_s &E, const double *u, const double *scale_dev, const double *v,
-                         double *y, const int *active, hipStream_t s);
+void eq_newton_prepare(cpg_equation_s &E, const double *u, double t, hipStream_t s);
+void eq_newton_apply_dev(const cpg_equation_s &E, const double *scale_dev, const double *v, double *y,
+                         const int *active, hipStream_t s, double scale = 0.0);
 
-// the Newton matrix M - dt a J(cu) applied at the quadrature points; cu and the scale stay in
-// device memory, so captured Krylov cycles are replayed for every stage and Newton iteration
+// the Newton matrix M - dt a J(cu) applied at the quadrature points, from the point data of
+// the last eq_newton_prepare; the scale stays in device memory, so captured Krylov cycles are
+// replayed for every stage and Newton iteration
 struct NewtonOp {
     const cpg_equation_s *E;
-    const double *u;
     const double *scale;
 };
 void newton_linop(void *ctx, const double *x, double *y, const int *active, hipStream_t s) {
     const auto *o = static_cast<const NewtonOp *>(ctx);
-    eq_newton_apply_dev(*o->E, o->u, o->scale, x, y, active, s);
+    eq_newton_apply_dev(*o->E, o->scale, x, y, active, s);
 }
 
 __global__ void k_set_scalar(double *p, double v) { *p = v; }
@@ -281,7 +282,7 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
                 CPG_CHECK(params->pc_refresh >= 0 && params->pc_refresh <= 2, CPG_ERR_ARG, "bad pc_refresh %d",
                           params->pc_refresh);
                 S->opscale.alloc(1);
-                S->nop = cpg::NewtonOp{E, S->cu.p, S->opscale.p};
+                S->nop = cpg::NewtonOp{E, S->opscale.p};
             } else if (method == CPG_STEP_IRK) {
                 CPG_CHECK(nstages == 2 || nstages == 3, CPG_ERR_UNSUPPORTED, "Radau IIA with %d stages", nstages);
                 CPG_CHECK(params->restart > 0 && params->maxit > 0 && params->newton_maxit > 0, CPG_ERR_ARG,
@@ -526,6 +527,7 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                 if (S->sp.newton_operator == 1) {
                     hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, S->opscale.p, -dt * S->A[stg * ns + stg]);
                     CPG_LAUNCH_CHECK();
+                    eq_newton_prepare(*S->E, S->cu.p, ts, s);
                     op = LinOp{newton_linop, &S->nop};
                 }
                 check_ctx(guard([&] { gmres_solve(S->K, op, S->P, S->r.p, S->x.p, true, &tol, &maxit, s); }), stg, it,

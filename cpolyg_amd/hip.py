@@ -307,6 +307,14 @@ class Equation:
         _chk(lib().cpg_newton_apply(self.h, _vp(ptr(u)), C.c_double(t), C.c_double(scale), _vp(ptr(v)),
                                     _vp(ptr(y)), _stream(stream)))
 
+    def newton_prepare(self, u, t, stream=None):
+        """The (u, t)-dependent point data of the Newton operator (``cpg_newton_prepare``)."""
+        _chk(lib().cpg_newton_prepare(self.h, _vp(ptr(u)), C.c_double(t), _stream(stream)))
+
+    def newton_apply_prepared(self, scale, v, y, stream=None):
+        """y = M v + scale J(u, t) v for the (u, t) of the last ``newton_prepare``."""
+        _chk(lib().cpg_newton_apply_prepared(self.h, C.c_double(scale), _vp(ptr(v)), _vp(ptr(y)), _stream(stream)))
+
     def mass_matvec(self, x, y, stream=None):
         _chk(lib().cpg_mass_matvec(self.h, _vp(ptr(x)), _vp(ptr(y)), _stream(stream)))
 

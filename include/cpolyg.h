@@ -160,6 +160,14 @@ int cpg_assemble_jacobian(cpg_equation E, const double *u, double t, double scal
  * by the equation's `jacobian` option, so the two agree to rounding.  y must not alias u or v. */
 int cpg_newton_apply(cpg_equation E, const double *u, double t, double scale, const double *v, double *y,
                      void *stream);
+/* cpg_newton_apply in two parts, for many products with one Newton matrix (no reference
+ * counterpart; the reference assembles the matrix once and calls BlockMatrix::matvec per GMRES
+ * iteration): cpg_newton_prepare evaluates what depends on (u, t) alone at every quadrature point
+ * and keeps it in the equation; cpg_newton_apply_prepared then forms y = M v + scale J(u, t) v for
+ * the (u, t) of the last prepare on E (v and y distinct).  Calls on one equation are ordered by the
+ * stream, as the equation's other calls are. */
+int cpg_newton_prepare(cpg_equation E, const double *u, double t, void *stream);
+int cpg_newton_apply_prepared(cpg_equation E, double scale, const double *v, double *y, void *stream);
 int cpg_mass_matvec(cpg_equation E, const double *x, double *y, void *stream);
 int cpg_mass_download(cpg_equation E, double *host_blocks);   /* n blocks bl x bl, synchronous */
 /* MeshFn::L2Norm (src/MeshFn.cpp:299-312) of u, nc values to host, synchronous */

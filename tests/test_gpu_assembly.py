@@ -102,6 +102,14 @@ def test_residual_and_jacobian(cuda, kind, deg, kw):
     zd = _dev(cuda, np.zeros_like(u))
     J.matvec(vd, zd)
     assert rel(yd.cpu().numpy(), zd.cpu().numpy()) < 1e-12
+    # the same in two parts: prepare for (u, t), then products for two scales
+    E.newton_prepare(ud, t)
+    for s2 in (sc, 2.0 * sc):
+        E.newton_apply_prepared(s2, vd, zd)
+        yo2 = np.einsum("scr,sc->sr", s2 * Jo, v.reshape(-1, bl)[orc.colind])
+        yo2 = np.add.reduceat(yo2, orc.rowptr[:-1], axis=0)
+        yo2 += np.einsum("icr,ic->ir", Mo, v.reshape(-1, bl))
+        assert rel(zd.cpu().numpy(), yo2.ravel()) < 1e-12
 
 
 def test_mass_matvec(cuda):
