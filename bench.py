@@ -8,10 +8,17 @@ Workload (one "step" = one DIRK3 time step, i.e. three Newton solves on the devi
   flexible GMRES(50) to 1e-10 with block DILU; exact periodic traces and exact Euler
   off-diagonal Jacobian blocks (the reference's truncated periodic trace is a parity option:
   on this workload it drives the third DIRK stage of the fourth step to non-physical states).
+Solver configuration of the headline: the Newton operator M - dt a J(u) is applied at the
+quadrature points (cpg_newton_prepare once per Newton matrix, cpg_newton_apply_prepared per
+GMRES iteration; equal to the assembled matrix's product to rounding), and the assembled matrix
+and its block DILU are refreshed once per step (--pc-refresh 2; the reference refreshes them at
+every Newton iteration, --pc-refresh 0).  The Newton iteration and the GMRES tolerance are those
+of the reference policy; the same number of Newton and GMRES iterations results.
 The headline value is the mixed-precision path (by default DILU factors stored in BF16 with one
 power-of-two scale per block and applied with FP32 partial sums, inside FP64 flexible GMRES); the
 FP64 path (FP64 factors, same solver) is timed in the same run and reported beside it, with the
-difference between the two solutions.
+difference between the two solutions, and both are timed again under the reference refresh
+policy with the matrix-free and with the assembled (BSR matvec) operator.
 
 Prints ONE JSON line.  Only one GPU is used: --gpus above 1 exits with a message.
 """
@@ -54,12 +61,12 @@ def parse():
                    help="GMRES operator: the Newton matrix applied at the quadrature points, or the BSR matvec of "
                         "the assembled matrix")
     p.add_argument("--trace", type=int, default=1, help="periodic trace: 1 exact, 0 reference (truncated)")
-    p.add_argument("--pc-refresh", type=int, default=0, choices=[0, 1, 2],
+    p.add_argument("--pc-refresh", type=int, default=2, choices=[0, 1, 2],
                    help="0: Newton matrix and preconditioner at every Newton iteration (reference); 1: once per stage; "
                         "2: once per step (matrix-free operator only)")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-compare", dest="compare_operators", action="store_false",
-                   help="skip the two runs with the other GMRES operator")
+                   help="skip the runs under the reference refresh policy (matrix-free and assembled operator)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-nx", type=int, default=128, help="lattice size of the CPU-baseline sample")
     p.add_argument("--workload", choices=["dirk3-euler", "batch", "pmc"], default="dirk3-euler",
@@ -340,13 +347,14 @@ def main():
     E = hip.Equation(hip.Mesh(mesh), spec, deg)
     bl, nnzb = E.bl, E.nnzb
 
-    def solver(prec, pc=PC_DILU, op=None):
+    def solver(prec, pc=PC_DILU, op=None, refresh=None):
         # each stage's Newton iteration starts from the previous stage's increment and halves an
         # update that would leave the physical states (the reference starts from zero; at CFL 20 its
         # first update on the third stage can overshoot to negative pressure)
         return SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=args.restart, rtol=1e-10, maxit=20000,
                           pc_kind=pc, newton_tol=5e-13, newton_maxit=100, precision=prec, inner_iters=args.inner,
-                          newton_guess=1, newton_backtrack=1, pc_refresh=args.pc_refresh,
+                          newton_guess=1, newton_backtrack=1,
+                          pc_refresh=args.pc_refresh if refresh is None else refresh,
                           newton_operator=(OP_MATRIX_FREE if args.operator == "matrix-free" else OP_ASSEMBLED)
                           if op is None else op)
 
@@ -367,17 +375,21 @@ def main():
         res["fp64_phase_ms_per_step"] = {k: sum(s[k] for s in st64) / steps for k in
                                          ("ms_residual", "ms_jacobian", "ms_factor", "ms_gmres")}
     if args.compare_operators:
-        # the same two paths with the other GMRES operator
-        other = OP_ASSEMBLED if args.operator == "matrix-free" else OP_MATRIX_FREE
-        cmp_ = {"operator": "assembled" if other == OP_ASSEMBLED else "matrix-free"}
-        ua, wa, _, _, _, _ = run_path(E, u0, dt, steps, args.warmup, solver(mixed_prec, mixed_pc, other))
-        cmp_["mixed_ms_per_step"] = 1e3 * wa / steps
-        cmp_["mixed_max_rel_diff"] = float(np.max(np.abs(ua - u_mix)) / np.max(np.abs(u_mix)))
-        if not args.no_fp64:
-            _, wa64, _, _, _, _ = run_path(E, u0, dt, steps, args.warmup, solver(PREC_FP64, PC_DILU, other))
-            cmp_["fp64_ms_per_step"] = 1e3 * wa64 / steps
-            cmp_["speedup_mixed_vs_fp64"] = wa64 / wa
-        res["other_operator"] = cmp_
+        # the same pair of paths under the reference's refresh policy (Newton matrix and factor at
+        # every Newton iteration), with the matrix-free and with the assembled (BSR) operator
+        for key, op in (("reference_refresh_matrix_free", OP_MATRIX_FREE), ("reference_refresh_assembled", OP_ASSEMBLED)):
+            if key.endswith("matrix_free") and args.pc_refresh == 0 and args.operator == "matrix-free":
+                continue
+            cmp_ = {"operator": "matrix-free" if op == OP_MATRIX_FREE else "assembled", "pc_refresh": 0}
+            ua, wa, sta, _, _, _ = run_path(E, u0, dt, steps, args.warmup, solver(mixed_prec, mixed_pc, op, 0))
+            cmp_["mixed_ms_per_step"] = 1e3 * wa / steps
+            cmp_["mixed_max_rel_diff_vs_headline"] = float(np.max(np.abs(ua - u_mix)) / np.max(np.abs(u_mix)))
+            cmp_["gmres_iters_per_step"] = sum(x["gmres_iters"] for x in sta) / steps
+            if not args.no_fp64:
+                _, wa64, _, _, _, _ = run_path(E, u0, dt, steps, args.warmup, solver(PREC_FP64, PC_DILU, op, 0))
+                cmp_["fp64_ms_per_step"] = 1e3 * wa64 / steps
+                cmp_["speedup_mixed_vs_fp64"] = wa64 / wa
+            res[key] = cmp_
     # per-launch rates on the last Newton matrix of the mixed run (stage-3 scale)
     rates = standalone_rates(E, u_last, t_last, -dt * a_ss)
     elem = {PREC_BF16: 2, PREC_FP32: 4}[mixed_prec]
@@ -420,7 +432,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"Euler isentropic vortex, periodic Voronoi {n} cells ({args.nx}^2 jittered "
                                f"lattice), p={deg} ({bl}x{bl} blocks), DIRK3 CFL {args.cfl:g}, FGMRES({args.restart}) "
-                               f"rtol 1e-10, block DILU, {args.operator} Newton operator", "cells": n, "deg": deg,
+                               f"rtol 1e-10, block DILU, {args.operator} Newton operator, preconditioner refreshed "
+                               f"{['every Newton iteration', 'once per stage', 'once per step'][args.pc_refresh]}",
+                   "cells": n, "deg": deg, "pc_refresh": args.pc_refresh,
                    "block": bl, "nnzb": nnzb, "dt": dt, "cfl": args.cfl, "mixed": args.mixed,
                    "operator": args.operator, "jacobian": "exact" if args.jacobian else "reference",
                    "periodic_trace": "exact" if args.trace else "reference", "colours": mesh.n_colours},

@@ -190,7 +190,9 @@ __device__ inline void cellop_contract(const double *D, const double *G, int jg,
 }
 
 // Point data of the Euler Newton operator: nv volume points (LPV fields each) and nep edge
-// points (LPE fields each), stored field-major so that a wavefront's loads are contiguous
+// points (LPE fields each), stored field-major so that a wavefront's loads are contiguous; read
+// once per product with non-temporal loads, so that they do not evict the Krylov vectors that
+// the preconditioner sweeps gather next
 struct LinData {
     const double *vol;   // vol[f * nv + q]
     const double *edg;   // edg[f * nep + e * NQL + q] for edge e (global edge index) and point q
@@ -241,7 +243,8 @@ __global__ __launch_bounds__(64, DEG <= 3 ? 3 : 2) void k_cellop(DevEq eq, Geo g
                 if constexpr (NC == 4) {
                     double p[LPV];
 #pragma unroll
-                    for (int f = 0; f < LPV; f++) p[f] = ld.vol[(size_t)f * ld.nv + q0 + lane];
+                    for (int f = 0; f < LPV; f++)
+                        p[f] = __builtin_nontemporal_load(ld.vol + (size_t)f * ld.nv + q0 + lane);
                     euler_jvp_p(eq.gamma, p, X, eq.jacobian == 0, Fx, Gy);
                 } else {
                     double bx, by;
@@ -315,7 +318,7 @@ __global__ __launch_bounds__(64, DEG <= 3 ? 3 : 2) void k_cellop(DevEq eq, Geo g
                     double rec[LPE];
                     const size_t ep = (size_t)(e0 + eb + el) * NQL + q;
 #pragma unroll
-                    for (int f = 0; f < LPE; f++) rec[f] = ld.edg[(size_t)f * ld.nep + ep];
+                    for (int f = 0; f < LPE; f++) rec[f] = __builtin_nontemporal_load(ld.edg + (size_t)f * ld.nep + ep);
                     euler_numflux_jvp_p(eq, rec, Xm, Xp, E.nx, E.ny, F);
                 } else {
                     F[0] = adv_numflux_jvp(eq, Xm[0], Xp[0], px, py, E.nx, E.ny);
