@@ -69,12 +69,15 @@ def parse():
                    help="skip the runs under the reference refresh policy (matrix-free and assembled operator)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-nx", type=int, default=128, help="lattice size of the CPU-baseline sample")
-    p.add_argument("--workload", choices=["dirk3-euler", "batch", "pmc"], default="dirk3-euler",
+    p.add_argument("--workload", choices=["dirk3-euler", "batch", "pmc", "irk"], default="dirk3-euler",
                    help="batch: 64 independent implicit advection solves advanced as one batch (prints its own "
                         "line); pmc: the short kernel sequence profiled with rocprofv3 --pmc for HBM traffic")
     p.add_argument("--batch-stages", type=int, default=2, help="batch: 1 backward Euler, 2 SDIRK2")
     p.add_argument("--batch-count", type=int, default=64)
     p.add_argument("--no-sequential", action="store_true", help="batch: skip the one-stepper-per-instance timing")
+    p.add_argument("--irk-nx", type=int, default=512, help="irk: lattice size (cells = nx^2)")
+    p.add_argument("--irk-cfl", type=float, default=10.0)
+    p.add_argument("--irk-restart", type=int, default=60)
     return p.parse_args()
 
 
@@ -286,6 +289,76 @@ def bench_batch(args):
     print(json.dumps(out))
 
 
+def bench_irk(args):
+    """Radau IIA (3 stages) for compressible Euler, density 1 + 0.2 sin(2 pi x) sin(2 pi y), velocity
+    (1, 0.5), pressure 1, periodic unit square, Voronoi mesh of a jittered nx^2 lattice (seed 5), p=3.
+    The stage-coupled Newton system (3 x 40 unknowns per cell) is applied matrix-free; block Jacobi on
+    the cell-interleaved 120 x 120 coupled diagonal blocks, factors in BF16 (headline) and FP64;
+    FP64 flexible GMRES(--irk-restart) to 1e-10; Newton to 1e-10 (the reference's 5e-13 is an absolute
+    L2 bound set for its small meshes)."""
+    import numpy as np
+    import torch
+
+    from cpolyg_amd import hip, setups
+    from cpolyg_amd.mesh import jittered_lattice_seeds, periodic_voronoi
+    from cpolyg_amd.spec import EquationSpec, GMRES_FLEXIBLE, OP_MATRIX_FREE, PC_BJACOBI, PREC_BF16, PREC_FP64, SolverSpec
+
+    nx, deg = args.irk_nx, 3
+    t0 = time.perf_counter()
+    mesh = periodic_voronoi(jittered_lattice_seeds(nx, nx, 1.0, 1.0, 0.3, 5), 1.0, 1.0)
+    u0 = setups.interp(mesh, deg, setups.sine_density(), 4)
+    t_setup = time.perf_counter() - t0
+    n = mesh.n_cells
+    dt = setups.dg_time_step(args.irk_cfl, 1.0 / nx, deg, setups.euler_max_speed(u0, deg))
+    E = hip.Equation(hip.Mesh(mesh), EquationSpec.euler(periodic_trace=1, jacobian=1), deg)
+    res = {}
+    free0, total = torch.cuda.mem_get_info()
+    for name, prec in (("bf16", PREC_BF16), ("fp64", PREC_FP64)):
+        sp = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=args.irk_restart, rtol=1e-10, maxit=6000,
+                        pc_kind=PC_BJACOBI, newton_tol=1e-10, newton_maxit=10, precision=prec,
+                        newton_operator=OP_MATRIX_FREE, pc_refresh=args.pc_refresh)
+        S = hip.Stepper(E, 1, 3, sp)
+        free1, _ = torch.cuda.mem_get_info()
+        u = torch.from_numpy(u0).cuda()
+        nxt = torch.empty_like(u)
+        t = 0.0
+        for _ in range(args.warmup):
+            S.advance(u, dt, t, nxt)
+            u, nxt = nxt, u
+            t += dt
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sts = []
+        for _ in range(args.steps):
+            sts.append(S.advance(u, dt, t, nxt).as_dict())
+            u, nxt = nxt, u
+            t += dt
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        res[name] = {"ms_per_step": wall * 1e3, "steps_per_s": 1.0 / wall,
+                     "newton_iters_per_step": sum(x["newton_iters"] for x in sts) / args.steps,
+                     "gmres_iters_per_step": sum(x["gmres_iters"] for x in sts) / args.steps,
+                     "phase_ms_per_step": {k: sum(x[k] for x in sts) / args.steps
+                                           for k in ("ms_residual", "ms_jacobian", "ms_factor", "ms_gmres")},
+                     "last_resid": sts[-1]["last_resid"],
+                     "stepper_device_bytes": free0 - free1, "u": u.cpu().numpy()}
+        del S
+        torch.cuda.synchronize()
+    diff = float(np.max(np.abs(res["bf16"]["u"] - res["fp64"]["u"])) / np.max(np.abs(res["fp64"]["u"])))
+    for r in res.values():
+        del r["u"]
+    out = {"metric": "Radau IIA (3-stage) steps/s, Euler p=3, 262k cells, coupled system matrix-free",
+           "value": res["bf16"]["steps_per_s"], "unit": "IRK steps/s", "steps": args.steps, "warmup": args.warmup,
+           "config": {"workload": f"Euler density wave, periodic Voronoi {n} cells ({nx}^2 jittered lattice), p={deg}, "
+                                  f"Radau IIA 3 stages at CFL {args.irk_cfl:g}, matrix-free coupled operator, block "
+                                  f"Jacobi on 120x120 coupled blocks, FGMRES({args.irk_restart}) rtol 1e-10",
+                      "cells": n, "dt": dt, "pc_refresh": args.pc_refresh, "colours": mesh.n_colours},
+           "bf16": res["bf16"], "fp64": res["fp64"], "speedup_bf16_vs_fp64": res["fp64"]["ms_per_step"] /
+           res["bf16"]["ms_per_step"], "bf16_vs_fp64_max_rel_diff": diff, "gpu_memory_total_bytes": total,
+           "setup_s": t_setup}
+    print(json.dumps(out))
+
+
 def pmc_workload(args):
     """Kernel sequence of the HBM-traffic profile: a 1 GiB device copy of known size (calibrates
     FETCH_SIZE / WRITE_SIZE), three FP64 BSR matvecs of the headline Newton matrix, one BF16 DILU
@@ -321,6 +394,12 @@ def main():
     args = parse()
     if args.workload == "pmc":
         pmc_workload(args)
+        return
+    if args.workload == "irk":
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        bench_irk(args)
         return
     if args.workload == "batch":
         if args.gpus != 1:

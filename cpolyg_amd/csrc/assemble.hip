@@ -199,13 +199,14 @@ struct LinData {
     int nv, nep;
 };
 
-// out = b(x) (residual, LIN = false; x = u) or out = M x + scale J(u) x (LIN = true; x = v, the
-// u-dependent part from `ld`)
+// out = b(x) (residual, LIN = false; x = u) or out = M xm + scale J(u) x (LIN = true; x = v, xm = v
+// when null, the u-dependent part from `ld`; the coupled stages of an implicit Runge-Kutta step
+// take the mass term and the Jacobian term from different vectors)
 template <int DEG, int NC, bool LIN>
 __global__ __launch_bounds__(64, DEG <= 3 ? 3 : 2) void k_cellop(DevEq eq, Geo g, const double *__restrict__ x, LinData ld,
                                                   double scale, const double *__restrict__ scale_dev,
                                                   const int *__restrict__ active, const double *__restrict__ mass,
-                                                  double *__restrict__ out) {
+                                                  const double *__restrict__ xm, double *__restrict__ out) {
     using L = CellOpLayout<DEG, NC, LIN>;
     if (active && !*active) return;   // a converged Krylov cycle replayed from a graph
     if (scale_dev) scale = *scale_dev;
@@ -355,8 +356,14 @@ __global__ __launch_bounds__(64, DEG <= 3 ? 3 : 2) void k_cellop(DevEq eq, Geo g
             const int oc = lane / BS, ok = lane - oc * BS;
             const double *Mi = mass + (size_t)i * BS * BS;
             double m = 0.0;
+            if (xm) {
+                const double *xi = xm + (size_t)i * BL + oc * BS;
 #pragma unroll
-            for (int l = 0; l < BS; l++) m = fma(Mi[l * BS + ok], sX[oc * BS + l], m);
+                for (int l = 0; l < BS; l++) m = fma(Mi[l * BS + ok], xi[l], m);
+            } else {
+#pragma unroll
+                for (int l = 0; l < BS; l++) m = fma(Mi[l * BS + ok], sX[oc * BS + l], m);
+            }
             r = fma(scale, r, m);
         }
         out[(size_t)i * BL + lane] = r;
@@ -793,34 +800,41 @@ template <int DEG, int NC>
 static void launch_residual(const cpg_equation_s &E, const double *u, double t, double *b, hipStream_t s) {
     launch_bc_states<DEG, NC>(E, t, s);
     hipLaunchKernelGGL((k_cellop<DEG, NC, false>), dim3(E.n), dim3(64), 0, s, E.eq, E.geo(), u, LinData{}, 0.0,
-                       nullptr, nullptr, nullptr, b);
+                       nullptr, nullptr, nullptr, nullptr, b);
     CPG_LAUNCH_CHECK();
 }
 
 // point data of the Newton operator for u (Euler; advection has none), with the Dirichlet
 // states already at the right time
+// point data of the Newton operator for u (Euler; advection has none) into vol / edg (the
+// equation's own buffers when null), with the Dirichlet states already at the right time
 template <int DEG, int NC>
-static void launch_newton_prepare(cpg_equation_s &E, const double *u, hipStream_t s) {
+static void launch_newton_prepare(cpg_equation_s &E, const double *u, double *vol, double *edg, hipStream_t s) {
     if constexpr (NC == 4) {
         const size_t nep = (size_t)E.nedges * Deg<DEG>::NQL;
-        if (!E.lin_vol.p) {
-            E.lin_vol.alloc((size_t)LPV * E.nvpts);
-            E.lin_edg.alloc((size_t)LPE * nep);
+        if (!vol) {
+            if (!E.lin_vol.p) {
+                E.lin_vol.alloc((size_t)LPV * E.nvpts);
+                E.lin_edg.alloc((size_t)LPE * nep);
+            }
+            vol = E.lin_vol.p;
+            edg = E.lin_edg.p;
         }
-        hipLaunchKernelGGL((k_linprep<DEG>), dim3(E.n), dim3(64), 0, s, E.eq, E.geo(), u, E.lin_vol.p, E.lin_edg.p,
-                           E.nvpts, (int)nep);
+        hipLaunchKernelGGL((k_linprep<DEG>), dim3(E.n), dim3(64), 0, s, E.eq, E.geo(), u, vol, edg, E.nvpts, (int)nep);
         CPG_LAUNCH_CHECK();
     }
 }
 
-// y = M v + scale J(u) v for the u of the last prepare; scale_dev (if non-null) overrides scale
+// y = M xm + scale J(u) v (xm = v when null) for the u whose point data vol / edg hold (the
+// equation's own, from its last prepare, when null); scale_dev (if non-null) overrides scale
 // from device memory, so a captured launch can be replayed for another Newton matrix
 template <int DEG, int NC>
 static void launch_newton_apply(const cpg_equation_s &E, double scale, const double *scale_dev, const double *v,
-                                double *y, const int *active, hipStream_t s) {
-    const LinData ld{E.lin_vol.p, E.lin_edg.p, E.nvpts, E.nedges * Deg<DEG>::NQL};
+                                const double *xm, double *y, const int *active, const double *vol, const double *edg,
+                                hipStream_t s) {
+    const LinData ld{vol ? vol : E.lin_vol.p, vol ? edg : E.lin_edg.p, E.nvpts, E.nedges * Deg<DEG>::NQL};
     hipLaunchKernelGGL((k_cellop<DEG, NC, true>), dim3(E.n), dim3(64), 0, s, E.eq, E.geo(), v, ld, scale, scale_dev,
-                       active, E.mass.p, y);
+                       active, E.mass.p, xm, y);
     CPG_LAUNCH_CHECK();
 }
 
@@ -863,25 +877,34 @@ void eq_residual(const cpg_equation_s &E, const double *u, double t, double *b, 
     CPG_DISPATCH(E, launch_residual, E, u, t, b, s);
 }
 
-// the u-dependent point data of the Newton operator at (u, t)
-void eq_newton_prepare(cpg_equation_s &E, const double *u, double t, hipStream_t s) {
+// the u-dependent point data of the Newton operator at (u, t), into vol / edg (sized by
+// eq_newton_data_size; the equation's own buffers when null)
+void eq_newton_prepare(cpg_equation_s &E, const double *u, double t, hipStream_t s, double *vol, double *edg) {
     CPG_DISPATCH(E, launch_bc_only, E, t, s);
-    CPG_DISPATCH(E, launch_newton_prepare, E, u, s);
+    CPG_DISPATCH(E, launch_newton_prepare, E, u, vol, edg, s);
+}
+
+// doubles of the operator's point data (volume, edges); 0 for advection
+void eq_newton_data_size(const cpg_equation_s &E, size_t *nvol, size_t *nedg) {
+    const int nql = 2 * E.deg <= 2 ? 2 : 2 * E.deg <= 4 ? 4 : 2 * E.deg <= 6 ? 6 : 10;
+    *nvol = E.nc == 4 ? (size_t)LPV * E.nvpts : 0;
+    *nedg = E.nc == 4 ? (size_t)LPE * E.nedges * nql : 0;
 }
 
 // y = M v + scale J(u, t) v without assembling J
 void eq_newton_apply(cpg_equation_s &E, const double *u, double t, double scale, const double *v, double *y,
                      hipStream_t s) {
-    eq_newton_prepare(E, u, t, s);
+    eq_newton_prepare(E, u, t, s, nullptr, nullptr);
     E.prepared = true;
-    CPG_DISPATCH(E, launch_newton_apply, E, scale, nullptr, v, y, nullptr, s);
+    CPG_DISPATCH(E, launch_newton_apply, E, scale, nullptr, v, nullptr, y, nullptr, nullptr, nullptr, s);
 }
 
 // the same for the (u, t) of the last eq_newton_prepare, with the scale in device memory (the
 // Krylov operator of a Newton step)
 void eq_newton_apply_dev(const cpg_equation_s &E, const double *scale_dev, const double *v, double *y,
-                         const int *active, hipStream_t s, double scale) {
-    CPG_DISPATCH(E, launch_newton_apply, E, scale, scale_dev, v, y, active, s);
+                         const int *active, hipStream_t s, double scale, const double *xm, const double *vol,
+                         const double *edg) {
+    CPG_DISPATCH(E, launch_newton_apply, E, scale, scale_dev, v, xm, y, active, vol, edg, s);
 }
 
 void eq_jacobian(const cpg_equation_s &E, const double *u, double t, double scale, int add_mass, double *J,
@@ -1225,7 +1248,7 @@ int cpg_newton_apply(cpg_equation E, const double *u, double t, double scale, co
 int cpg_newton_prepare(cpg_equation E, const double *u, double t, void *stream) {
     return guard([&] {
         CPG_CHECK(E && u, CPG_ERR_ARG, "null argument");
-        eq_newton_prepare(*E, u, t, S(stream));
+        eq_newton_prepare(*E, u, t, S(stream), nullptr, nullptr);
         E->prepared = true;
     });
 }
@@ -1234,7 +1257,7 @@ int cpg_newton_apply_prepared(cpg_equation E, double scale, const double *v, dou
     return guard([&] {
         CPG_CHECK(E && v && y && v != y, CPG_ERR_ARG, "bad argument");
         CPG_CHECK(E->prepared || E->nc != 4, CPG_ERR_ARG, "cpg_newton_apply_prepared before cpg_newton_prepare");
-        eq_newton_apply_dev(*E, nullptr, v, y, nullptr, S(stream), scale);
+        eq_newton_apply_dev(*E, nullptr, v, y, nullptr, S(stream), scale, nullptr, nullptr, nullptr);
     });
 }
 

@@ -128,4 +128,12 @@ void bsr_linop(void *ctx, const double *x, double *y, const int *active, hipStre
 // GMRES with the driver of K's mode (krylov.hip); x holds the initial guess unless x_zero
 void gmres_solve(cpg_krylov K, const LinOp &A, cpg_prec P, const double *b, double *x, bool x_zero, double *tol,
                  int *maxit, hipStream_t s);
+// out = M^-1 in (out distinct from in) for the flexible driver; `active` as for bsr_matvec
+struct PcOp {
+    void (*fn)(void *ctx, const double *in, double *out, const int *active, hipStream_t s);
+    void *ctx;
+};
+// flexible GMRES (K in CPG_GMRES_FLEXIBLE mode) with the preconditioner as a PcOp
+void gmres_flexible_op(cpg_krylov K, const LinOp &A, const PcOp &M, const double *b, double *x, bool x_zero,
+                       double *tol, int *maxit, hipStream_t s);
 }  // namespace cpg

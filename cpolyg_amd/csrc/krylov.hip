@@ -350,6 +350,7 @@ struct KOps {
     const LinOp &A;
     cpg_prec P;
     hipStream_t s;
+    const PcOp *M = nullptr;   // preconditioner as an operator (flexible driver), instead of P
 
     void matvec(const double *x, double *y, const int *active = nullptr) {
         K.mark(0, s);
@@ -365,6 +366,12 @@ struct KOps {
     }
     // out = M^-1 in (a copy when there is no preconditioner)
     void pc_to(const double *in, double *out, const int *active = nullptr) {
+        if (M) {
+            K.mark(1, s);
+            M->fn(M->ctx, in, out, active, s);
+            K.mark(1, s);
+            return;
+        }
         if (!P) {
             CPG_HIP(hipMemcpyAsync(out, in, K.N * sizeof(double), hipMemcpyDeviceToDevice, s));
             return;
@@ -587,16 +594,17 @@ void run_part(cpg_krylov_s &K, hipStream_t s, hipGraphExec_t &exec, Fn &&body) {
 
 }  // namespace
 
-void gmres_flexible(cpg_krylov_s *K, const LinOp &A, cpg_prec P, const double *b, double *x, bool x_zero,
-                    double *tol, int *maxit, hipStream_t caller) {
+void gmres_flexible(cpg_krylov_s *K, const LinOp &A, cpg_prec P, const PcOp *M, const double *b, double *x,
+                    bool x_zero, double *tol, int *maxit, hipStream_t caller) {
     // everything below runs on K->kst after the caller's queued work; the host waits for it
     // before returning, so later work on the caller's stream is ordered after it
     CPG_HIP(hipEventRecord(K->cev, caller));
     CPG_HIP(hipStreamWaitEvent(K->kst, K->cev, 0));
     hipStream_t s = K->kst;
-    KOps op{*K, A, P, s};
+    KOps op{*K, A, P, s, M};
     const int m = K->m;
-    const cpg_krylov_s::Sig sig{b, x, reinterpret_cast<const void *>(A.fn), A.ctx, P};
+    const cpg_krylov_s::Sig sig{b, x, reinterpret_cast<const void *>(A.fn), A.ctx,
+                                M ? reinterpret_cast<const void *>(M->ctx) : P};
     if (!(sig == K->sig)) {
         K->drop_graphs();
         K->sig = sig;
@@ -654,11 +662,17 @@ void gmres_flexible(cpg_krylov_s *K, const LinOp &A, cpg_prec P, const double *b
 void gmres_solve(cpg_krylov K, const LinOp &A, cpg_prec P, const double *b, double *x, bool x_zero, double *tol,
                  int *maxit, hipStream_t s) {
     if (K->mode == CPG_GMRES_FLEXIBLE) {
-        gmres_flexible(K, A, P, b, x, x_zero, tol, maxit, s);
+        gmres_flexible(K, A, P, nullptr, b, x, x_zero, tol, maxit, s);
         return;
     }
     if (x_zero) vec_fill(x, 0.0, K->N, s);
     gmres_left_ref(K, A, P, b, x, tol, maxit, s);
+}
+
+void gmres_flexible_op(cpg_krylov K, const LinOp &A, const PcOp &M, const double *b, double *x, bool x_zero,
+                       double *tol, int *maxit, hipStream_t s) {
+    CPG_CHECK(K->mode == CPG_GMRES_FLEXIBLE, CPG_ERR_ARG, "an operator preconditioner needs flexible GMRES");
+    gmres_flexible(K, A, nullptr, &M, b, x, x_zero, tol, maxit, s);
 }
 
 void bsr_linop(void *ctx, const double *x, double *y, const int *active, hipStream_t s) {

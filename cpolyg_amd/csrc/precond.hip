@@ -317,6 +317,113 @@ __global__ __launch_bounds__(WG) void k_bjacobi_factor(int bl, const int *__rest
     store_block<T, NE>(bl, F + (size_t)i * bb, scale ? scale + i : nullptr, c, scratch);
 }
 
+// block Jacobi for blocks too large for the augmented Gauss-Jordan (the stage-coupled
+// 120 x 120 blocks of a 3-stage Radau IIA step): in-place Gauss-Jordan with partial pivoting
+// (first maximal |pivot|) in LDS, row-major with a padded stride, then the column
+// permutation undone.  One workgroup per block; thread (h, c) of the elimination owns
+// column c and the rows h, h + NH, ... (no divisions in the inner loop).
+constexpr int IPMAX = 128;
+
+template <class T, int BL>
+__global__ __launch_bounds__(WG) void k_bjacobi_factor_ip(const int *__restrict__ diag, const double *__restrict__ A,
+                                                          T *__restrict__ F, float *__restrict__ scale,
+                                                          int *__restrict__ status) {
+    static_assert(BL <= IPMAX && 2 * BL <= WG, "block size");
+    constexpr int LD = BL + 1, NH = WG / BL;
+    extern __shared__ double lds[];
+    double *X = lds;                   // X[r * LD + c]
+    double *fcol = X + (size_t)BL * LD; // column k of the rows being eliminated
+    __shared__ int piv[IPMAX];
+    __shared__ double red[WG / 64];
+    __shared__ int bad;
+    const int i = blockIdx.x, tid = threadIdx.x;
+    constexpr size_t bb = (size_t)BL * BL;
+    const double *a = A + (size_t)diag[i] * bb;
+    for (int e = tid; e < (int)bb; e += WG) {
+        const int c = e / BL, r = e - c * BL;
+        X[r * LD + c] = a[e];
+    }
+    if (tid == 0) bad = 0;
+    const int hc = tid / BL, cc = tid - hc * BL;   // elimination role
+    const bool el = hc < NH;
+    __syncthreads();
+    for (int k = 0; k < BL; k++) {
+        // pivot row: first maximal |X[r][k]|, r >= k (wavefront 0)
+        if (tid < 64) {
+            double best = -1.0;
+            int br = BL;
+            for (int r = k + tid; r < BL; r += 64) {
+                const double v = fabs(X[r * LD + k]);
+                if (v > best) { best = v; br = r; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(best, off);
+                const int orr = __shfl_xor(br, off);
+                if (ob > best || (ob == best && orr < br)) { best = ob; br = orr; }
+            }
+            if (tid == 0) {
+                piv[k] = br;
+                if (!(best > 0.0)) bad = 1;
+            }
+        }
+        __syncthreads();
+        if (bad) break;
+        const int p = piv[k];
+        if (p != k && tid < BL) {
+            const double t = X[k * LD + tid];
+            X[k * LD + tid] = X[p * LD + tid];
+            X[p * LD + tid] = t;
+        }
+        __syncthreads();
+        const double d = 1.0 / X[k * LD + k];
+        if (tid < BL) fcol[tid] = tid == k ? 0.0 : X[tid * LD + k];
+        __syncthreads();
+        if (tid < BL) X[k * LD + tid] = (tid == k ? 1.0 : X[k * LD + tid]) * d;
+        __syncthreads();
+        // X[r][c] -= f_r X[k][c] for r != k, with column k taken as 0 before the update
+        if (el) {
+            const double pk = X[k * LD + cc];
+            for (int r = hc; r < BL; r += NH) {
+                if (r == k) continue;
+                const double x = cc == k ? 0.0 : X[r * LD + cc];
+                X[r * LD + cc] = x - fcol[r] * pk;
+            }
+        }
+        __syncthreads();
+    }
+    if (bad) {
+        if (tid == 0) atomicOr(status, 1);
+        return;
+    }
+    // undo the row interchanges on the columns of the inverse
+    for (int k = BL - 1; k >= 0; k--) {
+        const int p = piv[k];
+        if (p != k && tid < BL) {
+            const double t = X[tid * LD + k];
+            X[tid * LD + k] = X[tid * LD + p];
+            X[tid * LD + p] = t;
+        }
+        __syncthreads();
+    }
+    float inv = 1.0f;
+    if constexpr (Store<T>::prec == CPG_BF16) {
+        double mx = 0.0;
+        for (int e = tid; e < (int)bb; e += WG) {
+            const int c = e / BL, r = e - c * BL;
+            mx = fmax(mx, fabs(X[r * LD + c]));
+        }
+        mx = wg_max_abs(mx, red);
+        const float sc = pow2_scale(mx);
+        inv = 1.0f / sc;
+        if (tid == 0) scale[i] = sc;
+    }
+    T *f = F + (size_t)i * bb;
+    for (int e = tid; e < (int)bb; e += WG) {
+        const int c = e / BL, r = e - c * BL;
+        Store<T>::put(f, e, X[r * LD + c], inv);
+    }
+}
+
 // ---- wavefront-per-row factor: FP64 MFMA block products, in-register Gauss-Jordan -------
 //
 // Per level, one 64-lane wavefront per block row forms the L blocks and the Schur
@@ -1120,10 +1227,35 @@ void factor_dilu_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
     }
 }
 
+template <class T, int BL>
+void factor_bj_ip_bl(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
+    const size_t lds = ((size_t)BL * (BL + 1) + BL) * sizeof(double);
+    auto kern = k_bjacobi_factor_ip<T, BL>;
+    set_lds(kern, lds);
+    hipLaunchKernelGGL(kern, dim3(P.n), dim3(WG), lds, s, P.diag.p, A.blocks.p, Fp<T>(P),
+                       P.precision == CPG_BF16 ? P.scale.p : nullptr, P.status.p);
+    CPG_LAUNCH_CHECK();
+}
+
+// block sizes beyond the augmented kernel: the coupled Radau IIA blocks (ns stages x 4 x bs),
+// 120 for 3 stages at p = 3 and 2 stages at p = 4
+template <class T>
+void factor_bj_ip_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
+    if (P.bl == 120) {
+        factor_bj_ip_bl<T, 120>(P, A, s);
+        return;
+    }
+    CPG_CHECK(false, CPG_ERR_UNSUPPORTED, "block size %d too large for block Jacobi", P.bl);
+}
+
 template <class T, int NE>
 void factor_bj_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
     const int bl = P.bl;
     const size_t lds = (size_t)(2 * bl * bl + bl + 8) * sizeof(double);
+    if (lds > 160 * 1024) {
+        factor_bj_ip_t<T>(P, A, s);
+        return;
+    }
     CPG_CHECK(lds <= 160 * 1024, CPG_ERR_UNSUPPORTED, "block size %d too large for the LDS factor kernel", bl);
     auto kern = k_bjacobi_factor<T, NE>;
     set_lds(kern, lds);

@@ -26,9 +26,12 @@ double eq_newton_resid(const cpg_equation_s &E, const double *kv, const double *
                        double *out_nc, hipStream_t s);
 void krylov_timing(cpg_krylov K, bool on);
 void krylov_timing_read(cpg_krylov K, int *nm, double *msm, int *na, double *msa);
-void eq_newton_prepare(cpg_equation_s &E, const double *u, double t, hipStream_t s);
+void eq_newton_prepare(cpg_equation_s &E, const double *u, double t, hipStream_t s, double *vol = nullptr,
+                       double *edg = nullptr);
+void eq_newton_data_size(const cpg_equation_s &E, size_t *nvol, size_t *nedg);
 void eq_newton_apply_dev(const cpg_equation_s &E, const double *scale_dev, const double *v, double *y,
-                         const int *active, hipStream_t s, double scale = 0.0);
+                         const int *active, hipStream_t s, double scale = 0.0, const double *xm = nullptr,
+                         const double *vol = nullptr, const double *edg = nullptr);
 
 // the Newton matrix M - dt a J(cu) applied at the quadrature points, from the point data of
 // the last eq_newton_prepare; the scale stays in device memory, so captured Krylov cycles are
@@ -43,6 +46,99 @@ void newton_linop(void *ctx, const double *x, double *y, const int *active, hipS
 }
 
 __global__ void k_set_scalar(double *p, double v) { *p = v; }
+
+// ---- implicit Runge-Kutta with the matrix-free coupled operator ------------------------
+// The coupled Newton system of IRK::advance (src/TimeIntegration.cpp:188-281) in its
+// stage-major layout: y_s = M x_s - dt sum_t A[s][t] J(cu_s) x_t.  Row stage s is the Newton
+// operator of stage s's state applied to w_s = sum_t A[s][t] x_t, with its mass term on x_s.
+// Preconditioner: block Jacobi on the cell-interleaved coupled diagonal blocks (one
+// (ns bl) x (ns bl) block per cell, D_i[(s,.),(t,.)] = delta_st M_i - dt A[s][t] J_ii(cu_s)),
+// applied on an interleaved copy of the vector.
+
+struct IrkCoef3 {
+    double a[3];
+};
+
+// w = sum_t a[t] x_t  (stage-major x, ns stages of N)
+__global__ void k_irk_combine(IrkCoef3 c, int ns, const double *__restrict__ x, size_t N, double *__restrict__ w,
+                              const int *__restrict__ active) {
+    if (active && !*active) return;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N) return;
+    double v = 0.0;
+    for (int t = 0; t < ns; t++) v = fma(c.a[t], x[(size_t)t * N + e], v);
+    w[e] = v;
+}
+
+// stage-major [s][i][r] <-> cell-interleaved [i][s][r] (forward: to interleaved)
+__global__ void k_irk_interleave(int ns, int n, int bl, const double *__restrict__ in, double *__restrict__ out,
+                                 int forward, const int *__restrict__ active) {
+    if (active && !*active) return;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;   // index in the interleaved vector
+    const size_t tot = (size_t)ns * n * bl;
+    if (e >= tot) return;
+    const size_t i = e / ((size_t)ns * bl);
+    const int rem = (int)(e - i * ns * bl), st = rem / bl, r = rem - st * bl;
+    const size_t sm = ((size_t)st * n + i) * bl + r;
+    if (forward) out[e] = in[sm];
+    else out[sm] = in[e];
+}
+
+// row stripe s of every coupled diagonal block (column-major, side ns bl) from the diagonal
+// blocks of J(cu_s) (pattern: diagonal first in each row) and the mass blocks
+__global__ void k_irk_diag_scatter(int ns, int s, int n, int bl, int bs, const int *__restrict__ rowptr,
+                                   const double *__restrict__ J, const double *__restrict__ mass, IrkCoef3 adt,
+                                   double *__restrict__ D) {
+    const int L = ns * bl;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * bl * L) return;
+    const size_t i = e / ((size_t)bl * L);
+    const int rem = (int)(e - i * bl * L);
+    const int col = rem / bl, r = rem - col * bl;   // column of the coupled block, row within stripe s
+    const int t = col / bl, c = col - t * bl;
+    const double *Jii = J + (size_t)rowptr[i] * bl * bl;
+    double v = adt.a[t] * Jii[(size_t)c * bl + r];
+    if (t == s && r / bs == c / bs) v += mass[(size_t)i * bs * bs + (c % bs) * bs + (r % bs)];
+    D[(size_t)i * L * L + (size_t)col * L + s * bl + r] = v;
+}
+
+struct IrkOp {
+    const cpg_equation_s *E;
+    int ns, n, bl;
+    size_t N;
+    double A[9];
+    const double *negdt;          // -dt in device memory
+    double *w;                    // N scratch
+    double *vol[3], *edg[3];      // point data of each stage's state
+};
+void irk_linop(void *ctx, const double *x, double *y, const int *active, hipStream_t s) {
+    const auto *o = static_cast<const IrkOp *>(ctx);
+    for (int st = 0; st < o->ns; st++) {
+        IrkCoef3 c{};
+        for (int t = 0; t < o->ns; t++) c.a[t] = o->A[st * o->ns + t];
+        hipLaunchKernelGGL(k_irk_combine, dim3(ceil_div(o->N, 256)), dim3(256), 0, s, c, o->ns, x, o->N, o->w, active);
+        CPG_LAUNCH_CHECK();
+        eq_newton_apply_dev(*o->E, o->negdt, o->w, y + o->N * st, active, s, 0.0, x + o->N * st, o->vol[st],
+                            o->edg[st]);
+    }
+}
+
+struct IrkPc {
+    cpg_prec P;
+    int ns, n, bl;
+    double *pin, *pout;
+};
+void irk_pcop(void *ctx, const double *in, double *out, const int *active, hipStream_t s) {
+    const auto *o = static_cast<const IrkPc *>(ctx);
+    const size_t tot = (size_t)o->ns * o->n * o->bl;
+    hipLaunchKernelGGL(k_irk_interleave, dim3(ceil_div(tot, 256)), dim3(256), 0, s, o->ns, o->n, o->bl, in, o->pin, 1,
+                       active);
+    CPG_LAUNCH_CHECK();
+    prec_apply_to(o->P, o->pin, o->pout, s, active);
+    hipLaunchKernelGGL(k_irk_interleave, dim3(ceil_div(tot, 256)), dim3(256), 0, s, o->ns, o->n, o->bl, o->pout, out,
+                       0, active);
+    CPG_LAUNCH_CHECK();
+}
 
 struct LinComb {
     const double *v[6];
@@ -190,7 +286,12 @@ struct cpg_stepper_s {
     cpg_krylov K = nullptr;
     DevBuf<double> k, cu, rhs, r, x, minv, bigR, bigK;
     DevBuf<int> slot_row;
-    DevBuf<double> opscale;    // -dt a_ss of the matrix-free Newton operator
+    DevBuf<double> opscale;    // -dt a_ss of the matrix-free Newton operator (IRK: -dt)
+    // IRK with the matrix-free coupled operator
+    cpg_bsr Dc = nullptr;      // coupled diagonal blocks, one per cell
+    DevBuf<double> irk_data, pin, pout, wbuf;
+    cpg::IrkOp irkop{};
+    cpg::IrkPc irkpc{};
     bool factored = false;     // the preconditioner holds a factor of this step (pc_refresh 2)
     cpg::NewtonOp nop{};
     // CPOLYG_VERBOSE=1 in the environment at creation: one stderr line per Newton iteration,
@@ -201,6 +302,7 @@ struct cpg_stepper_s {
         if (K) cpg_krylov_destroy(K);
         if (J) cpg_bsr_destroy(J);
         if (big) cpg_bsr_destroy(big);
+        if (Dc) cpg_bsr_destroy(Dc);
     }
 };
 
@@ -283,6 +385,41 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
                           params->pc_refresh);
                 S->opscale.alloc(1);
                 S->nop = cpg::NewtonOp{E, S->opscale.p};
+            } else if (method == CPG_STEP_IRK && params->newton_operator == 1) {
+                CPG_CHECK(nstages == 2 || nstages == 3, CPG_ERR_UNSUPPORTED, "Radau IIA with %d stages", nstages);
+                CPG_CHECK(params->restart > 0 && params->maxit > 0 && params->newton_maxit > 0, CPG_ERR_ARG,
+                          "bad solver parameters");
+                CPG_CHECK(params->gmres_mode == CPG_GMRES_FLEXIBLE, CPG_ERR_ARG,
+                          "the matrix-free IRK operator uses flexible GMRES");
+                CPG_CHECK(params->pc_kind == CPG_PC_BJACOBI || params->pc_kind == CPG_PC_DILU, CPG_ERR_UNSUPPORTED,
+                          "the matrix-free IRK step takes the coupled block-Jacobi preconditioner");
+                CPG_CHECK(params->pc_refresh >= 0 && params->pc_refresh <= 2, CPG_ERR_ARG, "bad pc_refresh %d",
+                          params->pc_refresh);
+                const int ns = nstages;
+                check(cpg_jacobian_create(E, &S->J));
+                std::vector<int> rp(n + 1), ci(n);
+                for (int i = 0; i <= n; i++) rp[i] = i;
+                for (int i = 0; i < n; i++) ci[i] = i;
+                check(cpg_bsr_create(n, ns * bl, n, rp.data(), ci.data(), &S->Dc));
+                check(cpg_prec_create(S->Dc, CPG_PC_BJACOBI, params->precision, &S->P));
+                check(cpg_krylov_create(ns * n, bl, params->restart, CPG_GMRES_FLEXIBLE, &S->K));
+                S->bigR.alloc(S->N * ns);
+                S->bigK.alloc(S->N * ns);
+                S->k.alloc(S->N * ns);
+                S->pin.alloc(S->N * ns);
+                S->pout.alloc(S->N * ns);
+                S->wbuf.alloc(S->N);
+                S->opscale.alloc(1);
+                size_t nvol = 0, nedg = 0;
+                eq_newton_data_size(*E, &nvol, &nedg);
+                S->irk_data.alloc((nvol + nedg) * ns);
+                S->irkop = cpg::IrkOp{E, ns, n, bl, S->N, {}, S->opscale.p, S->wbuf.p, {}, {}};
+                for (int j = 0; j < ns * ns; j++) S->irkop.A[j] = S->A[j];
+                for (int st = 0; st < ns; st++) {
+                    S->irkop.vol[st] = nvol ? S->irk_data.p + (nvol + nedg) * st : nullptr;
+                    S->irkop.edg[st] = nvol ? S->irk_data.p + (nvol + nedg) * st + nvol : nullptr;
+                }
+                S->irkpc = cpg::IrkPc{S->P, ns, n, bl, S->pin.p, S->pout.p};
             } else if (method == CPG_STEP_IRK) {
                 CPG_CHECK(nstages == 2 || nstages == 3, CPG_ERR_UNSUPPORTED, "Radau IIA with %d stages", nstages);
                 CPG_CHECK(params->restart > 0 && params->maxit > 0 && params->newton_maxit > 0, CPG_ERR_ARG,
@@ -407,6 +544,88 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
             lincomb(*S, unp1, u, S->b, 4, k, s);
             CPG_HIP(hipStreamSynchronize(s));
             st.ms_total = ms_since(t_all);
+            return;
+        }
+        if (S->method == CPG_STEP_IRK && S->Dc) {
+            // IRK::advance (src/TimeIntegration.cpp:188-281) with the coupled Newton system
+            // applied matrix-free: per Newton iteration the stage residuals, (per pc_refresh)
+            // the stage Jacobians scattered into the coupled diagonal blocks and their
+            // block-Jacobi factor, the point data of every stage's state, then flexible GMRES
+            // warm-started from the previous correction as in the reference
+            const int ns = S->ns;
+            const int bs = equation_bs(*S->E);
+            const size_t NB = N * ns;
+            CPG_HIP(hipMemsetAsync(k, 0, NB * sizeof(double), s));
+            CPG_HIP(hipMemsetAsync(S->bigK.p, 0, NB * sizeof(double), s));
+            hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, S->opscale.p, -dt);
+            CPG_LAUNCH_CHECK();
+            bool converged = false, factored = false;
+            for (int it = 0; it < S->sp.newton_maxit; it++) {
+                auto t0 = clk::now();
+                double rn2 = 0.0;
+                for (int stg = 0; stg < ns; stg++) {
+                    lincomb(*S, S->cu.p, u, &S->A[stg * ns], ns, k, s);
+                    eq_residual(*S->E, S->cu.p, t + dt * S->c[stg], S->rhs.p, s);
+                    const double rs = eq_newton_resid(*S->E, k + N * stg, S->rhs.p, dt, S->bigR.p + N * stg, nullptr, s);
+                    rn2 += rs * rs;
+                }
+                const double rn = std::sqrt(rn2);
+                st.ms_residual += ms_since(t0);
+                st.last_resid = rn;
+                CPG_CHECK(std::isfinite(rn), CPG_ERR_NONFINITE, "non-finite IRK Newton residual (it %d)", it);
+                if (rn < S->sp.newton_tol) { converged = true; break; }
+                st.newton_iters++;
+                if (S->sp.pc_refresh == 0 || !factored) {
+                    t0 = clk::now();
+                    int nnzb = 0;
+                    check(cpg_equation_info(S->E, nullptr, nullptr, &nnzb));
+                    IrkCoef3 adt{};
+                    for (int stg = 0; stg < ns; stg++) {
+                        for (int tt = 0; tt < ns; tt++) adt.a[tt] = -dt * S->A[stg * ns + tt];
+                        lincomb(*S, S->cu.p, u, &S->A[stg * ns], ns, k, s);
+                        eq_jacobian(*S->E, S->cu.p, t + dt * S->c[stg], 1.0, 0, S->J->blocks.p, s);
+                        const size_t cnt = (size_t)S->n * S->bl * ns * S->bl;
+                        hipLaunchKernelGGL(k_irk_diag_scatter, dim3(ceil_div(cnt, 256)), dim3(256), 0, s, ns, stg, S->n,
+                                           S->bl, bs, S->J->rowptr.p, S->J->blocks.p, equation_mass(*S->E), adt,
+                                           S->Dc->blocks.p);
+                        CPG_LAUNCH_CHECK();
+                    }
+                    CPG_HIP(hipStreamSynchronize(s));
+                    st.ms_jacobian += ms_since(t0);
+                    t0 = clk::now();
+                    check(cpg_prec_factor(S->P, S->Dc, s));
+                    st.ms_factor += ms_since(t0);
+                    factored = true;
+                }
+                t0 = clk::now();
+                for (int stg = 0; stg < ns; stg++) {
+                    lincomb(*S, S->cu.p, u, &S->A[stg * ns], ns, k, s);
+                    eq_newton_prepare(*S->E, S->cu.p, t + dt * S->c[stg], s, S->irkop.vol[stg], S->irkop.edg[stg]);
+                }
+                double tol = S->sp.rtol;
+                int maxit = S->sp.maxit;
+                check(guard([&] {
+                    gmres_flexible_op(S->K, LinOp{irk_linop, &S->irkop}, PcOp{irk_pcop, &S->irkpc}, S->bigR.p,
+                                      S->bigK.p, false, &tol, &maxit, s);
+                }));
+                st.ms_gmres += ms_since(t0);
+                st.gmres_iters += maxit;
+                st.linear_solves++;
+                if (S->verbose)
+                    std::fprintf(stderr, "cpolyg: IRK Newton iteration %d, residual norm = %.6e, GMRES iterations %d\n",
+                                 it, rn, maxit);
+                hipLaunchKernelGGL(k_axpy, dim3(ceil_div(NB, 256)), dim3(256), 0, s, k, -1.0, S->bigK.p, NB);
+                CPG_LAUNCH_CHECK();
+            }
+            lincomb(*S, unp1, u, S->b, ns, k, s);
+            CPG_HIP(hipStreamSynchronize(s));
+            st.ms_total = ms_since(t_all);
+            if (!converged) {
+                st.status = CPG_ERR_NOT_CONVERGED;
+                set_error("IRK Newton did not reach %g in %d iterations (last %g)", S->sp.newton_tol,
+                          S->sp.newton_maxit, st.last_resid);
+                throw Error{CPG_ERR_NOT_CONVERGED};
+            }
             return;
         }
         if (S->method == CPG_STEP_IRK) {
