@@ -382,11 +382,17 @@ __global__ __launch_bounds__(WG) void k_bjacobi_factor_ip(const int *__restrict_
         __syncthreads();
         // X[r][c] -= f_r X[k][c] for r != k, with column k taken as 0 before the update
         if (el) {
+            // fixed trip count, unrolled: the LDS reads of many rows are in flight at once
+            constexpr int NR = (BL + NH - 1) / NH;
             const double pk = X[k * LD + cc];
-            for (int r = hc; r < BL; r += NH) {
-                if (r == k) continue;
-                const double x = cc == k ? 0.0 : X[r * LD + cc];
-                X[r * LD + cc] = x - fcol[r] * pk;
+            const bool ck = cc == k;
+#pragma unroll 12
+            for (int j = 0; j < NR; j++) {
+                const int r = hc + NH * j;
+                if (r < BL && r != k) {
+                    const double x = ck ? 0.0 : X[r * LD + cc];
+                    X[r * LD + cc] = x - fcol[r] * pk;
+                }
             }
         }
         __syncthreads();
