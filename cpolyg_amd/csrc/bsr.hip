@@ -25,7 +25,9 @@ void set_error(const char *fmt, ...) {
 
 // ---- row-stream launch shape --------------------------------------------------------
 // Waves per row of the compiled row-stream kernels, per storage element size (8, 4, 2) and
-// block size.  Defaults come from timing sweeps on MI355X (cpg_stream_tune re-measures them).
+// block size.  Defaults come from timing sweeps on MI355X (cpg_stream_autotune re-measures
+// them): four waves, two for BF16 blocks (the BF16 DILU apply at 65,536 rows of 40x40 blocks:
+// 304 us against 316 us with four).  Results do not depend on the choice.
 namespace {
 struct WprEntry {
     int elem, bl, wpr;
@@ -39,7 +41,7 @@ std::vector<WprEntry> &wpr_table() {
 int stream_wpr(int elem_bytes, int bl) {
     for (const auto &e : wpr_table())
         if (e.elem == elem_bytes && e.bl == bl) return e.wpr;
-    return 4;
+    return elem_bytes == 2 ? 2 : 4;
 }
 
 void set_stream_wpr(int elem_bytes, int bl, int wpr) {

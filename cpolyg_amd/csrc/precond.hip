@@ -1163,9 +1163,12 @@ struct cpg_prec_s {
     int nup = 0, nurows = 0;
     DevBuf<int> frows, brows;
     std::vector<int> flev, blev;  // level pointers into frows / brows
+    std::vector<int> frow0, brow0;  // first row of each level whose rows are contiguous, else -1
     // packed factor streams: pos[slot] = packed block; forward stream (L blocks) and backward
     // stream (D^-1, DU blocks) pointers per position in frows / brows; scol = source of each
-    // packed block (column block j >= 0, or -1-i for D_i^-1, which multiplies y_i)
+    // packed block in the tags of stream_source: the column block j (src1 = y forward, x
+    // backward), -1-i for D_i^-1 y_i (src0 = y), or the input vector (src2) where the source
+    // row has no lower blocks, so that y = x there and forward level 0 needs no launch
     DevBuf<int> pos, fptr, bptr, scol;
     int nbmax_f = 1, nbmax_b = 1;
     // storage
@@ -1352,18 +1355,34 @@ void apply_t(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s, const int *act
         return;
     }
     double *y = P.ybuf.p;
-    for (size_t lv = 0; lv + 1 < P.flev.size(); lv++) {
+    constexpr bool x64 = std::is_same<XT, double>::value;
+    auto stream_of = [](const DevBuf<int> &rows, const std::vector<int> &row0, const DevBuf<int> &ptr, int lv, int r0,
+                        const int *scol, const int *active) {
+        RowStream S{row0[lv] >= 0 ? nullptr : rows.p + r0, ptr.p + r0, scol, active};
+        S.row0 = row0[lv] >= 0 ? row0[lv] : 0;
+        return S;
+    };
+    // Level 0 rows have no lower blocks, y_i = x_i; the packed sources of y there name x
+    // (src2), so the level needs no launch for FP64 vectors.  FP32 vectors (the inner solver's)
+    // run it as a copy into y, which their forward sources then read.
+    for (size_t lv = x64 ? 1 : 0; lv + 1 < P.flev.size(); lv++) {
         const int r0 = P.flev[lv], nr = P.flev[lv + 1] - r0;
         if (!nr) continue;
         // y_i = x_i - sum_k L_ik y_k
         StreamArgs<double> a;
-        a.S = RowStream{P.frows.p + r0, P.fptr.p + r0, P.scol.p, active};
+        a.S = stream_of(P.frows, P.frow0, P.fptr, (int)lv, r0, P.scol.p, active);
         a.items = nr;
         a.nbmax = P.nbmax_f;
         a.src1 = y;
         a.c1 = -1.0;
-        if constexpr (std::is_same<XT, double>::value) a.base64 = in;
-        else a.base32 = in;
+        a.c2 = -1.0;
+        if constexpr (x64) {
+            a.base64 = in;
+            a.src2 = in;
+        } else {
+            a.base32 = in;
+            a.src2 = y;
+        }
         a.out = y;
         launch_rowstream<T, double>(bl, a, F, sc, s);
     }
@@ -1372,12 +1391,14 @@ void apply_t(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s, const int *act
         if (!nr) continue;
         // x_i = D_i^-1 y_i - sum_j DU_ij x_j
         StreamArgs<XT> a;
-        a.S = RowStream{P.brows.p + r0, P.bptr.p + r0, P.scol.p, active};
+        a.S = stream_of(P.brows, P.brow0, P.bptr, (int)lv, r0, P.scol.p, active);
         a.items = nr;
         a.nbmax = P.nbmax_b;
         a.src0 = y;
         a.src1 = out;
         a.c1 = -1.0;
+        a.src2 = in;   // in_i = out_i until row i itself writes it (in-place applies)
+        a.c2 = 1.0;
         a.out = out;
         launch_rowstream<T, XT>(bl, a, F, sc, s);
     }
@@ -1442,6 +1463,25 @@ void apply_inner(cpg_prec_s &P, const double *x, double *xout, hipStream_t s) {
     apply_factors<float>(P, t, t, s);                       // x = M^-1 Q y
     hipLaunchKernelGGL(k_f2d, dim3(g256), dim3(256), 0, s, t, xout, N);
     CPG_LAUNCH_CHECK();
+}
+
+// Contiguous levels: row i starts a new level when one of its lower neighbours lies in the
+// current one.  For a colour-ordered matrix every level ends no earlier than the colour it
+// starts in, so there are at most as many levels as colours, each a contiguous row range.
+// Returns the level pointers (row indices).
+std::vector<int> segments(int n, const std::vector<int> &lptr, const std::vector<int> &lcol) {
+    std::vector<int> ptr{0};
+    int start = 0;
+    for (int i = 0; i < n; i++) {
+        for (int q = lptr[i]; q < lptr[i + 1]; q++)
+            if (lcol[q] >= start) {
+                ptr.push_back(i);
+                start = i;
+                break;
+            }
+    }
+    if (n > 0) ptr.push_back(n);
+    return ptr;
 }
 
 // level schedule from the pattern: level(i) = 1 + max level(j) over dependencies
@@ -1538,19 +1578,49 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
             }
             std::vector<int> frows, brows;
             levels(n, lptr, lcol, true, P->flev, frows);
-            levels(n, uptr, ucol, false, P->blev, brows);
+            const std::vector<int> seg = segments(n, lptr, lcol);
+            if (P->dilu() && seg.size() <= P->flev.size()) {
+                // contiguous levels; the backward sweep takes them from the last (valid for the
+                // structurally symmetric pattern checked above: an upper neighbour of a row lies
+                // in a later level)
+                const int nl = (int)seg.size() - 1;
+                P->flev = seg;
+                for (int i = 0; i < n; i++) frows[i] = i;
+                P->blev.assign(1, 0);
+                brows.clear();
+                for (int l = nl - 1; l >= 0; l--) {
+                    for (int i = seg[l]; i < seg[l + 1]; i++) brows.push_back(i);
+                    P->blev.push_back((int)brows.size());
+                }
+            } else {
+                levels(n, uptr, ucol, false, P->blev, brows);
+            }
+            auto first_rows = [](const std::vector<int> &lev, const std::vector<int> &rows) {
+                std::vector<int> r0;
+                for (size_t l = 0; l + 1 < lev.size(); l++) {
+                    bool cont = lev[l + 1] > lev[l];
+                    for (int p = lev[l]; cont && p < lev[l + 1]; p++) cont = rows[p] == rows[lev[l]] + (p - lev[l]);
+                    r0.push_back(cont ? rows[lev[l]] : -1);
+                }
+                return r0;
+            };
+            P->frow0 = first_rows(P->flev, frows);
+            P->brow0 = first_rows(P->blev, brows);
             P->frows.upload(frows);
             P->brows.upload(brows);
             if (P->dilu()) {
                 // packed streams in sweep order
+                CPG_CHECK(n < kSrc2Tag - 1, CPG_ERR_UNSUPPORTED, "%d block rows: too many for the packed sources", n);
                 std::vector<int> pos(A->nnzb, -1), fptr(n + 1, 0), bptr(n + 1, 0), scol;
+                // a lower neighbour k with no lower blocks of its own has y_k = x_k: read x
+                auto src_of = [&](int k) { return lptr[k + 1] > lptr[k] ? k : -1 - (kSrc2Tag + k); };
                 int cur = 0;
                 for (int p = 0; p < n; p++) {
                     const int i = frows[p];
                     fptr[p] = cur;
                     for (int q = lptr[i]; q < lptr[i + 1]; q++) {
                         pos[lslot[q]] = cur++;
-                        scol.push_back(lcol[q]);
+                        scol.push_back(src_of(lcol[q]));
                     }
                     P->nbmax_f = std::max(P->nbmax_f, lptr[i + 1] - lptr[i]);
                 }
@@ -1559,7 +1629,7 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
                     const int i = brows[p];
                     bptr[p] = cur;
                     pos[diag[i]] = cur++;
-                    scol.push_back(-1 - i);
+                    scol.push_back(lptr[i + 1] > lptr[i] ? -1 - i : -1 - (kSrc2Tag + i));
                     for (int q = uptr[i]; q < uptr[i + 1]; q++) {
                         pos[uslot[q]] = cur++;
                         scol.push_back(ucol[q]);

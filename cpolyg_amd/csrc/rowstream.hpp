@@ -9,14 +9,17 @@
 //
 // One workgroup of WPR wavefronts owns one output row:
 //   1. gather: the nb source sub-vectors v_b (bl values each) are read once into LDS.  A
-//      block's source is src1[j] times c1 (j = bcol >= 0) or src0[-1-j] (bcol < 0); the
-//      BF16 per-block power-of-two scale is folded in here, so the stream loop does none of
-//      this work;
+//      block's source (bcol entry, see stream_source) is src1[j] times c1, src0[u], or
+//      src2[k] times c2; the BF16 per-block power-of-two scale is folded in here, so the
+//      stream loop does none of this work;
 //   2. stream: a group of TPC = bl/VW lanes covers one column with 16-byte loads (VW
 //      elements of one column per lane); the row's columns are dealt to G = 4 * 64/TPC
 //      logical groups (column cg to group cg mod G), each wavefront load is a contiguous
 //      ~1 KB segment of the stream, and every lane owns VW fixed rows of 4/WPR logical groups
-//      and keeps their partial sums in registers;
+//      and keeps their partial sums in registers.  Loads go out in chunks of four per lane,
+//      the next chunk issued before the current one is consumed and the first one before
+//      the gather: the short rows of the DILU sweeps (3-5 blocks) are then one round trip
+//      to memory instead of one per column a lane owns;
 //   3. reduce: the G partial vectors are summed through LDS in a fixed order and added to
 //      base_i.  No atomics, and the logical groups do not depend on WPR, so the result is
 //      bitwise the same for every launch shape the autotuner may pick.
@@ -32,15 +35,38 @@ struct bf16 {
     uint16_t v;
 };
 
-// packed streams: item w (blockIdx.x) writes row rows[w] (w if rows is null) from the packed
-// blocks [bptr[w], bptr[w+1]) (block w alone if bptr is null); bcol[b] selects the source
+// packed streams: item w (blockIdx.x) writes row rows[w] (row0 + w if rows is null) from the
+// packed blocks [bptr[w], bptr[w+1]) (block row0 + w alone if bptr is null); bcol[b] selects the source
 // of packed block b (the row itself from src1 if bcol is null)
 struct RowStream {
     const int *rows;
     const int *bptr;
     const int *bcol;
     const int *active = nullptr;   // device flag: when it reads 0 the launch does nothing
+    int row0 = 0;                  // rows null: item w writes row row0 + w (a contiguous level)
 };
+
+// Packed source of a block (bcol entry): j >= 0 reads c1 * src1[j]; -1 - u with u < kSrc2Tag
+// reads src0[u]; -1 - (kSrc2Tag + k) reads c2 * src2[k]
+constexpr int kSrc2Tag = 1 << 30;
+
+template <class XT>
+__device__ inline double stream_source(int j, int c, int bl, const double *__restrict__ src0, const XT *src1,
+                                       double c1, const XT *src2, double c2) {
+    if constexpr (sizeof(XT) == 8) {
+        // one load from a selected pointer: no branch between the column index and its load
+        const int u = -1 - j;
+        const bool s1 = j >= 0, s2 = !s1 && u >= kSrc2Tag;
+        const double *base = s1 ? (const double *)src1 : s2 ? (const double *)src2 : src0;
+        const int r = s1 ? j : s2 ? u - kSrc2Tag : u;
+        const double cf = s1 ? c1 : s2 ? c2 : 1.0;
+        return cf * base[(size_t)r * bl + c];
+    }
+    if (j >= 0) return c1 * (double)src1[(size_t)j * bl + c];
+    const int u = -1 - j;
+    if (u < kSrc2Tag) return src0[(size_t)u * bl + c];
+    return c2 * (double)src2[(size_t)(u - kSrc2Tag) * bl + c];
+}
 
 template <class T>
 struct StreamAcc {
@@ -59,49 +85,46 @@ constexpr int stream_vw() {
     return w;
 }
 
-typedef double stream_d2 __attribute__((ext_vector_type(2)));
-typedef float stream_f4 __attribute__((ext_vector_type(4)));
-typedef float stream_f2 __attribute__((ext_vector_type(2)));
 typedef uint32_t stream_u4 __attribute__((ext_vector_type(4)));
 typedef uint32_t stream_u2 __attribute__((ext_vector_type(2)));
 
-// the blocks are read once per sweep: non-temporal loads keep them from evicting the
-// gathered vectors from the caches
+// One lane's VW elements of a column as raw bytes (16 at most) in a 4-dword register tuple.
+// The blocks are read once per sweep: non-temporal loads keep them from evicting the
+// gathered vectors from the caches.
+template <class T, int VW>
+__device__ inline stream_u4 raw_load(const T *__restrict__ p) {
+    constexpr int B = (int)sizeof(T) * VW;
+    stream_u4 r = {0u, 0u, 0u, 0u};
+    if constexpr (B == 16) {
+        r = __builtin_nontemporal_load(reinterpret_cast<const stream_u4 *>(p));
+    } else if constexpr (B == 8) {
+        const stream_u2 v = __builtin_nontemporal_load(reinterpret_cast<const stream_u2 *>(p));
+        r.x = v.x;
+        r.y = v.y;
+    } else if constexpr (B == 4) {
+        r.x = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p));
+    } else {
+        r.x = (uint32_t)reinterpret_cast<const uint16_t *>(p)[0];
+    }
+    return r;
+}
+// the VW elements of a raw_load as accumulator values (BF16: the high half of an FP32 word,
+// element 2k the low half of word k)
 template <class T, int VW, class A>
-__device__ inline void stream_load(const T *__restrict__ p, A (&a)[VW]) {
+__device__ inline void raw_convert(stream_u4 r, A (&a)[VW]) {
+    uint32_t u[4] = {r.x, r.y, r.z, r.w};
     if constexpr (sizeof(T) == 8) {
-        if constexpr (VW == 2) {
-            const stream_d2 v = __builtin_nontemporal_load(reinterpret_cast<const stream_d2 *>(p));
-            a[0] = (A)v.x;
-            a[1] = (A)v.y;
-        } else {
-            a[0] = (A)__builtin_nontemporal_load(reinterpret_cast<const double *>(p));
+#pragma unroll
+        for (int k = 0; k < VW; k++) {
+            const uint64_t b = ((uint64_t)u[2 * k + 1] << 32) | u[2 * k];
+            double d;
+            __builtin_memcpy(&d, &b, 8);
+            a[k] = (A)d;
         }
     } else if constexpr (sizeof(T) == 4) {
-        if constexpr (VW == 4) {
-            const stream_f4 v = __builtin_nontemporal_load(reinterpret_cast<const stream_f4 *>(p));
-            a[0] = (A)v.x; a[1] = (A)v.y; a[2] = (A)v.z; a[3] = (A)v.w;
-        } else if constexpr (VW == 2) {
-            const stream_f2 v = __builtin_nontemporal_load(reinterpret_cast<const stream_f2 *>(p));
-            a[0] = (A)v.x; a[1] = (A)v.y;
-        } else {
-            a[0] = (A)__builtin_nontemporal_load(reinterpret_cast<const float *>(p));
-        }
+#pragma unroll
+        for (int k = 0; k < VW; k++) a[k] = (A)__uint_as_float(u[k]);
     } else {
-        // BF16: the high half of an FP32 word; element 2k is the low half of word k
-        constexpr int NW = (VW + 1) / 2;
-        uint32_t u[NW];
-        if constexpr (VW == 8) {
-            const stream_u4 v = __builtin_nontemporal_load(reinterpret_cast<const stream_u4 *>(p));
-            u[0] = v.x; u[1] = v.y; u[2] = v.z; u[3] = v.w;
-        } else if constexpr (VW == 4) {
-            const stream_u2 v = __builtin_nontemporal_load(reinterpret_cast<const stream_u2 *>(p));
-            u[0] = v.x; u[1] = v.y;
-        } else if constexpr (VW == 2) {
-            u[0] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p));
-        } else {
-            u[0] = (uint32_t)reinterpret_cast<const uint16_t *>(p)[0];
-        }
 #pragma unroll
         for (int k = 0; k < VW; k++) {
             const uint32_t bits = (k & 1) ? (u[k >> 1] & 0xffff0000u) : (u[k >> 1] << 16);
@@ -119,13 +142,14 @@ inline size_t rowstream_lds(int nbmax) {
     return (size_t)(G + (nbmax > 0 ? nbmax : 1)) * BL * sizeof(A);
 }
 
-// out may alias src1 (rows written by one launch are never sources of the same launch, and a
-// row that is its own source is gathered before it is written).  Exactly one of base64 /
-// base32 may be non-null.
+// out may alias src1 and src2 (rows written by one launch are never sources of the same
+// launch, and a row that is its own source is gathered before it is written).  Exactly one
+// of base64 / base32 may be non-null.
 template <class T, int BL, int WPR, class XT>
 __global__ __launch_bounds__(64 * WPR) void k_rowstream(RowStream S, const T *__restrict__ F,
                                                         const float *__restrict__ scale,
                                                         const double *__restrict__ src0, const XT *src1, double c1,
+                                                        const XT *src2, double c2,
                                                         const double *__restrict__ base64,
                                                         const float *__restrict__ base32, XT *out) {
     using A = typename StreamAcc<T>::type;
@@ -143,54 +167,65 @@ __global__ __launch_bounds__(64 * WPR) void k_rowstream(RowStream S, const T *__
     if (S.active && !*S.active) return;
     const int w = blockIdx.x;
     const int tid = threadIdx.x;
-    const int i = S.rows ? S.rows[w] : w;
-    const int b0 = S.bptr ? S.bptr[w] : w;
+    const int i = S.rows ? S.rows[w] : S.row0 + w;
+    const int b0 = S.bptr ? S.bptr[w] : S.row0 + w;
     const int nb = S.bptr ? S.bptr[w + 1] - b0 : 1;
     const int ncol = nb * BL;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int gl = lane / TPC;
+    const int rr = (lane - gl * TPC) * VW;
+    const int g0 = gl + GW * wv;
+    const T *p = F + (size_t)b0 * BB + rr;
+    // the stream phase takes the row in chunks of PF columns per logical-group slot: every
+    // chunk's loads are issued together (the next chunk's before this one is consumed), and
+    // the first chunk is issued before the gather, so its latency overlaps the gather's
+    constexpr int PF = 4 / LPW;
+    constexpr int CH = G * PF;   // columns per chunk
+    stream_u4 cur[LPW][PF];
+    auto issue = [&](stream_u4 (&r)[LPW][PF], int base) {
+#pragma unroll
+        for (int k = 0; k < PF; k++)
+#pragma unroll
+            for (int l = 0; l < LPW; l++) {
+                const int cg = base + g0 + GW * WPR * l + G * k;
+                if (cg < ncol) r[l][k] = raw_load<T, VW>(p + (size_t)cg * BL);
+            }
+    };
+    if (gl < GW) issue(cur, 0);
     for (int e = tid; e < ncol; e += NT) {
         const int b = e / BL, c = e - b * BL;
         const int j = S.bcol ? S.bcol[b0 + b] : i;
-        double v = j >= 0 ? c1 * (double)src1[(size_t)j * BL + c] : src0[(size_t)(-1 - j) * BL + c];
+        double v = stream_source<XT>(j, c, BL, src0, src1, c1, src2, c2);
         if constexpr (sizeof(T) == 2) v *= (double)scale[b0 + b];
         xs[e] = (A)v;
     }
     __syncthreads();
-    const int lane = tid & 63, wv = tid >> 6;
-    const int gl = lane / TPC;
-    const int rr = (lane - gl * TPC) * VW;
     if (gl < GW) {
-        // logical group of slot l of this lane: gl + GW (wv + WPR l)
         A acc[LPW][VW];
 #pragma unroll
         for (int l = 0; l < LPW; l++)
 #pragma unroll
             for (int v = 0; v < VW; v++) acc[l][v] = 0;
-        const T *p = F + (size_t)b0 * BB + rr;
-        const int g0 = gl + GW * wv;
-        if constexpr (LPW == 1) {
-#pragma unroll 4
-            for (int cg = g0; cg < ncol; cg += G) {
-                A a[VW];
-                stream_load<T, VW, A>(p + (size_t)cg * BL, a);
-                const A xv = xs[cg];
+        for (int base = 0; base < ncol; base += CH) {
+            stream_u4 nxt[LPW][PF];
+            if (base + CH < ncol) issue(nxt, base + CH);
 #pragma unroll
-                for (int v = 0; v < VW; v++) acc[0][v] = fma(a[v], xv, acc[0][v]);
-            }
-        } else {
-#pragma unroll 2
-            for (int base = 0; base < ncol; base += G) {
+            for (int k = 0; k < PF; k++)
 #pragma unroll
                 for (int l = 0; l < LPW; l++) {
-                    const int cg = base + g0 + GW * WPR * l;
+                    const int cg = base + g0 + GW * WPR * l + G * k;
                     if (cg < ncol) {
                         A a[VW];
-                        stream_load<T, VW, A>(p + (size_t)cg * BL, a);
+                        raw_convert<T, VW, A>(cur[l][k], a);
                         const A xv = xs[cg];
 #pragma unroll
                         for (int v = 0; v < VW; v++) acc[l][v] = fma(a[v], xv, acc[l][v]);
                     }
                 }
-            }
+#pragma unroll
+            for (int k = 0; k < PF; k++)
+#pragma unroll
+                for (int l = 0; l < LPW; l++) cur[l][k] = nxt[l][k];
         }
 #pragma unroll
         for (int l = 0; l < LPW; l++)
@@ -213,6 +248,7 @@ template <class T, class XT>
 __global__ __launch_bounds__(256) void k_rowstream_any(int bl, RowStream S, const T *__restrict__ F,
                                                        const float *__restrict__ scale,
                                                        const double *__restrict__ src0, const XT *src1, double c1,
+                                                       const XT *src2, double c2,
                                                        const double *__restrict__ base64,
                                                        const float *__restrict__ base32, XT *out) {
     extern __shared__ double lds_d[];
@@ -220,14 +256,14 @@ __global__ __launch_bounds__(256) void k_rowstream_any(int bl, RowStream S, cons
     if (S.active && !*S.active) return;
     const int w = blockIdx.x;
     const int tid = threadIdx.x;
-    const int i = S.rows ? S.rows[w] : w;
-    const int b0 = S.bptr ? S.bptr[w] : w;
+    const int i = S.rows ? S.rows[w] : S.row0 + w;
+    const int b0 = S.bptr ? S.bptr[w] : S.row0 + w;
     const int nb = S.bptr ? S.bptr[w + 1] - b0 : 1;
     const int ncol = nb * bl;
     for (int e = tid; e < ncol; e += 256) {
         const int b = e / bl, c = e - b * bl;
         const int j = S.bcol ? S.bcol[b0 + b] : i;
-        double v = j >= 0 ? c1 * (double)src1[(size_t)j * bl + c] : src0[(size_t)(-1 - j) * bl + c];
+        double v = stream_source<XT>(j, c, bl, src0, src1, c1, src2, c2);
         if constexpr (sizeof(T) == 2) v *= (double)scale[b0 + b];
         xs[e] = v;
     }
@@ -255,6 +291,8 @@ struct StreamArgs {
     const double *src0 = nullptr;
     const XT *src1 = nullptr;
     double c1 = 1.0;
+    const XT *src2 = nullptr;
+    double c2 = 1.0;
     const double *base64 = nullptr;
     const float *base32 = nullptr;
     XT *out = nullptr;
@@ -267,7 +305,7 @@ inline void launch_rowstream_wpr(const StreamArgs<XT> &a, const T *F, const floa
         CPG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_rowstream<T, BL, WPR, XT>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL((k_rowstream<T, BL, WPR, XT>), dim3(a.items), dim3(64 * WPR), lds, s, a.S, F, scale, a.src0,
-                       a.src1, a.c1, a.base64, a.base32, a.out);
+                       a.src1, a.c1, a.src2, a.c2, a.base64, a.base32, a.out);
     CPG_LAUNCH_CHECK();
 }
 
@@ -305,7 +343,7 @@ inline void launch_rowstream(int bl, const StreamArgs<XT> &a, const T *F, const 
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     CPG_CHECK(lds <= 160 * 1024, CPG_ERR_UNSUPPORTED, "row of %d blocks of %d too long", a.nbmax, bl);
     hipLaunchKernelGGL((k_rowstream_any<T, XT>), dim3(a.items), dim3(256), lds, s, bl, a.S, F, scale, a.src0, a.src1,
-                       a.c1, a.base64, a.base32, a.out);
+                       a.c1, a.src2, a.c2, a.base64, a.base32, a.out);
     CPG_LAUNCH_CHECK();
 }
 

@@ -51,6 +51,74 @@ __all__ = [
 ]
 
 
+def _first_fit(adj: list[list[int]]) -> np.ndarray:
+    """Greedy first-fit colouring in cell order."""
+    colour = np.full(len(adj), -1, np.int64)
+    for i in range(len(adj)):
+        used = {colour[j] for j in adj[i] if colour[j] >= 0}
+        c = 0
+        while c in used:
+            c += 1
+        colour[i] = c
+    return colour
+
+
+def _dsatur(adj: list[list[int]]) -> np.ndarray:
+    """Saturation-degree colouring (Brelaz 1979): repeatedly colour, with its smallest free
+    colour, the uncoloured cell with the most distinct colours among its neighbours (ties:
+    most neighbours, then lowest index)."""
+    import heapq
+
+    n = len(adj)
+    colour = np.full(n, -1, np.int64)
+    sat: list[set] = [set() for _ in range(n)]
+    heap = [(0, -len(adj[i]), i) for i in range(n)]
+    heapq.heapify(heap)
+    while heap:
+        s, d, v = heapq.heappop(heap)
+        if colour[v] >= 0 or -s != len(sat[v]):
+            continue   # stale entry
+        c = 0
+        while c in sat[v]:
+            c += 1
+        colour[v] = c
+        for j in adj[v]:
+            if colour[j] < 0 and c not in sat[j]:
+                sat[j].add(c)
+                heapq.heappush(heap, (-len(sat[j]), -len(adj[j]), j))
+    return colour
+
+
+def _balance(adj: list[list[int]], colour: np.ndarray, passes: int = 6) -> np.ndarray:
+    """Move cells out of classes above the mean size into the smallest class that none of
+    their neighbours holds, until no move is possible (the colouring stays proper)."""
+    colour = colour.copy()
+    nc = int(colour.max()) + 1 if len(colour) else 0
+    if nc < 2:
+        return colour
+    sizes = np.bincount(colour, minlength=nc)
+    mean = len(colour) / nc
+    for _ in range(passes):
+        moved = 0
+        for v in range(len(colour)):
+            c0 = colour[v]
+            if sizes[c0] <= mean:
+                continue
+            used = {colour[j] for j in adj[v]}
+            best = -1
+            for c in range(nc):
+                if c not in used and sizes[c] < sizes[c0] - 1 and (best < 0 or sizes[c] < sizes[best]):
+                    best = c
+            if best >= 0:
+                sizes[c0] -= 1
+                sizes[best] += 1
+                colour[v] = best
+                moved += 1
+        if not moved:
+            break
+    return colour
+
+
 @dataclasses.dataclass
 class PolyMesh:
     v: np.ndarray
@@ -127,60 +195,27 @@ class PolyMesh:
                          - t[:, 0] * t[:, 5] - t[:, 4] * t[:, 3] - t[:, 2] * t[:, 1])
         return np.add.reduceat(a, self.tptr[:-1])
 
-    def multicolour(self, order: str = "greedy") -> "PolyMesh":
-        """Return a copy renumbered colour-major (greedy first-fit colouring).
+    def multicolour(self, order: str = "dsatur") -> "PolyMesh":
+        """Return a copy renumbered colour-major.
+
+        ``order="dsatur"`` (default): saturation-degree colouring (Brelaz), then a balancing
+        pass that moves cells from the largest classes into smaller ones their neighbours
+        leave free.  Every class is one level, hence one launch, of each DILU sweep and of
+        the factor, so few classes of equal size keep every launch large (a 256^2 jittered
+        Voronoi lattice: 5 classes of ~13.1k cells, against 6 from first-fit, two of them
+        nearly empty).  ``order="greedy"``: first-fit in cell order.
 
         Cells keep their relative order inside a colour, so a lattice-ordered
         seed set keeps its locality inside each colour.
         """
         n = self.n_cells
         adj = self.adjacency()
-        colour = np.full(n, -1, np.int64)
-        for i in range(n):
-            used = {colour[j] for j in adj[i] if colour[j] >= 0}
-            c = 0
-            while c in used:
-                c += 1
-            colour[i] = c
-        # empty the smallest classes where their cells fit another colour: every class
-        # is one launch per DILU sweep, so a near-empty trailing class costs two launches
-        for _ in range(2):
-            nc = int(colour.max()) + 1
-            if nc <= 2:
-                break
-            sizes = np.bincount(colour, minlength=nc)
-            small = int(np.argmin(sizes))
-            members = np.nonzero(colour == small)[0]
-            moved = colour.copy()
-            ok = True
-            for i in members:
-                used = {moved[j] for j in adj[i]}
-                cand = [c for c in range(nc) if c != small and c not in used]
-                if not cand:
-                    # free a colour c held by a single neighbour w by moving w elsewhere
-                    for c in range(nc):
-                        if c == small:
-                            continue
-                        holders = [j for j in adj[i] if moved[j] == c]
-                        if len(holders) != 1:
-                            continue
-                        w = holders[0]
-                        wused = {moved[j] for j in adj[w]}
-                        alt = [d for d in range(nc) if d not in (small, c) and d not in wused]
-                        if alt:
-                            sizes[moved[w]] -= 1
-                            moved[w] = alt[0]
-                            sizes[alt[0]] += 1
-                            cand = [c]
-                            break
-                if not cand:
-                    ok = False
-                    break
-                moved[i] = min(cand, key=lambda c: sizes[c])
-                sizes[moved[i]] += 1
-            if not ok:
-                break
-            colour = np.where(moved > small, moved - 1, moved)
+        if order == "greedy":
+            colour = _first_fit(adj)
+        elif order == "dsatur":
+            colour = _balance(adj, _dsatur(adj))
+        else:
+            raise ValueError(f"unknown colouring order {order!r}")
         perm = np.argsort(colour, kind="stable")  # new -> old
         m = self.permute(perm)
         m.colour = colour[perm].astype(np.int32)

@@ -58,7 +58,7 @@ def test_dilu_fp64_factor_and_apply(cuda, deg):
     A = hip.Bsr(orc.rowptr, orc.colind, bl)
     A.upload(B)
     P = hip.Prec(A, PC_DILU, PREC_FP64)
-    assert P.levels() == m.n_colours
+    assert 2 <= P.levels() <= m.n_colours   # contiguous levels: at most one per colour
     P.factor()
     F = P.download()
     Fo, info = O.dilu_factor(orc.rowptr, orc.colind, B)

@@ -68,6 +68,24 @@ def test_multicolour_ordering_property():
     assert np.all(np.diff(m.colour) >= 0)
 
 
+@pytest.mark.parametrize("order", ["dsatur", "greedy"])
+def test_multicolour_is_proper_and_balanced(order):
+    """Both colourings are proper; the default one has at most 5 classes of near-equal size on a
+    jittered Voronoi lattice (each class is one launch per DILU sweep)."""
+    base = periodic_voronoi(jittered_lattice_seeds(24, 24, 1.0, 1.0, 0.3, 2), 1.0, 1.0, colour=False)
+    m = base.multicolour(order)
+    m.check()
+    adj = m.adjacency()
+    assert all(m.colour[i] != m.colour[j] for i in range(m.n_cells) for j in adj[i])
+    sizes = np.diff(m.colour_ptr)
+    assert sizes.sum() == base.n_cells
+    if order == "dsatur":
+        assert m.n_colours <= 5
+        assert sizes.max() - sizes.min() <= max(2, sizes.max() // 50)
+    # a renumbering of the same cells
+    np.testing.assert_allclose(np.sort(m.areas()), np.sort(base.areas()))
+
+
 def test_save_load_roundtrip(tmp_path):
     m = periodic_voronoi(jittered_lattice_seeds(6, 6, 1.0, 1.0, 0.3, 9), 1.0, 1.0)
     m.save(str(tmp_path / "m.npz"))

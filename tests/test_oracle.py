@@ -136,7 +136,10 @@ def test_polygon_quadrature_and_mass(mesh_fn):
 
     for i in (0, m.n_cells // 2):
         x0, y0, w, h = m.bb[i]
-        pts = m.v[m.polygon(i)]
+        # moments of the cell moved to its bounding-box corner (the mass block is invariant
+        # under the shift; absolute coordinates would cost the power series digits)
+        pts = m.v[m.polygon(i)] - np.array([x0, y0])
+        x0 = y0 = 0.0
         # phi_k as 2-D power series in (x, y): P_a(2(x-x0)/w - 1) P_b(2(y-y0)/h - 1)
         def series(n, s, z0):
             c = np.zeros(n + 1)
