@@ -317,116 +317,144 @@ __global__ __launch_bounds__(WG) void k_bjacobi_factor(int bl, const int *__rest
     store_block<T, NE>(bl, F + (size_t)i * bb, scale ? scale + i : nullptr, c, scratch);
 }
 
-// block Jacobi for blocks too large for the augmented Gauss-Jordan (the stage-coupled
-// 120 x 120 blocks of a 3-stage Radau IIA step): in-place Gauss-Jordan with partial pivoting
-// (first maximal |pivot|) in LDS, row-major with a padded stride, then the column
-// permutation undone.  One workgroup per block; thread (h, c) of the elimination owns
-// column c and the rows h, h + NH, ... (no divisions in the inner loop).
-constexpr int IPMAX = 128;
+// Block Jacobi for the stage-coupled 120 x 120 blocks, register-tiled: thread (rg, cg) of an
+// RG x CG grid keeps rows [RPT rg, RPT rg + RPT) x columns [CPT cg, CPT cg + CPT) of the block
+// in registers.  Gauss-Jordan in exchange form with partial pivoting (first maximal |pivot|
+// among the rows not yet pivotal): step k exchanges column k with pivot row p_k.  The owners
+// of column k post their best candidate and the column to LDS, every thread picks the pivot
+// from the RG candidates, the owners of row p_k post that row, and each thread updates its
+// tile: X[p][k] = 1/v, X[p][c] = -X[p][c]/v, X[r][k] = X[r][k]/v, X[r][c] -= X[r][k] X[p][c]/v
+// (v = X[p][k]).  Rows are never swapped: the inverse is A^-1[q(r)][p_c] = X[r][c] with
+// q(p_k) = k.  Two barriers per step; the tile is the only large state, so several blocks
+// share a CU (a block held in LDS, 116 KB, leaves room for one: 2.5 us per block against
+// 0.74 us here at 65,536 blocks).
+template <int BL>
+struct RTile;
+template <>
+struct RTile<120> {
+    static constexpr int RG = 12, CG = 20, RPT = 10, CPT = 6;
+};
 
 template <class T, int BL>
-__global__ __launch_bounds__(WG) void k_bjacobi_factor_ip(const int *__restrict__ diag, const double *__restrict__ A,
+__global__ __launch_bounds__(WG) void k_bjacobi_factor_rt(const int *__restrict__ diag, const double *__restrict__ A,
                                                           T *__restrict__ F, float *__restrict__ scale,
                                                           int *__restrict__ status) {
-    static_assert(BL <= IPMAX && 2 * BL <= WG, "block size");
-    constexpr int LD = BL + 1, NH = WG / BL;
-    extern __shared__ double lds[];
-    double *X = lds;                   // X[r * LD + c]
-    double *fcol = X + (size_t)BL * LD; // column k of the rows being eliminated
-    __shared__ int piv[IPMAX];
+    using R = RTile<BL>;
+    constexpr int RG = R::RG, CG = R::CG, RPT = R::RPT, CPT = R::CPT;
+    static_assert(RG * RPT == BL && CG * CPT == BL && RG * CG <= WG && RPT <= 32, "tiling");
+    __shared__ double cv[2][RG];
+    __shared__ int cr[2][RG];
+    __shared__ double fcol[2][BL], prow[2][BL];
+    __shared__ int pr[BL], qr[BL];
     __shared__ double red[WG / 64];
-    __shared__ int bad;
     const int i = blockIdx.x, tid = threadIdx.x;
+    const bool act = tid < RG * CG;
+    const int rg = act ? tid % RG : 0, cg = act ? tid / RG : 0;   // consecutive threads: row groups
+    const int r0 = rg * RPT, c0 = cg * CPT;
     constexpr size_t bb = (size_t)BL * BL;
     const double *a = A + (size_t)diag[i] * bb;
-    for (int e = tid; e < (int)bb; e += WG) {
-        const int c = e / BL, r = e - c * BL;
-        X[r * LD + c] = a[e];
-    }
-    if (tid == 0) bad = 0;
-    const int hc = tid / BL, cc = tid - hc * BL;   // elimination role
-    const bool el = hc < NH;
-    __syncthreads();
+    double x[RPT][CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; c++)
+#pragma unroll
+        for (int r = 0; r < RPT; r++) x[r][c] = act ? a[(size_t)(c0 + c) * BL + r0 + r] : 0.0;
+    unsigned used = 0;   // bit r: row r0 + r has been a pivot row
+    bool ok = true;
+#pragma unroll 1
     for (int k = 0; k < BL; k++) {
-        // pivot row: first maximal |X[r][k]|, r >= k (wavefront 0)
-        if (tid < 64) {
-            double best = -1.0;
-            int br = BL;
-            for (int r = k + tid; r < BL; r += 64) {
-                const double v = fabs(X[r * LD + k]);
-                if (v > best) { best = v; br = r; }
-            }
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ob = __shfl_xor(best, off);
-                const int orr = __shfl_xor(br, off);
-                if (ob > best || (ob == best && orr < br)) { best = ob; br = orr; }
-            }
-            if (tid == 0) {
-                piv[k] = br;
-                if (!(best > 0.0)) bad = 1;
-            }
-        }
-        __syncthreads();
-        if (bad) break;
-        const int p = piv[k];
-        if (p != k && tid < BL) {
-            const double t = X[k * LD + tid];
-            X[k * LD + tid] = X[p * LD + tid];
-            X[p * LD + tid] = t;
-        }
-        __syncthreads();
-        const double d = 1.0 / X[k * LD + k];
-        if (tid < BL) fcol[tid] = tid == k ? 0.0 : X[tid * LD + k];
-        __syncthreads();
-        if (tid < BL) X[k * LD + tid] = (tid == k ? 1.0 : X[k * LD + tid]) * d;
-        __syncthreads();
-        // X[r][c] -= f_r X[k][c] for r != k, with column k taken as 0 before the update
-        if (el) {
-            // fixed trip count, unrolled: the LDS reads of many rows are in flight at once
-            constexpr int NR = (BL + NH - 1) / NH;
-            const double pk = X[k * LD + cc];
-            const bool ck = cc == k;
-#pragma unroll 12
-            for (int j = 0; j < NR; j++) {
-                const int r = hc + NH * j;
-                if (r < BL && r != k) {
-                    const double x = ck ? 0.0 : X[r * LD + cc];
-                    X[r * LD + cc] = x - fcol[r] * pk;
+        const int b = k & 1;
+        const int kg = k / CPT, kc = k - kg * CPT;
+        if (act && cg == kg) {
+            double best = -1.0, bv = 0.0;
+            int br = -1;
+#pragma unroll
+            for (int r = 0; r < RPT; r++) {
+                double v = x[r][0];
+#pragma unroll
+                for (int c = 1; c < CPT; c++) v = kc == c ? x[r][c] : v;
+                fcol[b][r0 + r] = v;
+                const double m = fabs(v);
+                if (!((used >> r) & 1u) && m > best) {
+                    best = m;
+                    bv = v;
+                    br = r0 + r;
                 }
             }
+            cv[b][rg] = bv;
+            cr[b][rg] = br;
         }
         __syncthreads();
+        double pa = -1.0, pv = 0.0;
+        int p = -1;
+#pragma unroll
+        for (int g = 0; g < RG; g++) {
+            const int r = cr[b][g];
+            const double v = cv[b][g];
+            if (r >= 0 && fabs(v) > pa) {
+                pa = fabs(v);
+                pv = v;
+                p = r;
+            }
+        }
+        if (p < 0 || !(pa > 0.0) || !isfinite(pv)) {   // the same for every thread
+            ok = false;
+            break;
+        }
+        const int pg = p / RPT, pl = p - pg * RPT;
+        if (act && rg == pg) {
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                double v = x[0][c];
+#pragma unroll
+                for (int r = 1; r < RPT; r++) v = pl == r ? x[r][c] : v;
+                prow[b][c0 + c] = v;
+            }
+        }
+        if (tid == 0) pr[k] = p;
+        __syncthreads();
+        const double ip = 1.0 / pv;
+        double f[RPT], q[CPT];
+#pragma unroll
+        for (int r = 0; r < RPT; r++) f[r] = fcol[b][r0 + r];
+#pragma unroll
+        for (int c = 0; c < CPT; c++) q[c] = prow[b][c0 + c] * ip;
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            const bool prw = rg == pg && r == pl;
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const bool pcl = c0 + c == k;
+                x[r][c] = prw ? (pcl ? ip : -q[c]) : (pcl ? f[r] * ip : fma(-f[r], q[c], x[r][c]));
+            }
+        }
+        if (rg == pg) used |= 1u << pl;
     }
-    if (bad) {
+    if (!ok) {
         if (tid == 0) atomicOr(status, 1);
         return;
     }
-    // undo the row interchanges on the columns of the inverse
-    for (int k = BL - 1; k >= 0; k--) {
-        const int p = piv[k];
-        if (p != k && tid < BL) {
-            const double t = X[tid * LD + k];
-            X[tid * LD + k] = X[tid * LD + p];
-            X[tid * LD + p] = t;
-        }
-        __syncthreads();
-    }
+    if (tid < BL) qr[pr[tid]] = tid;
+    __syncthreads();
     float inv = 1.0f;
     if constexpr (Store<T>::prec == CPG_BF16) {
         double mx = 0.0;
-        for (int e = tid; e < (int)bb; e += WG) {
-            const int c = e / BL, r = e - c * BL;
-            mx = fmax(mx, fabs(X[r * LD + c]));
-        }
+#pragma unroll
+        for (int r = 0; r < RPT; r++)
+#pragma unroll
+            for (int c = 0; c < CPT; c++) mx = fmax(mx, act ? fabs(x[r][c]) : 0.0);
         mx = wg_max_abs(mx, red);
         const float sc = pow2_scale(mx);
         inv = 1.0f / sc;
         if (tid == 0) scale[i] = sc;
     }
-    T *f = F + (size_t)i * bb;
-    for (int e = tid; e < (int)bb; e += WG) {
-        const int c = e / BL, r = e - c * BL;
-        Store<T>::put(f, e, X[r * LD + c], inv);
+    if (act) {
+        T *f = F + (size_t)i * bb;
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const size_t col = (size_t)pr[c0 + c] * BL;
+#pragma unroll
+            for (int r = 0; r < RPT; r++) Store<T>::put(f, col + qr[r0 + r], x[r][c], inv);
+        }
     }
 }
 
@@ -1237,11 +1265,8 @@ void factor_dilu_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
 }
 
 template <class T, int BL>
-void factor_bj_ip_bl(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
-    const size_t lds = ((size_t)BL * (BL + 1) + BL) * sizeof(double);
-    auto kern = k_bjacobi_factor_ip<T, BL>;
-    set_lds(kern, lds);
-    hipLaunchKernelGGL(kern, dim3(P.n), dim3(WG), lds, s, P.diag.p, A.blocks.p, Fp<T>(P),
+void factor_bj_rt_bl(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
+    hipLaunchKernelGGL((k_bjacobi_factor_rt<T, BL>), dim3(P.n), dim3(WG), 0, s, P.diag.p, A.blocks.p, Fp<T>(P),
                        P.precision == CPG_BF16 ? P.scale.p : nullptr, P.status.p);
     CPG_LAUNCH_CHECK();
 }
@@ -1249,9 +1274,9 @@ void factor_bj_ip_bl(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
 // block sizes beyond the augmented kernel: the coupled Radau IIA blocks (ns stages x 4 x bs),
 // 120 for 3 stages at p = 3 and 2 stages at p = 4
 template <class T>
-void factor_bj_ip_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
+void factor_bj_big_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
     if (P.bl == 120) {
-        factor_bj_ip_bl<T, 120>(P, A, s);
+        factor_bj_rt_bl<T, 120>(P, A, s);
         return;
     }
     CPG_CHECK(false, CPG_ERR_UNSUPPORTED, "block size %d too large for block Jacobi", P.bl);
@@ -1262,7 +1287,7 @@ void factor_bj_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
     const int bl = P.bl;
     const size_t lds = (size_t)(2 * bl * bl + bl + 8) * sizeof(double);
     if (lds > 160 * 1024) {
-        factor_bj_ip_t<T>(P, A, s);
+        factor_bj_big_t<T>(P, A, s);
         return;
     }
     CPG_CHECK(lds <= 160 * 1024, CPG_ERR_UNSUPPORTED, "block size %d too large for the LDS factor kernel", bl);
