@@ -78,6 +78,8 @@ def parse():
     p.add_argument("--irk-nx", type=int, default=512, help="irk: lattice size (cells = nx^2)")
     p.add_argument("--irk-cfl", type=float, default=10.0)
     p.add_argument("--irk-restart", type=int, default=60)
+    p.add_argument("--irk-pc", default="bjacobi,dilu",
+                   help="irk: preconditioners, comma-separated (bjacobi: BF16 and FP64 factors; dilu: BF16 and FP32)")
     return p.parse_args()
 
 
@@ -293,16 +295,18 @@ def bench_batch(args):
 def bench_irk(args):
     """Radau IIA (3 stages) for compressible Euler, density 1 + 0.2 sin(2 pi x) sin(2 pi y), velocity
     (1, 0.5), pressure 1, periodic unit square, Voronoi mesh of a jittered nx^2 lattice (seed 5), p=3.
-    The stage-coupled Newton system (3 x 40 unknowns per cell) is applied matrix-free; block Jacobi on
-    the cell-interleaved 120 x 120 coupled diagonal blocks, factors in BF16 (headline) and FP64;
-    FP64 flexible GMRES(--irk-restart) to 1e-10; Newton to 1e-10 (the reference's 5e-13 is an absolute
-    L2 bound set for its small meshes)."""
+    The stage-coupled Newton system (3 x 40 unknowns per cell) is applied matrix-free and
+    preconditioned on its cell-interleaved form (120 x 120 blocks) by block Jacobi (factors in BF16
+    and FP64) and by block DILU formed from the three stage Jacobians (BF16 and FP32: FP64 DILU
+    factors of 262,144 cells are 211 GB); FP64 flexible GMRES(--irk-restart) to 1e-10; Newton to
+    1e-10 (the reference's 5e-13 is an absolute L2 bound set for its small meshes)."""
     import numpy as np
     import torch
 
     from cpolyg_amd import hip, setups
     from cpolyg_amd.mesh import jittered_lattice_seeds, periodic_voronoi
-    from cpolyg_amd.spec import EquationSpec, GMRES_FLEXIBLE, OP_MATRIX_FREE, PC_BJACOBI, PREC_BF16, PREC_FP64, SolverSpec
+    from cpolyg_amd.spec import (EquationSpec, GMRES_FLEXIBLE, OP_MATRIX_FREE, PC_BJACOBI, PC_DILU, PREC_BF16,
+                                 PREC_FP32, PREC_FP64, SolverSpec)
 
     nx, deg = args.irk_nx, 3
     t0 = time.perf_counter()
@@ -312,11 +316,19 @@ def bench_irk(args):
     n = mesh.n_cells
     dt = setups.dg_time_step(args.irk_cfl, 1.0 / nx, deg, setups.euler_max_speed(u0, deg))
     E = hip.Equation(hip.Mesh(mesh), EquationSpec.euler(periodic_trace=1, jacobian=1), deg)
+    runs = []
+    for pc in args.irk_pc.split(","):
+        if pc == "bjacobi":
+            runs += [("bjacobi_bf16", PC_BJACOBI, PREC_BF16), ("bjacobi_fp64", PC_BJACOBI, PREC_FP64)]
+        elif pc == "dilu":
+            runs += [("dilu_bf16", PC_DILU, PREC_BF16), ("dilu_fp32", PC_DILU, PREC_FP32)]
+        else:
+            raise SystemExit(f"unknown --irk-pc entry {pc!r}")
     res = {}
     free0, total = torch.cuda.mem_get_info()
-    for name, prec in (("bf16", PREC_BF16), ("fp64", PREC_FP64)):
+    for name, pc, prec in runs:
         sp = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=args.irk_restart, rtol=1e-10, maxit=6000,
-                        pc_kind=PC_BJACOBI, newton_tol=1e-10, newton_maxit=10, precision=prec,
+                        pc_kind=pc, newton_tol=1e-10, newton_maxit=10, precision=prec,
                         newton_operator=OP_MATRIX_FREE, pc_refresh=args.pc_refresh)
         S = hip.Stepper(E, 1, 3, sp)
         free1, _ = torch.cuda.mem_get_info()
@@ -343,20 +355,26 @@ def bench_irk(args):
                                            for k in ("ms_residual", "ms_jacobian", "ms_factor", "ms_gmres")},
                      "last_resid": sts[-1]["last_resid"],
                      "stepper_device_bytes": free0 - free1, "u": u.cpu().numpy()}
-        del S
+        print(f"# {name}: {res[name]['ms_per_step']:.1f} ms/step, {res[name]['gmres_iters_per_step']:.1f} "
+              f"GMRES iterations", file=sys.stderr, flush=True)
+        del S, u, nxt
         torch.cuda.synchronize()
-    diff = float(np.max(np.abs(res["bf16"]["u"] - res["fp64"]["u"])) / np.max(np.abs(res["fp64"]["u"])))
+    # every run against the first FP64-factor one (or the first run)
+    base = next((k for k in res if k.endswith("fp64")), next(iter(res)))
+    ub = res[base]["u"]
+    for r in res.values():
+        r["max_rel_diff_vs_" + base] = float(np.max(np.abs(r["u"] - ub)) / np.max(np.abs(ub)))
     for r in res.values():
         del r["u"]
+    head = next((k for k in ("dilu_bf16", "bjacobi_bf16") if k in res), base)
     out = {"metric": "Radau IIA (3-stage) steps/s, Euler p=3, 262k cells, coupled system matrix-free",
-           "value": res["bf16"]["steps_per_s"], "unit": "IRK steps/s", "steps": args.steps, "warmup": args.warmup,
+           "value": res[head]["steps_per_s"], "unit": "IRK steps/s", "value_run": head, "steps": args.steps,
+           "warmup": args.warmup,
            "config": {"workload": f"Euler density wave, periodic Voronoi {n} cells ({nx}^2 jittered lattice), p={deg}, "
-                                  f"Radau IIA 3 stages at CFL {args.irk_cfl:g}, matrix-free coupled operator, block "
-                                  f"Jacobi on 120x120 coupled blocks, FGMRES({args.irk_restart}) rtol 1e-10",
+                                  f"Radau IIA 3 stages at CFL {args.irk_cfl:g}, matrix-free coupled operator, "
+                                  f"cell-interleaved 120x120 coupled blocks, FGMRES({args.irk_restart}) rtol 1e-10",
                       "cells": n, "dt": dt, "pc_refresh": args.pc_refresh, "colours": mesh.n_colours},
-           "bf16": res["bf16"], "fp64": res["fp64"], "speedup_bf16_vs_fp64": res["fp64"]["ms_per_step"] /
-           res["bf16"]["ms_per_step"], "bf16_vs_fp64_max_rel_diff": diff, "gpu_memory_total_bytes": total,
-           "setup_s": t_setup}
+           "runs": res, "gpu_memory_total_bytes": total, "setup_s": t_setup}
     print(json.dumps(out))
 
 

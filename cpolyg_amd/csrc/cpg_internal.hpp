@@ -133,6 +133,18 @@ struct PcOp {
     void (*fn)(void *ctx, const double *in, double *out, const int *active, hipStream_t s);
     void *ctx;
 };
+// Block DILU of the stage-coupled Radau IIA system in its cell-interleaved form (3 stages of
+// 40 x 40 blocks: bl 120).  The coupled block of slot q is coef[s][t] J_s[q] (+ the mass block
+// on the diagonal), built on the fly from the stage Jacobians; P is a DILU handle created over
+// the 120-wide pattern of the stage Jacobians (precond.hip)
+struct CoupledBlocks {
+    const double *J[3];   // stage Jacobian blocks (bl 40, one BSR pattern)
+    const double *mass;   // per-cell bs x bs mass blocks
+    double coef[9];       // coef[3 s + t] = -dt A[s][t]
+    int bs;
+    int n, nnzb;          // block rows and slots of the stage pattern
+};
+void prec_factor_coupled(cpg_prec P, const CoupledBlocks &C, hipStream_t s);
 // flexible GMRES (K in CPG_GMRES_FLEXIBLE mode) with the preconditioner as a PcOp
 void gmres_flexible_op(cpg_krylov K, const LinOp &A, const PcOp &M, const double *b, double *x, bool x_zero,
                        double *tol, int *maxit, hipStream_t s);

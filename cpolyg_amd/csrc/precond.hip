@@ -335,10 +335,12 @@ struct RTile<120> {
     static constexpr int RG = 12, CG = 20, RPT = 10, CPT = 6;
 };
 
-template <class T, int BL>
-__global__ __launch_bounds__(WG) void k_bjacobi_factor_rt(const int *__restrict__ diag, const double *__restrict__ A,
-                                                          T *__restrict__ F, float *__restrict__ scale,
-                                                          int *__restrict__ status) {
+// inverse of the block at a (column-major, FP64) to outT in precision T (BF16: its scale to
+// *scale_out) and, with OUT64, in FP64 to out64 (which may equal a: every thread has read its
+// tile before the first barrier of the elimination); whole workgroup
+template <class T, int BL, bool OUT64>
+__device__ inline void rt_gauss_jordan(const double *a, double *out64, T *__restrict__ outT,
+                                       float *scale_out, int *__restrict__ status) {
     using R = RTile<BL>;
     constexpr int RG = R::RG, CG = R::CG, RPT = R::RPT, CPT = R::CPT;
     static_assert(RG * RPT == BL && CG * CPT == BL && RG * CG <= WG && RPT <= 32, "tiling");
@@ -347,12 +349,10 @@ __global__ __launch_bounds__(WG) void k_bjacobi_factor_rt(const int *__restrict_
     __shared__ double fcol[2][BL], prow[2][BL];
     __shared__ int pr[BL], qr[BL];
     __shared__ double red[WG / 64];
-    const int i = blockIdx.x, tid = threadIdx.x;
+    const int tid = threadIdx.x;
     const bool act = tid < RG * CG;
     const int rg = act ? tid % RG : 0, cg = act ? tid / RG : 0;   // consecutive threads: row groups
     const int r0 = rg * RPT, c0 = cg * CPT;
-    constexpr size_t bb = (size_t)BL * BL;
-    const double *a = A + (size_t)diag[i] * bb;
     double x[RPT][CPT];
 #pragma unroll
     for (int c = 0; c < CPT; c++)
@@ -445,17 +445,29 @@ __global__ __launch_bounds__(WG) void k_bjacobi_factor_rt(const int *__restrict_
         mx = wg_max_abs(mx, red);
         const float sc = pow2_scale(mx);
         inv = 1.0f / sc;
-        if (tid == 0) scale[i] = sc;
+        if (tid == 0) *scale_out = sc;
     }
     if (act) {
-        T *f = F + (size_t)i * bb;
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
             const size_t col = (size_t)pr[c0 + c] * BL;
 #pragma unroll
-            for (int r = 0; r < RPT; r++) Store<T>::put(f, col + qr[r0 + r], x[r][c], inv);
+            for (int r = 0; r < RPT; r++) {
+                Store<T>::put(outT, col + qr[r0 + r], x[r][c], inv);
+                if constexpr (OUT64) out64[col + qr[r0 + r]] = x[r][c];
+            }
         }
     }
+}
+
+template <class T, int BL>
+__global__ __launch_bounds__(WG) void k_bjacobi_factor_rt(const int *__restrict__ diag, const double *__restrict__ A,
+                                                          T *__restrict__ F, float *__restrict__ scale,
+                                                          int *__restrict__ status) {
+    constexpr size_t bb = (size_t)BL * BL;
+    const int i = blockIdx.x;
+    rt_gauss_jordan<T, BL, false>(A + (size_t)diag[i] * bb, nullptr, F + (size_t)i * bb,
+                                  scale ? scale + i : nullptr, status);
 }
 
 // ---- wavefront-per-row factor: FP64 MFMA block products, in-register Gauss-Jordan -------
@@ -1176,6 +1188,226 @@ __global__ __launch_bounds__(IWG) void k_f32_combine(const float *__restrict__ Q
     }
 }
 
+// ---- block DILU of the stage-coupled Radau IIA system (cell-interleaved 120 x 120 blocks) ---
+// The coupled block of slot q (block row i, block column j) is never stored:
+//   C_q[(s,r),(t,c)] = coef[s][t] J_s[q](r,c)  (+ m_i(r mod bs, c mod bs) when i = j, s = t and
+//   r, c lie in the same component),
+// built from the three stage Jacobians (40 x 40 column-major blocks of one pattern) and the
+// per-cell mass blocks.  The factor is the DILU above on these blocks, in FP64 on
+// v_mfma_f64_16x16x4f64 (8 x 8 tiles of 16 over the padded 128 x 128 block; wavefront w of the
+// workgroup owns tile columns w and w + 4, 16 tiles):
+//   Schur (one workgroup per row i of a level): S_i = C_ii in DD64[i], then per lower neighbour
+//     k: X <- D_k^-1, T = C_ik X (registers), X <- T (stored as L_ik), S_i -= X C_ki;
+//   inverse: D_i^-1 = S_i^-1 by the register-tiled Gauss-Jordan, to DD64[i] and the factors;
+//   upper (after all levels): DU_ij = D_i^-1 C_ij.
+// LDS: X (120 x 121 doubles) and the three stage blocks of one slot (3 x 40 x 41): 152 KB.
+struct CoupledSrc {
+    const double *J0, *J1, *J2;
+    const double *mass;
+    double coef[9];   // coef[3 s + t]
+    int bs;
+};
+constexpr int CBL = 120, CSB = 40, CLD = CSB + 1, CTM = 8, CXL = WF<CBL>::LD;
+constexpr size_t CBB = (size_t)CBL * CBL;
+constexpr int CSTAGE = CSB * CLD;   // LDS doubles of one staged stage block
+constexpr size_t CLDS = ((size_t)CBL * CXL + 3 * CSTAGE) * sizeof(double);
+
+__device__ inline double coupled_coef(const CoupledSrc &src, int s, int t) {
+    const int idx = 3 * s + t;
+    double v = src.coef[0];
+#pragma unroll
+    for (int j = 1; j < 9; j++) v = idx == j ? src.coef[j] : v;
+    return v;
+}
+
+// Jl <- the three stage blocks of slot q (column stride CLD), all threads
+__device__ inline void coupled_stage(double *Jl, const CoupledSrc &src, int q) {
+    constexpr int SB = CSB * CSB;
+    for (int e = threadIdx.x; e < 3 * SB; e += FWG) {
+        const int st = e / SB, rem = e - st * SB, c = rem / CSB, r = rem - c * CSB;
+        const double *J = st == 0 ? src.J0 : st == 1 ? src.J1 : src.J2;
+        Jl[st * CSTAGE + c * CLD + r] = J[(size_t)q * SB + rem];
+    }
+}
+
+// acc[2 tm + h] = tile (tm, w + 4 h) of C_q X (C_q from the staged Jl, X in LDS, stride CXL)
+__device__ inline void coupled_times_x(const double *Jl, const double *X, const CoupledSrc &src, dx4 (&acc)[16]) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < 16; j++) acc[j] = dx4{0.0, 0.0, 0.0, 0.0};
+    int sm[CTM], jb[CTM];   // stage and staged-row offset of this lane's row in tile row tm
+#pragma unroll
+    for (int tm = 0; tm < CTM; tm++) {
+        const int m = 16 * tm + lr;
+        sm[tm] = m < CBL ? m / CSB : 3;
+        jb[tm] = m < CBL ? sm[tm] * CSTAGE + (m - sm[tm] * CSB) : 0;
+    }
+    const int n0 = 16 * wv + lr, n1 = 16 * (wv + 4) + lr;
+    const bool v1 = n1 < CBL;
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        double cf[CTM];
+#pragma unroll
+        for (int tm = 0; tm < CTM; tm++) cf[tm] = sm[tm] < 3 ? coupled_coef(src, sm[tm], t) : 0.0;
+#pragma unroll
+        for (int kk = 0; kk < CSB / 4; kk++) {
+            const int c = 4 * kk + lk, k = CSB * t + c;
+            const double b0 = X[n0 * CXL + k];
+            const double b1 = v1 ? X[n1 * CXL + k] : 0.0;
+#pragma unroll
+            for (int tm = 0; tm < CTM; tm++) {
+                const double a = cf[tm] * Jl[jb[tm] + c * CLD];
+                acc[2 * tm] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[2 * tm], 0, 0, 0);
+                acc[2 * tm + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[2 * tm + 1], 0, 0, 0);
+            }
+        }
+    }
+}
+
+// acc[2 tm + h] += sgn * tile (tm, w + 4 h) of X C_q
+__device__ inline void x_times_coupled(const double *X, const double *Jl, const CoupledSrc &src, double sgn,
+                                       dx4 (&acc)[16]) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int n0 = 16 * wv + lr, n1 = 16 * (wv + 4) + lr;
+    const bool v1 = n1 < CBL;
+    const int t0 = n0 / CSB, c0 = n0 - t0 * CSB;
+    const int t1 = v1 ? n1 / CSB : 0, c1 = v1 ? n1 - t1 * CSB : 0;
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+        const double cf0 = sgn * coupled_coef(src, s, t0);
+        const double cf1 = v1 ? sgn * coupled_coef(src, s, t1) : 0.0;
+#pragma unroll
+        for (int kk = 0; kk < CSB / 4; kk++) {
+            const int r = 4 * kk + lk, k = CSB * s + r;
+            const double b0 = cf0 * Jl[s * CSTAGE + c0 * CLD + r];
+            const double b1 = cf1 * Jl[s * CSTAGE + c1 * CLD + r];
+#pragma unroll
+            for (int tm = 0; tm < CTM; tm++) {
+                const int m = 16 * tm + lr;
+                const double a = m < CBL ? X[k * CXL + m] : 0.0;
+                acc[2 * tm] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[2 * tm], 0, 0, 0);
+                acc[2 * tm + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[2 * tm + 1], 0, 0, 0);
+            }
+        }
+    }
+}
+
+__device__ inline void tile_store_global(double *g, int tm, int tn, const dx4 &t) {
+    const int l = threadIdx.x & 63;
+    const int r0 = l >> 4, c0 = l & 15;
+    const int col = 16 * tn + c0;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int row = 16 * tm + r0 + 4 * e;
+        if (row < CBL && col < CBL) g[(size_t)col * CBL + row] = t[e];
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(FWG) void k_cdilu_schur(RowLists L, CoupledSrc src, T *__restrict__ F,
+                                                     float *__restrict__ scale, double *__restrict__ DD64) {
+    extern __shared__ double lds_c[];
+    double *X = lds_c;
+    double *Jl = X + CBL * CXL;
+    __shared__ double red[4];
+    const int wv = threadIdx.x >> 6;
+    const int i = L.rows[blockIdx.x];
+    const int d = L.diag[i];
+    const int bs = src.bs;
+    double *S = DD64 + (size_t)i * CBB;
+    // S_i = C_ii, mass term included
+    for (int e = threadIdx.x; e < (int)CBB; e += FWG) {
+        const int col = e / CBL, m = e - col * CBL;
+        const int s = m / CSB, r = m - s * CSB, t = col / CSB, c = col - t * CSB;
+        const double *J = s == 0 ? src.J0 : s == 1 ? src.J1 : src.J2;
+        double v = coupled_coef(src, s, t) * J[(size_t)d * CSB * CSB + c * CSB + r];
+        if (s == t && r / bs == c / bs) v += src.mass[(size_t)i * bs * bs + (c % bs) * bs + (r % bs)];
+        S[e] = v;
+    }
+    for (int q = L.lptr[i]; q < L.lptr[i + 1]; q++) {
+        const int sl = L.lslot[q], k = L.lcol[q], st = L.ltrans[q];
+        __syncthreads();   // X and Jl free; S_i visible to every wavefront
+        coupled_stage(Jl, src, sl);
+        wg_block_to_lds<CBL>(X, DD64 + (size_t)k * CBB);
+        __syncthreads();
+        dx4 acc[16];
+        coupled_times_x(Jl, X, src, acc);   // L_ik = C_ik D_k^-1
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 16; j++) tile_to_lds<CBL>(X, j >> 1, wv + 4 * (j & 1), acc[j]);
+        coupled_stage(Jl, src, st);
+        __syncthreads();
+        wg_lds_block_store<T, CBL>(X, F + (size_t)L.pos[sl] * CBB, scale ? scale + L.pos[sl] : nullptr, red);
+#pragma unroll
+        for (int j = 0; j < 16; j++) tile_load<CBL>(acc[j], j >> 1, wv + 4 * (j & 1), S);
+        x_times_coupled(X, Jl, src, -1.0, acc);   // S_i -= L_ik C_ki
+#pragma unroll
+        for (int j = 0; j < 16; j++) tile_store_global(S, j >> 1, wv + 4 * (j & 1), acc[j]);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(WG) void k_cdilu_inv(const int *__restrict__ rows, const int *__restrict__ diag,
+                                                  const int *__restrict__ pos, double *DD64, T *__restrict__ F,
+                                                  float *__restrict__ scale, int *__restrict__ status) {
+    const int i = rows[blockIdx.x];
+    const int pd = pos[diag[i]];
+    double *S = DD64 + (size_t)i * CBB;
+    rt_gauss_jordan<T, CBL, true>(S, S, F + (size_t)pd * CBB, scale ? scale + pd : nullptr, status);
+}
+
+template <class T>
+__global__ __launch_bounds__(FWG) void k_cdilu_upper(const int *__restrict__ urows, const int *__restrict__ uptr,
+                                                     const int *__restrict__ uslot, const int *__restrict__ pos,
+                                                     CoupledSrc src, const double *__restrict__ DD64,
+                                                     T *__restrict__ F, float *__restrict__ scale) {
+    extern __shared__ double lds_c[];
+    double *X = lds_c;
+    double *Jl = X + CBL * CXL;
+    __shared__ double red[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = urows[blockIdx.x];
+    wg_block_to_lds<CBL>(X, DD64 + (size_t)i * CBB);
+    for (int q = uptr[i]; q < uptr[i + 1]; q++) {
+        const int sl = uslot[q];
+        __syncthreads();   // Jl and red free; X staged
+        coupled_stage(Jl, src, sl);
+        __syncthreads();
+        dx4 acc[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc[j] = dx4{0.0, 0.0, 0.0, 0.0};
+        x_times_coupled(X, Jl, src, 1.0, acc);   // DU_ij = D_i^-1 C_ij
+        float inv = 1.0f;
+        T *g = F + (size_t)pos[sl] * CBB;
+        if constexpr (Store<T>::prec == CPG_BF16) {
+            double mx = 0.0;
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) mx = fmax(mx, fabs(acc[j][e]));   // padding entries are 0
+            mx = wave_max(mx);
+            if (lane == 0) red[wv] = mx;
+            __syncthreads();
+            mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+            const float sc = pow2_scale(mx);
+            inv = 1.0f / sc;
+            if (threadIdx.x == 0) scale[pos[sl]] = sc;
+        }
+        const int r0 = lane >> 4, c0 = lane & 15;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int col = 16 * (wv + 4 * (j & 1)) + c0;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int row = 16 * (j >> 1) + r0 + 4 * e;
+                if (row < CBL && col < CBL) Store<T>::put(g, (size_t)col * CBL + row, acc[j][e], inv);
+            }
+        }
+    }
+}
+
 }  // namespace cpg
 
 using namespace cpg;
@@ -1323,6 +1555,32 @@ void factor_wf_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
     if (P.nup > 0) {
         hipLaunchKernelGGL((k_dilu_upper<T, BL>), dim3(P.nurows), dim3(FWG), 0, s, P.urows.p, P.uptr.p, P.uslot.p,
                            P.pos.p, A.blocks.p, P.DD64.p, Fp<T>(P), sc);
+        CPG_LAUNCH_CHECK();
+    }
+}
+
+// block DILU of the stage-coupled system: per level the Schur and inverse kernels, then the DU
+// blocks of every row
+template <class T>
+void factor_coupled_t(cpg_prec_s &P, const CoupledSrc &src, hipStream_t s) {
+    float *sc = P.precision == CPG_BF16 ? P.scale.p : nullptr;
+    set_lds(k_cdilu_schur<T>, CLDS);
+    set_lds(k_cdilu_upper<T>, CLDS);
+    RowLists L = P.lists(true);
+    for (size_t lv = 0; lv + 1 < P.flev.size(); lv++) {
+        const int nr = P.flev[lv + 1] - P.flev[lv];
+        if (!nr) continue;
+        RowLists Ll = L;
+        Ll.rows = P.frows.p + P.flev[lv];
+        hipLaunchKernelGGL(k_cdilu_schur<T>, dim3(nr), dim3(FWG), CLDS, s, Ll, src, Fp<T>(P), sc, P.DD64.p);
+        CPG_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_cdilu_inv<T>, dim3(nr), dim3(WG), 0, s, Ll.rows, P.diag.p, P.pos.p, P.DD64.p, Fp<T>(P), sc,
+                           P.status.p);
+        CPG_LAUNCH_CHECK();
+    }
+    if (P.nup > 0) {
+        hipLaunchKernelGGL(k_cdilu_upper<T>, dim3(P.nurows), dim3(FWG), CLDS, s, P.urows.p, P.uptr.p, P.uslot.p,
+                           P.pos.p, src, P.DD64.p, Fp<T>(P), sc);
         CPG_LAUNCH_CHECK();
     }
 }
@@ -1810,3 +2068,27 @@ int cpg_prec_download(cpg_prec P, double *host_blocks) {
 }
 
 }  // extern "C"
+
+namespace cpg {
+
+void prec_factor_coupled(cpg_prec P, const CoupledBlocks &C, hipStream_t s) {
+    CPG_CHECK(P && P->kind == CPG_PC_DILU && P->bl == CBL && C.bs > 0 && CSB % C.bs == 0, CPG_ERR_UNSUPPORTED,
+              "the coupled DILU takes 3 stages of 40 x 40 blocks");
+    CPG_CHECK(P->n == C.n && P->nnzb == C.nnzb, CPG_ERR_ARG, "preconditioner pattern (%d rows, %d slots) does not "
+              "match the stage Jacobians (%d, %d)", P->n, P->nnzb, C.n, C.nnzb);
+    if (P->n == 0) return;
+    CoupledSrc src{C.J[0], C.J[1], C.J[2], C.mass, {}, C.bs};
+    for (int j = 0; j < 9; j++) src.coef[j] = C.coef[j];
+    CPG_HIP(hipMemsetAsync(P->status.p, 0, sizeof(int), s));
+    switch (P->precision) {
+    case CPG_FP64: factor_coupled_t<double>(*P, src, s); break;
+    case CPG_FP32: factor_coupled_t<float>(*P, src, s); break;
+    default: factor_coupled_t<bf16>(*P, src, s); break;
+    }
+    int st = 0;
+    CPG_HIP(hipMemcpyAsync(&st, P->status.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    CPG_HIP(hipStreamSynchronize(s));
+    CPG_CHECK(st == 0, CPG_ERR_SINGULAR, "zero pivot in a coupled diagonal block inverse");
+}
+
+}  // namespace cpg

@@ -288,7 +288,10 @@ struct cpg_stepper_s {
     DevBuf<int> slot_row;
     DevBuf<double> opscale;    // -dt a_ss of the matrix-free Newton operator (IRK: -dt)
     // IRK with the matrix-free coupled operator
-    cpg_bsr Dc = nullptr;      // coupled diagonal blocks, one per cell
+    cpg_bsr Dc = nullptr;      // coupled diagonal blocks, one per cell (block Jacobi)
+    bool irk_mf = false;       // the matrix-free coupled path is set up
+    cpg_bsr Jst[2] = {nullptr, nullptr};   // coupled DILU: the Jacobians of stages 1, 2 (J: stage 0)
+    cpg_bsr Cpat = nullptr;    // coupled DILU: the (ns bl)-wide pattern (no blocks)
     DevBuf<double> irk_data, pin, pout, wbuf;
     cpg::IrkOp irkop{};
     cpg::IrkPc irkpc{};
@@ -303,6 +306,9 @@ struct cpg_stepper_s {
         if (J) cpg_bsr_destroy(J);
         if (big) cpg_bsr_destroy(big);
         if (Dc) cpg_bsr_destroy(Dc);
+        for (cpg_bsr j : Jst)
+            if (j) cpg_bsr_destroy(j);
+        if (Cpat) cpg_bsr_destroy(Cpat);
     }
 };
 
@@ -392,16 +398,36 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
                 CPG_CHECK(params->gmres_mode == CPG_GMRES_FLEXIBLE, CPG_ERR_ARG,
                           "the matrix-free IRK operator uses flexible GMRES");
                 CPG_CHECK(params->pc_kind == CPG_PC_BJACOBI || params->pc_kind == CPG_PC_DILU, CPG_ERR_UNSUPPORTED,
-                          "the matrix-free IRK step takes the coupled block-Jacobi preconditioner");
+                          "the matrix-free IRK step takes the coupled block-Jacobi or block DILU preconditioner");
                 CPG_CHECK(params->pc_refresh >= 0 && params->pc_refresh <= 2, CPG_ERR_ARG, "bad pc_refresh %d",
                           params->pc_refresh);
                 const int ns = nstages;
                 check(cpg_jacobian_create(E, &S->J));
-                std::vector<int> rp(n + 1), ci(n);
-                for (int i = 0; i <= n; i++) rp[i] = i;
-                for (int i = 0; i < n; i++) ci[i] = i;
-                check(cpg_bsr_create(n, ns * bl, n, rp.data(), ci.data(), &S->Dc));
-                check(cpg_prec_create(S->Dc, CPG_PC_BJACOBI, params->precision, &S->P));
+                if (params->pc_kind == CPG_PC_DILU) {
+                    // block DILU on the cell-interleaved coupled system: its blocks are formed in the
+                    // factor from the three stage Jacobians
+                    CPG_CHECK(ns == 3 && bl == 40, CPG_ERR_UNSUPPORTED,
+                              "the coupled DILU takes 3 stages of 40 x 40 blocks (got %d x %d)", ns, bl);
+                    for (cpg_bsr &j : S->Jst) check(cpg_jacobian_create(E, &j));
+                    auto *C = new cpg_bsr_s;
+                    S->Cpat = C;
+                    C->n = n;
+                    C->bl = ns * bl;
+                    C->nnzb = S->J->nnzb;
+                    C->nbmax = S->J->nbmax;
+                    C->h_rowptr = S->J->h_rowptr;
+                    C->h_colind = S->J->h_colind;
+                    C->rowptr.upload(C->h_rowptr);
+                    C->colind.upload(C->h_colind);
+                    check(cpg_prec_create(S->Cpat, CPG_PC_DILU, params->precision, &S->P));
+                } else {
+                    std::vector<int> rp(n + 1), ci(n);
+                    for (int i = 0; i <= n; i++) rp[i] = i;
+                    for (int i = 0; i < n; i++) ci[i] = i;
+                    check(cpg_bsr_create(n, ns * bl, n, rp.data(), ci.data(), &S->Dc));
+                    check(cpg_prec_create(S->Dc, CPG_PC_BJACOBI, params->precision, &S->P));
+                }
+                S->irk_mf = true;
                 check(cpg_krylov_create(ns * n, bl, params->restart, CPG_GMRES_FLEXIBLE, &S->K));
                 S->bigR.alloc(S->N * ns);
                 S->bigK.alloc(S->N * ns);
@@ -497,6 +523,27 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
     });
 }
 
+int cpg_prec_factor_coupled(cpg_prec P, cpg_equation E, cpg_bsr J0, cpg_bsr J1, cpg_bsr J2, const double *coef,
+                            void *stream) {
+    return guard([&] {
+        CPG_CHECK(P && E && J0 && J1 && J2 && coef, CPG_ERR_ARG, "null argument");
+        const cpg_bsr Js[3] = {J0, J1, J2};
+        int n = 0, bl = 0, nnzb = 0;
+        check(cpg_equation_info(E, &n, &bl, &nnzb));
+        for (const cpg_bsr J : Js)
+            CPG_CHECK(J->n == n && J->bl == bl && J->nnzb == nnzb, CPG_ERR_ARG, "stage Jacobian does not match E");
+        for (const cpg_bsr J : Js) CPG_CHECK(J->h_colind == J0->h_colind, CPG_ERR_ARG, "stage Jacobian patterns differ");
+        CoupledBlocks cb{};
+        for (int st = 0; st < 3; st++) cb.J[st] = Js[st]->blocks.p;
+        cb.mass = equation_mass(*E);
+        for (int j = 0; j < 9; j++) cb.coef[j] = coef[j];
+        cb.bs = equation_bs(*E);
+        cb.n = n;
+        cb.nnzb = nnzb;
+        prec_factor_coupled(P, cb, S(stream));
+    });
+}
+
 int cpg_stepper_destroy(cpg_stepper S) {
     return guard([&] { delete S; });
 }
@@ -546,7 +593,7 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
             st.ms_total = ms_since(t_all);
             return;
         }
-        if (S->method == CPG_STEP_IRK && S->Dc) {
+        if (S->method == CPG_STEP_IRK && S->irk_mf) {
             // IRK::advance (src/TimeIntegration.cpp:188-281) with the coupled Newton system
             // applied matrix-free: per Newton iteration the stage residuals, (per pc_refresh)
             // the stage Jacobians scattered into the coupled diagonal blocks and their
@@ -575,7 +622,28 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                 CPG_CHECK(std::isfinite(rn), CPG_ERR_NONFINITE, "non-finite IRK Newton residual (it %d)", it);
                 if (rn < S->sp.newton_tol) { converged = true; break; }
                 st.newton_iters++;
-                if (S->sp.pc_refresh == 0 || !factored) {
+                if ((S->sp.pc_refresh == 0 || !factored) && S->Cpat) {
+                    // the stage Jacobians, then the coupled DILU formed from them
+                    t0 = clk::now();
+                    cpg_bsr Js[3] = {S->J, S->Jst[0], S->Jst[1]};
+                    for (int stg = 0; stg < ns; stg++) {
+                        lincomb(*S, S->cu.p, u, &S->A[stg * ns], ns, k, s);
+                        eq_jacobian(*S->E, S->cu.p, t + dt * S->c[stg], 1.0, 0, Js[stg]->blocks.p, s);
+                    }
+                    CPG_HIP(hipStreamSynchronize(s));
+                    st.ms_jacobian += ms_since(t0);
+                    t0 = clk::now();
+                    CoupledBlocks cb{};
+                    for (int stg = 0; stg < ns; stg++) cb.J[stg] = Js[stg]->blocks.p;
+                    cb.mass = equation_mass(*S->E);
+                    for (int j = 0; j < 9; j++) cb.coef[j] = -dt * S->A[j];
+                    cb.bs = bs;
+                    cb.n = S->n;
+                    cb.nnzb = S->J->nnzb;
+                    check(guard([&] { prec_factor_coupled(S->P, cb, s); }));
+                    st.ms_factor += ms_since(t0);
+                    factored = true;
+                } else if (S->sp.pc_refresh == 0 || !factored) {
                     t0 = clk::now();
                     int nnzb = 0;
                     check(cpg_equation_info(S->E, nullptr, nullptr, &nnzb));

@@ -192,6 +192,13 @@ class Prec:
     def apply(self, x, stream=None):
         _chk(lib().cpg_prec_apply(self.h, _vp(ptr(x)), _stream(stream)))
 
+    def factor_coupled(self, E, stage_jacobians, coef, stream=None):
+        """Block DILU of the Radau IIA stage-coupled Newton matrix (``cpg_prec_factor_coupled``):
+        block (s, t) of slot q is coef[s, t] J_s[q] (+ E's mass on the diagonal when s == t)."""
+        J0, J1, J2 = stage_jacobians
+        c = np.ascontiguousarray(np.asarray(coef, np.float64).reshape(9))
+        _chk(lib().cpg_prec_factor_coupled(self.h, E.h, J0.h, J1.h, J2.h, c.ctypes.data_as(_dp), _stream(stream)))
+
     def download(self):
         nblk = self.A.nnzb if self.kind == _spec.PC_DILU else self.A.n
         out = np.zeros((nblk, self.A.bl, self.A.bl))

@@ -97,6 +97,17 @@ int cpg_prec_set_inner(cpg_prec P, int iterations);
 int cpg_prec_destroy(cpg_prec P);
 int cpg_prec_factor(cpg_prec P, cpg_bsr A, void *stream);
 int cpg_prec_apply(cpg_prec P, double *x, void *stream);   /* x <- M^-1 x, in place */
+/* Block DILU of the Radau IIA stage-coupled Newton matrix in cell-interleaved order (3 stages
+ * x bl = 40: 120-wide blocks), with the coupled blocks formed inside the factor: block (s, t)
+ * of slot q is coef[3 s + t] * J_s[q], plus E's mass block on the diagonal slots when s == t.
+ * Replaces BlockMatrix::blockBlockMatrix (src/BlockMatrix.cpp:228-258) followed by the
+ * BlockILU0 constructor (src/Preconditioners.cpp:49-123) for IRK::advance
+ * (src/TimeIntegration.cpp:233-245), on the interleaved ordering (a different DILU from the
+ * stage-major one) and without storing the coupled matrix.  P: kind CPG_PC_DILU created over a
+ * BSR of block size 120 with the stage Jacobians' pattern; J0..J2: that pattern at block size
+ * 40; coef: 9 host doubles (-dt A for the Newton matrix M - dt (A x J)); synchronous. */
+int cpg_prec_factor_coupled(cpg_prec P, cpg_equation E, cpg_bsr J0, cpg_bsr J1, cpg_bsr J2, const double *coef,
+                            void *stream);
 /* factors converted to FP64 in A's slot layout: diag = D^-1, lower = L, upper = DU
  * (DILU) or n inverse diagonal blocks (block Jacobi); synchronous */
 int cpg_prec_download(cpg_prec P, double *host_blocks);

@@ -89,6 +89,55 @@ def test_dilu_reduced_precision(cuda, prec, tol):
     assert rel(xd.cpu().numpy(), O.dilu_apply(orc.rowptr, orc.colind, Fo, x)) < tol
 
 
+# Radau IIA, 3 stages (Hairer & Wanner, Solving ODEs II, table IV.5.6)
+_R6 = np.sqrt(6.0)
+RADAU3 = np.array([[(88 - 7 * _R6) / 360, (296 - 169 * _R6) / 1800, (-2 + 3 * _R6) / 225],
+                   [(296 + 169 * _R6) / 1800, (88 + 7 * _R6) / 360, (-2 - 3 * _R6) / 225],
+                   [(16 - _R6) / 36, (16 + _R6) / 36, 1 / 9]])
+
+
+@pytest.mark.parametrize("prec,ftol,atol", [(PREC_FP64, 1e-11, 1e-11), (PREC_FP32, 1e-6, 5e-6),
+                                            (PREC_BF16, 8e-3, 2e-2)])
+def test_coupled_dilu_matches_oracle(cuda, prec, ftol, atol):
+    """Block DILU of the Radau IIA stage-coupled Newton matrix in cell-interleaved order, its
+    120 x 120 blocks formed inside the factor from three stage Jacobians and the mass matrix,
+    against the oracle's DILU of the same matrix assembled explicitly (block (s, t) of slot q =
+    -dt A[s][t] J_s[q], plus the mass block on the diagonal when s == t)."""
+    deg, bl = 3, 40
+    m = vortex_mesh(6)
+    orc = O.Oracle(m)
+    spec = EquationSpec.euler()
+    E = hip.Equation(hip.Mesh(m), spec, deg)
+    coef = -0.3 * RADAU3
+    Jo = [orc.jacobian(spec, deg, vortex_state(m, deg, perturb=0.02, seed=s)) for s in range(3)]
+    M = orc.mass(deg, 4)
+    nnzb = len(orc.colind)
+    # blocks are column-major: B[q][c][r] is entry (r, c)
+    Bc = np.zeros((nnzb, 3 * bl, 3 * bl))
+    for s in range(3):
+        for t in range(3):
+            Bc[:, bl * t:bl * (t + 1), bl * s:bl * (s + 1)] = coef[s, t] * Jo[s]
+    for i in range(m.n_cells):
+        for s in range(3):
+            Bc[orc.rowptr[i], bl * s:bl * (s + 1), bl * s:bl * (s + 1)] += M[i]
+    Js = []
+    for s in range(3):
+        J = hip.Bsr(orc.rowptr, orc.colind, bl)
+        J.upload(Jo[s])
+        Js.append(J)
+    P = hip.Prec(hip.Bsr(orc.rowptr, orc.colind, 3 * bl), PC_DILU, prec)
+    P.factor_coupled(E, Js, coef)
+    F = P.download()
+    Fo, info = O.dilu_factor(orc.rowptr, orc.colind, Bc)
+    assert info == 0
+    err = np.abs(F - Fo).reshape(nnzb, -1).max(1) / np.abs(Fo).reshape(nnzb, -1).max(1)
+    assert err.max() < ftol
+    x = np.random.default_rng(4).standard_normal(m.n_cells * 3 * bl)
+    xd = _dev(cuda, x)
+    P.apply(xd)
+    assert rel(xd.cpu().numpy(), O.dilu_apply(orc.rowptr, orc.colind, Fo, x)) < atol
+
+
 @pytest.mark.parametrize("prec,tol", [(PREC_FP64, 1e-11), (PREC_FP32, 5e-6), (PREC_BF16, 2e-2)])
 def test_block_jacobi(cuda, prec, tol):
     m, orc, B = _euler_system(3)

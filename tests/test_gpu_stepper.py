@@ -128,11 +128,14 @@ def test_irk_radau_matches_oracle(cuda, ns):
     assert rel(outd.cpu().numpy(), uo) < 1e-9
 
 
-@pytest.mark.parametrize("ns,prec,refresh", [(3, PREC_FP64, 0), (3, PREC_BF16, 2), (2, PREC_FP32, 0)])
-def test_irk_matrix_free_coupled_bjacobi(cuda, ns, prec, refresh):
+@pytest.mark.parametrize("ns,prec,refresh,pc", [(3, PREC_FP64, 0, PC_BJACOBI), (3, PREC_BF16, 2, PC_BJACOBI),
+                                                (2, PREC_FP32, 0, PC_BJACOBI), (3, PREC_FP64, 0, PC_DILU),
+                                                (3, PREC_BF16, 2, PC_DILU), (3, PREC_FP32, 1, PC_DILU)])
+def test_irk_matrix_free_coupled(cuda, ns, prec, refresh, pc):
     """Radau IIA with the coupled Newton system applied matrix-free and preconditioned by block
     Jacobi on the cell-interleaved (ns bl) x (ns bl) diagonal blocks (120 x 120 for 3 stages at
-    p = 3): the converged step equals the oracle's (stage-major assembled system, DILU, left GMRES)."""
+    p = 3) or by block DILU of the cell-interleaved coupled matrix: the converged step equals the
+    oracle's (stage-major assembled system, DILU, left GMRES)."""
     m = vortex_mesh(6)
     deg = 3
     spec = EquationSpec.euler(periodic_trace=1, jacobian=1)
@@ -141,7 +144,7 @@ def test_irk_matrix_free_coupled_bjacobi(cuda, ns, prec, refresh):
                      newton_tol=1e-12, newton_maxit=30)
     dt = 0.4
     uo, st = O.Oracle(m).irk_step(spec, deg, ns, u, dt, 0.0, ref)
-    solver = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=40, rtol=1e-11, maxit=20000, pc_kind=PC_BJACOBI,
+    solver = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=40, rtol=1e-11, maxit=20000, pc_kind=pc,
                         newton_tol=1e-12, newton_maxit=30, precision=prec, newton_operator=OP_MATRIX_FREE,
                         pc_refresh=refresh)
     S = hip.Stepper(hip.Equation(hip.Mesh(m), spec, deg), 1, ns, solver)
