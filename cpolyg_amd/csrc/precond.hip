@@ -367,11 +367,17 @@ __device__ inline void rt_gauss_jordan(const double *a, double *out64, T *__rest
         if (act && cg == kg) {
             double best = -1.0, bv = 0.0;
             int br = -1;
+            double col[RPT];
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                if (c == kc) {
+#pragma unroll
+                    for (int r = 0; r < RPT; r++) col[r] = x[r][c];
+                }
+            }
 #pragma unroll
             for (int r = 0; r < RPT; r++) {
-                double v = x[r][0];
-#pragma unroll
-                for (int c = 1; c < CPT; c++) v = kc == c ? x[r][c] : v;
+                const double v = col[r];
                 fcol[b][r0 + r] = v;
                 const double m = fabs(v);
                 if (!((used >> r) & 1u) && m > best) {
@@ -384,30 +390,40 @@ __device__ inline void rt_gauss_jordan(const double *a, double *out64, T *__rest
             cr[b][rg] = br;
         }
         __syncthreads();
+        // all candidates loaded first (their reads in flight together), then reduced in order
+        int crr[RG];
+        double cvv[RG];
+#pragma unroll
+        for (int g = 0; g < RG; g++) {
+            crr[g] = cr[b][g];
+            cvv[g] = cv[b][g];
+        }
         double pa = -1.0, pv = 0.0;
         int p = -1;
 #pragma unroll
         for (int g = 0; g < RG; g++) {
-            const int r = cr[b][g];
-            const double v = cv[b][g];
-            if (r >= 0 && fabs(v) > pa) {
-                pa = fabs(v);
-                pv = v;
-                p = r;
-            }
+            const double m = fabs(cvv[g]);
+            const bool better = crr[g] >= 0 && m > pa;
+            pa = better ? m : pa;
+            pv = better ? cvv[g] : pv;
+            p = better ? crr[g] : p;
         }
         if (p < 0 || !(pa > 0.0) || !isfinite(pv)) {   // the same for every thread
             ok = false;
             break;
         }
+        // p is the same in every lane: scalar, so that the pivot row and column below are
+        // selected by uniform branches and the update itself has none
+        p = __builtin_amdgcn_readfirstlane(p);
         const int pg = p / RPT, pl = p - pg * RPT;
-        if (act && rg == pg) {
+        const bool rowhit = act && rg == pg, colhit = act && cg == kg;
+        if (rowhit) {
 #pragma unroll
-            for (int c = 0; c < CPT; c++) {
-                double v = x[0][c];
+            for (int r = 0; r < RPT; r++) {
+                if (r == pl) {
 #pragma unroll
-                for (int r = 1; r < RPT; r++) v = pl == r ? x[r][c] : v;
-                prow[b][c0 + c] = v;
+                    for (int c = 0; c < CPT; c++) prow[b][c0 + c] = x[r][c];
+                }
             }
         }
         if (tid == 0) pr[k] = p;
@@ -418,16 +434,26 @@ __device__ inline void rt_gauss_jordan(const double *a, double *out64, T *__rest
         for (int r = 0; r < RPT; r++) f[r] = fcol[b][r0 + r];
 #pragma unroll
         for (int c = 0; c < CPT; c++) q[c] = prow[b][c0 + c] * ip;
+        // X[r][c] -= X[r][k] X[p][c] / v everywhere, then the pivot column and row overwritten
 #pragma unroll
-        for (int r = 0; r < RPT; r++) {
-            const bool prw = rg == pg && r == pl;
+        for (int r = 0; r < RPT; r++)
 #pragma unroll
-            for (int c = 0; c < CPT; c++) {
-                const bool pcl = c0 + c == k;
-                x[r][c] = prw ? (pcl ? ip : -q[c]) : (pcl ? f[r] * ip : fma(-f[r], q[c], x[r][c]));
+            for (int c = 0; c < CPT; c++) x[r][c] = fma(-f[r], q[c], x[r][c]);
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            if (c == kc) {
+#pragma unroll
+                for (int r = 0; r < RPT; r++) x[r][c] = colhit ? f[r] * ip : x[r][c];
             }
         }
-        if (rg == pg) used |= 1u << pl;
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            if (r == pl) {
+#pragma unroll
+                for (int c = 0; c < CPT; c++) x[r][c] = rowhit ? (colhit && c == kc ? ip : -q[c]) : x[r][c];
+            }
+        }
+        if (rowhit) used |= 1u << pl;
     }
     if (!ok) {
         if (tid == 0) atomicOr(status, 1);
