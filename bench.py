@@ -330,7 +330,12 @@ def bench_irk(args):
         sp = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=args.irk_restart, rtol=1e-10, maxit=6000,
                         pc_kind=pc, newton_tol=1e-10, newton_maxit=10, precision=prec,
                         newton_operator=OP_MATRIX_FREE, pc_refresh=args.pc_refresh)
-        S = hip.Stepper(E, 1, 3, sp)
+        try:
+            S = hip.Stepper(E, 1, 3, sp)
+        except hip.CpgError as e:   # e.g. the factors of this precision do not fit
+            res[name] = {"error": str(e)}
+            print(f"# {name}: {e}", file=sys.stderr, flush=True)
+            continue
         free1, _ = torch.cuda.mem_get_info()
         u = torch.from_numpy(u0).cuda()
         nxt = torch.empty_like(u)
@@ -360,13 +365,14 @@ def bench_irk(args):
         del S, u, nxt
         torch.cuda.synchronize()
     # every run against the first FP64-factor one (or the first run)
-    base = next((k for k in res if k.endswith("fp64")), next(iter(res)))
+    done = [k for k in res if "u" in res[k]]
+    base = next((k for k in done if k.endswith("fp64")), done[0])
     ub = res[base]["u"]
-    for r in res.values():
-        r["max_rel_diff_vs_" + base] = float(np.max(np.abs(r["u"] - ub)) / np.max(np.abs(ub)))
-    for r in res.values():
-        del r["u"]
-    head = next((k for k in ("dilu_bf16", "bjacobi_bf16") if k in res), base)
+    for k in done:
+        res[k]["max_rel_diff_vs_" + base] = float(np.max(np.abs(res[k]["u"] - ub)) / np.max(np.abs(ub)))
+    for k in done:
+        del res[k]["u"]
+    head = next((k for k in ("dilu_bf16", "bjacobi_bf16") if k in done), base)
     out = {"metric": "Radau IIA (3-stage) steps/s, Euler p=3, 262k cells, coupled system matrix-free",
            "value": res[head]["steps_per_s"], "unit": "IRK steps/s", "value_run": head, "steps": args.steps,
            "warmup": args.warmup,
