@@ -751,6 +751,128 @@ __global__ void k_colsum_final(const double *__restrict__ partial, int nb, int n
     out[c] = s;
 }
 
+// ---- analytic fields: MeshFn::interp (src/MeshFn.cpp:56-145) and MeshFn::L2Error
+// (src/MeshFn.cpp:277-296) on the device.  The reference evaluates a host functor; the fields
+// here are a fixed set (CPG_FIELD_*) evaluated in the kernels, or values supplied at the points.
+struct FieldArgs {
+    int kind;
+    double par[8];
+    double t;
+};
+
+// nearest periodic image of a difference (no wrap for a non-positive period)
+__device__ inline double wrap_diff(double d, double L) { return L > 0.0 ? d - L * rint(d / L) : d; }
+
+template <int NC>
+__device__ inline void field_eval(const FieldArgs &f, double x, double y, double *out) {
+    const double *p = f.par;
+    const double t = f.t;
+    if constexpr (NC == 1) {
+        // gaussian bump moving with (vx, vy)
+        const double dx = wrap_diff(x - p[0] - p[3] * t, p[5]), dy = wrap_diff(y - p[1] - p[4] * t, p[6]);
+        out[0] = exp(-p[2] * (dx * dx + dy * dy));
+    } else if (f.kind == CPG_FIELD_VORTEX) {
+        // isentropic vortex (Yee, Sandham and Djomehri 1999) advected by the free stream
+        const double beta = p[0], uinf = p[3], vinf = p[4], gam = p[5];
+        const double dx = wrap_diff(x - p[1] - uinf * t, p[6]), dy = wrap_diff(y - p[2] - vinf * t, p[7]);
+        const double ex = exp(0.5 * (1.0 - dx * dx - dy * dy));
+        const double b2p = beta / (2.0 * M_PI);
+        const double u = uinf - b2p * dy * ex, v = vinf + b2p * dx * ex;
+        const double T = 1.0 - (gam - 1.0) * beta * beta / (8.0 * gam * M_PI * M_PI) * ex * ex;
+        const double rho = pow(T, 1.0 / (gam - 1.0));
+        out[0] = rho;
+        out[1] = rho * u;
+        out[2] = rho * v;
+        out[3] = rho * T / (gam - 1.0) + 0.5 * rho * (u * u + v * v);
+    } else {
+        // density wave advected by a uniform flow
+        const double ux = p[0], uy = p[1], pr = p[2], gam = p[3], amp = p[4];
+        const double rho = 1.0 + amp * sin(2.0 * M_PI * (x - ux * t)) * sin(2.0 * M_PI * (y - uy * t));
+        out[0] = rho;
+        out[1] = rho * ux;
+        out[2] = rho * uy;
+        out[3] = pr / (gam - 1.0) + 0.5 * rho * (ux * ux + uy * uy);
+    }
+}
+
+// one thread per cell: u_i = Gp vals_i, Gp the least-squares pseudo-inverse (bs x (p+1)^2) of the
+// basis at the tensor Gauss points; vals from the field or from `vals` (n x Q x NC)
+template <int DEG, int NC>
+__global__ void k_interp(const double4 *__restrict__ bb, int n, FieldArgs f, const double *__restrict__ vals,
+                         const double *__restrict__ nodes, const double *__restrict__ Gp, double *__restrict__ u) {
+    constexpr int M = DEG + 1, Q = M * M, BS = Deg<DEG>::BS;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 B = bb[i];
+    double acc[NC][BS];
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int k = 0; k < BS; k++) acc[c][k] = 0.0;
+    for (int qi = 0; qi < M; qi++)
+        for (int qj = 0; qj < M; qj++) {
+            const int q = qi * M + qj;
+            double fv[NC];
+            if (vals) {
+#pragma unroll
+                for (int c = 0; c < NC; c++) fv[c] = vals[((size_t)i * Q + q) * NC + c];
+            } else {
+                field_eval<NC>(f, (nodes[qi] + 1.0) * B.z / 2.0 + B.x, (nodes[qj] + 1.0) * B.w / 2.0 + B.y, fv);
+            }
+#pragma unroll
+            for (int k = 0; k < BS; k++) {
+                const double g = Gp[k * Q + q];
+#pragma unroll
+                for (int c = 0; c < NC; c++) acc[c][k] = fma(g, fv[c], acc[c][k]);
+            }
+        }
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int k = 0; k < BS; k++) u[(size_t)i * NC * BS + c * BS + k] = acc[c][k];
+}
+
+template <int DEG>
+__global__ void k_interp_points(const double4 *__restrict__ bb, int n, const double *__restrict__ nodes,
+                                double *__restrict__ xy) {
+    constexpr int M = DEG + 1, Q = M * M;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * Q) return;
+    const int i = e / Q, q = e - i * Q, qi = q / M, qj = q - qi * M;
+    const double4 B = bb[i];
+    xy[2 * (size_t)e] = (nodes[qi] + 1.0) * B.z / 2.0 + B.x;
+    xy[2 * (size_t)e + 1] = (nodes[qj] + 1.0) * B.w / 2.0 + B.y;
+}
+
+// one thread per cell: err_i[c] = sum_q W_q (u_h(x_q) - f(x_q))_c^2 over the volume points
+template <int DEG, int NC>
+__global__ void k_l2_error(Geo g, int n, FieldArgs f, const double *__restrict__ u, double *__restrict__ err) {
+    constexpr int BS = Deg<DEG>::BS;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 B = g.bb[i];
+    double acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) acc[c] = 0.0;
+    for (int q = g.vptr[i]; q < g.vptr[i + 1]; q++) {
+        const double4 P = g.vpts[q];
+        double phi[BS];
+        basis<DEG>(2.0 * (P.x - B.x) / B.z - 1.0, 2.0 * (P.y - B.y) / B.w - 1.0, phi, nullptr, nullptr);
+        double fv[NC];
+        field_eval<NC>(f, P.x, P.y, fv);
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            double uh = 0.0;
+#pragma unroll
+            for (int k = 0; k < BS; k++) uh = fma(u[(size_t)i * NC * BS + c * BS + k], phi[k], uh);
+            const double d = uh - fv[c];
+            acc[c] = fma(P.z * d, d, acc[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) err[(size_t)i * NC + c] = acc[c];
+}
+
 }  // namespace cpg
 
 using namespace cpg;
@@ -780,6 +902,7 @@ struct cpg_equation_s {
     DevBuf<double> mass;        // n x bs x bs
     DevBuf<double> normc, partial, sums;
     int npartial = 0;
+    DevBuf<double> fit;         // interp: Gauss nodes (p+1), then the pseudo-inverse (bs x (p+1)^2)
     Geo geo() const { return Geo{d_bb.p, d_vptr.p, d_vpts.p, d_eptr.p, d_edges.p, bcst.p}; }
 };
 
@@ -979,6 +1102,102 @@ double eq_newton_resid(const cpg_equation_s &E, const double *kv, const double *
 void eq_newton_resid_cells(const cpg_equation_s &E, const double *kv, const double *b, double dt, const double *cfac,
                            double *r, hipStream_t s) {
     CPG_DISPATCH_BS(E, launch_newton_resid, E, kv, b, dt, cfac, r, s);
+}
+
+// MeshFn::interp's fitting data: the p+1 Gauss-Legendre nodes (ascending, Newton's method on
+// P_{p+1}) and the pseudo-inverse (G^T G)^-1 G^T of the basis at the tensor points
+// (G[i (p+1) + j][k] = phi_k(x_i, x_j)), the least-squares solve the reference does per cell
+static void ensure_fit(cpg_equation_s &E) {
+    if (E.fit.p) return;
+    const int M = E.deg + 1, Q = M * M, BS = E.bs;
+    std::vector<double> nodes(M);
+    for (int i = 0; i < M; i++) {
+        double x = std::cos(M_PI * (i + 0.75) / (M + 0.5));
+        for (int it = 0; it < 100; it++) {
+            double p0 = 1.0, p1 = x;
+            for (int k = 2; k <= M; k++) {
+                const double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            const double dp = M == 1 ? 1.0 : M * (x * p1 - p0) / (x * x - 1.0);
+            const double dx = p1 / dp;
+            x -= dx;
+            if (std::fabs(dx) < 1e-16) break;
+        }
+        nodes[M - 1 - i] = x;
+    }
+    auto leg = [](int a, double x) {
+        double p0 = 1.0, p1 = x;
+        if (a == 0) return p0;
+        for (int k = 2; k <= a; k++) {
+            const double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        return p1;
+    };
+    std::vector<double> G((size_t)Q * BS);
+    for (int qi = 0; qi < M; qi++)
+        for (int qj = 0; qj < M; qj++) {
+            int k = 0;
+            for (int a = 0; a <= E.deg; a++)
+                for (int b = 0; b <= E.deg - a; b++) G[(size_t)(qi * M + qj) * BS + k++] = leg(a, nodes[qi]) * leg(b, nodes[qj]);
+        }
+    // [G^T G | I] -> [I | (G^T G)^-1], partial pivoting
+    std::vector<double> aug((size_t)BS * 2 * BS, 0.0);
+    for (int r = 0; r < BS; r++) {
+        for (int c = 0; c < BS; c++) {
+            double v = 0.0;
+            for (int q = 0; q < Q; q++) v += G[(size_t)q * BS + r] * G[(size_t)q * BS + c];
+            aug[(size_t)r * 2 * BS + c] = v;
+        }
+        aug[(size_t)r * 2 * BS + BS + r] = 1.0;
+    }
+    for (int c = 0; c < BS; c++) {
+        int pr = c;
+        for (int r = c + 1; r < BS; r++)
+            if (std::fabs(aug[(size_t)r * 2 * BS + c]) > std::fabs(aug[(size_t)pr * 2 * BS + c])) pr = r;
+        for (int j = 0; j < 2 * BS; j++) std::swap(aug[(size_t)c * 2 * BS + j], aug[(size_t)pr * 2 * BS + j]);
+        const double d = aug[(size_t)c * 2 * BS + c];
+        CPG_CHECK(d != 0.0, CPG_ERR_SINGULAR, "interpolation normal matrix is singular");
+        for (int j = 0; j < 2 * BS; j++) aug[(size_t)c * 2 * BS + j] /= d;
+        for (int r = 0; r < BS; r++) {
+            if (r == c) continue;
+            const double f = aug[(size_t)r * 2 * BS + c];
+            for (int j = 0; j < 2 * BS; j++) aug[(size_t)r * 2 * BS + j] -= f * aug[(size_t)c * 2 * BS + j];
+        }
+    }
+    std::vector<double> fit(M + (size_t)BS * Q, 0.0);
+    for (int i = 0; i < M; i++) fit[i] = nodes[i];
+    for (int k = 0; k < BS; k++)
+        for (int q = 0; q < Q; q++) {
+            double v = 0.0;
+            for (int l = 0; l < BS; l++) v += aug[(size_t)k * 2 * BS + BS + l] * G[(size_t)q * BS + l];
+            fit[M + (size_t)k * Q + q] = v;
+        }
+    E.fit.upload(fit);
+}
+
+template <int DEG, int NC>
+static void launch_interp(const cpg_equation_s &E, const FieldArgs &f, const double *vals, double *u, hipStream_t s) {
+    hipLaunchKernelGGL((k_interp<DEG, NC>), dim3(ceil_div(E.n, 128)), dim3(128), 0, s, E.d_bb.p, E.n, f, vals,
+                       E.fit.p, E.fit.p + DEG + 1, u);
+    CPG_LAUNCH_CHECK();
+}
+
+template <int DEG, int NC>
+static void launch_interp_points(const cpg_equation_s &E, double *xy, hipStream_t s) {
+    const int cnt = E.n * (DEG + 1) * (DEG + 1);
+    hipLaunchKernelGGL((k_interp_points<DEG>), dim3(ceil_div(cnt, 256)), dim3(256), 0, s, E.d_bb.p, E.n, E.fit.p, xy);
+    CPG_LAUNCH_CHECK();
+}
+
+template <int DEG, int NC>
+static void launch_l2_error(const cpg_equation_s &E, const FieldArgs &f, const double *u, hipStream_t s) {
+    hipLaunchKernelGGL((k_l2_error<DEG, NC>), dim3(ceil_div(E.n, 128)), dim3(128), 0, s, E.geo(), E.n, f, u,
+                       E.normc.p);
+    CPG_LAUNCH_CHECK();
 }
 
 const double *eq_normc(const cpg_equation_s &E) { return E.normc.p; }
@@ -1288,6 +1507,64 @@ int cpg_l2norm(cpg_equation E, const double *u, double *out, void *stream) {
     return guard([&] {
         CPG_CHECK(E && u && out, CPG_ERR_ARG, "null argument");
         eq_newton_resid(*E, nullptr, u, 0.0, nullptr, out, S(stream));
+    });
+}
+
+static FieldArgs field_args(const cpg_equation_s &E, int field, const double *par, double t) {
+    CPG_CHECK(field == CPG_FIELD_GAUSSIAN || field == CPG_FIELD_VORTEX || field == CPG_FIELD_SINE_DENSITY, CPG_ERR_ARG,
+              "unknown field %d", field);
+    CPG_CHECK((field == CPG_FIELD_GAUSSIAN) == (E.nc == 1), CPG_ERR_ARG, "field %d does not have %d components",
+              field, E.nc);
+    CPG_CHECK(par, CPG_ERR_ARG, "null parameters");
+    FieldArgs f{};
+    f.kind = field;
+    const int np = field == CPG_FIELD_SINE_DENSITY ? 5 : field == CPG_FIELD_GAUSSIAN ? 7 : 8;
+    for (int j = 0; j < np; j++) f.par[j] = par[j];
+    f.t = t;
+    return f;
+}
+
+int cpg_interp_field(cpg_equation E, int field, const double *par, double t, double *u, void *stream) {
+    return guard([&] {
+        CPG_CHECK(E && u, CPG_ERR_ARG, "null argument");
+        const FieldArgs f = field_args(*E, field, par, t);
+        ensure_fit(*E);
+        CPG_DISPATCH(*E, launch_interp, *E, f, nullptr, u, S(stream));
+    });
+}
+
+int cpg_interp_points(cpg_equation E, double *xy, void *stream) {
+    return guard([&] {
+        CPG_CHECK(E && xy, CPG_ERR_ARG, "null argument");
+        ensure_fit(*E);
+        CPG_DISPATCH(*E, launch_interp_points, *E, xy, S(stream));
+    });
+}
+
+int cpg_interp_values(cpg_equation E, const double *vals, double *u, void *stream) {
+    return guard([&] {
+        CPG_CHECK(E && vals && u, CPG_ERR_ARG, "null argument");
+        ensure_fit(*E);
+        CPG_DISPATCH(*E, launch_interp, *E, FieldArgs{}, vals, u, S(stream));
+    });
+}
+
+int cpg_l2_error_field(cpg_equation E, const double *u, int field, const double *par, double t, double *err,
+                       void *stream) {
+    return guard([&] {
+        CPG_CHECK(E && u && err, CPG_ERR_ARG, "null argument");
+        const FieldArgs f = field_args(*E, field, par, t);
+        hipStream_t s = S(stream);
+        CPG_DISPATCH(*E, launch_l2_error, *E, f, u, s);
+        hipLaunchKernelGGL(k_colsum_partial, dim3(E->npartial, E->nc), dim3(256), 0, s, E->normc.p, E->n, E->nc,
+                           E->partial.p);
+        CPG_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(64), 0, s, E->partial.p, E->npartial, E->nc, E->sums.p);
+        CPG_LAUNCH_CHECK();
+        double h[4] = {0, 0, 0, 0};
+        CPG_HIP(hipMemcpyAsync(h, E->sums.p, sizeof(double) * E->nc, hipMemcpyDeviceToHost, s));
+        CPG_HIP(hipStreamSynchronize(s));
+        for (int c = 0; c < E->nc; c++) err[c] = std::sqrt(h[c]);
     });
 }
 

@@ -335,6 +335,34 @@ class Equation:
         _chk(lib().cpg_l2norm(self.h, _vp(ptr(u)), out.ctypes.data_as(_dp), _stream(stream)))
         return out
 
+    @staticmethod
+    def _field_par(par):
+        p = np.zeros(8)
+        p[:len(par)] = par
+        return p
+
+    def interp_field(self, field, par, t, u, stream=None):
+        """u = MeshFn::interp of an analytic field (``cpg_interp_field``; spec.FIELD_*)."""
+        p = self._field_par(par)
+        _chk(lib().cpg_interp_field(self.h, int(field), p.ctypes.data_as(_dp), C.c_double(t), _vp(ptr(u)),
+                                    _stream(stream)))
+
+    def interp_points(self, xy, stream=None):
+        """xy (device, n (p+1)^2 x 2) = the fitting points of MeshFn::interp."""
+        _chk(lib().cpg_interp_points(self.h, _vp(ptr(xy)), _stream(stream)))
+
+    def interp_values(self, vals, u, stream=None):
+        """u = the least-squares fit to vals (device, n (p+1)^2 x nc) at the fitting points."""
+        _chk(lib().cpg_interp_values(self.h, _vp(ptr(vals)), _vp(ptr(u)), _stream(stream)))
+
+    def l2_error_field(self, u, field, par, t, stream=None):
+        """MeshFn::L2Error of u against an analytic field at time t, per component."""
+        p = self._field_par(par)
+        out = np.zeros(self.spec.nc)
+        _chk(lib().cpg_l2_error_field(self.h, _vp(ptr(u)), int(field), p.ctypes.data_as(_dp), C.c_double(t),
+                                      out.ctypes.data_as(_dp), _stream(stream)))
+        return out
+
 
 class Stepper:
     """Device ``TimeStepper`` (DIRK / backward Euler / IRK / RK4)."""

@@ -31,6 +31,10 @@ PREC_BF16 = 2
 
 OP_ASSEMBLED = 0     # GMRES operator: BSR matvec of the assembled Newton matrix (reference)
 OP_MATRIX_FREE = 1   # the same matrix applied at the quadrature points (cpg_newton_apply)
+# analytic fields of cpg_interp_field / cpg_l2_error_field (parameters in include/cpolyg.h)
+FIELD_GAUSSIAN = 0        # cx, cy, a, vx, vy, Lx, Ly
+FIELD_VORTEX = 1          # beta, x0, y0, uinf, vinf, gamma, Lx, Ly
+FIELD_SINE_DENSITY = 2    # ux, uy, p, gamma, amp
 
 
 @dataclasses.dataclass

@@ -184,6 +184,29 @@ int cpg_mass_download(cpg_equation E, double *host_blocks);   /* n blocks bl x b
 /* MeshFn::L2Norm (src/MeshFn.cpp:299-312) of u, nc values to host, synchronous */
 int cpg_l2norm(cpg_equation E, const double *u, double *out, void *stream);
 
+/* ---- analytic fields: projection and L2 error on the device ----------------
+ * MeshFn::interp (src/MeshFn.cpp:56-145): per cell the least-squares fit of the basis to the
+ * field at the (p+1)^2 tensor Gauss-Legendre points of the cell's bounding box.
+ * MeshFn::L2Error (src/MeshFn.cpp:277-296): per component sqrt(sum_i int_i (u_h - f)^2) with the
+ * cell's degree-2p polygon rule.  The reference takes a host functor; here the field is one of
+ * a fixed set evaluated in the kernel, or values the caller computed at the points.
+ *   CPG_FIELD_GAUSSIAN (nc 1): par = cx, cy, a, vx, vy, Lx, Ly:
+ *     exp(-a |x - (c + v t)|^2), nearest periodic image when Lx, Ly > 0
+ *   CPG_FIELD_VORTEX (nc 4): par = beta, x0, y0, uinf, vinf, gamma, Lx, Ly: the isentropic vortex
+ *     (conserved variables) centred at (x0, y0) + (uinf, vinf) t, nearest image when Lx, Ly > 0
+ *   CPG_FIELD_SINE_DENSITY (nc 4): par = ux, uy, p, gamma, amp: density
+ *     1 + amp sin 2pi(x - ux t) sin 2pi(y - uy t), velocity (ux, uy), pressure p
+ * u, vals, xy: device arrays; the field's nc must be E's. */
+enum { CPG_FIELD_GAUSSIAN = 0, CPG_FIELD_VORTEX = 1, CPG_FIELD_SINE_DENSITY = 2 };
+int cpg_interp_field(cpg_equation E, int field, const double *par, double t, double *u, void *stream);
+/* the n (p+1)^2 fitting points, (x, y) pairs in cell-major order */
+int cpg_interp_points(cpg_equation E, double *xy, void *stream);
+/* the fit of caller values at those points (vals: n (p+1)^2 nc, component fastest) */
+int cpg_interp_values(cpg_equation E, const double *vals, double *u, void *stream);
+/* err: nc host doubles; synchronous */
+int cpg_l2_error_field(cpg_equation E, const double *u, int field, const double *par, double t, double *err,
+                       void *stream);
+
 /* ---- time stepping ----------------------------------------------------------
  * TimeStepper::advance (src/TimeIntegration.h:9-18) for DIRK (1 = backward
  * Euler, 2 = SDIRK2, 3, 5 stages; src/TimeIntegration.cpp:31-151) with the

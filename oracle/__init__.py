@@ -127,6 +127,34 @@ class Oracle:
         lib().o_l2norm(C.byref(self.m), deg, nc, _d(u), _d(out))
         return out
 
+    def interp_points(self, deg):
+        """MeshFn::interp's fitting points, (np, (deg+1)^2, 2)."""
+        out = np.zeros((self.n, (deg + 1) ** 2, 2))
+        lib().o_interp_points(C.byref(self.m), deg, _d(out))
+        return out
+
+    def interp_values(self, deg, nc, vals):
+        """Least-squares fit (MeshFn::interp) to values (np, (deg+1)^2, nc) at interp_points."""
+        vals = np.ascontiguousarray(vals, np.float64)
+        out = np.zeros(self.n * nc * (deg + 1) * (deg + 2) // 2)
+        assert lib().o_interp_values(C.byref(self.m), deg, nc, _d(vals), _d(out)) == 0
+        return out
+
+    def quad_points(self, deg2):
+        """The degree-deg2 polygon rule points of all cells: (count, 3) of x, y, weight x area."""
+        cnt = lib().o_quad_count(C.byref(self.m), deg2)
+        out = np.zeros((cnt, 3))
+        lib().o_quad_points(C.byref(self.m), deg2, _d(out))
+        return out
+
+    def l2_error_values(self, deg, nc, u, f):
+        """MeshFn::L2Error of u against values f (count, nc) at quad_points(2 deg)."""
+        out = np.zeros(nc)
+        u = np.ascontiguousarray(u, np.float64)
+        f = np.ascontiguousarray(f, np.float64)
+        lib().o_l2_error_values(C.byref(self.m), deg, nc, _d(u), _d(f), _d(out))
+        return out
+
     def dirk_step(self, spec, deg, nstages, u, dt, t, solver):
         u = np.ascontiguousarray(u, np.float64)
         out = np.zeros_like(u)

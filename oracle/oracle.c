@@ -529,6 +529,164 @@ void o_l2norm(const o_mesh *m, int deg, int nc, const double *u, double *out)
     for (int c = 0; c < nc; c++) out[c] = sqrt(acc[c]);
 }
 
+/* ---------------------------------------------------------------------------
+ * MeshFn::interp (src/MeshFn.cpp:56-145) and MeshFn::L2Error (src/MeshFn.cpp:277-296)
+ */
+
+/* the m Gauss-Legendre nodes, ascending (gsl_integration_glfixed_table in the reference):
+ * bisection on P_m between the roots of P_{m-1}, which interlace with those of P_m */
+void o_gauss_nodes(int m, double *x)
+{
+    double prev[16], cur[16];
+    int np = 0;
+    for (int n = 1; n <= m; n++) {
+        /* brackets: -1, roots of P_{n-1}, 1 */
+        double br[18];
+        int nb = 0;
+        br[nb++] = -1.0;
+        for (int k = 0; k < np; k++) br[nb++] = prev[k];
+        br[nb++] = 1.0;
+        for (int k = 0; k + 1 < nb; k++) {
+            double a = br[k], b = br[k + 1], Pa, dPa[MAXB + 2], Pv[MAXB + 2];
+            o_legendre(n, a, Pv, dPa);
+            Pa = Pv[n];
+            for (int it = 0; it < 200; it++) {
+                double c = 0.5 * (a + b), Pc;
+                o_legendre(n, c, Pv, dPa);
+                Pc = Pv[n];
+                if ((Pc < 0) == (Pa < 0)) { a = c; Pa = Pc; } else b = c;
+            }
+            cur[k] = 0.5 * (a + b);
+        }
+        np = n;
+        for (int k = 0; k < np; k++) prev[k] = cur[k];
+    }
+    for (int k = 0; k < m; k++) x[k] = prev[k];
+}
+
+void o_interp_points(const o_mesh *m, int deg, double *xy)
+{
+    int M = deg + 1, Q = M * M;
+    double nodes[16];
+    o_gauss_nodes(M, nodes);
+    for (int i = 0; i < m->np; i++) {
+        const double *bb = m->bb + 4 * i;
+        for (int qi = 0; qi < M; qi++)
+            for (int qj = 0; qj < M; qj++) {
+                double *o = xy + 2 * ((size_t)i * Q + qi * M + qj);
+                o[0] = (nodes[qi] + 1.0) * bb[2] / 2.0 + bb[0];
+                o[1] = (nodes[qj] + 1.0) * bb[3] / 2.0 + bb[1];
+            }
+    }
+}
+
+/* least squares of G c = v by Householder QR of G (Q x bs, the basis at the tensor points) */
+int o_interp_values(const o_mesh *m, int deg, int nc, const double *vals, double *u)
+{
+    int M = deg + 1, Q = M * M, bs = (deg + 1) * (deg + 2) / 2;
+    double nodes[16], G[25 * MAXB], hv[MAXB][25], beta[MAXB], phi[MAXB];
+    o_gauss_nodes(M, nodes);
+    for (int qi = 0; qi < M; qi++)
+        for (int qj = 0; qj < M; qj++) {
+            o_basis(deg, nodes[qi], nodes[qj], phi, NULL, NULL);
+            for (int k = 0; k < bs; k++) G[(qi * M + qj) * bs + k] = phi[k];
+        }
+    /* Householder: column k reflected onto e_k; G becomes R in its upper triangle */
+    for (int k = 0; k < bs; k++) {
+        double nrm = 0;
+        for (int r = k; r < Q; r++) nrm += G[r * bs + k] * G[r * bs + k];
+        nrm = sqrt(nrm);
+        if (nrm == 0) return 1;
+        double alpha = G[k * bs + k] > 0 ? -nrm : nrm, vn = 0;
+        for (int r = 0; r < Q; r++) hv[k][r] = r < k ? 0.0 : G[r * bs + k];
+        hv[k][k] -= alpha;
+        for (int r = k; r < Q; r++) vn += hv[k][r] * hv[k][r];
+        beta[k] = vn > 0 ? 2.0 / vn : 0.0;
+        for (int c = k; c < bs; c++) {
+            double d = 0;
+            for (int r = k; r < Q; r++) d += hv[k][r] * G[r * bs + c];
+            for (int r = k; r < Q; r++) G[r * bs + c] -= beta[k] * d * hv[k][r];
+        }
+    }
+    int bl = nc * bs;
+    for (int i = 0; i < m->np; i++)
+        for (int c = 0; c < nc; c++) {
+            double v[25];
+            for (int q = 0; q < Q; q++) v[q] = vals[((size_t)i * Q + q) * nc + c];
+            for (int k = 0; k < bs; k++) {
+                double d = 0;
+                for (int r = k; r < Q; r++) d += hv[k][r] * v[r];
+                for (int r = k; r < Q; r++) v[r] -= beta[k] * d * hv[k][r];
+            }
+            double *o = u + (size_t)i * bl + c * bs;
+            for (int k = bs - 1; k >= 0; k--) {
+                double s = v[k];
+                for (int l = k + 1; l < bs; l++) s -= G[k * bs + l] * o[l];
+                o[k] = s / G[k * bs + k];
+            }
+        }
+    return 0;
+}
+
+int o_quad_count(const o_mesh *m, int deg2)
+{
+    const double *tr, *ln;
+    int nt, nl;
+    pick_rules(deg2, &tr, &nt, &ln, &nl);
+    return m->tptr[m->np] * nt;
+}
+
+void o_quad_points(const o_mesh *m, int deg2, double *xyw)
+{
+    const double *tr, *ln;
+    int nt, nl;
+    pick_rules(deg2, &tr, &nt, &ln, &nl);
+    size_t e = 0;
+    for (int t = 0; t < m->tptr[m->np]; t++) {
+        const double *T = m->txy + 6 * t;
+        double area = tri_area(T);
+        for (int q = 0; q < nt; q++, e++) {
+            double r = tr[3 * q], s = tr[3 * q + 1];
+            xyw[3 * e] = T[0] * (1 - r - s) + T[2] * r + T[4] * s;
+            xyw[3 * e + 1] = T[1] * (1 - r - s) + T[3] * r + T[5] * s;
+            xyw[3 * e + 2] = tr[3 * q + 2] * area;
+        }
+    }
+}
+
+void o_l2_error_values(const o_mesh *m, int deg, int nc, const double *u, const double *f, double *out)
+{
+    int bs = (deg + 1) * (deg + 2) / 2, bl = nc * bs;
+    const double *tr, *ln;
+    int nt, nl;
+    pick_rules(2 * deg, &tr, &nt, &ln, &nl);
+    double acc[MAXNC] = {0, 0, 0, 0}, phi[MAXB], val[MAXNC];
+    size_t e = 0;
+    for (int i = 0; i < m->np; i++) {
+        double cell[MAXNC] = {0, 0, 0, 0};
+        for (int t = m->tptr[i]; t < m->tptr[i + 1]; t++) {
+            const double *T = m->txy + 6 * t;
+            double area = tri_area(T), z[MAXNC] = {0, 0, 0, 0};
+            for (int q = 0; q < nt; q++, e++) {
+                double r = tr[3 * q], s = tr[3 * q + 1];
+                double x = T[0] * (1 - r - s) + T[2] * r + T[4] * s;
+                double y = T[1] * (1 - r - s) + T[3] * r + T[5] * s;
+                double xi, eta;
+                local_coords(m, i, x, y, &xi, &eta);
+                o_basis(deg, xi, eta, phi, NULL, NULL);
+                eval_state(bs, nc, u + (size_t)i * bl, phi, val);
+                for (int c = 0; c < nc; c++) {
+                    double d = val[c] - f[e * nc + c];
+                    z[c] += d * d * tr[3 * q + 2];
+                }
+            }
+            for (int c = 0; c < nc; c++) cell[c] += z[c] * area;
+        }
+        for (int c = 0; c < nc; c++) acc[c] += cell[c];
+    }
+    for (int c = 0; c < nc; c++) out[c] = sqrt(acc[c]);
+}
+
 /* exterior state U+ at a boundary point of cell i across edge with neighbour id nb */
 static void outer_state(const o_mesh *m, const o_eq *eq, int deg, const double *u, int nb,
                         double x, double y, double t, double *Up, double *phip)

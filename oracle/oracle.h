@@ -63,6 +63,18 @@ int o_jacobian_pattern(const o_mesh *m, int *rowptr, int *colind);
 int o_jacobian(const o_mesh *m, const o_eq *eq, int deg, const double *u, double t,
                const int *rowptr, double *blocks);
 void o_l2norm(const o_mesh *m, int deg, int nc, const double *u, double *out);
+/* MeshFn::interp (src/MeshFn.cpp:56-145): the (deg+1)^2 fitting points of every cell (x, y
+ * pairs, cell-major), and the per-cell least-squares fit to values at them (vals: np x
+ * (deg+1)^2 x nc) */
+void o_gauss_nodes(int m, double *x);
+void o_interp_points(const o_mesh *m, int deg, double *xy);
+int o_interp_values(const o_mesh *m, int deg, int nc, const double *vals, double *u);
+/* MeshFn::L2Error (src/MeshFn.cpp:277-296): the degree-deg2 rule points of every cell (x, y,
+ * weight x sub-triangle area; o_quad_count of them), and the per-component error of u against
+ * values f at those points for deg2 = 2 deg */
+int o_quad_count(const o_mesh *m, int deg2);
+void o_quad_points(const o_mesh *m, int deg2, double *xyw);
+void o_l2_error_values(const o_mesh *m, int deg, int nc, const double *u, const double *f, double *out);
 void o_eval_bc(const o_eq *eq, double x, double y, double t, double *out);
 
 /* block sparse algebra (src/BlockMatrix.cpp, src/Preconditioners.cpp) */

@@ -133,3 +133,69 @@ def test_free_stream_preserved(cuda):
     ud, bd = _dev(cuda, u), _dev(cuda, np.zeros_like(u))
     E.residual(ud, 0.0, bd)
     assert np.abs(bd.cpu().numpy()).max() < 1e-11
+
+
+# ---- analytic fields: MeshFn::interp and MeshFn::L2Error on the device ---------------------
+def _np_field(kind, par, t, x, y):
+    """The fields of cpg_interp_field, in numpy (nearest periodic image where Lx, Ly > 0)."""
+    from cpolyg_amd.spec import FIELD_GAUSSIAN, FIELD_VORTEX, isentropic_vortex
+
+    def wrap(d, L):
+        return d - L * np.rint(d / L) if L > 0 else d
+
+    if kind == FIELD_GAUSSIAN:
+        cx, cy, a, vx, vy, Lx, Ly = par
+        dx, dy = wrap(x - cx - vx * t, Lx), wrap(y - cy - vy * t, Ly)
+        return np.exp(-a * (dx * dx + dy * dy))[..., None]
+    if kind == FIELD_VORTEX:
+        beta, x0, y0, uinf, vinf, gam, Lx, Ly = par
+        # the centre's nearest image, then the unwrapped formula at t = 0
+        dx, dy = wrap(x - x0 - uinf * t, Lx), wrap(y - y0 - vinf * t, Ly)
+        return isentropic_vortex(dx, dy, 0.0, beta, 0.0, 0.0, uinf, vinf, gam)
+    ux, uy, p, gam, amp = par
+    rho = 1 + amp * np.sin(2 * np.pi * (x - ux * t)) * np.sin(2 * np.pi * (y - uy * t))
+    return np.stack([rho, rho * ux, rho * uy, p / (gam - 1) + 0.5 * rho * (ux * ux + uy * uy) + 0 * rho], axis=-1)
+
+
+@pytest.mark.parametrize("kind,deg", [(0, 2), (0, 4), (1, 3), (1, 1), (2, 4), (2, 2)])
+def test_interp_and_l2_error_match_oracle(cuda, kind, deg):
+    """Projection of an analytic field (and of caller values at the fitting points) and the L2
+    error against the field at a later time equal the oracle's restatement (Householder least
+    squares, its own sub-triangle rule) on the same mesh."""
+    import torch
+    from cpolyg_amd.spec import FIELD_GAUSSIAN, FIELD_VORTEX
+
+    if kind == FIELD_GAUSSIAN:
+        m = unit_periodic_mesh(8)
+        spec = EquationSpec.reference_advection()
+        par = (0.5, 0.4, 60.0, 0.9, -0.6, 1.0, 1.0)   # crosses the periodic boundary by t = 0.7
+    elif kind == FIELD_VORTEX:
+        m = vortex_mesh(8)
+        spec = EquationSpec.euler()
+        par = (5.0, 5.0, 5.0, 1.0, 1.0, 1.4, 10.0, 10.0)
+    else:
+        m = unit_periodic_mesh(8)
+        spec = EquationSpec.euler()
+        par = (1.0, 0.5, 1.0, 1.4, 0.2)
+    E = hip.Equation(hip.Mesh(m), spec, deg)
+    orc = O.Oracle(m)
+    nc, bs, n = spec.nc, bs_of(deg), m.n_cells
+    Q = (deg + 1) ** 2
+    xy = torch.empty(n * Q * 2, dtype=torch.float64, device=cuda)
+    E.interp_points(xy)
+    pts = orc.interp_points(deg)
+    assert np.abs(xy.cpu().numpy().reshape(n, Q, 2) - pts).max() < 1e-13
+    t = 0.7
+    u = torch.empty(n * nc * bs, dtype=torch.float64, device=cuda)
+    E.interp_field(kind, par, t, u)
+    vals = _np_field(kind, par, t, pts[..., 0], pts[..., 1])
+    uo = orc.interp_values(deg, nc, vals)
+    assert rel(u.cpu().numpy(), uo) < 1e-12
+    u2 = torch.empty_like(u)
+    E.interp_values(_dev(cuda, vals), u2)
+    assert rel(u2.cpu().numpy(), uo) < 1e-12
+    for t2 in (t, t + 0.05):
+        q = orc.quad_points(2 * deg)
+        eo = orc.l2_error_values(deg, nc, uo, _np_field(kind, par, t2, q[:, 0], q[:, 1]))
+        eg = E.l2_error_field(u, kind, par, t2)
+        assert np.all(np.abs(eg - eo) <= 1e-9 * eo + 1e-14), (t2, eg, eo)

@@ -473,3 +473,52 @@ def test_rk4_linear_advection_dense():
     k4 = dt * L @ (u + k3)
     want = u + (k1 + 2 * k2 + 2 * k3 + k4) / 6
     assert rel(orc.rk4_step(spec, deg, u, dt, 0.0), want) < 1e-12
+
+
+# ---- MeshFn::interp and MeshFn::L2Error (src/MeshFn.cpp:56-145, 277-296) ---------------
+
+@pytest.mark.parametrize("m", [1, 2, 3, 4, 5])
+def test_oracle_gauss_nodes(m):
+    x = np.zeros(m)
+    O.lib().o_gauss_nodes(m, x.ctypes.data_as(O._dp))
+    assert np.abs(x - np.polynomial.legendre.leggauss(m)[0]).max() < 1e-15
+
+
+def _poly(coef, x, y):
+    """sum c_ab x^a y^b over a + b <= p (coef indexed by the basis order)."""
+    out = 0.0
+    for (a, b), c in coef.items():
+        out = out + c * x ** a * y ** b
+    return out
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+def test_oracle_interp_reproduces_polynomials_and_l2_error(deg):
+    """The fit reproduces a polynomial of total degree <= p exactly (it lies in the basis span),
+    so its L2 error against the polynomial vanishes; the error of u = 0 against a constant c is
+    |c| sqrt(area)."""
+    m = vortex_mesh(4)
+    orc = O.Oracle(m)
+    rng = np.random.default_rng(deg)
+    coef = {(a, b): rng.standard_normal() for a in range(deg + 1) for b in range(deg + 1 - a)}
+    pts = orc.interp_points(deg)
+    # cell-local polynomial around the domain centre keeps the values O(1..100)
+    vals = _poly(coef, (pts[..., 0] - 5) / 5, (pts[..., 1] - 5) / 5)[..., None]
+    u = orc.interp_values(deg, 1, vals)
+    q = orc.quad_points(2 * deg)
+    f = _poly(coef, (q[:, 0] - 5) / 5, (q[:, 1] - 5) / 5)[:, None]
+    err = orc.l2_error_values(deg, 1, u, f)[0]
+    norm = np.sqrt(np.sum(q[:, 2] * f[:, 0] ** 2))
+    assert err < 1e-12 * norm
+    c = 2.5
+    err0 = orc.l2_error_values(deg, 1, np.zeros_like(u), np.full((len(q), 1), c))[0]
+    assert abs(err0 - c * np.sqrt(m.areas().sum())) < 1e-12 * c * 10
+    # and the fit of a non-polynomial is the least-squares one: numpy's lstsq on the same system
+    from cpolyg_amd.setups import legendre_basis
+    g = np.exp(-0.3 * ((pts[..., 0] - 4) ** 2 + (pts[..., 1] - 6) ** 2))
+    ug = orc.interp_values(deg, 1, g[..., None])
+    nodes = np.polynomial.legendre.leggauss(deg + 1)[0]
+    qi, qj = np.meshgrid(nodes, nodes, indexing="ij")
+    G = legendre_basis(deg, qi.ravel(), qj.ravel())
+    ref = np.linalg.lstsq(G, g.T, rcond=None)[0].T.ravel()
+    assert np.abs(ug - ref).max() < 1e-13 * max(1.0, np.abs(ref).max())
