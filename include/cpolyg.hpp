@@ -274,6 +274,24 @@ class Equation {
         return y;
     }
 
+    // MeshFn::interp (src/MeshFn.cpp:56-145) of a built-in field (CPG_FIELD_*, parameters as in
+    // cpolyg.h) at time t: the least-squares fit at the bounding-box Gauss points
+    MeshFn interp(int field, const std::vector<double> &par, double t = 0) const {
+        std::vector<double> p(8, 0.0);
+        for (size_t j = 0; j < par.size() && j < 8; j++) p[j] = par[j];
+        MeshFn u(np_, deg_, nc_);
+        check(cpg_interp_field(handle(), field, p.data(), t, u.data(), nullptr));
+        check(cpg_stream_sync(nullptr));
+        return u;
+    }
+    // MeshFn::L2Error (src/MeshFn.cpp:277-296) against the same fields, per component
+    std::vector<double> L2Error(const MeshFn &f, int field, const std::vector<double> &par, double t = 0) const {
+        std::vector<double> p(8, 0.0), err(nc_);
+        for (size_t j = 0; j < par.size() && j < 8; j++) p[j] = par[j];
+        check(cpg_l2_error_field(handle(), f.data(), field, p.data(), t, err.data(), nullptr));
+        return err;
+    }
+
   protected:
     int deg_, nc_, np_ = 0;
     std::shared_ptr<cpg_equation_s> h_;

@@ -136,10 +136,34 @@ int main() {
     o_dirk_step(&om, &oe, deg, 3, u0.data(), dt, 0.0, &os, uo.data(), &ost);
     const double e_dirk = rel(u1.a->download(), uo);
 
-    std::printf("residual %.3e jacobian %.3e gmres(it %d vs %d) %.3e dirk3 %.3e (newton %d vs %d)\n", e_res, e_jac,
-                J.last_iterations, it, e_sol, e_dirk, dirk.last_stats.newton_iters, ost.newton_iters);
+    // MeshFn::interp of a gaussian bump and MeshFn::L2Error against it, with the oracle's
+    // restatement on the same points
+    const std::vector<double> gpar = {0.5, 0.5, 20.0, 0.0, 0.0, 1.0, 1.0};
+    MeshFn g = eqn.interp(CPG_FIELD_GAUSSIAN, gpar);
+    const int Q = (deg + 1) * (deg + 1);
+    std::vector<double> xy((size_t)np * Q * 2), gv((size_t)np * Q), go((size_t)np * bs);
+    o_interp_points(&om, deg, xy.data());
+    auto bump = [](double x, double y) {
+        const double dx = x - 0.5 - std::rint(x - 0.5), dy = y - 0.5 - std::rint(y - 0.5);
+        return std::exp(-20.0 * (dx * dx + dy * dy));
+    };
+    for (int e = 0; e < np * Q; e++) gv[e] = bump(xy[2 * e], xy[2 * e + 1]);
+    o_interp_values(&om, deg, 1, gv.data(), go.data());
+    const double e_interp = rel(g.a->download(), go);
+    const double err = eqn.L2Error(g, CPG_FIELD_GAUSSIAN, gpar)[0];
+    const int nq = o_quad_count(&om, 2 * deg);
+    std::vector<double> qp((size_t)nq * 3), fq(nq);
+    o_quad_points(&om, 2 * deg, qp.data());
+    for (int e = 0; e < nq; e++) fq[e] = bump(qp[3 * e], qp[3 * e + 1]);
+    double erro = 0.0;
+    o_l2_error_values(&om, deg, 1, go.data(), fq.data(), &erro);
+    const double e_l2 = std::abs(err - erro) / erro;
+
+    std::printf("residual %.3e jacobian %.3e gmres(it %d vs %d) %.3e dirk3 %.3e (newton %d vs %d) interp %.3e "
+                "l2error %.3e\n", e_res, e_jac, J.last_iterations, it, e_sol, e_dirk, dirk.last_stats.newton_iters,
+                ost.newton_iters, e_interp, e_l2);
     const bool ok = e_res < 1e-12 && e_jac < 1e-12 && e_sol < 1e-9 && e_dirk < 1e-10 &&
-                    std::abs(J.last_iterations - it) <= 1;
+                    std::abs(J.last_iterations - it) <= 1 && e_interp < 1e-12 && e_l2 < 1e-9;
     std::printf(ok ? "PASS\n" : "FAIL\n");
     return ok ? 0 : 1;
 }
