@@ -78,8 +78,9 @@ def parse():
     p.add_argument("--irk-nx", type=int, default=512, help="irk: lattice size (cells = nx^2)")
     p.add_argument("--irk-cfl", type=float, default=10.0)
     p.add_argument("--irk-restart", type=int, default=60)
-    p.add_argument("--irk-pc", default="bjacobi,dilu",
-                   help="irk: preconditioners, comma-separated (bjacobi: BF16 and FP64 factors; dilu: BF16 and FP32)")
+    p.add_argument("--irk-pc", default="bjacobi,dilu,wform",
+                   help="irk: preconditioners, comma-separated (bjacobi: BF16 and FP64 factors; dilu: BF16 and FP32; "
+                        "wform: the W form with stage-block-diagonal DILU, BF16)")
     return p.parse_args()
 
 
@@ -322,6 +323,9 @@ def bench_irk(args):
             runs += [("bjacobi_bf16", PC_BJACOBI, PREC_BF16), ("bjacobi_fp64", PC_BJACOBI, PREC_FP64)]
         elif pc == "dilu":
             runs += [("dilu_bf16", PC_DILU, PREC_BF16), ("dilu_fp32", PC_DILU, PREC_FP32)]
+        elif pc == "wform":
+            # IRK::newAdvance's form: stage-block-diagonal DILU (40 x 40 blocks)
+            runs += [("wform_dilu_bf16", PC_DILU, PREC_BF16)]
         else:
             raise SystemExit(f"unknown --irk-pc entry {pc!r}")
     res = {}
@@ -331,7 +335,7 @@ def bench_irk(args):
                         pc_kind=pc, newton_tol=1e-10, newton_maxit=10, precision=prec,
                         newton_operator=OP_MATRIX_FREE, pc_refresh=args.pc_refresh)
         try:
-            S = hip.Stepper(E, 1, 3, sp)
+            S = hip.Stepper(E, 3 if name.startswith("wform") else 1, 3, sp)
         except hip.CpgError as e:   # e.g. the factors of this precision do not fit
             res[name] = {"error": str(e)}
             print(f"# {name}: {e}", file=sys.stderr, flush=True)

@@ -140,6 +140,42 @@ void irk_pcop(void *ctx, const double *in, double *out, const int *active, hipSt
     CPG_LAUNCH_CHECK();
 }
 
+// ---- Radau IIA in the W form (IRK::newAdvance, src/TimeIntegration.cpp:283-395) -----------
+// Unknowns w_s (stage states u + w_s); row s of the Newton matrix applied to x is
+// M (sum_j Ainv[s][j] x_j) - dt J(u + w_s) x_s: the Newton operator of stage s's state on x_s
+// with its mass term on the combination.  Preconditioner: the factor of each stage's diagonal
+// block Ainv[s][s] M - dt J_s on its own part of the vector.
+struct IrkWOp {
+    const cpg_equation_s *E;
+    int ns;
+    size_t N;
+    double Ainv[9];
+    const double *negdt;          // -dt in device memory
+    double *z;                    // N scratch: the mass term's vector
+    double *vol[3], *edg[3];      // point data of each stage's state
+};
+void irkw_linop(void *ctx, const double *x, double *y, const int *active, hipStream_t s) {
+    const auto *o = static_cast<const IrkWOp *>(ctx);
+    for (int st = 0; st < o->ns; st++) {
+        IrkCoef3 c{};
+        for (int j = 0; j < o->ns; j++) c.a[j] = o->Ainv[st * o->ns + j];
+        hipLaunchKernelGGL(k_irk_combine, dim3(ceil_div(o->N, 256)), dim3(256), 0, s, c, o->ns, x, o->N, o->z, active);
+        CPG_LAUNCH_CHECK();
+        eq_newton_apply_dev(*o->E, o->negdt, x + o->N * st, y + o->N * st, active, s, 0.0, o->z, o->vol[st],
+                            o->edg[st]);
+    }
+}
+
+struct IrkWPc {
+    cpg_prec P[3];
+    int ns;
+    size_t N;
+};
+void irkw_pcop(void *ctx, const double *in, double *out, const int *active, hipStream_t s) {
+    const auto *o = static_cast<const IrkWPc *>(ctx);
+    for (int st = 0; st < o->ns; st++) prec_apply_to(o->P[st], in + o->N * st, out + o->N * st, s, active);
+}
+
 struct LinComb {
     const double *v[6];
     double a[6];
@@ -292,6 +328,13 @@ struct cpg_stepper_s {
     bool irk_mf = false;       // the matrix-free coupled path is set up
     cpg_bsr Jst[2] = {nullptr, nullptr};   // coupled DILU: the Jacobians of stages 1, 2 (J: stage 0)
     cpg_bsr Cpat = nullptr;    // coupled DILU: the (ns bl)-wide pattern (no blocks)
+    // IRK in the W form
+    cpg_prec Pw[3] = {nullptr, nullptr, nullptr};   // factors of the stage diagonal blocks
+    double Ainv[9] = {};       // A^-1 (row-major, ns x ns)
+    double btAinv[3] = {};     // b^T A^-1: u_{n+1} = u + sum_s btAinv[s] w_s
+    DevBuf<double> wv, zbuf;   // w (ns N), the operator's mass-term vector (N)
+    cpg::IrkWOp irkwop{};
+    cpg::IrkWPc irkwpc{};
     DevBuf<double> irk_data, pin, pout, wbuf;
     cpg::IrkOp irkop{};
     cpg::IrkPc irkpc{};
@@ -308,6 +351,8 @@ struct cpg_stepper_s {
         if (Dc) cpg_bsr_destroy(Dc);
         for (cpg_bsr j : Jst)
             if (j) cpg_bsr_destroy(j);
+        for (cpg_prec p : Pw)
+            if (p) cpg_prec_destroy(p);
         if (Cpat) cpg_bsr_destroy(Cpat);
     }
 };
@@ -354,8 +399,8 @@ extern "C" {
 int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver_params *params, cpg_stepper *out) {
     return guard([&] {
         CPG_CHECK(E && params && out, CPG_ERR_ARG, "null argument");
-        CPG_CHECK(method == CPG_STEP_DIRK || method == CPG_STEP_RK4 || method == CPG_STEP_IRK, CPG_ERR_UNSUPPORTED,
-                  "stepper method %d not available", method);
+        CPG_CHECK(method == CPG_STEP_DIRK || method == CPG_STEP_RK4 || method == CPG_STEP_IRK ||
+                      method == CPG_STEP_IRK_W, CPG_ERR_UNSUPPORTED, "stepper method %d not available", method);
         auto *S = new cpg_stepper_s;
         if (const char *v = std::getenv("CPOLYG_VERBOSE")) S->verbose = v[0] != '\0' && v[0] != '0';
         try {
@@ -363,7 +408,7 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
             S->method = method;
             S->ns = method == CPG_STEP_RK4 ? 4 : nstages;
             S->sp = *params;
-            tableau(method, S->ns, S->A, S->b, S->c);
+            tableau(method == CPG_STEP_IRK_W ? CPG_STEP_IRK : method, S->ns, S->A, S->b, S->c);
             int n, bl, nnzb;
             check(cpg_equation_info(E, &n, &bl, &nnzb));
             S->n = n;
@@ -391,6 +436,60 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
                           params->pc_refresh);
                 S->opscale.alloc(1);
                 S->nop = cpg::NewtonOp{E, S->opscale.p};
+            } else if (method == CPG_STEP_IRK_W) {
+                CPG_CHECK(nstages == 2 || nstages == 3, CPG_ERR_UNSUPPORTED, "Radau IIA with %d stages", nstages);
+                CPG_CHECK(params->restart > 0 && params->maxit > 0 && params->newton_maxit > 0, CPG_ERR_ARG,
+                          "bad solver parameters");
+                CPG_CHECK(params->gmres_mode == CPG_GMRES_FLEXIBLE, CPG_ERR_ARG,
+                          "the W-form IRK step applies its matrix matrix-free: use flexible GMRES");
+                CPG_CHECK(params->pc_kind == CPG_PC_BJACOBI || params->pc_kind == CPG_PC_DILU, CPG_ERR_UNSUPPORTED,
+                          "the W-form IRK step takes block Jacobi or block DILU per stage");
+                CPG_CHECK(params->pc_refresh >= 0 && params->pc_refresh <= 2, CPG_ERR_ARG, "bad pc_refresh %d",
+                          params->pc_refresh);
+                const int ns = nstages;
+                // A^-1 by Gauss-Jordan (ns <= 3), and b^T A^-1
+                double a[3][6] = {};
+                for (int i = 0; i < ns; i++) {
+                    for (int j = 0; j < ns; j++) a[i][j] = S->A[i * ns + j];
+                    a[i][ns + i] = 1.0;
+                }
+                for (int c = 0; c < ns; c++) {
+                    int pr = c;
+                    for (int r = c + 1; r < ns; r++)
+                        if (std::fabs(a[r][c]) > std::fabs(a[pr][c])) pr = r;
+                    for (int j = 0; j < 2 * ns; j++) std::swap(a[c][j], a[pr][j]);
+                    const double d = a[c][c];
+                    for (int j = 0; j < 2 * ns; j++) a[c][j] /= d;
+                    for (int r = 0; r < ns; r++) {
+                        if (r == c) continue;
+                        const double f = a[r][c];
+                        for (int j = 0; j < 2 * ns; j++) a[r][j] -= f * a[c][j];
+                    }
+                }
+                for (int i = 0; i < ns; i++)
+                    for (int j = 0; j < ns; j++) S->Ainv[i * ns + j] = a[i][ns + j];
+                for (int j = 0; j < ns; j++) {
+                    S->btAinv[j] = 0.0;
+                    for (int i = 0; i < ns; i++) S->btAinv[j] += S->b[i] * S->Ainv[i * ns + j];
+                }
+                check(cpg_jacobian_create(E, &S->J));
+                for (int st = 0; st < ns; st++) check(cpg_prec_create(S->J, params->pc_kind, params->precision, &S->Pw[st]));
+                check(cpg_krylov_create(ns * n, bl, params->restart, CPG_GMRES_FLEXIBLE, &S->K));
+                S->bigR.alloc(S->N * ns);
+                S->bigK.alloc(S->N * ns);
+                S->wv.alloc(S->N * ns);
+                S->zbuf.alloc(S->N);
+                S->opscale.alloc(1);
+                size_t nvol = 0, nedg = 0;
+                eq_newton_data_size(*E, &nvol, &nedg);
+                S->irk_data.alloc((nvol + nedg) * ns);
+                S->irkwop = cpg::IrkWOp{E, ns, S->N, {}, S->opscale.p, S->zbuf.p, {}, {}};
+                for (int j = 0; j < ns * ns; j++) S->irkwop.Ainv[j] = S->Ainv[j];
+                for (int st = 0; st < ns; st++) {
+                    S->irkwop.vol[st] = nvol ? S->irk_data.p + (nvol + nedg) * st : nullptr;
+                    S->irkwop.edg[st] = nvol ? S->irk_data.p + (nvol + nedg) * st + nvol : nullptr;
+                }
+                S->irkwpc = cpg::IrkWPc{{S->Pw[0], S->Pw[1], S->Pw[2]}, ns, S->N};
             } else if (method == CPG_STEP_IRK && params->newton_operator == 1) {
                 CPG_CHECK(nstages == 2 || nstages == 3, CPG_ERR_UNSUPPORTED, "Radau IIA with %d stages", nstages);
                 CPG_CHECK(params->restart > 0 && params->maxit > 0 && params->newton_maxit > 0, CPG_ERR_ARG,
@@ -591,6 +690,86 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
             lincomb(*S, unp1, u, S->b, 4, k, s);
             CPG_HIP(hipStreamSynchronize(s));
             st.ms_total = ms_since(t_all);
+            return;
+        }
+        if (S->method == CPG_STEP_IRK_W) {
+            // IRK::newAdvance (src/TimeIntegration.cpp:283-395): Newton on w with k = A^-1 w, the
+            // stage-block-diagonal factors refreshed per pc_refresh, flexible GMRES warm-started
+            // from the previous correction (bigU is never reset in the reference)
+            const int ns = S->ns;
+            const size_t NB = N * ns;
+            double *w = S->wv.p;
+            CPG_HIP(hipMemsetAsync(w, 0, NB * sizeof(double), s));
+            CPG_HIP(hipMemsetAsync(S->bigK.p, 0, NB * sizeof(double), s));
+            hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, S->opscale.p, -dt);
+            CPG_LAUNCH_CHECK();
+            const double one[1] = {1.0};
+            bool converged = false, factored = false;
+            for (int it = 0; it < S->sp.newton_maxit; it++) {
+                auto t0 = clk::now();
+                double rn2 = 0.0;
+                for (int stg = 0; stg < ns; stg++) {
+                    IrkCoef3 c{};
+                    for (int j = 0; j < ns; j++) c.a[j] = S->Ainv[stg * ns + j];
+                    hipLaunchKernelGGL(k_irk_combine, dim3(ceil_div(N, 256)), dim3(256), 0, s, c, ns, w, N,
+                                       k + N * stg, nullptr);
+                    CPG_LAUNCH_CHECK();
+                    lincomb(*S, S->cu.p, u, one, 1, w + N * stg, s);
+                    eq_residual(*S->E, S->cu.p, t + dt * S->c[stg], S->rhs.p, s);
+                    const double rs = eq_newton_resid(*S->E, k + N * stg, S->rhs.p, dt, S->bigR.p + N * stg, nullptr, s);
+                    rn2 += rs * rs;
+                }
+                const double rn = std::sqrt(rn2);
+                st.ms_residual += ms_since(t0);
+                st.last_resid = rn;
+                CPG_CHECK(std::isfinite(rn), CPG_ERR_NONFINITE, "non-finite IRK Newton residual (it %d)", it);
+                if (rn < S->sp.newton_tol) { converged = true; break; }
+                st.newton_iters++;
+                if (S->sp.pc_refresh == 0 || !factored) {
+                    // stage s: Ainv[s][s] M - dt J(u + w_s), formed as a (M - dt / a J)
+                    for (int stg = 0; stg < ns; stg++) {
+                        t0 = clk::now();
+                        const double a = S->Ainv[stg * ns + stg];
+                        lincomb(*S, S->cu.p, u, one, 1, w + N * stg, s);
+                        eq_jacobian(*S->E, S->cu.p, t + dt * S->c[stg], -dt / a, 1, S->J->blocks.p, s);
+                        check(cpg_bsr_scale(S->J, a, s));
+                        CPG_HIP(hipStreamSynchronize(s));
+                        st.ms_jacobian += ms_since(t0);
+                        t0 = clk::now();
+                        check(cpg_prec_factor(S->Pw[stg], S->J, s));
+                        st.ms_factor += ms_since(t0);
+                    }
+                    factored = true;
+                }
+                t0 = clk::now();
+                for (int stg = 0; stg < ns; stg++) {
+                    lincomb(*S, S->cu.p, u, one, 1, w + N * stg, s);
+                    eq_newton_prepare(*S->E, S->cu.p, t + dt * S->c[stg], s, S->irkwop.vol[stg], S->irkwop.edg[stg]);
+                }
+                double tol = S->sp.rtol;
+                int maxit = S->sp.maxit;
+                check(guard([&] {
+                    gmres_flexible_op(S->K, LinOp{irkw_linop, &S->irkwop}, PcOp{irkw_pcop, &S->irkwpc}, S->bigR.p,
+                                      S->bigK.p, false, &tol, &maxit, s);
+                }));
+                st.ms_gmres += ms_since(t0);
+                st.gmres_iters += maxit;
+                st.linear_solves++;
+                if (S->verbose)
+                    std::fprintf(stderr, "cpolyg: IRK (W form) Newton iteration %d, residual norm = %.6e, GMRES "
+                                 "iterations %d\n", it, rn, maxit);
+                hipLaunchKernelGGL(k_axpy, dim3(ceil_div(NB, 256)), dim3(256), 0, s, w, -1.0, S->bigK.p, NB);
+                CPG_LAUNCH_CHECK();
+            }
+            lincomb(*S, unp1, u, S->btAinv, ns, w, s);
+            CPG_HIP(hipStreamSynchronize(s));
+            st.ms_total = ms_since(t_all);
+            if (!converged) {
+                st.status = CPG_ERR_NOT_CONVERGED;
+                set_error("IRK Newton did not reach %g in %d iterations (last %g)", S->sp.newton_tol,
+                          S->sp.newton_maxit, st.last_resid);
+                throw Error{CPG_ERR_NOT_CONVERGED};
+            }
             return;
         }
         if (S->method == CPG_STEP_IRK && S->irk_mf) {

@@ -211,7 +211,12 @@ int cpg_l2_error_field(cpg_equation E, const double *u, int field, const double 
  * TimeStepper::advance (src/TimeIntegration.h:9-18) for DIRK (1 = backward
  * Euler, 2 = SDIRK2, 3, 5 stages; src/TimeIntegration.cpp:31-151) with the
  * Newton/GMRES loop on the device. */
-enum { CPG_STEP_DIRK = 0, CPG_STEP_IRK = 1, CPG_STEP_RK4 = 2 };
+enum { CPG_STEP_DIRK = 0, CPG_STEP_IRK = 1, CPG_STEP_RK4 = 2, CPG_STEP_IRK_W = 3 };
+/* CPG_STEP_IRK_W: Radau IIA in the form of IRK::newAdvance (src/TimeIntegration.cpp:283-395):
+ * unknowns w_s = sum_t A[s][t] k_t (stage states u + w_s), residuals M (A^-1 w)_s - dt b(u + w_s),
+ * Newton matrix blocks A^-1[s][j] M (+ -dt J_s on the diagonal), preconditioned by the
+ * stage-block-diagonal factors of A^-1[s][s] M - dt J_s (pc_kind DILU or block Jacobi, any
+ * precision); the coupled matrix is applied matrix-free, so gmres_mode must be flexible. */
 typedef struct {
     int gmres_mode;         /* CPG_GMRES_* */
     int restart;

@@ -152,3 +152,30 @@ def test_irk_matrix_free_coupled(cuda, ns, prec, refresh, pc):
     gs = S.advance(ud, dt, 0.0, outd)
     assert gs.last_resid < 1e-12
     assert rel(outd.cpu().numpy(), uo) < 1e-9
+
+
+STEP_IRK_W = 3
+
+
+@pytest.mark.parametrize("ns,prec,pc", [(3, PREC_FP64, PC_DILU), (3, PREC_BF16, PC_DILU), (2, PREC_FP64, PC_BJACOBI),
+                                        (3, PREC_FP32, PC_BJACOBI)])
+def test_irk_w_form_matches_oracle(cuda, ns, prec, pc):
+    """Radau IIA in the form of IRK::newAdvance (unknowns w = A k, stage-block-diagonal
+    preconditioner of A^-1[s][s] M - dt J_s, coupled matrix applied matrix-free): the converged
+    step solves the same stage equations as IRK::advance, so it equals the oracle's step."""
+    m = vortex_mesh(6)
+    deg = 3
+    spec = EquationSpec.euler(periodic_trace=1, jacobian=1)
+    u = vortex_state(m, deg)
+    ref = SolverSpec(gmres_mode=GMRES_LEFT_REF, restart=20, rtol=1e-11, maxit=4000, pc_kind=PC_DILU,
+                     newton_tol=1e-12, newton_maxit=30)
+    dt = 0.4
+    uo, st = O.Oracle(m).irk_step(spec, deg, ns, u, dt, 0.0, ref)
+    solver = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=40, rtol=1e-11, maxit=20000, pc_kind=pc,
+                        newton_tol=1e-12, newton_maxit=30, precision=prec, newton_operator=OP_MATRIX_FREE,
+                        pc_refresh=2)
+    S = hip.Stepper(hip.Equation(hip.Mesh(m), spec, deg), STEP_IRK_W, ns, solver)
+    ud, outd = _dev(cuda, u), _dev(cuda, np.zeros_like(u))
+    gs = S.advance(ud, dt, 0.0, outd)
+    assert gs.last_resid < 1e-12
+    assert rel(outd.cpu().numpy(), uo) < 1e-9
