@@ -515,11 +515,12 @@ def main():
     gm = sum(s["gmres_iters"] for s in st_mix) / steps
     if mixed_pc == PC_DILU:
         # dominant kernel of the mixed step: the DILU apply (one row-stream launch per colour level
-        # and sweep, forward level 0 excepted, timed as one unit inside GMRES)
+        # and sweep, forward level 0 and backward level 0 excepted, timed as one unit inside GMRES)
         ap_gbs = gbs(b_ap, ap_ms)
         lev = rates["dilu_levels"]
-        roof = {"bound": "hbm", "kernel": f"dilu_apply ({args.mixed} factors, 40x40 blocks, {2 * lev - 1} row-stream "
-                                          f"launches: {lev} colour levels x 2 sweeps, forward level 0 a no-op)",
+        roof = {"bound": "hbm", "kernel": f"dilu_apply ({args.mixed} factors, 40x40 blocks, {2 * lev - 2} row-stream "
+                                          f"launches: {lev} colour levels x 2 sweeps, forward level 0 a no-op, the last "
+                                          f"forward level finishing its rows' backward step)",
                 "achieved": ap_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ap_gbs / HBM_PEAK_GBS,
                 "traffic": APPLY_TRAFFIC.get((n, deg, args.mixed)), "algorithmic_bytes": b_ap, "avg_ms": ap_ms,
                 "launches": na}

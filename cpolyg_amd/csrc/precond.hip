@@ -1674,6 +1674,12 @@ void apply_t(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s, const int *act
     // Level 0 rows have no lower blocks, y_i = x_i; the packed sources of y there name x
     // (src2), so the level needs no launch for FP64 vectors.  FP32 vectors (the inner solver's)
     // run it as a copy into y, which their forward sources then read.
+    // The last forward level has no upper blocks, so its backward step is x_i = D_i^-1 y_i; with
+    // contiguous levels the first backward level is the same rows in the same order, and the
+    // forward launch of the last level finishes them (FP64 vectors).
+    const int nlf = (int)P.flev.size() - 1;
+    const bool fuse = x64 && rowstream_compiled(bl) && nlf >= 2 && P.frow0[nlf - 1] >= 0 && P.brow0.size() > 0 &&
+                      P.brow0[0] == P.frow0[nlf - 1] && P.blev[1] - P.blev[0] == P.flev[nlf] - P.flev[nlf - 1];
     for (size_t lv = x64 ? 1 : 0; lv + 1 < P.flev.size(); lv++) {
         const int r0 = P.flev[lv], nr = P.flev[lv + 1] - r0;
         if (!nr) continue;
@@ -1693,9 +1699,15 @@ void apply_t(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s, const int *act
             a.src2 = y;
         }
         a.out = y;
+        if constexpr (x64) {
+            if (fuse && (int)lv == nlf - 1) {
+                a.S.dptr = P.bptr.p + P.blev[0];
+                a.out = out;
+            }
+        }
         launch_rowstream<T, double>(bl, a, F, sc, s);
     }
-    for (size_t lv = 0; lv + 1 < P.blev.size(); lv++) {
+    for (size_t lv = fuse ? 1 : 0; lv + 1 < P.blev.size(); lv++) {
         const int r0 = P.blev[lv], nr = P.blev[lv + 1] - r0;
         if (!nr) continue;
         // x_i = D_i^-1 y_i - sum_j DU_ij x_j

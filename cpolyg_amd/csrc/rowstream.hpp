@@ -44,6 +44,9 @@ struct RowStream {
     const int *bcol;
     const int *active = nullptr;   // device flag: when it reads 0 the launch does nothing
     int row0 = 0;                  // rows null: item w writes row row0 + w (a contiguous level)
+    // set: item w's row result z is followed by the packed block at dptr[w] (D^-1 of a DILU row
+    // without upper blocks) and the launch writes that block times z instead of z
+    const int *dptr = nullptr;
 };
 
 // Packed source of a block (bcol entry): j >= 0 reads c1 * src1[j]; -1 - u with u < kSrc2Tag
@@ -233,12 +236,60 @@ __global__ __launch_bounds__(64 * WPR) void k_rowstream(RowStream S, const T *__
             for (int v = 0; v < VW; v++) red[(g0 + GW * WPR * l) * BL + rr + v] = acc[l][v];
     }
     __syncthreads();
+    if (!S.dptr) {
+        for (int t = tid; t < BL; t += NT) {
+            double s = base64 ? base64[(size_t)i * BL + t] : base32 ? (double)base32[(size_t)i * BL + t] : 0.0;
+            double r = 0.0;
+#pragma unroll
+            for (int gg = 0; gg < G; gg++) r += (double)red[gg * BL + t];
+            out[(size_t)i * BL + t] = (XT)(s + r);
+        }
+        return;
+    }
+    // the row result becomes the source of one more block: gathered (and BF16-scaled) exactly as
+    // a separate launch would gather it, then streamed and reduced in the same order
+    const int bd = S.dptr[w];
+    double dsc = 1.0;
+    if constexpr (sizeof(T) == 2) dsc = (double)scale[bd];
     for (int t = tid; t < BL; t += NT) {
         double s = base64 ? base64[(size_t)i * BL + t] : base32 ? (double)base32[(size_t)i * BL + t] : 0.0;
         double r = 0.0;
 #pragma unroll
         for (int gg = 0; gg < G; gg++) r += (double)red[gg * BL + t];
-        out[(size_t)i * BL + t] = (XT)(s + r);
+        xs[t] = (A)((s + r) * dsc);
+    }
+    __syncthreads();
+    if (gl < GW) {
+        A acc[LPW][VW];
+#pragma unroll
+        for (int l = 0; l < LPW; l++)
+#pragma unroll
+            for (int v = 0; v < VW; v++) acc[l][v] = 0;
+        const T *pd = F + (size_t)bd * BB + rr;
+        for (int base = 0; base < BL; base += G) {
+#pragma unroll
+            for (int l = 0; l < LPW; l++) {
+                const int cg = base + g0 + GW * WPR * l;
+                if (cg < BL) {
+                    A a[VW];
+                    raw_convert<T, VW, A>(raw_load<T, VW>(pd + (size_t)cg * BL), a);
+                    const A xv = xs[cg];
+#pragma unroll
+                    for (int v = 0; v < VW; v++) acc[l][v] = fma(a[v], xv, acc[l][v]);
+                }
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < LPW; l++)
+#pragma unroll
+            for (int v = 0; v < VW; v++) red[(g0 + GW * WPR * l) * BL + rr + v] = acc[l][v];
+    }
+    __syncthreads();
+    for (int t = tid; t < BL; t += NT) {
+        double r = 0.0;
+#pragma unroll
+        for (int gg = 0; gg < G; gg++) r += (double)red[gg * BL + t];
+        out[(size_t)i * BL + t] = (XT)r;
     }
 }
 
@@ -318,6 +369,11 @@ inline void launch_rowstream_bl(const StreamArgs<XT> &a, const T *F, const float
     }
 }
 
+// block sizes with a compiled k_rowstream (the others run k_rowstream_any, which has no dptr)
+inline bool rowstream_compiled(int bl) {
+    return bl == 3 || bl == 6 || bl == 10 || bl == 12 || bl == 15 || bl == 24 || bl == 40 || bl == 60 || bl == 120;
+}
+
 // one sweep over a.items rows; wpr <= 0 takes the tuned value
 template <class T, class XT>
 inline void launch_rowstream(int bl, const StreamArgs<XT> &a, const T *F, const float *scale, hipStream_t s,
@@ -337,6 +393,7 @@ inline void launch_rowstream(int bl, const StreamArgs<XT> &a, const T *F, const 
     default: break;
     }
     CPG_CHECK(bl <= 256, CPG_ERR_UNSUPPORTED, "block size %d > 256", bl);
+    CPG_CHECK(!a.S.dptr, CPG_ERR_UNSUPPORTED, "block size %d: no fused D^-1 step", bl);
     const size_t lds = (size_t)a.nbmax * bl * sizeof(double);
     if (lds > 64 * 1024)
         CPG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_rowstream_any<T, XT>),
