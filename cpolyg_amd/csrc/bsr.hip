@@ -157,7 +157,7 @@ int cpg_bsr_create(int n_rows, int bl, int nnzb, const int *rowptr, const int *c
             A->rowptr.upload(A->h_rowptr);
             A->colind.upload(A->h_colind);
             A->blocks.alloc((size_t)nnzb * bl * bl);
-            CPG_HIP(hipMemset(A->blocks.p, 0, A->blocks.n * sizeof(double)));
+            if (A->blocks.n) CPG_HIP(hipMemset(A->blocks.p, 0, A->blocks.n * sizeof(double)));
         } catch (...) {
             delete A;
             throw;
@@ -214,7 +214,9 @@ int cpg_bsr_scale(cpg_bsr A, double s, void *stream) {
 
 int cpg_bsr_matvec(cpg_bsr A, const double *x, double *y, void *stream) {
     return guard([&] {
-        CPG_CHECK(A && x && y, CPG_ERR_ARG, "null argument");
+        CPG_CHECK(A, CPG_ERR_ARG, "null argument");
+        if (A->n == 0) return;   // zero-length vectors may be null
+        CPG_CHECK(x && y, CPG_ERR_ARG, "null argument");
         CPG_CHECK(x != y, CPG_ERR_ARG, "x and y alias");
         bsr_matvec(*A, x, y, S(stream));
     });

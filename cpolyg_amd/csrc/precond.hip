@@ -2036,8 +2036,9 @@ int cpg_prec_factor(cpg_prec P, cpg_bsr A, void *stream) {
 
 int cpg_prec_apply(cpg_prec P, double *x, void *stream) {
     return guard([&] {
-        CPG_CHECK(P && x, CPG_ERR_ARG, "null argument");
-        if (P->kind == CPG_PC_NONE || P->n == 0) return;
+        CPG_CHECK(P, CPG_ERR_ARG, "null argument");
+        if (P->kind == CPG_PC_NONE || P->n == 0) return;   // zero-length vectors may be null
+        CPG_CHECK(x, CPG_ERR_ARG, "null argument");
         hipStream_t s = S(stream);
         if (P->kind == CPG_PC_INNER_GMRES) {
             CPG_CHECK(P->A, CPG_ERR_ARG, "inner GMRES preconditioner applied before its first factor");

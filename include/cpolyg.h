@@ -62,7 +62,7 @@ int cpg_bsr_upload(cpg_bsr A, const double *host_blocks, void *stream);
 int cpg_bsr_download(cpg_bsr A, double *host_blocks, void *stream);   /* synchronous */
 int cpg_bsr_device_blocks(cpg_bsr A, double **dev_blocks);
 int cpg_bsr_scale(cpg_bsr A, double s, void *stream);
-/* y = A x; x and y are device vectors of n_rows*bl doubles and must not alias */
+/* y = A x; x and y are device vectors of n_rows*bl doubles and must not alias (null when n_rows = 0) */
 int cpg_bsr_matvec(cpg_bsr A, const double *x, double *y, void *stream);
 
 /* ---- launch shapes of the block-row streams ----------------------------------

@@ -256,3 +256,36 @@ def test_factor_is_deterministic(cuda, kind):
     if kind == PC_DILU:
         Fo, _ = O.dilu_factor(orc.rowptr, orc.colind, B)
         assert rel(F0, Fo) < 1e-10
+
+
+def test_empty_matrix_operations_are_no_ops(cuda):
+    """A BSR with no rows: matvec, DILU / block-Jacobi factor and apply return at once."""
+    import torch
+    A = hip.Bsr(np.zeros(1, np.int32), np.zeros(0, np.int32), 40)
+    x = torch.zeros(0, dtype=torch.float64, device=cuda)
+    y = torch.zeros(0, dtype=torch.float64, device=cuda)
+    A.matvec(x, y)
+    for kind in (PC_DILU, PC_BJACOBI):
+        P = hip.Prec(A, kind, PREC_BF16)
+        P.factor()
+        P.apply(x)
+
+
+def test_coupled_dilu_rejects_mismatched_inputs(cuda):
+    """The coupled factor checks its handles before any launch: a 40-wide preconditioner, or a
+    120-wide one over another pattern, is refused with a status."""
+    deg, bl = 3, 40
+    m = vortex_mesh(4)
+    orc = O.Oracle(m)
+    E = hip.Equation(hip.Mesh(m), EquationSpec.euler(), deg)
+    Js = [hip.Bsr(orc.rowptr, orc.colind, bl) for _ in range(3)]
+    P40 = hip.Prec(hip.Bsr(orc.rowptr, orc.colind, bl), PC_DILU, PREC_FP64)
+    with pytest.raises(hip.CpgError) as e:
+        P40.factor_coupled(E, Js, -0.1 * RADAU3)
+    assert e.value.status == 7
+    n = m.n_cells
+    diag_only = hip.Bsr(np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), 3 * bl)
+    Pd = hip.Prec(diag_only, PC_DILU, PREC_FP64)
+    with pytest.raises(hip.CpgError) as e:
+        Pd.factor_coupled(E, Js, -0.1 * RADAU3)
+    assert e.value.status == 1

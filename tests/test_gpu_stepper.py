@@ -179,3 +179,37 @@ def test_irk_w_form_matches_oracle(cuda, ns, prec, pc):
     gs = S.advance(ud, dt, 0.0, outd)
     assert gs.last_resid < 1e-12
     assert rel(outd.cpu().numpy(), uo) < 1e-9
+
+
+# ---- failure paths: each ends with a status, every kernel exits normally -------------------
+@pytest.mark.parametrize("method,ns,pc", [(0, 3, PC_DILU), (STEP_IRK_W, 3, PC_DILU), (1, 3, PC_BJACOBI)])
+def test_nonfinite_state_reports_status(cuda, method, ns, pc):
+    """A NaN in the state makes the first Newton residual non-finite: the step returns
+    CPG_ERR_NONFINITE before any linear solve."""
+    m = vortex_mesh(4)
+    deg = 1
+    spec = EquationSpec.euler(periodic_trace=1, jacobian=1)
+    u = vortex_state(m, deg)
+    u[7] = np.nan
+    solver = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=20, rtol=1e-10, maxit=200, pc_kind=pc,
+                        newton_tol=1e-12, newton_maxit=5, newton_operator=OP_MATRIX_FREE)
+    S = hip.Stepper(hip.Equation(hip.Mesh(m), spec, deg), method, ns, solver)
+    with pytest.raises(hip.CpgError) as e:
+        S.advance(_dev(cuda, u), 0.1, 0.0, _dev(cuda, np.zeros_like(u)))
+    assert e.value.status == 5
+
+
+def test_newton_iteration_limit_reports_status(cuda):
+    """One Newton iteration against a 1e-30 tolerance: CPG_ERR_NOT_CONVERGED, with the
+    iterations done and the last residual in the message."""
+    m = vortex_mesh(4)
+    deg = 2
+    spec = EquationSpec.euler(periodic_trace=1, jacobian=1)
+    u = vortex_state(m, deg)
+    solver = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=20, rtol=1e-10, maxit=400, pc_kind=PC_DILU,
+                        newton_tol=1e-30, newton_maxit=1, newton_operator=OP_MATRIX_FREE)
+    S = hip.Stepper(hip.Equation(hip.Mesh(m), spec, deg), 0, 3, solver)
+    with pytest.raises(hip.CpgError) as e:
+        S.advance(_dev(cuda, u), 0.2, 0.0, _dev(cuda, np.zeros_like(u)))
+    assert e.value.status == 6
+    assert "Newton" in str(e.value)
