@@ -61,9 +61,10 @@ def parse():
                    help="GMRES operator: the Newton matrix applied at the quadrature points, or the BSR matvec of "
                         "the assembled matrix")
     p.add_argument("--trace", type=int, default=1, help="periodic trace: 1 exact, 0 reference (truncated)")
-    p.add_argument("--pc-refresh", type=int, default=2, choices=[0, 1, 2],
+    p.add_argument("--pc-refresh", type=int, default=2, choices=[0, 1, 2, 3],
                    help="0: Newton matrix and preconditioner at every Newton iteration (reference); 1: once per stage; "
-                        "2: once per step (matrix-free operator only)")
+                        "2: once per step; 3: kept across steps, renewed when GMRES slows (matrix-free operator "
+                        "only)")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-compare", dest="compare_operators", action="store_false",
                    help="skip the runs under the reference refresh policy (matrix-free and assembled operator)")
@@ -544,7 +545,7 @@ def main():
         "config": {"workload": f"Euler isentropic vortex, periodic Voronoi {n} cells ({args.nx}^2 jittered "
                                f"lattice), p={deg} ({bl}x{bl} blocks), DIRK3 CFL {args.cfl:g}, FGMRES({args.restart}) "
                                f"rtol 1e-10, block DILU, {args.operator} Newton operator, preconditioner refreshed "
-                               f"{['every Newton iteration', 'once per stage', 'once per step'][args.pc_refresh]}",
+                               f"{['every Newton iteration', 'once per stage', 'once per step', 'kept across steps'][args.pc_refresh]}",
                    "cells": n, "deg": deg, "pc_refresh": args.pc_refresh,
                    "block": bl, "nnzb": nnzb, "dt": dt, "cfl": args.cfl, "mixed": args.mixed,
                    "operator": args.operator, "jacobian": "exact" if args.jacobian else "reference",

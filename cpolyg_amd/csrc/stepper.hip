@@ -338,7 +338,12 @@ struct cpg_stepper_s {
     DevBuf<double> irk_data, pin, pout, wbuf;
     cpg::IrkOp irkop{};
     cpg::IrkPc irkpc{};
-    bool factored = false;     // the preconditioner holds a factor of this step (pc_refresh 2)
+    bool factored = false;     // the preconditioner holds a factor of this step (pc_refresh 2, 3)
+    // pc_refresh 3: the factor is kept across steps and renewed when a step's GMRES iteration
+    // count exceeds 1.5 x (+2) that of the first step after the last renewal, or dt changes
+    double dt_factored = 0.0;
+    int its_ref = 0, last_its = 0;
+    bool refreshed = false;    // a factor was formed during the current step
     cpg::NewtonOp nop{};
     // CPOLYG_VERBOSE=1 in the environment at creation: one stderr line per Newton iteration,
     // the progress the reference prints (TimeIntegration.cpp:209-264)
@@ -392,6 +397,30 @@ void check_ctx(int st, int stage, int it, double resid) {
     throw Error{st};
 }
 
+// start of a step: whether the preconditioner's factor is kept (pc_refresh 3) or formed anew
+void begin_step(cpg_stepper_s &S, double dt) {
+    S.refreshed = false;
+    if (S.sp.pc_refresh != 3) {
+        S.factored = false;
+        return;
+    }
+    if (!S.factored || dt != S.dt_factored || (S.its_ref > 0 && S.last_its > 1.5 * S.its_ref + 2))
+        S.factored = false;
+}
+
+// a factor was formed in this step
+void mark_factored(cpg_stepper_s &S, double dt) {
+    S.factored = true;
+    S.refreshed = true;
+    S.dt_factored = dt;
+}
+
+// end of a converged step: its GMRES iterations, the reference count after a renewal
+void end_step(cpg_stepper_s &S, int gmres_its) {
+    S.last_its = gmres_its;
+    if (S.refreshed) S.its_ref = gmres_its;
+}
+
 }  // namespace
 
 extern "C" {
@@ -432,7 +461,7 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
                 check(cpg_krylov_create(n, bl, params->restart, params->gmres_mode, &S->K));
                 CPG_CHECK(params->newton_operator == 0 || params->newton_operator == 1, CPG_ERR_ARG,
                           "bad newton_operator %d", params->newton_operator);
-                CPG_CHECK(params->pc_refresh >= 0 && params->pc_refresh <= 2, CPG_ERR_ARG, "bad pc_refresh %d",
+                CPG_CHECK(params->pc_refresh >= 0 && params->pc_refresh <= 3, CPG_ERR_ARG, "bad pc_refresh %d",
                           params->pc_refresh);
                 S->opscale.alloc(1);
                 S->nop = cpg::NewtonOp{E, S->opscale.p};
@@ -444,7 +473,7 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
                           "the W-form IRK step applies its matrix matrix-free: use flexible GMRES");
                 CPG_CHECK(params->pc_kind == CPG_PC_BJACOBI || params->pc_kind == CPG_PC_DILU, CPG_ERR_UNSUPPORTED,
                           "the W-form IRK step takes block Jacobi or block DILU per stage");
-                CPG_CHECK(params->pc_refresh >= 0 && params->pc_refresh <= 2, CPG_ERR_ARG, "bad pc_refresh %d",
+                CPG_CHECK(params->pc_refresh >= 0 && params->pc_refresh <= 3, CPG_ERR_ARG, "bad pc_refresh %d",
                           params->pc_refresh);
                 const int ns = nstages;
                 // A^-1 by Gauss-Jordan (ns <= 3), and b^T A^-1
@@ -498,7 +527,7 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
                           "the matrix-free IRK operator uses flexible GMRES");
                 CPG_CHECK(params->pc_kind == CPG_PC_BJACOBI || params->pc_kind == CPG_PC_DILU, CPG_ERR_UNSUPPORTED,
                           "the matrix-free IRK step takes the coupled block-Jacobi or block DILU preconditioner");
-                CPG_CHECK(params->pc_refresh >= 0 && params->pc_refresh <= 2, CPG_ERR_ARG, "bad pc_refresh %d",
+                CPG_CHECK(params->pc_refresh >= 0 && params->pc_refresh <= 3, CPG_ERR_ARG, "bad pc_refresh %d",
                           params->pc_refresh);
                 const int ns = nstages;
                 check(cpg_jacobian_create(E, &S->J));
@@ -704,7 +733,9 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
             hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, S->opscale.p, -dt);
             CPG_LAUNCH_CHECK();
             const double one[1] = {1.0};
-            bool converged = false, factored = false;
+            begin_step(*S, dt);
+            bool converged = false;
+            bool &factored = S->factored;
             for (int it = 0; it < S->sp.newton_maxit; it++) {
                 auto t0 = clk::now();
                 double rn2 = 0.0;
@@ -739,7 +770,7 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                         check(cpg_prec_factor(S->Pw[stg], S->J, s));
                         st.ms_factor += ms_since(t0);
                     }
-                    factored = true;
+                    mark_factored(*S, dt);
                 }
                 t0 = clk::now();
                 for (int stg = 0; stg < ns; stg++) {
@@ -770,6 +801,7 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                           S->sp.newton_maxit, st.last_resid);
                 throw Error{CPG_ERR_NOT_CONVERGED};
             }
+            end_step(*S, st.gmres_iters);
             return;
         }
         if (S->method == CPG_STEP_IRK && S->irk_mf) {
@@ -785,7 +817,9 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
             CPG_HIP(hipMemsetAsync(S->bigK.p, 0, NB * sizeof(double), s));
             hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, S->opscale.p, -dt);
             CPG_LAUNCH_CHECK();
-            bool converged = false, factored = false;
+            begin_step(*S, dt);
+            bool converged = false;
+            bool &factored = S->factored;
             for (int it = 0; it < S->sp.newton_maxit; it++) {
                 auto t0 = clk::now();
                 double rn2 = 0.0;
@@ -821,7 +855,7 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                     cb.nnzb = S->J->nnzb;
                     check(guard([&] { prec_factor_coupled(S->P, cb, s); }));
                     st.ms_factor += ms_since(t0);
-                    factored = true;
+                    mark_factored(*S, dt);
                 } else if (S->sp.pc_refresh == 0 || !factored) {
                     t0 = clk::now();
                     int nnzb = 0;
@@ -842,7 +876,7 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                     t0 = clk::now();
                     check(cpg_prec_factor(S->P, S->Dc, s));
                     st.ms_factor += ms_since(t0);
-                    factored = true;
+                    mark_factored(*S, dt);
                 }
                 t0 = clk::now();
                 for (int stg = 0; stg < ns; stg++) {
@@ -873,6 +907,7 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                           S->sp.newton_maxit, st.last_resid);
                 throw Error{CPG_ERR_NOT_CONVERGED};
             }
+            end_step(*S, st.gmres_iters);
             return;
         }
         if (S->method == CPG_STEP_IRK) {
@@ -937,11 +972,12 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                           S->sp.newton_maxit, st.last_resid);
                 throw Error{CPG_ERR_NOT_CONVERGED};
             }
+            end_step(*S, st.gmres_iters);
             return;
         }
         const int ns = S->ns;
         bool all_converged = true;
-        S->factored = false;
+        begin_step(*S, dt);
         CPG_HIP(hipMemsetAsync(k, 0, N * ns * sizeof(double), s));
         for (int stg = 0; stg < ns; stg++) {
             double *ks = k + N * stg;
@@ -973,7 +1009,8 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                 bool refresh = true;
                 if (S->sp.newton_operator == 1 && S->sp.pc_refresh == 1) refresh = it == 0;
                 if (S->sp.newton_operator == 1 && S->sp.pc_refresh == 2) refresh = it == 0 && (stg == 0 || !S->factored);
-                if (refresh) S->factored = true;
+                if (S->sp.newton_operator == 1 && S->sp.pc_refresh == 3) refresh = !S->factored;
+                if (refresh) mark_factored(*S, dt);
                 if (refresh) {
                     t0 = clk::now();
                     eq_jacobian(*S->E, S->cu.p, ts, -dt * S->A[stg * ns + stg], 1, S->J->blocks.p, s);
@@ -1018,6 +1055,7 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                       st.last_resid);
             throw Error{CPG_ERR_NOT_CONVERGED};
         }
+        end_step(*S, st.gmres_iters);
     });
     st.status = ret;
     if (stats) *stats = st;

@@ -238,8 +238,11 @@ typedef struct {
     int pc_refresh;         /* DIRK/backward Euler: 0 assemble the Newton matrix and factor the preconditioner
                                at every Newton iteration (reference, src/TimeIntegration.cpp:127-136); with
                                newton_operator = 1 only: 1 at the first Newton iteration of each stage, 2 at
-                               the first Newton iteration of the step (the Newton iteration itself is
-                               unchanged: only the preconditioner lags) */
+                               the first Newton iteration of the step, 3 keep the factor across steps and
+                               renew it when a step needs more than 1.5 x (+2) the GMRES iterations of the
+                               first step after the last renewal, or when dt changes (the Newton iteration
+                               itself is unchanged: only the preconditioner lags).  Radau IIA: 0 every
+                               Newton iteration, 1 and 2 once per step, 3 as for DIRK */
 } cpg_solver_params;
 
 typedef struct {

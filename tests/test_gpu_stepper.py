@@ -213,3 +213,32 @@ def test_newton_iteration_limit_reports_status(cuda):
         S.advance(_dev(cuda, u), 0.2, 0.0, _dev(cuda, np.zeros_like(u)))
     assert e.value.status == 6
     assert "Newton" in str(e.value)
+
+
+@pytest.mark.parametrize("method,ns,pc,prec", [(0, 3, PC_DILU, PREC_BF16), (1, 3, PC_BJACOBI, PREC_BF16),
+                                               (1, 3, PC_DILU, PREC_FP32), (STEP_IRK_W, 2, PC_DILU, PREC_FP64)])
+def test_factor_kept_across_steps(cuda, method, ns, pc, prec):
+    """pc_refresh 3 keeps the preconditioner's factor across steps (renewed when GMRES slows
+    down or dt changes): three steps, the third with another dt, match three oracle steps."""
+    m = vortex_mesh(6)
+    deg = 3
+    spec = EquationSpec.euler(periodic_trace=1, jacobian=1)
+    u = vortex_state(m, deg)
+    ref = SolverSpec(gmres_mode=GMRES_LEFT_REF, restart=20, rtol=1e-11, maxit=4000, pc_kind=PC_DILU,
+                     newton_tol=1e-12, newton_maxit=30)
+    orc = O.Oracle(m)
+    solver = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=40, rtol=1e-11, maxit=20000, pc_kind=pc,
+                        newton_tol=1e-12, newton_maxit=30, precision=prec, newton_operator=OP_MATRIX_FREE,
+                        pc_refresh=3, newton_guess=1 if method == 0 else 0)
+    S = hip.Stepper(hip.Equation(hip.Mesh(m), spec, deg), method, ns, solver)
+    uo, ud, t = u.copy(), _dev(cuda, u), 0.0
+    for dt in (0.3, 0.3, 0.2):
+        if method == 0:
+            uo, _ = orc.dirk_step(spec, deg, ns, uo, dt, t, ref)
+        else:
+            uo, _ = orc.irk_step(spec, deg, ns, uo, dt, t, ref)
+        out = _dev(cuda, np.zeros_like(u))
+        gs = S.advance(ud, dt, t, out)
+        assert gs.last_resid < 1e-12
+        ud, t = out, t + dt
+        assert rel(ud.cpu().numpy(), uo) < 1e-9
