@@ -38,9 +38,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per BSR-matvec launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
 # kernel (bench.py --workload pmc), FETCH scaled x2 and WRITE x1 by a 1 GiB copy of known size;
 # keyed by (cells, degree).  profiles/bsr_matvec_euler_p3_65k_pmc_traffic.txt has the passes.
-MATVEC_TRAFFIC = {(65536, 3): 6.153e9}
-# the same for one DILU apply (all 9 launches), keyed by (cells, degree, factor precision)
-APPLY_TRAFFIC = {(65536, 3, "bf16"): 1.725e9}
+MATVEC_TRAFFIC = {(65536, 3): 6.152e9}
+# the same for one DILU apply (all 8 launches), keyed by (cells, degree, factor precision)
+APPLY_TRAFFIC = {(65536, 3, "bf16"): 1.715e9}
 
 
 def parse():
