@@ -2,10 +2,11 @@
 //
 // Reference: BlockILU0 (src/Preconditioners.cpp:49-158) is block DILU:
 //   D_j = A_jj - sum_{i<j} A_ji D_i^-1 A_ij,  L_ji = A_ji D_i^-1,  DU_ij = D_i^-1 A_ij,
-// applied as x <- (I + DU)^-1 D^-1 (I + L)^-1 x.  Rows are processed in levels:
-// a row's forward level is one more than the deepest lower neighbour's, its backward
-// level one more than the deepest upper neighbour's.  For a colour-ordered matrix
-// (every neighbour pair i<j has colour(i) < colour(j)) the levels are the colours.
+// applied as x <- (I + DU)^-1 D^-1 (I + L)^-1 x.  Rows are processed in levels.  For a
+// colour-ordered matrix (every neighbour pair i<j has colour(i) < colour(j)) the levels are
+// contiguous row ranges, one per colour (see segments()), and the backward sweep takes them
+// from the last; otherwise a row's forward level is one more than the deepest lower
+// neighbour's and its backward level one more than the deepest upper neighbour's.
 //
 // Factor (left-looking, one workgroup per row of a forward level):
 //   AD = A_ii - sum_k L_ik A_ki with L_ik = A_ik D_k^-1 (k lower neighbour),
@@ -17,7 +18,8 @@
 // the forward stream holds, level by level and row by row, the L blocks of each row; the
 // backward stream, level by level (from the last), each row's D^-1 followed by its DU
 // blocks.  pos[slot] maps a slot of A to its packed block.
-// Apply (row-stream launches, one per level and direction):
+// Apply (row-stream launches, one per level and direction; forward level 0 is y = x and the
+// last forward level also forms x = D^-1 y for its rows, see apply_t):
 //   forward  y_i = x_i - sum_k L_ik y_k,
 //   backward x_i = D_i^-1 y_i - sum_j DU_ij x_j,
 // which is (I + DU)^-1 D^-1 (I + L)^-1 x with the D^-1 product moved into the backward
