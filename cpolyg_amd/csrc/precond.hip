@@ -760,20 +760,47 @@ struct FT {
     static constexpr int V = BL % 2 == 0 ? 2 : 1;          // doubles per load
 };
 
+// use(e, load(e)) for every e < N of this thread (e = threadIdx.x + FWG j, all threads of the
+// workgroup), the loads issued B at a time ahead of their uses so that B global loads per thread
+// are in flight together (a loop that loads and then uses one value waits out each load's
+// latency in turn); the rounds stay a loop so that no more than B loaded values are live; loads
+// past the end of the last round read index N - 1 and are not used
+template <int N, int B = 8, class Load, class Use>
+__device__ inline void wg_batched(Load &&load, Use &&use) {
+    constexpr int IT = (N + FWG - 1) / FWG;
+    constexpr int BB = IT < B ? IT : B;
+    using V = decltype(load(0));
+    const int t = threadIdx.x;
+#pragma unroll 1
+    for (int j0 = 0; j0 < IT; j0 += BB) {
+        V v[BB];
+#pragma unroll
+        for (int j = 0; j < BB; j++) v[j] = load(min(t + FWG * (j0 + j), N - 1));
+#pragma unroll
+        for (int j = 0; j < BB; j++) {
+            const int e = t + FWG * (j0 + j);
+            if (e < N) use(e, v[j]);
+        }
+    }
+}
+
 // X (column stride LD) <- the bl x bl block at g, all threads of the workgroup
 template <int BL>
 __device__ inline void wg_block_to_lds(double *X, const double *__restrict__ g) {
     using W = WF<BL>;
-    constexpr int V = FT<BL>::V;
-    for (int e = threadIdx.x * V; e < W::BB; e += FWG * V) {
-        const int c = e / BL, r = e - c * BL;
-        if constexpr (V == 2) {
-            const double2 a = *reinterpret_cast<const double2 *>(g + e);
-            X[c * W::LD + r] = a.x;
-            X[c * W::LD + r + 1] = a.y;
-        } else {
-            X[c * W::LD + r] = g[e];
-        }
+    if constexpr (FT<BL>::V == 2) {
+        wg_batched<W::BB / 2>([&](int u) { return *reinterpret_cast<const double2 *>(g + 2 * u); },
+                              [&](int u, double2 a) {
+                                  const int c = (2 * u) / BL, r = 2 * u - c * BL;
+                                  X[c * W::LD + r] = a.x;
+                                  X[c * W::LD + r + 1] = a.y;
+                              });
+    } else {
+        wg_batched<W::BB>([&](int e) { return g[e]; },
+                          [&](int e, double a) {
+                              const int c = e / BL, r = e - c * BL;
+                              X[c * W::LD + r] = a;
+                          });
     }
 }
 
@@ -1225,9 +1252,12 @@ __global__ __launch_bounds__(IWG) void k_f32_combine(const float *__restrict__ Q
 // v_mfma_f64_16x16x4f64 (8 x 8 tiles of 16 over the padded 128 x 128 block; wavefront w of the
 // workgroup owns tile columns w and w + 4, 16 tiles):
 //   Schur (one workgroup per row i of a level): S_i = C_ii in DD64[i], then per lower neighbour
-//     k: X <- D_k^-1, T = C_ik X (registers), X <- T (stored as L_ik), S_i -= X C_ki;
+//     k: X <- (coef x I) D_k^-1, T = BD(J_ik) X (registers), X <- T (stored as L_ik),
+//     Y = X BD(J_ki), S_i -= Y (coef x I) element by element;
 //   inverse: D_i^-1 = S_i^-1 by the register-tiled Gauss-Jordan, to DD64[i] and the factors;
-//   upper (after all levels): DU_ij = D_i^-1 C_ij.
+//   upper (after all levels): DU_ij = (D_i^-1 BD(J_ij)) (coef x I).
+// The block-diagonal products take 180 (BD X) and at most 320 (X BD) MFMAs per wavefront
+// against 480 for a dense 120 x 120 product.
 // LDS: X (120 x 121 doubles) and the three stage blocks of one slot (3 x 40 x 41): 152 KB.
 struct CoupledSrc {
     const double *J0, *J1, *J2;
@@ -1251,33 +1281,47 @@ __device__ inline double coupled_coef(const CoupledSrc &src, int s, int t) {
 // Jl <- the three stage blocks of slot q (column stride CLD), all threads
 __device__ inline void coupled_stage(double *Jl, const CoupledSrc &src, int q) {
     constexpr int SB = CSB * CSB;
-    for (int e = threadIdx.x; e < 3 * SB; e += FWG) {
-        const int st = e / SB, rem = e - st * SB, c = rem / CSB, r = rem - c * CSB;
-        const double *J = st == 0 ? src.J0 : st == 1 ? src.J1 : src.J2;
-        Jl[st * CSTAGE + c * CLD + r] = J[(size_t)q * SB + rem];
+    wg_batched<3 * SB / 2>(
+        [&](int u) {
+            const int st = (2 * u) / SB, rem = 2 * u - st * SB;
+            const double *J = st == 0 ? src.J0 : st == 1 ? src.J1 : src.J2;
+            return *reinterpret_cast<const double2 *>(J + (size_t)q * SB + rem);
+        },
+        [&](int u, double2 a) {
+            const int st = (2 * u) / SB, rem = 2 * u - st * SB, c = rem / CSB, r = rem - c * CSB;
+            Jl[st * CSTAGE + c * CLD + r] = a.x;
+            Jl[st * CSTAGE + c * CLD + r + 1] = a.y;
+        });
+}
+
+// The coupled block factors as C_q = BD(J_q) (coef x I): BD the block diagonal of the three
+// stage blocks (row stripe s of C_q holds J_s), (coef x I) the 3 x 3 stage mixing.  Products with
+// BD need only the k-steps of a tile's own stage block(s): 8 x 8 output tiles of 16 over the
+// padded 128 x 128 block, wavefront w owns tile columns w and w + 4; tile row or column index j
+// covers rows 16 j .. 16 j + 15, i.e. stages (16 j) / 40 .. min((16 j + 15) / 40, 2).
+
+// X <- (coef x I) X in place: rows (s, r) of every column mixed over s
+__device__ inline void coupled_mix_rows(double *X, const CoupledSrc &src) {
+    for (int e = threadIdx.x; e < CSB * CBL; e += FWG) {
+        const int n = e / CSB, r = e - n * CSB;
+        double *x = X + (size_t)n * CXL + r;
+        const double x0 = x[0], x1 = x[CSB], x2 = x[2 * CSB];
+#pragma unroll
+        for (int st = 0; st < 3; st++)
+            x[st * CSB] = fma(src.coef[3 * st], x0, fma(src.coef[3 * st + 1], x1, src.coef[3 * st + 2] * x2));
     }
 }
 
-// acc[2 tm + h] = tile (tm, w + 4 h) of C_q X (C_q from the staged Jl, X in LDS, stride CXL)
-__device__ inline void coupled_times_x(const double *Jl, const double *X, const CoupledSrc &src, dx4 (&acc)[16]) {
+// acc[2 tm + h] = tile (tm, w + 4 h) of BD(J) X (J from the staged Jl, X in LDS, stride CXL)
+__device__ inline void bd_times_x(const double *Jl, const double *X, dx4 (&acc)[16]) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int lr = lane & 15, lk = lane >> 4;
 #pragma unroll
     for (int j = 0; j < 16; j++) acc[j] = dx4{0.0, 0.0, 0.0, 0.0};
-    int sm[CTM], jb[CTM];   // stage and staged-row offset of this lane's row in tile row tm
-#pragma unroll
-    for (int tm = 0; tm < CTM; tm++) {
-        const int m = 16 * tm + lr;
-        sm[tm] = m < CBL ? m / CSB : 3;
-        jb[tm] = m < CBL ? sm[tm] * CSTAGE + (m - sm[tm] * CSB) : 0;
-    }
     const int n0 = 16 * wv + lr, n1 = 16 * (wv + 4) + lr;
     const bool v1 = n1 < CBL;
 #pragma unroll
     for (int t = 0; t < 3; t++) {
-        double cf[CTM];
-#pragma unroll
-        for (int tm = 0; tm < CTM; tm++) cf[tm] = sm[tm] < 3 ? coupled_coef(src, sm[tm], t) : 0.0;
 #pragma unroll
         for (int kk = 0; kk < CSB / 4; kk++) {
             const int c = 4 * kk + lk, k = CSB * t + c;
@@ -1285,7 +1329,10 @@ __device__ inline void coupled_times_x(const double *Jl, const double *X, const 
             const double b1 = v1 ? X[n1 * CXL + k] : 0.0;
 #pragma unroll
             for (int tm = 0; tm < CTM; tm++) {
-                const double a = cf[tm] * Jl[jb[tm] + c * CLD];
+                // tile rows of stage t only (compile-time per tm, t)
+                if ((16 * tm) / CSB > t || (16 * tm + 15) / CSB < t) continue;
+                const int m = 16 * tm + lr;
+                const double a = (m < CBL && m / CSB == t) ? Jl[t * CSTAGE + c * CLD + (m - t * CSB)] : 0.0;
                 acc[2 * tm] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[2 * tm], 0, 0, 0);
                 acc[2 * tm + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[2 * tm + 1], 0, 0, 0);
             }
@@ -1293,44 +1340,51 @@ __device__ inline void coupled_times_x(const double *Jl, const double *X, const 
     }
 }
 
-// acc[2 tm + h] += sgn * tile (tm, w + 4 h) of X C_q
-__device__ inline void x_times_coupled(const double *X, const double *Jl, const CoupledSrc &src, double sgn,
-                                       dx4 (&acc)[16]) {
+// stage s's k-steps of X BD(J) for the tile columns enabled by H0, H1
+template <bool H0, bool H1>
+__device__ inline void x_bd_stage(const double *X, const double *Jl, int s, int c0, int t0, int c1, int t1, int lr,
+                                  int lk, dx4 (&acc)[16]) {
+#pragma unroll
+    for (int kk = 0; kk < CSB / 4; kk++) {
+        const int r = 4 * kk + lk, k = CSB * s + r;
+        const double b0 = H0 && t0 == s ? Jl[s * CSTAGE + c0 * CLD + r] : 0.0;
+        const double b1 = H1 && t1 == s ? Jl[s * CSTAGE + c1 * CLD + r] : 0.0;
+#pragma unroll
+        for (int tm = 0; tm < CTM; tm++) {
+            const int m = 16 * tm + lr;
+            const double a = m < CBL ? X[k * CXL + m] : 0.0;
+            if (H0) acc[2 * tm] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[2 * tm], 0, 0, 0);
+            if (H1) acc[2 * tm + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[2 * tm + 1], 0, 0, 0);
+        }
+    }
+}
+
+// acc[2 tm + h] = tile (tm, w + 4 h) of X BD(J)
+__device__ inline void x_times_bd(const double *X, const double *Jl, dx4 (&acc)[16]) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < 16; j++) acc[j] = dx4{0.0, 0.0, 0.0, 0.0};
     const int n0 = 16 * wv + lr, n1 = 16 * (wv + 4) + lr;
     const bool v1 = n1 < CBL;
     const int t0 = n0 / CSB, c0 = n0 - t0 * CSB;
-    const int t1 = v1 ? n1 / CSB : 0, c1 = v1 ? n1 - t1 * CSB : 0;
-#pragma unroll
+    const int t1 = v1 ? n1 / CSB : 3, c1 = v1 ? n1 - t1 * CSB : 0;
     for (int s = 0; s < 3; s++) {
-        const double cf0 = sgn * coupled_coef(src, s, t0);
-        const double cf1 = v1 ? sgn * coupled_coef(src, s, t1) : 0.0;
-#pragma unroll
-        for (int kk = 0; kk < CSB / 4; kk++) {
-            const int r = 4 * kk + lk, k = CSB * s + r;
-            const double b0 = cf0 * Jl[s * CSTAGE + c0 * CLD + r];
-            const double b1 = cf1 * Jl[s * CSTAGE + c1 * CLD + r];
-#pragma unroll
-            for (int tm = 0; tm < CTM; tm++) {
-                const int m = 16 * tm + lr;
-                const double a = m < CBL ? X[k * CXL + m] : 0.0;
-                acc[2 * tm] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[2 * tm], 0, 0, 0);
-                acc[2 * tm + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[2 * tm + 1], 0, 0, 0);
-            }
-        }
+        // stages of this wavefront's two tile columns (uniform per wavefront)
+        const bool h0 = (16 * wv) / CSB <= s && s <= (16 * wv + 15) / CSB;
+        const bool h1 = (16 * (wv + 4)) / CSB <= s && s <= (16 * (wv + 4) + 15) / CSB;
+        if (h0 && h1) x_bd_stage<true, true>(X, Jl, s, c0, t0, c1, t1, lr, lk, acc);
+        else if (h0) x_bd_stage<true, false>(X, Jl, s, c0, t0, c1, t1, lr, lk, acc);
+        else if (h1) x_bd_stage<false, true>(X, Jl, s, c0, t0, c1, t1, lr, lk, acc);
     }
 }
 
-__device__ inline void tile_store_global(double *g, int tm, int tn, const dx4 &t) {
-    const int l = threadIdx.x & 63;
-    const int r0 = l >> 4, c0 = l & 15;
-    const int col = 16 * tn + c0;
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-        const int row = 16 * tm + r0 + 4 * e;
-        if (row < CBL && col < CBL) g[(size_t)col * CBL + row] = t[e];
-    }
+// element e (column-major) of Y (coef x I) for Y in LDS (stride CXL)
+__device__ inline double coupled_mix_col(const double *Y, const CoupledSrc &src, int e) {
+    const int col = e / CBL, m = e - col * CBL, t = col / CSB, c = col - t * CSB;
+    const double *y = Y + (size_t)c * CXL + m;
+    return fma(coupled_coef(src, 0, t), y[0],
+               fma(coupled_coef(src, 1, t), y[(size_t)CSB * CXL], coupled_coef(src, 2, t) * y[(size_t)2 * CSB * CXL]));
 }
 
 template <class T>
@@ -1346,33 +1400,41 @@ __global__ __launch_bounds__(FWG) void k_cdilu_schur(RowLists L, CoupledSrc src,
     const int bs = src.bs;
     double *S = DD64 + (size_t)i * CBB;
     // S_i = C_ii, mass term included
-    for (int e = threadIdx.x; e < (int)CBB; e += FWG) {
-        const int col = e / CBL, m = e - col * CBL;
-        const int s = m / CSB, r = m - s * CSB, t = col / CSB, c = col - t * CSB;
-        const double *J = s == 0 ? src.J0 : s == 1 ? src.J1 : src.J2;
-        double v = coupled_coef(src, s, t) * J[(size_t)d * CSB * CSB + c * CSB + r];
-        if (s == t && r / bs == c / bs) v += src.mass[(size_t)i * bs * bs + (c % bs) * bs + (r % bs)];
-        S[e] = v;
-    }
+    wg_batched<(int)CBB>(
+        [&](int e) {
+            const int col = e / CBL, m = e - col * CBL;
+            const int st = m / CSB, r = m - st * CSB, c = col % CSB;
+            const double *J = st == 0 ? src.J0 : st == 1 ? src.J1 : src.J2;
+            return double2{J[(size_t)d * CSB * CSB + c * CSB + r], src.mass[(size_t)i * bs * bs + (c % bs) * bs + (r % bs)]};
+        },
+        [&](int e, double2 a) {
+            const int col = e / CBL, m = e - col * CBL;
+            const int st = m / CSB, r = m - st * CSB, t = col / CSB, c = col - t * CSB;
+            S[e] = fma(coupled_coef(src, st, t), a.x, st == t && r / bs == c / bs ? a.y : 0.0);
+        });
     for (int q = L.lptr[i]; q < L.lptr[i + 1]; q++) {
         const int sl = L.lslot[q], k = L.lcol[q], st = L.ltrans[q];
         __syncthreads();   // X and Jl free; S_i visible to every wavefront
         coupled_stage(Jl, src, sl);
         wg_block_to_lds<CBL>(X, DD64 + (size_t)k * CBB);
         __syncthreads();
+        coupled_mix_rows(X, src);   // W = (coef x I) D_k^-1
+        __syncthreads();
         dx4 acc[16];
-        coupled_times_x(Jl, X, src, acc);   // L_ik = C_ik D_k^-1
+        bd_times_x(Jl, X, acc);     // L_ik = C_ik D_k^-1 = BD(J_ik) W
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < 16; j++) tile_to_lds<CBL>(X, j >> 1, wv + 4 * (j & 1), acc[j]);
         coupled_stage(Jl, src, st);
         __syncthreads();
         wg_lds_block_store<T, CBL>(X, F + (size_t)L.pos[sl] * CBB, scale ? scale + L.pos[sl] : nullptr, red);
+        x_times_bd(X, Jl, acc);     // Y = L_ik BD(J_ki)
+        __syncthreads();
 #pragma unroll
-        for (int j = 0; j < 16; j++) tile_load<CBL>(acc[j], j >> 1, wv + 4 * (j & 1), S);
-        x_times_coupled(X, Jl, src, -1.0, acc);   // S_i -= L_ik C_ki
-#pragma unroll
-        for (int j = 0; j < 16; j++) tile_store_global(S, j >> 1, wv + 4 * (j & 1), acc[j]);
+        for (int j = 0; j < 16; j++) tile_to_lds<CBL>(X, j >> 1, wv + 4 * (j & 1), acc[j]);
+        __syncthreads();
+        wg_batched<(int)CBB>([&](int e) { return S[e]; },   // S_i -= Y (coef x I)
+                             [&](int e, double v) { S[e] = v - coupled_mix_col(X, src, e); });
     }
 }
 
@@ -1397,24 +1459,24 @@ __global__ __launch_bounds__(FWG) void k_cdilu_upper(const int *__restrict__ uro
     __shared__ double red[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = urows[blockIdx.x];
-    wg_block_to_lds<CBL>(X, DD64 + (size_t)i * CBB);
     for (int q = uptr[i]; q < uptr[i + 1]; q++) {
         const int sl = uslot[q];
-        __syncthreads();   // Jl and red free; X staged
+        __syncthreads();   // X, Jl and red free
+        wg_block_to_lds<CBL>(X, DD64 + (size_t)i * CBB);
         coupled_stage(Jl, src, sl);
         __syncthreads();
         dx4 acc[16];
+        x_times_bd(X, Jl, acc);   // Y = D_i^-1 BD(J_ij)
+        __syncthreads();
 #pragma unroll
-        for (int j = 0; j < 16; j++) acc[j] = dx4{0.0, 0.0, 0.0, 0.0};
-        x_times_coupled(X, Jl, src, 1.0, acc);   // DU_ij = D_i^-1 C_ij
+        for (int j = 0; j < 16; j++) tile_to_lds<CBL>(X, j >> 1, wv + 4 * (j & 1), acc[j]);
+        __syncthreads();
+        // DU_ij = Y (coef x I), in precision T (BF16: one power-of-two scale for the block)
         float inv = 1.0f;
         T *g = F + (size_t)pos[sl] * CBB;
         if constexpr (Store<T>::prec == CPG_BF16) {
             double mx = 0.0;
-#pragma unroll
-            for (int j = 0; j < 16; j++)
-#pragma unroll
-                for (int e = 0; e < 4; e++) mx = fmax(mx, fabs(acc[j][e]));   // padding entries are 0
+            for (int e = threadIdx.x; e < (int)CBB; e += FWG) mx = fmax(mx, fabs(coupled_mix_col(X, src, e)));
             mx = wave_max(mx);
             if (lane == 0) red[wv] = mx;
             __syncthreads();
@@ -1423,16 +1485,7 @@ __global__ __launch_bounds__(FWG) void k_cdilu_upper(const int *__restrict__ uro
             inv = 1.0f / sc;
             if (threadIdx.x == 0) scale[pos[sl]] = sc;
         }
-        const int r0 = lane >> 4, c0 = lane & 15;
-#pragma unroll
-        for (int j = 0; j < 16; j++) {
-            const int col = 16 * (wv + 4 * (j & 1)) + c0;
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int row = 16 * (j >> 1) + r0 + 4 * e;
-                if (row < CBL && col < CBL) Store<T>::put(g, (size_t)col * CBL + row, acc[j][e], inv);
-            }
-        }
+        for (int e = threadIdx.x; e < (int)CBB; e += FWG) Store<T>::put(g, (size_t)e, coupled_mix_col(X, src, e), inv);
     }
 }
 
