@@ -1251,9 +1251,9 @@ __global__ __launch_bounds__(IWG) void k_f32_combine(const float *__restrict__ Q
 // per-cell mass blocks.  The factor is the DILU above on these blocks, in FP64 on
 // v_mfma_f64_16x16x4f64 (8 x 8 tiles of 16 over the padded 128 x 128 block; wavefront w of the
 // workgroup owns tile columns w and w + 4, 16 tiles):
-//   Schur (one workgroup per row i of a level): S_i = C_ii in DD64[i], then per lower neighbour
-//     k: X <- (coef x I) D_k^-1, T = BD(J_ik) X (registers), X <- T (stored as L_ik),
-//     Y = X BD(J_ki), S_i -= Y (coef x I) element by element;
+//   Schur (one workgroup per row i of a level): per lower neighbour k: X <- (coef x I) D_k^-1,
+//     T = BD(J_ik) X (registers), X <- T (stored as L_ik), accY += X BD(J_ki) (a second set of
+//     accumulators); then S_i = C_ii - accY (coef x I) to DD64[i] element by element;
 //   inverse: D_i^-1 = S_i^-1 by the register-tiled Gauss-Jordan, to DD64[i] and the factors;
 //   upper (after all levels): DU_ij = (D_i^-1 BD(J_ij)) (coef x I).
 // The block-diagonal products take 180 (BD X) and at most 320 (X BD) MFMAs per wavefront
@@ -1359,12 +1359,10 @@ __device__ inline void x_bd_stage(const double *X, const double *Jl, int s, int 
     }
 }
 
-// acc[2 tm + h] = tile (tm, w + 4 h) of X BD(J)
+// acc[2 tm + h] += tile (tm, w + 4 h) of X BD(J)
 __device__ inline void x_times_bd(const double *X, const double *Jl, dx4 (&acc)[16]) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int lr = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int j = 0; j < 16; j++) acc[j] = dx4{0.0, 0.0, 0.0, 0.0};
     const int n0 = 16 * wv + lr, n1 = 16 * (wv + 4) + lr;
     const bool v1 = n1 < CBL;
     const int t0 = n0 / CSB, c0 = n0 - t0 * CSB;
@@ -1396,25 +1394,14 @@ __global__ __launch_bounds__(FWG) void k_cdilu_schur(RowLists L, CoupledSrc src,
     __shared__ double red[4];
     const int wv = threadIdx.x >> 6;
     const int i = L.rows[blockIdx.x];
-    const int d = L.diag[i];
-    const int bs = src.bs;
-    double *S = DD64 + (size_t)i * CBB;
-    // S_i = C_ii, mass term included
-    wg_batched<(int)CBB>(
-        [&](int e) {
-            const int col = e / CBL, m = e - col * CBL;
-            const int st = m / CSB, r = m - st * CSB, c = col % CSB;
-            const double *J = st == 0 ? src.J0 : st == 1 ? src.J1 : src.J2;
-            return double2{J[(size_t)d * CSB * CSB + c * CSB + r], src.mass[(size_t)i * bs * bs + (c % bs) * bs + (r % bs)]};
-        },
-        [&](int e, double2 a) {
-            const int col = e / CBL, m = e - col * CBL;
-            const int st = m / CSB, r = m - st * CSB, t = col / CSB, c = col - t * CSB;
-            S[e] = fma(coupled_coef(src, st, t), a.x, st == t && r / bs == c / bs ? a.y : 0.0);
-        });
+    // accY = sum over lower neighbours k of L_ik BD(J_ki), kept in the accumulators; then
+    // S_i = C_ii - accY (coef x I) in one pass
+    dx4 accY[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) accY[j] = dx4{0.0, 0.0, 0.0, 0.0};
     for (int q = L.lptr[i]; q < L.lptr[i + 1]; q++) {
         const int sl = L.lslot[q], k = L.lcol[q], st = L.ltrans[q];
-        __syncthreads();   // X and Jl free; S_i visible to every wavefront
+        __syncthreads();   // X and Jl free
         coupled_stage(Jl, src, sl);
         wg_block_to_lds<CBL>(X, DD64 + (size_t)k * CBB);
         __syncthreads();
@@ -1428,14 +1415,28 @@ __global__ __launch_bounds__(FWG) void k_cdilu_schur(RowLists L, CoupledSrc src,
         coupled_stage(Jl, src, st);
         __syncthreads();
         wg_lds_block_store<T, CBL>(X, F + (size_t)L.pos[sl] * CBB, scale ? scale + L.pos[sl] : nullptr, red);
-        x_times_bd(X, Jl, acc);     // Y = L_ik BD(J_ki)
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 16; j++) tile_to_lds<CBL>(X, j >> 1, wv + 4 * (j & 1), acc[j]);
-        __syncthreads();
-        wg_batched<(int)CBB>([&](int e) { return S[e]; },   // S_i -= Y (coef x I)
-                             [&](int e, double v) { S[e] = v - coupled_mix_col(X, src, e); });
+        x_times_bd(X, Jl, accY);    // accY += L_ik BD(J_ki)
     }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; j++) tile_to_lds<CBL>(X, j >> 1, wv + 4 * (j & 1), accY[j]);
+    __syncthreads();
+    // S_i = C_ii - accY (coef x I), mass term included
+    const int d = L.diag[i];
+    const int bs = src.bs;
+    double *S = DD64 + (size_t)i * CBB;
+    wg_batched<(int)CBB>(
+        [&](int e) {
+            const int col = e / CBL, m = e - col * CBL;
+            const int st = m / CSB, r = m - st * CSB, c = col % CSB;
+            const double *J = st == 0 ? src.J0 : st == 1 ? src.J1 : src.J2;
+            return double2{J[(size_t)d * CSB * CSB + c * CSB + r], src.mass[(size_t)i * bs * bs + (c % bs) * bs + (r % bs)]};
+        },
+        [&](int e, double2 a) {
+            const int col = e / CBL, m = e - col * CBL;
+            const int st = m / CSB, r = m - st * CSB, t = col / CSB, c = col - t * CSB;
+            S[e] = fma(coupled_coef(src, st, t), a.x, st == t && r / bs == c / bs ? a.y : 0.0) - coupled_mix_col(X, src, e);
+        });
 }
 
 template <class T>
@@ -1466,6 +1467,8 @@ __global__ __launch_bounds__(FWG) void k_cdilu_upper(const int *__restrict__ uro
         coupled_stage(Jl, src, sl);
         __syncthreads();
         dx4 acc[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc[j] = dx4{0.0, 0.0, 0.0, 0.0};
         x_times_bd(X, Jl, acc);   // Y = D_i^-1 BD(J_ij)
         __syncthreads();
 #pragma unroll
