@@ -765,7 +765,7 @@ struct FT {
 // are in flight together (a loop that loads and then uses one value waits out each load's
 // latency in turn); the rounds stay a loop so that no more than B loaded values are live; loads
 // past the end of the last round read index N - 1 and are not used
-template <int N, int B = 8, class Load, class Use>
+template <int N, int B = 16, class Load, class Use>
 __device__ inline void wg_batched(Load &&load, Use &&use) {
     constexpr int IT = (N + FWG - 1) / FWG;
     constexpr int BB = IT < B ? IT : B;
@@ -1377,6 +1377,18 @@ __device__ inline void x_times_bd(const double *X, const double *Jl, dx4 (&acc)[
     }
 }
 
+// Y <- Y (coef x I) in place: columns (s, c) of every row mixed over s
+__device__ inline void coupled_mix_cols(double *Y, const CoupledSrc &src) {
+    for (int e = threadIdx.x; e < CSB * CBL; e += FWG) {
+        const int c = e / CBL, m = e - c * CBL;
+        double *y = Y + (size_t)c * CXL + m;
+        const double y0 = y[0], y1 = y[(size_t)CSB * CXL], y2 = y[(size_t)2 * CSB * CXL];
+#pragma unroll
+        for (int t = 0; t < 3; t++)
+            y[(size_t)t * CSB * CXL] = fma(src.coef[t], y0, fma(src.coef[3 + t], y1, src.coef[6 + t] * y2));
+    }
+}
+
 // element e (column-major) of Y (coef x I) for Y in LDS (stride CXL)
 __device__ inline double coupled_mix_col(const double *Y, const CoupledSrc &src, int e) {
     const int col = e / CBL, m = e - col * CBL, t = col / CSB, c = col - t * CSB;
@@ -1458,7 +1470,7 @@ __global__ __launch_bounds__(FWG) void k_cdilu_upper(const int *__restrict__ uro
     double *X = lds_c;
     double *Jl = X + CBL * CXL;
     __shared__ double red[4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int wv = threadIdx.x >> 6;
     const int i = urows[blockIdx.x];
     for (int q = uptr[i]; q < uptr[i + 1]; q++) {
         const int sl = uslot[q];
@@ -1474,21 +1486,9 @@ __global__ __launch_bounds__(FWG) void k_cdilu_upper(const int *__restrict__ uro
 #pragma unroll
         for (int j = 0; j < 16; j++) tile_to_lds<CBL>(X, j >> 1, wv + 4 * (j & 1), acc[j]);
         __syncthreads();
-        // DU_ij = Y (coef x I), in precision T (BF16: one power-of-two scale for the block)
-        float inv = 1.0f;
-        T *g = F + (size_t)pos[sl] * CBB;
-        if constexpr (Store<T>::prec == CPG_BF16) {
-            double mx = 0.0;
-            for (int e = threadIdx.x; e < (int)CBB; e += FWG) mx = fmax(mx, fabs(coupled_mix_col(X, src, e)));
-            mx = wave_max(mx);
-            if (lane == 0) red[wv] = mx;
-            __syncthreads();
-            mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-            const float sc = pow2_scale(mx);
-            inv = 1.0f / sc;
-            if (threadIdx.x == 0) scale[pos[sl]] = sc;
-        }
-        for (int e = threadIdx.x; e < (int)CBB; e += FWG) Store<T>::put(g, (size_t)e, coupled_mix_col(X, src, e), inv);
+        coupled_mix_cols(X, src);   // DU_ij = Y (coef x I)
+        __syncthreads();
+        wg_lds_block_store<T, CBL>(X, F + (size_t)pos[sl] * CBB, scale ? scale + pos[sl] : nullptr, red);
     }
 }
 
