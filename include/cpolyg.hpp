@@ -10,7 +10,7 @@
 //   MeshFn        MeshFn.h:11-84             MeshFn (device coefficient vector)
 //   MassMatrix    MassMatrix.h:9-27          MassMatrix
 //   Equation      Equation.h:8-67            Equation / Euler / Advection / ConstantAdvection
-//   TimeStepper   TimeIntegration.h:9-76     TimeStepper / DIRK / BackwardEuler / RK4
+//   TimeStepper   TimeIntegration.h:9-76     TimeStepper / DIRK / BackwardEuler / RK4 / IRK
 //
 // Differences a caller sees: vectors are device pointers; every failing call
 // throws cpolyg::hip::Error carrying the C status; the GMRES restart and
@@ -395,6 +395,34 @@ struct BackwardEuler : TimeStepper {
 
 struct RK4 : TimeStepper {
     explicit RK4(Equation &eqn) : TimeStepper(eqn, CPG_STEP_RK4, 4, reference_solver()) {}
+};
+
+// IRK (src/TimeIntegration.h:64-76): Radau IIA with nStages = 2 or 3.  advance solves the
+// stage-coupled Newton system (IRK::advance, src/TimeIntegration.cpp:153-281); newAdvance the
+// W form in the unknowns w = A k (IRK::newAdvance, :283-395), on a second stepper made at its
+// first call with the same parameters but flexible GMRES (the W form applies its matrix
+// matrix-free).  The reference's constructor takes the mass matrix; here the equation's own is
+// used, so the MassMatrix overload only keeps its call sites unchanged.
+struct IRK : TimeStepper {
+    IRK(int nStages, Equation &eqn, const cpg_solver_params &sp = reference_solver())
+        : TimeStepper(eqn, CPG_STEP_IRK, nStages, sp), ns_(nStages), sp_(sp) {}
+    IRK(int nStages, MassMatrix &, Equation &eqn, const cpg_solver_params &sp = reference_solver())
+        : IRK(nStages, eqn, sp) {}
+    MeshFn newAdvance(const MeshFn &u, double dt, double t = 0) {
+        if (!w_) {
+            cpg_solver_params w = sp_;
+            w.gmres_mode = CPG_GMRES_FLEXIBLE;
+            w_ = std::make_unique<TimeStepper>(*eqn_, CPG_STEP_IRK_W, ns_, w);
+        }
+        MeshFn out = w_->advance(u, dt, t);
+        last_stats = w_->last_stats;
+        return out;
+    }
+
+  private:
+    int ns_;
+    cpg_solver_params sp_;
+    std::unique_ptr<TimeStepper> w_;
 };
 
 }  // namespace hip

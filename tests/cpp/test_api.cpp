@@ -136,6 +136,16 @@ int main() {
     o_dirk_step(&om, &oe, deg, 3, u0.data(), dt, 0.0, &os, uo.data(), &ost);
     const double e_dirk = rel(u1.a->download(), uo);
 
+    // IRK::advance (coupled Newton system) and IRK::newAdvance (W form): both solve the
+    // 2-stage Radau IIA step that the oracle takes
+    IRK irk(2, eqn, sp);
+    MeshFn ui = irk.advance(u, dt, 0.0);
+    MeshFn uw = irk.newAdvance(u, dt, 0.0);
+    std::vector<double> uio(np * bs);
+    o_stats osi{};
+    o_irk_step(&om, &oe, deg, 2, u0.data(), dt, 0.0, &os, uio.data(), &osi);
+    const double e_irk = rel(ui.a->download(), uio), e_irkw = rel(uw.a->download(), uio);
+
     // MeshFn::interp of a gaussian bump and MeshFn::L2Error against it, with the oracle's
     // restatement on the same points
     const std::vector<double> gpar = {0.5, 0.5, 20.0, 0.0, 0.0, 1.0, 1.0};
@@ -160,10 +170,11 @@ int main() {
     const double e_l2 = std::abs(err - erro) / erro;
 
     std::printf("residual %.3e jacobian %.3e gmres(it %d vs %d) %.3e dirk3 %.3e (newton %d vs %d) interp %.3e "
-                "l2error %.3e\n", e_res, e_jac, J.last_iterations, it, e_sol, e_dirk, dirk.last_stats.newton_iters,
-                ost.newton_iters, e_interp, e_l2);
+                "l2error %.3e irk2 %.3e irk2-w %.3e\n", e_res, e_jac, J.last_iterations, it, e_sol, e_dirk,
+                dirk.last_stats.newton_iters, ost.newton_iters, e_interp, e_l2, e_irk, e_irkw);
     const bool ok = e_res < 1e-12 && e_jac < 1e-12 && e_sol < 1e-9 && e_dirk < 1e-10 &&
-                    std::abs(J.last_iterations - it) <= 1 && e_interp < 1e-12 && e_l2 < 1e-9;
+                    std::abs(J.last_iterations - it) <= 1 && e_interp < 1e-12 && e_l2 < 1e-9 && e_irk < 1e-10 &&
+                    e_irkw < 1e-10;
     std::printf(ok ? "PASS\n" : "FAIL\n");
     return ok ? 0 : 1;
 }
