@@ -298,7 +298,25 @@ void tableau(int method, int ns, double *A, double *b, double *c) {
         c[0] = 0; c[1] = 0.5; c[2] = 0.5; c[3] = 1.0;
         return;
     }
+    if (method == CPG_STEP_RK2) {   // RK2::advance (src/TimeIntegration.cpp:24-29)
+        b[0] = 0.5; b[1] = 0.5;
+        c[0] = 0; c[1] = 0.5;
+        return;
+    }
+    if (method == CPG_STEP_FORWARD_EULER) {   // ForwardEuler::advance (src/TimeIntegration.cpp:10-12)
+        b[0] = 1.0;
+        c[0] = 0;
+        return;
+    }
     CPG_CHECK(false, CPG_ERR_UNSUPPORTED, "method %d with %d stages", method, ns);
+}
+
+// the explicit methods: stage input u + c_q k_{q-1}, one block mass solve per stage
+inline bool explicit_method(int method) {
+    return method == CPG_STEP_RK4 || method == CPG_STEP_RK2 || method == CPG_STEP_FORWARD_EULER;
+}
+inline int explicit_stages(int method) {
+    return method == CPG_STEP_RK4 ? 4 : method == CPG_STEP_RK2 ? 2 : 1;
 }
 
 using clk = std::chrono::steady_clock;
@@ -428,14 +446,14 @@ extern "C" {
 int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver_params *params, cpg_stepper *out) {
     return guard([&] {
         CPG_CHECK(E && params && out, CPG_ERR_ARG, "null argument");
-        CPG_CHECK(method == CPG_STEP_DIRK || method == CPG_STEP_RK4 || method == CPG_STEP_IRK ||
+        CPG_CHECK(method == CPG_STEP_DIRK || explicit_method(method) || method == CPG_STEP_IRK ||
                       method == CPG_STEP_IRK_W, CPG_ERR_UNSUPPORTED, "stepper method %d not available", method);
         auto *S = new cpg_stepper_s;
         if (const char *v = std::getenv("CPOLYG_VERBOSE")) S->verbose = v[0] != '\0' && v[0] != '0';
         try {
             S->E = E;
             S->method = method;
-            S->ns = method == CPG_STEP_RK4 ? 4 : nstages;
+            S->ns = explicit_method(method) ? explicit_stages(method) : nstages;
             S->sp = *params;
             tableau(method == CPG_STEP_IRK_W ? CPG_STEP_IRK : method, S->ns, S->A, S->b, S->c);
             int n, bl, nnzb;
@@ -701,9 +719,9 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
         const auto t_all = clk::now();
         const size_t N = S->N;
         double *k = S->k.p;
-        if (S->method == CPG_STEP_RK4) {
+        if (explicit_method(S->method)) {
             const int nc = (int)S->A[0], bs = (int)S->A[1];
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < S->ns; q++) {
                 const double coef[1] = {S->c[q]};
                 // stage input u + c_q k_{q-1}
                 if (q == 0) CPG_HIP(hipMemcpyAsync(S->cu.p, u, N * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -716,7 +734,7 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                                    LinComb{{k + N * q}, {dt}, 1}, k + N * q, N);
                 CPG_LAUNCH_CHECK();
             }
-            lincomb(*S, unp1, u, S->b, 4, k, s);
+            lincomb(*S, unp1, u, S->b, S->ns, k, s);
             CPG_HIP(hipStreamSynchronize(s));
             st.ms_total = ms_since(t_all);
             return;

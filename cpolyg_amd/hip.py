@@ -365,7 +365,7 @@ class Equation:
 
 
 class Stepper:
-    """Device ``TimeStepper`` (DIRK / backward Euler / IRK / RK4)."""
+    """Device ``TimeStepper`` (DIRK / backward Euler / IRK / RK4 / RK2 / forward Euler)."""
 
     def __init__(self, eq: Equation, method: int, nstages: int, solver: _spec.SolverSpec):
         self.eq = eq

@@ -211,7 +211,14 @@ int cpg_l2_error_field(cpg_equation E, const double *u, int field, const double 
  * TimeStepper::advance (src/TimeIntegration.h:9-18) for DIRK (1 = backward
  * Euler, 2 = SDIRK2, 3, 5 stages; src/TimeIntegration.cpp:31-151) with the
  * Newton/GMRES loop on the device. */
-enum { CPG_STEP_DIRK = 0, CPG_STEP_IRK = 1, CPG_STEP_RK4 = 2, CPG_STEP_IRK_W = 3 };
+enum { CPG_STEP_DIRK = 0, CPG_STEP_IRK = 1, CPG_STEP_RK4 = 2, CPG_STEP_IRK_W = 3, CPG_STEP_RK2 = 4,
+       CPG_STEP_FORWARD_EULER = 5 };
+/* Explicit steps, k_q = dt M^-1 b(u + c_q k_{q-1}, t + c_q dt), u_{n+1} = u + sum_q w_q k_q:
+ * CPG_STEP_RK4 (RK4::advance, src/TimeIntegration.cpp:14-22), CPG_STEP_RK2 (RK2::advance, :24-29:
+ * c = (0, 1/2), w = (1/2, 1/2), as the reference computes it), CPG_STEP_FORWARD_EULER
+ * (ForwardEuler::advance, :10-12).  M^-1 is applied per component (the reference's
+ * MassMatrix::solve indexes past its output for nc > 1, src/MassMatrix.cpp:108-109); nstages
+ * and the solver parameters are not used. */
 /* CPG_STEP_IRK_W: Radau IIA in the form of IRK::newAdvance (src/TimeIntegration.cpp:283-395):
  * unknowns w_s = sum_t A[s][t] k_t (stage states u + w_s), residuals M (A^-1 w)_s - dt b(u + w_s),
  * Newton matrix blocks A^-1[s][j] M (+ -dt J_s on the diagonal), preconditioned by the

@@ -10,7 +10,8 @@
 //   MeshFn        MeshFn.h:11-84             MeshFn (device coefficient vector)
 //   MassMatrix    MassMatrix.h:9-27          MassMatrix
 //   Equation      Equation.h:8-67            Equation / Euler / Advection / ConstantAdvection
-//   TimeStepper   TimeIntegration.h:9-76     TimeStepper / DIRK / BackwardEuler / RK4 / IRK
+//   TimeStepper   TimeIntegration.h:9-76     TimeStepper / DIRK / BackwardEuler / RK4 / RK2 /
+//                                            ForwardEuler / IRK
 //
 // Differences a caller sees: vectors are device pointers; every failing call
 // throws cpolyg::hip::Error carrying the C status; the GMRES restart and
@@ -395,6 +396,16 @@ struct BackwardEuler : TimeStepper {
 
 struct RK4 : TimeStepper {
     explicit RK4(Equation &eqn) : TimeStepper(eqn, CPG_STEP_RK4, 4, reference_solver()) {}
+};
+
+// RK2 (src/TimeIntegration.h:41-46): the midpoint stage with weights 1/2, 1/2, as the reference
+struct RK2 : TimeStepper {
+    explicit RK2(Equation &eqn) : TimeStepper(eqn, CPG_STEP_RK2, 2, reference_solver()) {}
+};
+
+// ForwardEuler (src/TimeIntegration.h:20-25)
+struct ForwardEuler : TimeStepper {
+    explicit ForwardEuler(Equation &eqn) : TimeStepper(eqn, CPG_STEP_FORWARD_EULER, 1, reference_solver()) {}
 };
 
 // IRK (src/TimeIntegration.h:64-76): Radau IIA with nStages = 2 or 3.  advance solves the

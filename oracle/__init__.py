@@ -171,6 +171,20 @@ class Oracle:
                          C.c_double(t), C.byref(osolver(solver)), _d(out), C.byref(st))
         return out, st
 
+    def forward_euler_step(self, spec, deg, u, dt, t):
+        u = np.ascontiguousarray(u, np.float64)
+        out = np.zeros_like(u)
+        lib().o_forward_euler_step(C.byref(self.m), C.byref(self.eq(spec)), deg, _d(u), C.c_double(dt),
+                                   C.c_double(t), _d(out))
+        return out
+
+    def rk2_step(self, spec, deg, u, dt, t):
+        u = np.ascontiguousarray(u, np.float64)
+        out = np.zeros_like(u)
+        lib().o_rk2_step(C.byref(self.m), C.byref(self.eq(spec)), deg, _d(u), C.c_double(dt), C.c_double(t),
+                         _d(out))
+        return out
+
     def rk4_step(self, spec, deg, u, dt, t):
         u = np.ascontiguousarray(u, np.float64)
         out = np.zeros_like(u)

@@ -1451,6 +1451,55 @@ int o_irk_step(const o_mesh *m, const o_eq *eq, int deg, int ns, const double *u
     return 0;
 }
 
+/* dt M^-1 b(v, tv) into k, M's per-cell LU factors in Mf/piv */
+static void explicit_stage(const o_mesh *m, const o_eq *eq, int deg, const double *Mf, const int *piv,
+                           const double *v, double tv, double dt, double *k)
+{
+    int n = m->np, bs = (deg + 1) * (deg + 2) / 2, bl = o_nc(eq) * bs;
+    size_t N = (size_t)n * bl, bb2 = (size_t)bl * bl;
+    o_assemble(m, eq, deg, v, tv, k);
+    for (int i = 0; i < n; i++) lu_solve(bl, Mf + i * bb2, piv + (size_t)i * bl, k + (size_t)i * bl);
+    for (size_t e = 0; e < N; e++) k[e] *= dt;
+}
+
+/* ForwardEuler::advance (src/TimeIntegration.cpp:10-12): u + dt M^-1 b(u, t), with a
+ * per-component block mass solve (the reference's MassMatrix::solve indexes past its output for
+ * nc > 1, src/MassMatrix.cpp:108-109) */
+int o_forward_euler_step(const o_mesh *m, const o_eq *eq, int deg, const double *u, double dt, double t,
+                         double *unp1)
+{
+    int n = m->np, bs = (deg + 1) * (deg + 2) / 2, bl = o_nc(eq) * bs;
+    size_t N = (size_t)n * bl, bb2 = (size_t)bl * bl;
+    double *M = malloc(sizeof(double) * bb2 * n), *k = malloc(sizeof(double) * N);
+    int *piv = malloc(sizeof(int) * N);
+    o_mass(m, deg, o_nc(eq), M);
+    for (int i = 0; i < n; i++) lu_factor(bl, M + i * bb2, piv + (size_t)i * bl);
+    explicit_stage(m, eq, deg, M, piv, u, t, dt, k);
+    for (size_t e = 0; e < N; e++) unp1[e] = u[e] + k[e];
+    free(M); free(k); free(piv);
+    return 0;
+}
+
+/* RK2::advance (src/TimeIntegration.cpp:24-29): k1 = dt M^-1 b(u, t),
+ * k2 = dt M^-1 b(u + k1/2, t + dt/2), u + (k1 + k2)/2 -- the midpoint stage with the trapezoidal
+ * weights, as the reference has it (first order) */
+int o_rk2_step(const o_mesh *m, const o_eq *eq, int deg, const double *u, double dt, double t, double *unp1)
+{
+    int n = m->np, bs = (deg + 1) * (deg + 2) / 2, bl = o_nc(eq) * bs;
+    size_t N = (size_t)n * bl, bb2 = (size_t)bl * bl;
+    double *M = malloc(sizeof(double) * bb2 * n), *k1 = malloc(sizeof(double) * N);
+    double *k2 = malloc(sizeof(double) * N), *v = malloc(sizeof(double) * N);
+    int *piv = malloc(sizeof(int) * N);
+    o_mass(m, deg, o_nc(eq), M);
+    for (int i = 0; i < n; i++) lu_factor(bl, M + i * bb2, piv + (size_t)i * bl);
+    explicit_stage(m, eq, deg, M, piv, u, t, dt, k1);
+    for (size_t e = 0; e < N; e++) v[e] = u[e] + 0.5 * k1[e];
+    explicit_stage(m, eq, deg, M, piv, v, t + 0.5 * dt, dt, k2);
+    for (size_t e = 0; e < N; e++) unp1[e] = u[e] + 0.5 * (k1[e] + k2[e]);
+    free(M); free(k1); free(k2); free(v); free(piv);
+    return 0;
+}
+
 /* RK4::advance (src/TimeIntegration.cpp:14-22) with a per-component block mass solve */
 int o_rk4_step(const o_mesh *m, const o_eq *eq, int deg, const double *u, double dt, double t,
                double *unp1)

@@ -122,6 +122,9 @@ int o_dirk_step(const o_mesh *m, const o_eq *eq, int deg, int nstages, const dou
                 double dt, double t, const o_solver *s, double *unp1, o_stats *st);
 int o_irk_step(const o_mesh *m, const o_eq *eq, int deg, int nstages, const double *u,
                double dt, double t, const o_solver *s, double *unp1, o_stats *st);
+int o_forward_euler_step(const o_mesh *m, const o_eq *eq, int deg, const double *u, double dt, double t,
+                         double *unp1);
+int o_rk2_step(const o_mesh *m, const o_eq *eq, int deg, const double *u, double dt, double t, double *unp1);
 int o_rk4_step(const o_mesh *m, const o_eq *eq, int deg, const double *u, double dt, double t,
                double *unp1);
 

@@ -146,6 +146,15 @@ int main() {
     o_irk_step(&om, &oe, deg, 2, u0.data(), dt, 0.0, &os, uio.data(), &osi);
     const double e_irk = rel(ui.a->download(), uio), e_irkw = rel(uw.a->download(), uio);
 
+    // RK2::advance and ForwardEuler::advance (explicit, one block mass solve per stage)
+    RK2 rk2(eqn);
+    ForwardEuler fe(eqn);
+    std::vector<double> ur2o(np * bs), ufeo(np * bs);
+    o_rk2_step(&om, &oe, deg, u0.data(), dt, 0.0, ur2o.data());
+    o_forward_euler_step(&om, &oe, deg, u0.data(), dt, 0.0, ufeo.data());
+    const double e_rk2 = rel(rk2.advance(u, dt, 0.0).a->download(), ur2o);
+    const double e_fe = rel(fe.advance(u, dt, 0.0).a->download(), ufeo);
+
     // MeshFn::interp of a gaussian bump and MeshFn::L2Error against it, with the oracle's
     // restatement on the same points
     const std::vector<double> gpar = {0.5, 0.5, 20.0, 0.0, 0.0, 1.0, 1.0};
@@ -170,11 +179,12 @@ int main() {
     const double e_l2 = std::abs(err - erro) / erro;
 
     std::printf("residual %.3e jacobian %.3e gmres(it %d vs %d) %.3e dirk3 %.3e (newton %d vs %d) interp %.3e "
-                "l2error %.3e irk2 %.3e irk2-w %.3e\n", e_res, e_jac, J.last_iterations, it, e_sol, e_dirk,
-                dirk.last_stats.newton_iters, ost.newton_iters, e_interp, e_l2, e_irk, e_irkw);
+                "l2error %.3e irk2 %.3e irk2-w %.3e rk2 %.3e forward-euler %.3e\n", e_res, e_jac, J.last_iterations, it,
+                e_sol, e_dirk, dirk.last_stats.newton_iters, ost.newton_iters, e_interp, e_l2, e_irk, e_irkw, e_rk2,
+                e_fe);
     const bool ok = e_res < 1e-12 && e_jac < 1e-12 && e_sol < 1e-9 && e_dirk < 1e-10 &&
                     std::abs(J.last_iterations - it) <= 1 && e_interp < 1e-12 && e_l2 < 1e-9 && e_irk < 1e-10 &&
-                    e_irkw < 1e-10;
+                    e_irkw < 1e-10 && e_rk2 < 1e-12 && e_fe < 1e-12;
     std::printf(ok ? "PASS\n" : "FAIL\n");
     return ok ? 0 : 1;
 }

@@ -1,4 +1,4 @@
-"""GPU DIRK / backward-Euler / RK4 steps against the oracle's steppers."""
+"""GPU DIRK / backward-Euler / IRK / explicit (RK4, RK2, forward Euler) steps against the oracle's steppers."""
 import numpy as np
 import pytest
 
@@ -94,15 +94,17 @@ def test_backward_euler_advection_bjacobi(cuda):
     assert rel(outd.cpu().numpy(), uo) < 1e-10
 
 
-def test_rk4_advection(cuda):
+@pytest.mark.parametrize("method,name", [(2, "rk4_step"), (4, "rk2_step"), (5, "forward_euler_step")])
+def test_rk4_advection(cuda, method, name):
+    """The explicit steps (RK4, the reference's RK2, forward Euler) against the oracle's."""
     m = unit_periodic_mesh(8)
     deg = 2
     spec = EquationSpec.constant_advection(1.0, 1.0)
     u = setups.interp(m, deg, setups.gaussian(), 1)
     dt = 0.1 * m.h_min()
-    uo = O.Oracle(m).rk4_step(spec, deg, u, dt, 0.0)
+    uo = getattr(O.Oracle(m), name)(spec, deg, u, dt, 0.0)
     E = hip.Equation(hip.Mesh(m), spec, deg)
-    S = hip.Stepper(E, 2, 4, SolverSpec())
+    S = hip.Stepper(E, method, 4, SolverSpec())
     ud, outd = _dev(cuda, u), _dev(cuda, np.zeros_like(u))
     S.advance(ud, dt, 0.0, outd)
     assert rel(outd.cpu().numpy(), uo) < 1e-12

@@ -475,6 +475,20 @@ def test_rk4_linear_advection_dense():
     assert rel(orc.rk4_step(spec, deg, u, dt, 0.0), want) < 1e-12
 
 
+def test_forward_euler_and_rk2_linear_advection_dense():
+    """ForwardEuler and the reference's RK2 (midpoint stage, weights 1/2 and 1/2) on a linear
+    residual b(u) = J u: u + dt L u and u + dt L u + dt^2/4 L^2 u with L = M^-1 J, from the
+    oracle's dense Jacobian and mass matrix."""
+    m, deg, spec, orc, u, M, Jd = _linear_setup()
+    dt = 0.01
+    L = np.linalg.solve(M, Jd)
+    Lu = L @ u
+    assert rel(orc.forward_euler_step(spec, deg, u, dt, 0.0), u + dt * Lu) < 1e-12
+    assert rel(orc.rk2_step(spec, deg, u, dt, 0.0), u + dt * Lu + 0.25 * dt * dt * (L @ Lu)) < 1e-12
+    # the second-order term is what separates RK2 from forward Euler at this dt
+    assert rel(orc.rk2_step(spec, deg, u, dt, 0.0), u + dt * Lu) > 1e-6
+
+
 # ---- MeshFn::interp and MeshFn::L2Error (src/MeshFn.cpp:56-145, 277-296) ---------------
 
 @pytest.mark.parametrize("m", [1, 2, 3, 4, 5])
