@@ -675,6 +675,37 @@ void gmres_flexible_op(cpg_krylov K, const LinOp &A, const PcOp &M, const double
     gmres_flexible(K, A, nullptr, &M, b, x, x_zero, tol, maxit, s);
 }
 
+// partial[b] = sum over chunk b of (t - b)^2
+__global__ __launch_bounds__(KWG) void k_diff_sq(const double *__restrict__ t, const double *__restrict__ b,
+                                                 size_t N, double *__restrict__ partial) {
+    __shared__ double red[KWG / 64];
+    const size_t base = (size_t)blockIdx.x * KCH + threadIdx.x;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < KPT; k++) {
+        const size_t e = base + (size_t)k * KWG;
+        if (e < N) {
+            const double d = t[e] - b[e];
+            acc = fma(d, d, acc);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double q = 0.0;
+        for (int w = 0; w < KWG / 64; w++) q += red[w];
+        partial[blockIdx.x] = q;
+    }
+}
+
+// x = (omega Db + x) - omega t: the Jacobi update of BlockMatrix::jacobi (src/BlockMatrix.cpp:73)
+__global__ void k_jacobi_update(double *__restrict__ x, const double *__restrict__ Db, const double *__restrict__ t,
+                                size_t N, double omega) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < N) x[e] = (omega * Db[e] + x[e]) - omega * t[e];
+}
+
 void bsr_linop(void *ctx, const double *x, double *y, const int *active, hipStream_t s) {
     bsr_matvec(*static_cast<const cpg_bsr_s *>(ctx), x, y, s, active);
 }
@@ -743,6 +774,46 @@ int cpg_gmres(cpg_krylov K, cpg_bsr A, cpg_prec P, const double *b, double *x, d
         CPG_CHECK(A->n == K->n && A->bl == K->bl, CPG_ERR_ARG, "matrix does not match the workspace");
         const LinOp op{bsr_linop, A};
         gmres_solve(K, op, P, b, x, false, tol, maxit, S(stream));
+    });
+}
+
+int cpg_jacobi(cpg_krylov K, cpg_bsr A, cpg_prec P, const double *b, double *x, double *tol, int *maxit,
+               void *stream) {
+    return guard([&] {
+        CPG_CHECK(K && A && b && x && tol && maxit, CPG_ERR_ARG, "null argument");
+        CPG_CHECK(A->n == K->n && A->bl == K->bl, CPG_ERR_ARG, "matrix does not match the workspace");
+        CPG_CHECK(*maxit >= 0, CPG_ERR_ARG, "negative iteration limit");
+        hipStream_t s = S(stream);
+        const size_t N = K->N;
+        double *Db = K->V.p, *t = K->V.p + N;   // V holds restart + 1 >= 2 vectors
+        const double omega = 1.0;
+        // Dinvb = P^-1 b
+        CPG_HIP(hipMemcpyAsync(Db, b, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (P) {
+            const int st = cpg_prec_apply(P, Db, s);
+            if (st) throw Error{st};
+        }
+        double rn = 0.0;   // the reference's norm of an empty residual when no iteration runs
+        int it = 0;
+        for (; it < *maxit; it++) {
+            bsr_matvec(*A, x, t, s, nullptr);   // t = A x
+            hipLaunchKernelGGL(k_diff_sq, dim3(K->nblk), dim3(KWG), 0, s, t, b, N, K->partial.p);
+            CPG_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_reduce, dim3(1), dim3(KWG), 0, s, K->partial.p, K->nblk, 1, K->scal.p, 1, nullptr);
+            CPG_LAUNCH_CHECK();
+            CPG_HIP(hipMemcpyAsync(&rn, K->scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+            CPG_HIP(hipStreamSynchronize(s));
+            if (rn < *tol) break;   // ||A x - b||_2, absolute
+            if (P) {
+                const int st = cpg_prec_apply(P, t, s);
+                if (st) throw Error{st};
+            }
+            hipLaunchKernelGGL(k_jacobi_update, dim3(ceil_div(N, 256)), dim3(256), 0, s, x, Db, t, N, omega);
+            CPG_LAUNCH_CHECK();
+        }
+        CPG_HIP(hipStreamSynchronize(s));
+        *tol = rn;
+        *maxit = it;
     });
 }
 

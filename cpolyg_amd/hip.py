@@ -229,6 +229,14 @@ class Krylov:
                              C.byref(it), _stream(stream)))
         return t.value, it.value
 
+    def jacobi(self, A: Bsr, P: Prec | None, b, x, tol, maxit, stream=None):
+        """BlockMatrix::jacobi (src/BlockMatrix.cpp:51-77); returns (||A x - b||_2, iterations)."""
+        t = C.c_double(tol)
+        it = C.c_int(maxit)
+        _chk(lib().cpg_jacobi(self.h, A.h, P.h if P is not None else None, _vp(ptr(b)), _vp(ptr(x)), C.byref(t),
+                              C.byref(it), _stream(stream)))
+        return t.value, it.value
+
 
 def eq_params(s: _spec.EquationSpec) -> EqParams:
     e = EqParams()

@@ -125,6 +125,13 @@ int cpg_krylov_create(int n_rows, int bl, int restart, int mode, cpg_krylov *out
 int cpg_krylov_destroy(cpg_krylov K);
 int cpg_gmres(cpg_krylov K, cpg_bsr A, cpg_prec P, const double *b, double *x, double *tol, int *maxit,
               void *stream);
+/* Replaces BlockMatrix::jacobi (src/BlockMatrix.cpp:51-77): x <- (P^-1 b + x) - P^-1 A x (omega = 1)
+ * until ||A x - b||_2 < tol (absolute), at most *maxit iterations.  On return *tol is ||A x - b||_2
+ * of the last check and *maxit the iterations taken, as in the reference (0 and 0 when *maxit is
+ * 0).  P is any preconditioner of A (the reference passes its BlockJacobi) or null for P = I; x
+ * holds the initial guess.  Uses K's vectors (any mode); synchronous on return. */
+int cpg_jacobi(cpg_krylov K, cpg_bsr A, cpg_prec P, const double *b, double *x, double *tol, int *maxit,
+               void *stream);
 
 /* ---- mesh and equations -----------------------------------------------------
  * Mesh arrays are PolyMesh's public fields (src/PolyMesh.h:22-28) flattened:

@@ -147,6 +147,8 @@ class BlockMatrix {
     // BlockMatrix::gmres (src/BlockMatrix.cpp:79-163): tol, maxIt in/out; x = initial guess in, solution out
     void gmres(const double *b, double *x, int m, double &tol, int &maxIt, Preconditioner &pc,
                int mode = CPG_GMRES_LEFT_REF);
+    // BlockMatrix::jacobi (src/BlockMatrix.cpp:51-77): tol (absolute ||A x - b||_2), maxIt in/out
+    void jacobi(const double *b, double *x, double &tol, int &maxIt, Preconditioner &pc);
 
   protected:
     void adopt(cpg_bsr h) { h_ = std::shared_ptr<cpg_bsr_s>(h, [](cpg_bsr b) { cpg_bsr_destroy(b); }); }
@@ -200,12 +202,21 @@ inline void BlockMatrix::gmres(const double *b, double *x, int m, double &tol, i
     check(cpg_gmres(K, handle(), pc.handle(), b, x, &tol, &maxIt, nullptr));
 }
 
+inline void BlockMatrix::jacobi(const double *b, double *x, double &tol, int &maxIt, Preconditioner &pc) {
+    cpg_krylov K = nullptr;
+    check(cpg_krylov_create(n_rows, bl, 1, CPG_GMRES_LEFT_REF, &K));
+    std::unique_ptr<cpg_krylov_s, int (*)(cpg_krylov)> guard(K, cpg_krylov_destroy);
+    check(cpg_jacobi(K, handle(), pc.handle(), b, x, &tol, &maxIt, nullptr));
+}
+
 // ---- equations ---------------------------------------------------------------------
-enum Solver { kGMRESSolver, kJacobiSolver };
 
 class Equation;
 
 // Jacobian (src/Jacobian.h): BlockMatrix with the mesh-adjacency pattern
+// Jacobian::solve's choice of solver (src/Jacobian.h:9-13)
+enum Solver { kGMRESSolver, kJacobiSolver };
+
 class Jacobian : public BlockMatrix {
   public:
     int nc = 0, deg = 0;
@@ -230,6 +241,20 @@ class Jacobian : public BlockMatrix {
         std::vector<double> z(x.size(), 0.0);
         x.a->upload(z);
         gmres(b.data(), x.data(), m, tol, maxIt, pc, mode);
+        last_tol = tol;
+        last_iterations = maxIt;
+        return x;
+    }
+    // Jacobian::solve(b, pc, s) (src/Jacobian.cpp:68-97): zero initial guess, tol 1e-14,
+    // maxIt 90000, GMRES(20) or the Jacobi iteration
+    MeshFn solve(const MeshFn &b, Preconditioner &pc, Solver s) {
+        if (s == kGMRESSolver) return solve(b, pc);
+        MeshFn x(b.np, b.deg, b.nc);
+        std::vector<double> z(x.size(), 0.0);
+        x.a->upload(z);
+        double tol = 1e-14;
+        int maxIt = 90000;
+        jacobi(b.data(), x.data(), tol, maxIt, pc);
         last_tol = tol;
         last_iterations = maxIt;
         return x;

@@ -122,6 +122,22 @@ int main() {
             xo.data(), 20, &tol, &it, O_GMRES_LEFT_REF);
     const double e_sol = rel(x.a->download(), xo);
 
+    // BlockMatrix::jacobi with the block-Jacobi preconditioner of the same B
+    BlockJacobi bj(J);
+    MeshFn xj(np, deg, 1);
+    xj.a->upload(std::vector<double>(np * bs, 0.0));
+    double jtol = 1e-10;
+    int jit = 1000;
+    J.jacobi(b.data(), xj.data(), jtol, jit, bj);
+    std::vector<double> LU(Bh.size()), xjo(np * bs, 0.0);
+    std::vector<int> piv(np * bs);
+    o_bjacobi_factor(np, 6, J.rowBlock.data(), J.colIndices.data(), Bh.data(), LU.data(), piv.data());
+    double jtolo = 1e-10;
+    int jito = 1000;
+    o_jacobi_solve(np, 6, J.rowBlock.data(), J.colIndices.data(), Bh.data(), LU.data(), piv.data(), bo.data(),
+                   xjo.data(), &jtolo, &jito);
+    const double e_jacobi = rel(xj.a->download(), xjo);
+
     // DIRK::advance
     cpg_solver_params sp = reference_solver();
     sp.rtol = 1e-12;
@@ -179,12 +195,13 @@ int main() {
     const double e_l2 = std::abs(err - erro) / erro;
 
     std::printf("residual %.3e jacobian %.3e gmres(it %d vs %d) %.3e dirk3 %.3e (newton %d vs %d) interp %.3e "
-                "l2error %.3e irk2 %.3e irk2-w %.3e rk2 %.3e forward-euler %.3e\n", e_res, e_jac, J.last_iterations, it,
-                e_sol, e_dirk, dirk.last_stats.newton_iters, ost.newton_iters, e_interp, e_l2, e_irk, e_irkw, e_rk2,
-                e_fe);
+                "l2error %.3e irk2 %.3e irk2-w %.3e rk2 %.3e forward-euler %.3e jacobi(it %d vs %d) %.3e\n", e_res, e_jac,
+                J.last_iterations, it, e_sol, e_dirk, dirk.last_stats.newton_iters, ost.newton_iters, e_interp, e_l2,
+                e_irk, e_irkw, e_rk2, e_fe, jit, jito, e_jacobi);
     const bool ok = e_res < 1e-12 && e_jac < 1e-12 && e_sol < 1e-9 && e_dirk < 1e-10 &&
                     std::abs(J.last_iterations - it) <= 1 && e_interp < 1e-12 && e_l2 < 1e-9 && e_irk < 1e-10 &&
-                    e_irkw < 1e-10 && e_rk2 < 1e-12 && e_fe < 1e-12;
+                    e_irkw < 1e-10 && e_rk2 < 1e-12 && e_fe < 1e-12 &&
+                    jit == jito && jit < 1000 && e_jacobi < 1e-9;
     std::printf(ok ? "PASS\n" : "FAIL\n");
     return ok ? 0 : 1;
 }

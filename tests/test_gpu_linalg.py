@@ -7,7 +7,7 @@ import oracle as O
 from cpolyg_amd import hip
 from cpolyg_amd.spec import (EquationSpec, PC_BJACOBI, PC_DILU, PREC_BF16, PREC_FP32, PREC_FP64,
                              GMRES_FLEXIBLE, GMRES_LEFT_REF)
-from problems import newton_matrix, rel, vortex_mesh, vortex_state, unit_periodic_mesh
+from problems import dense, newton_matrix, rel, vortex_mesh, vortex_state, unit_periodic_mesh
 
 pytestmark = pytest.mark.gpu
 
@@ -151,6 +151,47 @@ def test_block_jacobi(cuda, prec, tol):
     xd = _dev(cuda, x)
     P.apply(xd)
     assert rel(xd.cpu().numpy(), O.bjacobi_apply(LU, piv, x)) < tol
+
+
+@pytest.mark.parametrize("pc", [PC_BJACOBI, None])
+def test_jacobi_iteration_matches_oracle(cuda, pc):
+    """cpg_jacobi (BlockMatrix::jacobi, omega = 1) against the oracle: a fixed number of steps
+    (tol 0) gives the same iterate and residual norm; to convergence, the same iteration count."""
+    m, orc, B = _adv_system(2)
+    bl = B.shape[1]
+    A = hip.Bsr(orc.rowptr, orc.colind, bl)
+    A.upload(B)
+    P = None
+    if pc is not None:
+        P = hip.Prec(A, PC_BJACOBI, PREC_FP64)
+        P.factor()
+    K = hip.Krylov(m.n_cells, bl, 4, GMRES_LEFT_REF)
+    b = np.random.default_rng(8).standard_normal(m.n_cells * bl)
+    bd = _dev(cuda, b)
+    if pc is None:
+        # P = I: x <- x + (b - A x), a Richardson iteration; scale the system so that it converges
+        B = B / np.abs(B).max() * 0.5
+        A.upload(B)
+        xd = _dev(cuda, np.zeros_like(b))
+        tol, its = K.jacobi(A, None, bd, xd, 0.0, 5)
+        x = np.zeros_like(b)
+        Ad = dense(orc.rowptr, orc.colind, B)
+        for _ in range(5):
+            r = Ad @ x - b
+            x = x - r
+        assert its == 5 and rel(xd.cpu().numpy(), x) < 1e-12
+        return
+    xd = _dev(cuda, np.zeros_like(b))
+    tol, its = K.jacobi(A, P, bd, xd, 0.0, 9)
+    xo, tolo, itso = O.jacobi_solve(orc.rowptr, orc.colind, B, b, tol=0.0, maxit=9)
+    assert its == itso == 9 and rel(xd.cpu().numpy(), xo) < 1e-12 and abs(tol - tolo) < 1e-10 * tolo
+    xd = _dev(cuda, np.zeros_like(b))
+    tol, its = K.jacobi(A, P, bd, xd, 1e-10, 1000)
+    xo, tolo, itso = O.jacobi_solve(orc.rowptr, orc.colind, B, b, tol=1e-10, maxit=1000)
+    assert its == itso < 1000 and tol < 1e-10 and rel(xd.cpu().numpy(), xo) < 1e-9
+    # no iterations: the reference's empty residual
+    xd = _dev(cuda, np.zeros_like(b))
+    assert K.jacobi(A, P, bd, xd, 1e-10, 0) == (0.0, 0)
 
 
 @pytest.mark.parametrize("mode", [GMRES_LEFT_REF, GMRES_FLEXIBLE])

@@ -475,6 +475,34 @@ def test_rk4_linear_advection_dense():
     assert rel(orc.rk4_step(spec, deg, u, dt, 0.0), want) < 1e-12
 
 
+def test_jacobi_iteration_matches_splitting_form():
+    """BlockMatrix::jacobi (omega = 1) in the reference's residual-correction form equals the
+    block-Jacobi splitting x_{k+1} = D^-1 (b - R x_k) (A = D + R, dense): the same iterates after
+    a fixed number of steps, tol = ||A x_k - b||_2 of the last check, and convergence to A^-1 b."""
+    from problems import newton_matrix
+    m = unit_periodic_mesh(5)
+    deg = 2
+    orc = O.Oracle(m)
+    spec = EquationSpec.reference_advection()
+    u = np.random.default_rng(3).standard_normal(m.n_cells * bs_of(deg))
+    B = newton_matrix(orc, spec, deg, u, dt=0.05)
+    Ad = dense(orc.rowptr, orc.colind, B)
+    n, bl = m.n_cells, B.shape[1]
+    D = np.zeros_like(Ad)
+    for i in range(n):
+        D[i * bl:(i + 1) * bl, i * bl:(i + 1) * bl] = Ad[i * bl:(i + 1) * bl, i * bl:(i + 1) * bl]
+    R = Ad - D
+    b = np.random.default_rng(4).standard_normal(n * bl)
+    xs = np.zeros_like(b)
+    for k in range(7):
+        last_r = np.linalg.norm(Ad @ xs - b)
+        xs = np.linalg.solve(D, b - R @ xs)
+    x, tol, its = O.jacobi_solve(orc.rowptr, orc.colind, B, b, tol=0.0, maxit=7)
+    assert its == 7 and rel(x, xs) < 1e-12 and abs(tol - last_r) < 1e-12 * last_r
+    x, tol, its = O.jacobi_solve(orc.rowptr, orc.colind, B, b, tol=1e-11, maxit=500)
+    assert its < 500 and tol < 1e-11 and rel(x, np.linalg.solve(Ad, b)) < 1e-10
+
+
 def test_forward_euler_and_rk2_linear_advection_dense():
     """ForwardEuler and the reference's RK2 (midpoint stage, weights 1/2 and 1/2) on a linear
     residual b(u) = J u: u + dt L u and u + dt L u + dt^2/4 L^2 u with L = M^-1 J, from the
