@@ -75,6 +75,12 @@ __global__ void k_scale(double *x, double v, size_t n) {
     if (i < n) x[i] *= v;
 }
 
+// out = a x + b y (y null: out = a x); out may alias x or y (each element is read before it is written)
+__global__ void k_axpby(size_t n, double a, const double *x, double b, const double *y, double *out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = y ? a * x[i] + b * y[i] : a * x[i];
+}
+
 void vec_fill(double *x, double v, size_t n, hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill, dim3(ceil_div(n, 256)), dim3(256), 0, s, x, v, n);
@@ -208,6 +214,17 @@ int cpg_bsr_scale(cpg_bsr A, double s, void *stream) {
         if (!A->blocks.n) return;
         hipLaunchKernelGGL(k_scale, dim3(ceil_div(A->blocks.n, 256)), dim3(256), 0, S(stream), A->blocks.p, s,
                            A->blocks.n);
+        CPG_LAUNCH_CHECK();
+    });
+}
+
+int cpg_vec_axpby(unsigned long long n, double a, const double *x, double b, const double *y, double *out,
+                  void *stream) {
+    return guard([&] {
+        if (!n) return;
+        CPG_CHECK(x && out, CPG_ERR_ARG, "null argument");
+        hipLaunchKernelGGL(k_axpby, dim3(ceil_div((size_t)n, 256)), dim3(256), 0, S(stream), (size_t)n, a, x, b, y,
+                           out);
         CPG_LAUNCH_CHECK();
     });
 }

@@ -49,6 +49,10 @@ int cpg_device_free(void *ptr);
 int cpg_memcpy_h2d(void *dst, const void *src, unsigned long long bytes, void *stream);
 int cpg_memcpy_d2h(void *dst, const void *src, unsigned long long bytes, void *stream);   /* synchronous */
 int cpg_memcpy_d2d(void *dst, const void *src, unsigned long long bytes, void *stream);
+/* out = a x + b y over n doubles (y null: out = a x; out may alias x or y): MeshFn's arithmetic
+ * (operator+, -, *, /, +=, -=; src/MeshFn.h:59-84, src/MeshFn.cpp:209-236) */
+int cpg_vec_axpby(unsigned long long n, double a, const double *x, double b, const double *y, double *out,
+                  void *stream);
 int cpg_stream_sync(void *stream);
 
 /* ---- block sparse row matrix ------------------------------------------------

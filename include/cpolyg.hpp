@@ -95,9 +95,14 @@ struct PolyMesh {
 
 // ---- MeshFn ------------------------------------------------------------------------
 // arma::cube(bs, nc, np) coefficients, here a device vector u[i*bl + c*bs + k]
+// Copies share the device vector; the arithmetic below never writes into a shared vector (+=
+// and -= rebind to a new one), so a copy keeps its value as the reference's would.  `eq` is the
+// equation whose mesh and basis the coefficients belong to (set by Equation's results, carried
+// by the arithmetic); L2Norm needs it.
 struct MeshFn {
     int deg = 0, nc = 1, np = 0;
     std::shared_ptr<DeviceArray> a;
+    std::shared_ptr<cpg_equation_s> eq;
     MeshFn() = default;
     MeshFn(int a_np, int a_deg, int a_nc) : deg(a_deg), nc(a_nc), np(a_np),
         a(std::make_shared<DeviceArray>((size_t)a_np * a_nc * (a_deg + 1) * (a_deg + 2) / 2)) {}
@@ -105,7 +110,33 @@ struct MeshFn {
     size_t size() const { return (size_t)np * nc * bs(); }
     double *data() { return a->data(); }
     const double *data() const { return a->data(); }
+
+    // MeshFn::L2Norm (src/MeshFn.cpp:299-312): one value per component
+    std::vector<double> L2Norm() const {
+        if (!eq) throw Error(CPG_ERR_ARG, "L2Norm: the coefficients carry no equation");
+        std::vector<double> out(nc);
+        check(cpg_l2norm(eq.get(), data(), out.data(), nullptr));
+        return out;
+    }
+    // operator+, -, * (src/MeshFn.cpp:209-236), / and the in-place forms (src/MeshFn.h:59-79)
+    MeshFn operator+(const MeshFn &o) const { return axpby(1.0, &o, 1.0); }
+    MeshFn operator-(const MeshFn &o) const { return axpby(1.0, &o, -1.0); }
+    MeshFn operator*(double s) const { return axpby(s, nullptr, 0.0); }
+    MeshFn operator/(double s) const { return (*this) * (1.0 / s); }
+    MeshFn &operator+=(const MeshFn &o) { return *this = *this + o; }
+    MeshFn &operator-=(const MeshFn &o) { return *this = *this - o; }
+
+  private:
+    MeshFn axpby(double ca, const MeshFn *o, double cb) const {
+        if (o && (o->deg != deg || o->nc != nc || o->np != np)) throw Error(CPG_ERR_ARG, "MeshFn shapes differ");
+        MeshFn r(np, deg, nc);
+        r.eq = eq ? eq : (o ? o->eq : nullptr);
+        check(cpg_vec_axpby(size(), ca, data(), cb, o ? o->data() : nullptr, r.data(), nullptr));
+        check(cpg_stream_sync(nullptr));
+        return r;
+    }
 };
+inline MeshFn operator*(double s, const MeshFn &fn) { return fn * s; }
 
 // ---- BlockMatrix -------------------------------------------------------------------
 class Preconditioner;
@@ -238,6 +269,7 @@ class Jacobian : public BlockMatrix {
     MeshFn solve(const MeshFn &b, Preconditioner &pc, int m = 20, double tol = 1e-14, int maxIt = 90000,
                  int mode = CPG_GMRES_LEFT_REF) {
         MeshFn x(b.np, b.deg, b.nc);
+        x.eq = b.eq;
         std::vector<double> z(x.size(), 0.0);
         x.a->upload(z);
         gmres(b.data(), x.data(), m, tol, maxIt, pc, mode);
@@ -250,6 +282,7 @@ class Jacobian : public BlockMatrix {
     MeshFn solve(const MeshFn &b, Preconditioner &pc, Solver s) {
         if (s == kGMRESSolver) return solve(b, pc);
         MeshFn x(b.np, b.deg, b.nc);
+        x.eq = b.eq;
         std::vector<double> z(x.size(), 0.0);
         x.a->upload(z);
         double tol = 1e-14;
@@ -280,6 +313,7 @@ class Equation {
     // Equation::assemble (src/Equation.cpp:152-191)
     virtual MeshFn assemble(const MeshFn &f, double t) {
         MeshFn b(np_, deg_, nc_);
+        b.eq = h_;
         check(cpg_assemble_residual(handle(), f.data(), t, b.data(), nullptr));
         check(cpg_stream_sync(nullptr));
         return b;
@@ -295,6 +329,7 @@ class Equation {
     // `B = jacobian(f, t); B *= scale; B += M` (src/TimeIntegration.cpp:127-129)
     MeshFn newtonApply(const MeshFn &f, double t, double scale, const MeshFn &v) const {
         MeshFn y(np_, deg_, nc_);
+        y.eq = h_;
         check(cpg_newton_apply(handle(), f.data(), t, scale, v.data(), y.data(), nullptr));
         check(cpg_stream_sync(nullptr));
         return y;
@@ -306,6 +341,7 @@ class Equation {
         std::vector<double> p(8, 0.0);
         for (size_t j = 0; j < par.size() && j < 8; j++) p[j] = par[j];
         MeshFn u(np_, deg_, nc_);
+        u.eq = h_;
         check(cpg_interp_field(handle(), field, p.data(), t, u.data(), nullptr));
         check(cpg_stream_sync(nullptr));
         return u;
@@ -365,6 +401,7 @@ class MassMatrix {
     explicit MassMatrix(const Equation &E) : E_(&E) {}
     MeshFn matvec(const MeshFn &fn) const {
         MeshFn y(fn.np, fn.deg, fn.nc);
+        y.eq = fn.eq;
         check(cpg_mass_matvec(E_->handle(), fn.data(), y.data(), nullptr));
         check(cpg_stream_sync(nullptr));
         return y;
@@ -399,6 +436,7 @@ class TimeStepper {
     // TimeStepper::advance (src/TimeIntegration.h:16)
     virtual MeshFn advance(const MeshFn &u, double dt, double t = 0) {
         MeshFn out(u.np, u.deg, u.nc);
+        out.eq = u.eq;
         check(cpg_stepper_advance(h_.get(), u.data(), dt, t, out.data(), &last_stats, nullptr));
         return out;
     }

@@ -194,14 +194,37 @@ int main() {
     o_l2_error_values(&om, deg, 1, go.data(), fq.data(), &erro);
     const double e_l2 = std::abs(err - erro) / erro;
 
+    // MeshFn arithmetic and MeshFn::L2Norm: (2 g + b - g) / 2 - g / 2 = b / 2, and the norm of
+    // g carries its equation; a copy keeps its value when the original is updated in place
+    MeshFn h = (2.0 * g + b - g) / 2.0 - g * 0.5;
+    MeshFn gc = g;
+    g += b;
+    std::vector<double> hw = b.a->download(), gw = g.a->download(), gcw = gc.a->download();
+    for (auto &v : hw) v *= 0.5;
+    const double e_arith = rel(h.a->download(), hw) + rel(gcw, go);
+    std::vector<double> gsum = go;
+    for (size_t e = 0; e < gsum.size(); e++) gsum[e] += b.a->download()[e];
+    const double e_inplace = rel(gw, gsum);
+    // the L2 norm of the projected bump: sqrt of sum_i c_i^T M_i c_i with the oracle's mass blocks
+    std::vector<double> Mg((size_t)np * bs * bs);
+    o_mass(&om, deg, 1, Mg.data());
+    double nrm = 0.0;
+    for (int i = 0; i < np; i++)
+        for (int r = 0; r < bs; r++)
+            for (int c = 0; c < bs; c++)
+                nrm += go[(size_t)i * bs + r] * Mg[(size_t)i * bs * bs + (size_t)c * bs + r] * go[(size_t)i * bs + c];
+    const double e_norm = std::abs(gc.L2Norm()[0] - std::sqrt(nrm)) / std::sqrt(nrm);
+
     std::printf("residual %.3e jacobian %.3e gmres(it %d vs %d) %.3e dirk3 %.3e (newton %d vs %d) interp %.3e "
                 "l2error %.3e irk2 %.3e irk2-w %.3e rk2 %.3e forward-euler %.3e jacobi(it %d vs %d) %.3e\n", e_res, e_jac,
                 J.last_iterations, it, e_sol, e_dirk, dirk.last_stats.newton_iters, ost.newton_iters, e_interp, e_l2,
                 e_irk, e_irkw, e_rk2, e_fe, jit, jito, e_jacobi);
+    std::printf("meshfn arithmetic %.3e in-place %.3e l2norm %.3e\n", e_arith, e_inplace, e_norm);
     const bool ok = e_res < 1e-12 && e_jac < 1e-12 && e_sol < 1e-9 && e_dirk < 1e-10 &&
                     std::abs(J.last_iterations - it) <= 1 && e_interp < 1e-12 && e_l2 < 1e-9 && e_irk < 1e-10 &&
                     e_irkw < 1e-10 && e_rk2 < 1e-12 && e_fe < 1e-12 &&
-                    jit == jito && jit < 1000 && e_jacobi < 1e-9;
+                    jit == jito && jit < 1000 && e_jacobi < 1e-9 &&
+                    e_arith < 1e-12 && e_inplace < 1e-14 && e_norm < 1e-12;
     std::printf(ok ? "PASS\n" : "FAIL\n");
     return ok ? 0 : 1;
 }
