@@ -438,8 +438,8 @@ def main():
         bench_batch(args)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is described in "
-              f"DESIGN.md and not measured.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the path is measured on a single GPU "
+              f"and has no multi-GPU run.", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
