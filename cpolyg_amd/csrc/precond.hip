@@ -1294,6 +1294,32 @@ __device__ inline void coupled_stage(double *Jl, const CoupledSrc &src, int q) {
         });
 }
 
+// the three stage blocks of slot q into registers (CSU double2 per thread), and from there to Jl:
+// the loads of a stage set issued before a product phase and stored after it
+constexpr int CSU = (3 * CSB * CSB / 2 + FWG - 1) / FWG;
+__device__ inline void coupled_stage_load(const CoupledSrc &src, int q, double2 (&v)[CSU]) {
+    constexpr int SB = CSB * CSB, NU = 3 * SB / 2;
+#pragma unroll
+    for (int j = 0; j < CSU; j++) {
+        const int u = min((int)threadIdx.x + FWG * j, NU - 1);
+        const int st = (2 * u) / SB, rem = 2 * u - st * SB;
+        const double *J = st == 0 ? src.J0 : st == 1 ? src.J1 : src.J2;
+        v[j] = *reinterpret_cast<const double2 *>(J + (size_t)q * SB + rem);
+    }
+}
+__device__ inline void coupled_stage_store(double *Jl, const double2 (&v)[CSU]) {
+    constexpr int SB = CSB * CSB, NU = 3 * SB / 2;
+#pragma unroll
+    for (int j = 0; j < CSU; j++) {
+        const int u = (int)threadIdx.x + FWG * j;
+        if (u < NU) {
+            const int st = (2 * u) / SB, rem = 2 * u - st * SB, c = rem / CSB, r = rem - c * CSB;
+            Jl[st * CSTAGE + c * CLD + r] = v[j].x;
+            Jl[st * CSTAGE + c * CLD + r + 1] = v[j].y;
+        }
+    }
+}
+
 // The coupled block factors as C_q = BD(J_q) (coef x I): BD the block diagonal of the three
 // stage blocks (row stripe s of C_q holds J_s), (coef x I) the 3 x 3 stage mixing.  Products with
 // BD need only the k-steps of a tile's own stage block(s): 8 x 8 output tiles of 16 over the
@@ -1419,12 +1445,14 @@ __global__ __launch_bounds__(FWG) void k_cdilu_schur(RowLists L, CoupledSrc src,
         __syncthreads();
         coupled_mix_rows(X, src);   // W = (coef x I) D_k^-1
         __syncthreads();
+        double2 nxt[CSU];
+        coupled_stage_load(src, st, nxt);   // J_ki's stage blocks, in flight during the product
         dx4 acc[16];
         bd_times_x(Jl, X, acc);     // L_ik = C_ik D_k^-1 = BD(J_ik) W
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < 16; j++) tile_to_lds<CBL>(X, j >> 1, wv + 4 * (j & 1), acc[j]);
-        coupled_stage(Jl, src, st);
+        coupled_stage_store(Jl, nxt);
         __syncthreads();
         wg_lds_block_store<T, CBL>(X, F + (size_t)L.pos[sl] * CBB, scale ? scale + L.pos[sl] : nullptr, red);
         x_times_bd(X, Jl, accY);    // accY += L_ik BD(J_ki)
