@@ -1437,16 +1437,19 @@ __global__ __launch_bounds__(FWG) void k_cdilu_schur(RowLists L, CoupledSrc src,
     dx4 accY[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) accY[j] = dx4{0.0, 0.0, 0.0, 0.0};
-    for (int q = L.lptr[i]; q < L.lptr[i + 1]; q++) {
-        const int sl = L.lslot[q], k = L.lcol[q], st = L.ltrans[q];
+    // the stage sets pass through registers: each is loaded one product phase before it is stored
+    double2 nxt[CSU];
+    const int q0 = L.lptr[i], q1 = L.lptr[i + 1];
+    if (q0 < q1) coupled_stage_load(src, L.lslot[q0], nxt);
+    for (int q = q0; q < q1; q++) {
+        const int k = L.lcol[q], st = L.ltrans[q];
         __syncthreads();   // X and Jl free
-        coupled_stage(Jl, src, sl);
+        coupled_stage_store(Jl, nxt);   // J_ik's stage blocks
         wg_block_to_lds<CBL>(X, DD64 + (size_t)k * CBB);
         __syncthreads();
         coupled_mix_rows(X, src);   // W = (coef x I) D_k^-1
         __syncthreads();
-        double2 nxt[CSU];
-        coupled_stage_load(src, st, nxt);   // J_ki's stage blocks, in flight during the product
+        coupled_stage_load(src, st, nxt);   // J_ki's, in flight during the product
         dx4 acc[16];
         bd_times_x(Jl, X, acc);     // L_ik = C_ik D_k^-1 = BD(J_ik) W
         __syncthreads();
@@ -1454,7 +1457,9 @@ __global__ __launch_bounds__(FWG) void k_cdilu_schur(RowLists L, CoupledSrc src,
         for (int j = 0; j < 16; j++) tile_to_lds<CBL>(X, j >> 1, wv + 4 * (j & 1), acc[j]);
         coupled_stage_store(Jl, nxt);
         __syncthreads();
+        const int sl = L.lslot[q];
         wg_lds_block_store<T, CBL>(X, F + (size_t)L.pos[sl] * CBB, scale ? scale + L.pos[sl] : nullptr, red);
+        if (q + 1 < q1) coupled_stage_load(src, L.lslot[q + 1], nxt);   // the next J_ik's
         x_times_bd(X, Jl, accY);    // accY += L_ik BD(J_ki)
     }
     __syncthreads();
