@@ -1505,12 +1505,17 @@ __global__ __launch_bounds__(FWG) void k_cdilu_upper(const int *__restrict__ uro
     __shared__ double red[4];
     const int wv = threadIdx.x >> 6;
     const int i = urows[blockIdx.x];
-    for (int q = uptr[i]; q < uptr[i + 1]; q++) {
+    // each slot's stage set passes through registers, loaded during the previous slot's product
+    double2 nxt[CSU];
+    const int q0 = uptr[i], q1 = uptr[i + 1];
+    if (q0 < q1) coupled_stage_load(src, uslot[q0], nxt);
+    for (int q = q0; q < q1; q++) {
         const int sl = uslot[q];
         __syncthreads();   // X, Jl and red free
+        coupled_stage_store(Jl, nxt);
         wg_block_to_lds<CBL>(X, DD64 + (size_t)i * CBB);
-        coupled_stage(Jl, src, sl);
         __syncthreads();
+        if (q + 1 < q1) coupled_stage_load(src, uslot[q + 1], nxt);
         dx4 acc[16];
 #pragma unroll
         for (int j = 0; j < 16; j++) acc[j] = dx4{0.0, 0.0, 0.0, 0.0};
