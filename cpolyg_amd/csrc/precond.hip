@@ -1278,24 +1278,8 @@ __device__ inline double coupled_coef(const CoupledSrc &src, int s, int t) {
     return v;
 }
 
-// Jl <- the three stage blocks of slot q (column stride CLD), all threads
-__device__ inline void coupled_stage(double *Jl, const CoupledSrc &src, int q) {
-    constexpr int SB = CSB * CSB;
-    wg_batched<3 * SB / 2>(
-        [&](int u) {
-            const int st = (2 * u) / SB, rem = 2 * u - st * SB;
-            const double *J = st == 0 ? src.J0 : st == 1 ? src.J1 : src.J2;
-            return *reinterpret_cast<const double2 *>(J + (size_t)q * SB + rem);
-        },
-        [&](int u, double2 a) {
-            const int st = (2 * u) / SB, rem = 2 * u - st * SB, c = rem / CSB, r = rem - c * CSB;
-            Jl[st * CSTAGE + c * CLD + r] = a.x;
-            Jl[st * CSTAGE + c * CLD + r + 1] = a.y;
-        });
-}
-
-// the three stage blocks of slot q into registers (CSU double2 per thread), and from there to Jl:
-// the loads of a stage set issued before a product phase and stored after it
+// the three stage blocks of slot q into registers (CSU double2 per thread), and from there to Jl
+// (column stride CLD): the loads of a stage set are issued before a product phase and stored after it
 constexpr int CSU = (3 * CSB * CSB / 2 + FWG - 1) / FWG;
 __device__ inline void coupled_stage_load(const CoupledSrc &src, int q, double2 (&v)[CSU]) {
     constexpr int SB = CSB * CSB, NU = 3 * SB / 2;
