@@ -4,10 +4,11 @@ Workload (one "step" = one DIRK3 time step, i.e. three Newton solves on the devi
   isentropic vortex (strength 5, centre (5,5), free stream (1,1), gamma 1.4) on [0,10]^2,
   periodic; Voronoi mesh from a 256x256 lattice with uniform jitter 0.3h (seed 3),
   multicolour-ordered; Legendre p=3 (40x40 blocks); DIRK3 at CFL 20
-  (dt = 20 h / ((2p+1) max(|v|+c)), h = 10/256); Newton to 5e-13 on the L2 residual norm;
-  flexible GMRES(50) to 1e-10 with block DILU; exact periodic traces and exact Euler
-  off-diagonal Jacobian blocks (the reference's truncated periodic trace is a parity option:
-  on this workload it drives the third DIRK stage of the fourth step to non-physical states).
+  (dt = 20 h / ((2p+1) max(|v|+c)), h = 10/256) with Alexander's L-stable SDIRK3 tableau (the
+  reference's 3-stage tableau, --dirk3 reference, is not A-stable: |R(-inf)| = 1.78, and at
+  CFL 20 it amplifies the under-resolved modes every step until the state is non-physical);
+  Newton to 5e-13 on the L2 residual norm; flexible GMRES(50) to 1e-10 with block DILU; exact
+  periodic traces and exact Euler off-diagonal Jacobian blocks.
 Solver configuration of the headline: the Newton operator M - dt a J(u) is applied at the
 quadrature points (cpg_newton_prepare once per Newton matrix, cpg_newton_apply_prepared per
 GMRES iteration; equal to the assembled matrix's product to rounding), and the assembled matrix
@@ -17,8 +18,8 @@ of the reference policy; the same number of Newton and GMRES iterations results.
 The headline value is the mixed-precision path (by default DILU factors stored in BF16 with one
 power-of-two scale per block and applied with FP32 partial sums, inside FP64 flexible GMRES); the
 FP64 path (FP64 factors, same solver) is timed in the same run and reported beside it, with the
-difference between the two solutions, and both are timed again under the reference refresh
-policy with the matrix-free and with the assembled (BSR matvec) operator.
+difference between the two solutions after every timed step, and both are timed again under the
+reference refresh policy with the matrix-free and with the assembled (BSR matvec) operator.
 
 Prints ONE JSON line.  Only one GPU is used: --gpus above 1 exits with a message.
 """
@@ -35,6 +36,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "DIRK steps/s, Euler p=3 65k cells: mixed-prec vs FP64 on GPU at 1e-10 resid"
 HBM_PEAK_GBS = 8000.0
+# diagonal coefficient a_ss of the two 3-stage DIRK tableaux (the Newton matrix is M - dt a_ss J)
+A_SS = {"reference": 0.5 * (1 + 0.435866521508459), "l-stable": 0.43586652150845899942}
 # HBM bytes per BSR-matvec launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
 # kernel (bench.py --workload pmc), FETCH scaled x2 and WRITE x1 by a 1 GiB copy of known size;
 # keyed by (cells, degree).  profiles/bsr_matvec_euler_p3_65k_pmc_traffic.txt has the passes.
@@ -51,6 +54,8 @@ def parse():
     p.add_argument("--nx", type=int, default=256, help="lattice size (cells = nx^2)")
     p.add_argument("--deg", type=int, default=3)
     p.add_argument("--cfl", type=float, default=20.0)
+    p.add_argument("--dirk3", choices=["l-stable", "reference"], default="l-stable",
+                   help="3-stage DIRK tableau: Alexander's L-stable SDIRK3, or the reference's (not A-stable)")
     p.add_argument("--restart", type=int, default=50)
     p.add_argument("--mixed", choices=["fp32", "bf16", "inner-fp32", "inner-bf16"], default="bf16",
                    help="fp32/bf16: DILU factors stored in that precision inside FP64 FGMRES; inner-*: FP32 "
@@ -102,10 +107,11 @@ def algorithmic_bytes(n, nnzb, bl, elem, scales):
     return matvec, apply
 
 
-def run_path(E, u0, dt, steps, warmup, solver):
+def run_path(E, u0, dt, steps, warmup, solver, keep=False, label=""):
     """Warmup, then `steps` timed DIRK3 steps (the production path: captured GMRES cycles),
     then one more step with HIP events around every operator and preconditioner application
-    (eager launches; not part of the timed region's wall time)."""
+    (eager launches; not part of the timed region's wall time).  keep: a device copy of the
+    state after every timed step (one 21 MB device-to-device copy per step at 65,536 cells)."""
     import torch
     from cpolyg_amd import hip
 
@@ -121,18 +127,26 @@ def run_path(E, u0, dt, steps, warmup, solver):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     stats = []
+    states = []
     for _ in range(steps):
         st = S.advance(u, dt, t, nxt, stream)
         stats.append(st.as_dict())
         u, nxt = nxt, u
+        if keep:
+            states.append(u.clone())
         t += dt
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    print(f"# {label}: {1e3 * wall / max(steps, 1):.1f} ms/step, "
+          f"{sum(x['gmres_iters'] for x in stats) / max(steps, 1):.1f} GMRES iterations/step", file=sys.stderr,
+          flush=True)
     S.enable_kernel_timing(True)
     S.advance(u, dt, t, nxt, stream)
     kt = S.kernel_timing()
     S.enable_kernel_timing(False)
     torch.cuda.synchronize()
+    if keep:
+        stats[-1]["states"] = states
     return u.cpu().numpy(), wall, stats, kt, u, t
 
 
@@ -176,7 +190,7 @@ def standalone_rates(E, u, t, scale, reps=20):
     return out
 
 
-def cpu_baseline(nx, deg, cfl, counts, n_target):
+def cpu_baseline(nx, deg, cfl, counts, n_target, a_ss):
     """Oracle (single-threaded C port of the reference algorithm) timed per phase on a
     bounded sample of the same workload at nx^2 cells, scaled linearly to n_target cells
     and composed with the per-step phase counts of the GPU run."""
@@ -190,7 +204,7 @@ def cpu_baseline(nx, deg, cfl, counts, n_target):
     orc = O.Oracle(mesh)
     spec = EquationSpec.euler(periodic_trace=counts["trace"], jacobian=counts["jacobian"])
     dt = setups.dg_time_step(cfl, 10.0 / nx, deg, setups.euler_max_speed(u, deg))
-    a = 0.5 * (1 + 0.435866521508459)
+    a = a_ss
     t0 = time.perf_counter()
     b = orc.assemble(spec, deg, u, 0.0)
     t_res = time.perf_counter() - t0
@@ -403,7 +417,7 @@ def pmc_workload(args):
     J = E.jacobian_matrix()
     u = torch.from_numpy(u0).cuda()
     dt = setups.dg_time_step(args.cfl, 10.0 / args.nx, args.deg, setups.euler_max_speed(u0, args.deg))
-    E.jacobian(u, 0.0, J, scale=-dt * 0.5 * (1 + 0.435866521508459), add_mass=True)
+    E.jacobian(u, 0.0, J, scale=-dt * A_SS[args.dirk3], add_mass=True)
     P = hip.Prec(J, PC_DILU, PREC_BF16)
     P.factor(J)
     x = torch.randn(E.n * E.bl, dtype=torch.float64, device="cuda")
@@ -445,16 +459,21 @@ def main():
     import torch
 
     from cpolyg_amd import hip, setups
-    from cpolyg_amd.spec import (EquationSpec, GMRES_FLEXIBLE, OP_ASSEMBLED, OP_MATRIX_FREE, PC_DILU, PC_INNER_GMRES,
-                                 PREC_BF16, PREC_FP32, PREC_FP64, SolverSpec)
+    from cpolyg_amd.spec import (DIRK3_L_STABLE, DIRK3_REFERENCE, EquationSpec, GMRES_FLEXIBLE, OP_ASSEMBLED,
+                                 OP_MATRIX_FREE, PC_DILU, PC_INNER_GMRES, PREC_BF16, PREC_FP32, PREC_FP64, SolverSpec)
 
     deg = args.deg
+    t_setup = time.perf_counter()
     mesh, u0 = workload(args.nx, deg)
     n = mesh.n_cells
+    print(f"# mesh {n} cells, {mesh.n_colours} colours, initial state: {time.perf_counter() - t_setup:.1f} s",
+          file=sys.stderr, flush=True)
     dt = setups.dg_time_step(args.cfl, 10.0 / args.nx, deg, setups.euler_max_speed(u0, deg))
     spec = EquationSpec.euler(periodic_trace=args.trace, jacobian=args.jacobian)
     E = hip.Equation(hip.Mesh(mesh), spec, deg)
     bl, nnzb = E.bl, E.nnzb
+
+    dirk3 = DIRK3_L_STABLE if args.dirk3 == "l-stable" else DIRK3_REFERENCE
 
     def solver(prec, pc=PC_DILU, op=None, refresh=None):
         # each stage's Newton iteration starts from the previous stage's increment and halves an
@@ -462,7 +481,7 @@ def main():
         # first update on the third stage can overshoot to negative pressure)
         return SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=args.restart, rtol=1e-10, maxit=20000,
                           pc_kind=pc, newton_tol=5e-13, newton_maxit=100, precision=prec, inner_iters=args.inner,
-                          newton_guess=1, newton_backtrack=1,
+                          newton_guess=1, newton_backtrack=1, dirk_tableau=dirk3,
                           pc_refresh=args.pc_refresh if refresh is None else refresh,
                           newton_operator=(OP_MATRIX_FREE if args.operator == "matrix-free" else OP_ASSEMBLED)
                           if op is None else op)
@@ -470,15 +489,24 @@ def main():
     mixed_prec = PREC_BF16 if args.mixed.endswith("bf16") else PREC_FP32
     mixed_pc = PC_INNER_GMRES if args.mixed.startswith("inner") else PC_DILU
     steps = args.steps
-    a_ss = 0.5 * (1 + 0.435866521508459)
-    u_mix, wall_mix, st_mix, kt, u_last, t_last = run_path(E, u0, dt, steps, args.warmup, solver(mixed_prec, mixed_pc))
+    a_ss = A_SS[args.dirk3]
+    u_mix, wall_mix, st_mix, kt, u_last, t_last = run_path(E, u0, dt, steps, args.warmup,
+                                                           solver(mixed_prec, mixed_pc), keep=not args.no_fp64,
+                                                           label=f"mixed {args.mixed}")
     res = {}
     if not args.no_fp64:
-        u64, wall64, st64, kt64, _, _ = run_path(E, u0, dt, steps, args.warmup, solver(PREC_FP64))
+        u64, wall64, st64, kt64, _, _ = run_path(E, u0, dt, steps, args.warmup, solver(PREC_FP64), keep=True,
+                                              label="fp64")
         res["fp64_steps_per_s"] = steps / wall64
         res["fp64_ms_per_step"] = 1e3 * wall64 / steps
         res["speedup_mixed_vs_fp64"] = wall64 / wall_mix
         res["mixed_vs_fp64_max_rel_diff"] = float(np.max(np.abs(u_mix - u64)) / np.max(np.abs(u64)))
+        # the mixed-precision solution against the FP64 path's after every timed step (both
+        # start from the same state and take the same dt): max |u_mixed - u_fp64| / max |u_fp64|
+        sm, s64 = st_mix[-1].pop("states"), st64[-1].pop("states")
+        res["mixed_vs_fp64_max_rel_diff_per_step"] = [float((a - b).abs().max() / b.abs().max())
+                                                      for a, b in zip(sm, s64)]
+        del sm, s64
         res["fp64_gmres_iters_per_step"] = sum(s["gmres_iters"] for s in st64) / steps
         res["fp64_newton_iters_per_step"] = sum(s["newton_iters"] for s in st64) / steps
         res["fp64_phase_ms_per_step"] = {k: sum(s[k] for s in st64) / steps for k in
@@ -490,14 +518,20 @@ def main():
             if key.endswith("matrix_free") and args.pc_refresh == 0 and args.operator == "matrix-free":
                 continue
             cmp_ = {"operator": "matrix-free" if op == OP_MATRIX_FREE else "assembled", "pc_refresh": 0}
-            ua, wa, sta, _, _, _ = run_path(E, u0, dt, steps, args.warmup, solver(mixed_prec, mixed_pc, op, 0))
-            cmp_["mixed_ms_per_step"] = 1e3 * wa / steps
-            cmp_["mixed_max_rel_diff_vs_headline"] = float(np.max(np.abs(ua - u_mix)) / np.max(np.abs(u_mix)))
-            cmp_["gmres_iters_per_step"] = sum(x["gmres_iters"] for x in sta) / steps
-            if not args.no_fp64:
-                _, wa64, _, _, _, _ = run_path(E, u0, dt, steps, args.warmup, solver(PREC_FP64, PC_DILU, op, 0))
-                cmp_["fp64_ms_per_step"] = 1e3 * wa64 / steps
-                cmp_["speedup_mixed_vs_fp64"] = wa64 / wa
+            try:
+                ua, wa, sta, _, _, _ = run_path(E, u0, dt, steps, args.warmup, solver(mixed_prec, mixed_pc, op, 0),
+                                                label=f"{key} mixed")
+                cmp_["mixed_ms_per_step"] = 1e3 * wa / steps
+                cmp_["mixed_max_rel_diff_vs_headline"] = float(np.max(np.abs(ua - u_mix)) / np.max(np.abs(u_mix)))
+                cmp_["gmres_iters_per_step"] = sum(x["gmres_iters"] for x in sta) / steps
+                if not args.no_fp64:
+                    _, wa64, _, _, _, _ = run_path(E, u0, dt, steps, args.warmup, solver(PREC_FP64, PC_DILU, op, 0),
+                                                   label=f"{key} fp64")
+                    cmp_["fp64_ms_per_step"] = 1e3 * wa64 / steps
+                    cmp_["speedup_mixed_vs_fp64"] = wa64 / wa
+            except hip.CpgError as e:   # a comparison leg that fails is reported, not fatal
+                cmp_["error"] = str(e)
+                print(f"# {key}: {e}", file=sys.stderr, flush=True)
             res[key] = cmp_
     # per-launch rates on the last Newton matrix of the mixed run (stage-3 scale)
     rates = standalone_rates(E, u_last, t_last, -dt * a_ss)
@@ -543,10 +577,12 @@ def main():
         "dtype": f"f64 outer / {args.mixed} preconditioner factors",
         "data": "synthetic",
         "config": {"workload": f"Euler isentropic vortex, periodic Voronoi {n} cells ({args.nx}^2 jittered "
-                               f"lattice), p={deg} ({bl}x{bl} blocks), DIRK3 CFL {args.cfl:g}, FGMRES({args.restart}) "
+                               f"lattice), p={deg} ({bl}x{bl} blocks), DIRK3 "
+                               f"({'Alexander L-stable SDIRK3' if args.dirk3 == 'l-stable' else 'reference tableau'}) "
+                               f"CFL {args.cfl:g}, FGMRES({args.restart}) "
                                f"rtol 1e-10, block DILU, {args.operator} Newton operator, preconditioner refreshed "
                                f"{['every Newton iteration', 'once per stage', 'once per step', 'kept across steps'][args.pc_refresh]}",
-                   "cells": n, "deg": deg, "pc_refresh": args.pc_refresh,
+                   "cells": n, "deg": deg, "pc_refresh": args.pc_refresh, "dirk3_tableau": args.dirk3,
                    "block": bl, "nnzb": nnzb, "dt": dt, "cfl": args.cfl, "mixed": args.mixed,
                    "operator": args.operator, "jacobian": "exact" if args.jacobian else "reference",
                    "periodic_trace": "exact" if args.trace else "reference", "colours": mesh.n_colours},
@@ -571,7 +607,7 @@ def main():
     if not args.no_cpu:
         counts = {"residuals": newton + 3, "newton": newton, "gmres": gm, "trace": args.trace,
                   "jacobian": args.jacobian}
-        out["cpu_baseline"] = cpu_baseline(args.cpu_nx, deg, args.cfl, counts, n)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_nx, deg, args.cfl, counts, n, a_ss)
     print(json.dumps(out))
 
 

@@ -34,7 +34,7 @@ void eq_newton_resid_cells(const cpg_equation_s &E, const double *kv, const doub
                            double *r, hipStream_t s);
 const double *eq_normc(const cpg_equation_s &E);
 int equation_bs(const cpg_equation_s &E);
-void tableau(int method, int ns, double *A, double *b, double *c);
+void tableau(int method, int ns, double *A, double *b, double *c, int dirk3);
 
 constexpr int BWG = 256;
 constexpr int BPT = 8;
@@ -589,7 +589,7 @@ int cpg_batch_create(int nclass, const cpg_equation *eqs, const int *ninst, cons
             B->sp = *params;
             B->ns = nstages;
             B->m = params->restart;
-            tableau(CPG_STEP_DIRK, nstages, B->A, B->b, B->c);
+            tableau(CPG_STEP_DIRK, nstages, B->A, B->b, B->c, params->dirk_tableau);
             B->use_graph = use_graph != 0;
             CPG_HIP(hipStreamCreateWithFlags(&B->s, hipStreamNonBlocking));
             size_t off = 0;

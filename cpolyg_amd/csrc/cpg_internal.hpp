@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -95,10 +96,20 @@ inline hipStream_t S(void *stream) { return reinterpret_cast<hipStream_t>(stream
 
 inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 
+// Process-wide counter.  A handle takes a new value when it is created and whenever device
+// buffers that captured graphs point into are reallocated, so a graph cache keyed on handle
+// addresses tells a handle re-created at a reused address (or a reallocated one) from the one
+// whose buffers it captured.
+inline uint64_t next_generation() {
+    static std::atomic<uint64_t> g{0};
+    return ++g;
+}
+
 }  // namespace cpg
 
 // ---- handle types ------------------------------------------------------------------
 struct cpg_bsr_s {
+    uint64_t gen = cpg::next_generation();
     int n = 0, bl = 0, nnzb = 0;
     int nbmax = 1;   // longest block row
     std::vector<int> h_rowptr, h_colind;
@@ -117,6 +128,8 @@ void set_stream_wpr(int elem_bytes, int bl, int wpr);
 // out = M^-1 in for a factored preconditioner (in and out distinct); `active` as for bsr_matvec
 // (DILU and block-Jacobi sweeps only)
 void prec_apply_to(cpg_prec P, const double *in, double *out, hipStream_t s, const int *active = nullptr);
+// the preconditioner's generation (next_generation): changes when its buffers are reallocated
+uint64_t prec_generation(cpg_prec P);
 
 // y = A x for the Krylov solvers; `active` as for bsr_matvec
 struct LinOp {

@@ -282,8 +282,9 @@ struct cpg_krylov_s {
         const double *b = nullptr;
         double *x = nullptr;
         const void *fn = nullptr, *ctx = nullptr, *P = nullptr;
+        uint64_t agen = 0, pgen = 0;   // generations of a BSR operator and of P (0 for other kinds)
         bool operator==(const Sig &o) const {
-            return b == o.b && x == o.x && fn == o.fn && ctx == o.ctx && P == o.P;
+            return b == o.b && x == o.x && fn == o.fn && ctx == o.ctx && P == o.P && agen == o.agen && pgen == o.pgen;
         }
     } sig;
     hipGraphExec_t g_start0 = nullptr, g_start = nullptr, g_finish = nullptr;
@@ -603,8 +604,12 @@ void gmres_flexible(cpg_krylov_s *K, const LinOp &A, cpg_prec P, const PcOp *M, 
     hipStream_t s = K->kst;
     KOps op{*K, A, P, s, M};
     const int m = K->m;
+    // the graphs hold device pointers of the operator and the preconditioner: they are replayed
+    // only for the same handles with the same buffers
     const cpg_krylov_s::Sig sig{b, x, reinterpret_cast<const void *>(A.fn), A.ctx,
-                                M ? reinterpret_cast<const void *>(M->ctx) : P};
+                                M ? reinterpret_cast<const void *>(M->ctx) : P,
+                                A.fn == bsr_linop ? static_cast<const cpg_bsr_s *>(A.ctx)->gen : 0,
+                                M ? 0 : prec_generation(P)};
     if (!(sig == K->sig)) {
         K->drop_graphs();
         K->sig = sig;

@@ -1520,6 +1520,7 @@ using namespace cpg;
 
 // ---- handle --------------------------------------------------------------------------
 struct cpg_prec_s {
+    uint64_t gen = cpg::next_generation();   // renewed by cpg_prec_set_inner, which reallocates
     int kind = CPG_PC_NONE, precision = CPG_FP64;
     int n = 0, bl = 0, nnzb = 0;
     // pattern-derived lists
@@ -1568,6 +1569,10 @@ struct cpg_prec_s {
         return L;
     }
 };
+
+namespace cpg {
+uint64_t prec_generation(cpg_prec P) { return P ? P->gen : 0; }
+}  // namespace cpg
 
 namespace {
 
@@ -2155,6 +2160,7 @@ int cpg_prec_set_inner(cpg_prec P, int iterations) {
         CPG_CHECK(P->kind == CPG_PC_INNER_GMRES, CPG_ERR_ARG, "not an inner-GMRES preconditioner");
         const int k = iterations;
         P->inner_k = k;
+        P->gen = next_generation();
         P->Q.alloc(P->N * (k + 1));
         P->ipart.alloc((size_t)P->nchunk * (k + 1));
         P->H.alloc((size_t)(k + 1) * k);

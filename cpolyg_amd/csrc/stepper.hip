@@ -243,7 +243,7 @@ __global__ void k_irk_scatter(int ns, int si, int nnzb, int bl, int bs, const in
     big[dst] = v;
 }
 
-void tableau(int method, int ns, double *A, double *b, double *c) {
+void tableau(int method, int ns, double *A, double *b, double *c, int dirk3) {
     std::memset(A, 0, sizeof(double) * 25);
     if (method == CPG_STEP_IRK) {   // Radau IIA (src/TimeIntegration.cpp:153-186)
         if (ns == 2) {
@@ -271,7 +271,21 @@ void tableau(int method, int ns, double *A, double *b, double *c) {
             b[0] = 1 - g; b[1] = g; c[0] = g; c[1] = 1;
             return;
         }
+        if (ns == 3 && dirk3 == CPG_DIRK3_L_STABLE) {
+            // Alexander's L-stable SDIRK3 (SIAM J. Numer. Anal. 14 (1977) 1006-1021), not in the
+            // reference: g is the root in (1/6, 1/2) of x^3 - 3x^2 + 3x/2 - 1/6; the last row of A is b
+            // (stiffly accurate), so R(-inf) = 0 and |R(iy)| <= 1
+            const double g = 0.43586652150845899942;
+            const double b1 = -(6 * g * g - 16 * g + 1) / 4.0, b2 = (6 * g * g - 20 * g + 5) / 4.0;
+            const double a[9] = {g, 0, 0, (1 - g) / 2.0, g, 0, b1, b2, g};
+            std::memcpy(A, a, sizeof a);
+            b[0] = b1; b[1] = b2; b[2] = g;
+            c[0] = g; c[1] = (1 + g) / 2.0; c[2] = 1;
+            return;
+        }
         if (ns == 3) {
+            // the reference's tableau (src/TimeIntegration.cpp:71-77): Crouzeix's form with
+            // Alexander's gamma as alpha, second order and not A-stable (R(-inf) = -1.78)
             const double al = 0.435866521508459;
             const double a[9] = {(1 + al) / 2.0, 0, 0, -al / 2.0, (1 + al) / 2.0, 0, 1 + al, -(1 + 2 * al), (1 + al) / 2.0};
             std::memcpy(A, a, sizeof a);
@@ -407,6 +421,21 @@ int check(int st) {
     return st;
 }
 
+// The state a step returns is checked before the step reports success.  The residual at u_{n+1}
+// evaluates the state at every quadrature point and the Lax-Friedrichs sound speed at every edge
+// point, so a NaN or infinity anywhere, or a state whose sound speed is not real, makes its norm
+// non-finite, and the step that produced it fails instead of the one after it.
+void check_state(cpg_stepper_s &S, const double *unp1, double t_end, hipStream_t s, cpg_step_stats &st) {
+    const auto t0 = clk::now();
+    eq_residual(*S.E, unp1, t_end, S.rhs.p, s);
+    const double rn = eq_newton_resid(*S.E, unp1, S.rhs.p, 1.0, S.r.p, nullptr, s);
+    const double ms = ms_since(t0);
+    st.ms_residual += ms;
+    st.ms_total += ms;
+    CPG_CHECK(std::isfinite(rn), CPG_ERR_NONFINITE,
+              "the step to t = %.17g returned a non-physical state (the residual at u_{n+1} is not finite)", t_end);
+}
+
 // re-raise a failed library call with the stage / Newton iteration it happened in
 void check_ctx(int st, int stage, int it, double resid) {
     if (!st) return;
@@ -455,7 +484,9 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
             S->method = method;
             S->ns = explicit_method(method) ? explicit_stages(method) : nstages;
             S->sp = *params;
-            tableau(method == CPG_STEP_IRK_W ? CPG_STEP_IRK : method, S->ns, S->A, S->b, S->c);
+            CPG_CHECK(params->dirk_tableau == CPG_DIRK3_REFERENCE || params->dirk_tableau == CPG_DIRK3_L_STABLE,
+                      CPG_ERR_ARG, "bad dirk_tableau %d", params->dirk_tableau);
+            tableau(method == CPG_STEP_IRK_W ? CPG_STEP_IRK : method, S->ns, S->A, S->b, S->c, params->dirk_tableau);
             int n, bl, nnzb;
             check(cpg_equation_info(E, &n, &bl, &nnzb));
             S->n = n;
@@ -737,6 +768,7 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
             lincomb(*S, unp1, u, S->b, S->ns, k, s);
             CPG_HIP(hipStreamSynchronize(s));
             st.ms_total = ms_since(t_all);
+            check_state(*S, unp1, t + dt, s, st);
             return;
         }
         if (S->method == CPG_STEP_IRK_W) {
@@ -819,6 +851,7 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                           S->sp.newton_maxit, st.last_resid);
                 throw Error{CPG_ERR_NOT_CONVERGED};
             }
+            check_state(*S, unp1, t + dt, s, st);
             end_step(*S, st.gmres_iters);
             return;
         }
@@ -925,6 +958,7 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                           S->sp.newton_maxit, st.last_resid);
                 throw Error{CPG_ERR_NOT_CONVERGED};
             }
+            check_state(*S, unp1, t + dt, s, st);
             end_step(*S, st.gmres_iters);
             return;
         }
@@ -990,6 +1024,7 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                           S->sp.newton_maxit, st.last_resid);
                 throw Error{CPG_ERR_NOT_CONVERGED};
             }
+            check_state(*S, unp1, t + dt, s, st);
             end_step(*S, st.gmres_iters);
             return;
         }
@@ -1073,6 +1108,7 @@ int cpg_stepper_advance(cpg_stepper S, const double *u, double dt, double t, dou
                       st.last_resid);
             throw Error{CPG_ERR_NOT_CONVERGED};
         }
+        check_state(*S, unp1, t + dt, s, st);
         end_step(*S, st.gmres_iters);
     });
     st.status = ret;

@@ -41,7 +41,8 @@ class SolverParams(C.Structure):
     _fields_ = [("gmres_mode", C.c_int), ("restart", C.c_int), ("rtol", C.c_double), ("maxit", C.c_int),
                 ("pc_kind", C.c_int), ("precision", C.c_int), ("newton_tol", C.c_double),
                 ("newton_maxit", C.c_int), ("inner_iters", C.c_int), ("newton_guess", C.c_int),
-                ("newton_backtrack", C.c_int), ("newton_operator", C.c_int), ("pc_refresh", C.c_int)]
+                ("newton_backtrack", C.c_int), ("newton_operator", C.c_int), ("pc_refresh", C.c_int),
+                ("dirk_tableau", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -255,7 +256,7 @@ def eq_params(s: _spec.EquationSpec) -> EqParams:
 def solver_params(s: _spec.SolverSpec) -> SolverParams:
     return SolverParams(s.gmres_mode, s.restart, s.rtol, s.maxit, s.pc_kind, s.precision, s.newton_tol,
                         s.newton_maxit, s.inner_iters, s.newton_guess, s.newton_backtrack, s.newton_operator,
-                        s.pc_refresh)
+                        s.pc_refresh, s.dirk_tableau)
 
 
 class Mesh:

@@ -29,6 +29,9 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16 = 2
 
+DIRK3_REFERENCE = 0  # the reference's 3-stage tableau (src/TimeIntegration.cpp:71-77; not A-stable)
+DIRK3_L_STABLE = 1   # Alexander's L-stable SDIRK3 (not in the reference)
+
 OP_ASSEMBLED = 0     # GMRES operator: BSR matvec of the assembled Newton matrix (reference)
 OP_MATRIX_FREE = 1   # the same matrix applied at the quadrature points (cpg_newton_apply)
 # analytic fields of cpg_interp_field / cpg_l2_error_field (parameters in include/cpolyg.h)
@@ -100,6 +103,7 @@ class SolverSpec:
     newton_backtrack: int = 0    # 1: halve updates that lead to a non-finite residual
     newton_operator: int = OP_ASSEMBLED
     pc_refresh: int = 0          # 1: with OP_MATRIX_FREE, refresh the preconditioner once per stage
+    dirk_tableau: int = DIRK3_REFERENCE   # 3-stage DIRK: reference tableau or Alexander's L-stable SDIRK3
 
 
 def bs_of(deg: int) -> int:

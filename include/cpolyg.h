@@ -235,6 +235,7 @@ enum { CPG_STEP_DIRK = 0, CPG_STEP_IRK = 1, CPG_STEP_RK4 = 2, CPG_STEP_IRK_W = 3
  * Newton matrix blocks A^-1[s][j] M (+ -dt J_s on the diagonal), preconditioned by the
  * stage-block-diagonal factors of A^-1[s][s] M - dt J_s (pc_kind DILU or block Jacobi, any
  * precision); the coupled matrix is applied matrix-free, so gmres_mode must be flexible. */
+enum { CPG_DIRK3_REFERENCE = 0, CPG_DIRK3_L_STABLE = 1 };
 typedef struct {
     int gmres_mode;         /* CPG_GMRES_* */
     int restart;
@@ -261,6 +262,10 @@ typedef struct {
                                first step after the last renewal, or when dt changes (the Newton iteration
                                itself is unchanged: only the preconditioner lags).  Radau IIA: 0 every
                                Newton iteration, 1 and 2 once per step, 3 as for DIRK */
+    int dirk_tableau;       /* DIRK with 3 stages: CPG_DIRK3_REFERENCE the reference's tableau
+                               (src/TimeIntegration.cpp:71-77: second order, not A-stable, R(-inf) = -1.78);
+                               CPG_DIRK3_L_STABLE Alexander's L-stable SDIRK3 (third order, R(-inf) = 0;
+                               no reference counterpart) */
 } cpg_solver_params;
 
 typedef struct {

@@ -33,7 +33,8 @@ class OEq(C.Structure):
 
 class OSolver(C.Structure):
     _fields_ = [("gmres_mode", C.c_int), ("restart", C.c_int), ("rtol", C.c_double), ("maxit", C.c_int),
-                ("pc_kind", C.c_int), ("newton_tol", C.c_double), ("newton_maxit", C.c_int)]
+                ("pc_kind", C.c_int), ("newton_tol", C.c_double), ("newton_maxit", C.c_int),
+                ("dirk_tableau", C.c_int)]
 
 
 class OStats(C.Structure):
@@ -194,7 +195,8 @@ class Oracle:
 
 
 def osolver(s) -> OSolver:
-    return OSolver(s.gmres_mode, s.restart, s.rtol, s.maxit, s.pc_kind, s.newton_tol, s.newton_maxit)
+    return OSolver(s.gmres_mode, s.restart, s.rtol, s.maxit, s.pc_kind, s.newton_tol, s.newton_maxit,
+                   getattr(s, "dirk_tableau", 0))
 
 
 # ---- block-sparse algebra ---------------------------------------------------------
@@ -313,9 +315,10 @@ def rule_tables(deg2):
     return tri, lin
 
 
-def dirk_tableau(ns):
+def dirk_tableau(ns, variant=0):
+    """variant (3 stages): 0 the reference's tableau, 1 Alexander's L-stable SDIRK3."""
     A, b, c = np.zeros(25), np.zeros(5), np.zeros(5)
-    lib().o_dirk_tableau(ns, _d(A), _d(b), _d(c))
+    lib().o_dirk_tableau_v(ns, variant, _d(A), _d(b), _d(c))
     return A[:ns * ns].reshape(ns, ns), b[:ns], c[:ns]
 
 

@@ -1228,6 +1228,26 @@ int o_jacobi_solve(int n, int bl, const int *rowptr, const int *colind, const do
 /* ---------------------------------------------------------------------------
  * Time stepping (src/TimeIntegration.cpp)
  */
+/* Alexander's L-stable 3-stage SDIRK (R. Alexander, SIAM J. Numer. Anal. 14 (1977) 1006-1021):
+ * gamma is the root in (1/6, 1/2) of x^3 - 3x^2 + 3x/2 - 1/6, c = (gamma, (1+gamma)/2, 1), and the
+ * last row of A is b (stiffly accurate), so R(-inf) = 0.  Not in the reference: the reference's
+ * 3-stage tableau (src/TimeIntegration.cpp:71-77) is not A-stable (|R(-inf)| = 1.78). */
+static void alexander_sdirk3(double *A, double *b, double *c)
+{
+    const double g = 0.43586652150845899942;
+    const double b1 = -(6 * g * g - 16 * g + 1) / 4.0, b2 = (6 * g * g - 20 * g + 5) / 4.0;
+    double a[9] = {g, 0, 0, (1 - g) / 2.0, g, 0, b1, b2, g};
+    memcpy(A, a, sizeof a);
+    b[0] = b1; b[1] = b2; b[2] = g;
+    c[0] = g; c[1] = (1 + g) / 2.0; c[2] = 1;
+}
+
+void o_dirk_tableau_v(int nstages, int variant, double *A, double *b, double *c)
+{
+    if (nstages == 3 && variant == O_DIRK3_L_STABLE) { alexander_sdirk3(A, b, c); return; }
+    o_dirk_tableau(nstages, A, b, c);
+}
+
 void o_dirk_tableau(int nstages, double *A, double *b, double *c)
 {
     if (nstages == 1) { A[0] = 1; b[0] = 1; c[0] = 1; return; }   /* backward Euler */
@@ -1320,7 +1340,7 @@ int o_dirk_step(const o_mesh *m, const o_eq *eq, int deg, int nstages, const dou
     int n = m->np, bs = (deg + 1) * (deg + 2) / 2, nc = o_nc(eq), bl = nc * bs;
     size_t N = (size_t)n * bl, bb2 = (size_t)bl * bl;
     double A[25], b[5], c[5];
-    o_dirk_tableau(nstages, A, b, c);
+    o_dirk_tableau_v(nstages, s->dirk_tableau, A, b, c);
     int *rowptr = malloc(sizeof(int) * (n + 1));
     int nnzb = o_jacobian_pattern(m, rowptr, NULL);
     int *colind = malloc(sizeof(int) * nnzb);

@@ -107,6 +107,7 @@ typedef struct {
     int pc_kind;            /* O_PC_* */
     double newton_tol;      /* absolute, on max_c L2Norm (kNewtonTolerance) */
     int newton_maxit;       /* kNewtonMaxIterations */
+    int dirk_tableau;       /* 3-stage DIRK: O_DIRK3_REFERENCE or O_DIRK3_L_STABLE */
 } o_solver;
 
 typedef struct {
@@ -116,7 +117,11 @@ typedef struct {
     double last_resid;
 } o_stats;
 
+/* 3-stage DIRK tableaux: the reference's (src/TimeIntegration.cpp:71-77) and Alexander's
+ * L-stable SDIRK3 (not in the reference) */
+enum { O_DIRK3_REFERENCE = 0, O_DIRK3_L_STABLE = 1 };
 void o_dirk_tableau(int nstages, double *A, double *b, double *c);
+void o_dirk_tableau_v(int nstages, int variant, double *A, double *b, double *c);
 void o_irk_tableau(int nstages, double *A, double *b, double *c);
 int o_dirk_step(const o_mesh *m, const o_eq *eq, int deg, int nstages, const double *u,
                 double dt, double t, const o_solver *s, double *unp1, o_stats *st);

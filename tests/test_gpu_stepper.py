@@ -4,8 +4,8 @@ import pytest
 
 import oracle as O
 from cpolyg_amd import hip, setups
-from cpolyg_amd.spec import (EquationSpec, SolverSpec, GMRES_FLEXIBLE, GMRES_LEFT_REF, OP_ASSEMBLED, OP_MATRIX_FREE,
-                             PC_DILU, PC_BJACOBI, PREC_BF16, PREC_FP32, PREC_FP64, bs_of)
+from cpolyg_amd.spec import (DIRK3_L_STABLE, EquationSpec, SolverSpec, GMRES_FLEXIBLE, GMRES_LEFT_REF, OP_ASSEMBLED,
+                             OP_MATRIX_FREE, PC_DILU, PC_BJACOBI, PREC_BF16, PREC_FP32, PREC_FP64, bs_of)
 from problems import rel, unit_periodic_mesh, vortex_mesh, vortex_state
 
 pytestmark = pytest.mark.gpu
@@ -56,6 +56,79 @@ def test_dirk_flexible_mixed_precision(cuda, prec, op):
     gs = S.advance(ud, dt, 0.0, outd)
     assert gs.last_resid < 1e-11
     assert rel(outd.cpu().numpy(), uo) < 1e-9
+
+
+def test_dirk3_l_stable_25_steps_mixed_bf16(cuda):
+    """25 steps at CFL 20 with the benchmark's solver (Alexander's L-stable SDIRK3, BF16 DILU factors
+    in FP64 flexible GMRES(50) to 1e-10, matrix-free Newton operator, preconditioner refreshed once
+    per step, each stage's Newton started from the previous stage's increment, backtracking on)
+    on a 64-cell vortex at p = 3, every step against the oracle's trajectory (FP64 DILU, Newton
+    from zero, the same tableau) started from the same state."""
+    m = vortex_mesh(8)
+    deg = 3
+    spec = EquationSpec.euler(periodic_trace=1, jacobian=1)
+    u = vortex_state(m, deg)
+    dt = setups.dg_time_step(20.0, 10.0 / 8, deg, setups.euler_max_speed(u, deg))
+    ref = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=50, rtol=1e-10, maxit=4000, pc_kind=PC_DILU,
+                     newton_tol=5e-13, newton_maxit=40, dirk_tableau=DIRK3_L_STABLE)
+    solver = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=50, rtol=1e-10, maxit=20000, pc_kind=PC_DILU,
+                        newton_tol=5e-13, newton_maxit=100, precision=PREC_BF16, newton_guess=1, newton_backtrack=1,
+                        newton_operator=OP_MATRIX_FREE, pc_refresh=2, dirk_tableau=DIRK3_L_STABLE)
+    orc = O.Oracle(m)
+    S = hip.Stepper(hip.Equation(hip.Mesh(m), spec, deg), 0, 3, solver)
+    uo, ud, t = u.copy(), _dev(cuda, u), 0.0
+    worst = 0.0
+    for k in range(25):
+        uo, _ = orc.dirk_step(spec, deg, 3, uo, dt, t, ref)
+        out = _dev(cuda, np.zeros_like(u))
+        gs = S.advance(ud, dt, t, out)
+        assert gs.last_resid < 5e-13
+        ud, t = out, t + dt
+        err = rel(ud.cpu().numpy(), uo)
+        worst = max(worst, err)
+        assert err < 1e-9, (k, err)
+    print(f"25 L-stable SDIRK3 steps at CFL 20 (dt {dt:.4f}): max rel. difference from the oracle {worst:.2e}")
+
+
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32, PREC_BF16])
+def test_dirk3_euler_p4_matches_oracle(cuda, prec):
+    """Euler at p = 4 (60 x 60 blocks): one DIRK3 step (L-stable tableau) with FP64, FP32 and BF16
+    DILU factors in flexible GMRES, matrix-free Newton operator, against the oracle's step."""
+    m = vortex_mesh(6)
+    deg = 4
+    spec = EquationSpec.euler(periodic_trace=1, jacobian=1)
+    u = vortex_state(m, deg)
+    dt = setups.dg_time_step(20.0, 10.0 / 6, deg, setups.euler_max_speed(u, deg))
+    kw = dict(gmres_mode=GMRES_FLEXIBLE, restart=50, rtol=1e-10, maxit=4000, pc_kind=PC_DILU, newton_tol=1e-12,
+              newton_maxit=40, dirk_tableau=DIRK3_L_STABLE)
+    uo, st = O.Oracle(m).dirk_step(spec, deg, 3, u, dt, 0.0, SolverSpec(**kw))
+    S = hip.Stepper(hip.Equation(hip.Mesh(m), spec, deg), 0, 3,
+                    SolverSpec(**kw, precision=prec, newton_operator=OP_MATRIX_FREE, pc_refresh=2))
+    ud, outd = _dev(cuda, u), _dev(cuda, np.zeros_like(u))
+    gs = S.advance(ud, dt, 0.0, outd)
+    assert gs.last_resid < 1e-12
+    assert rel(outd.cpu().numpy(), uo) < 1e-9
+
+
+@pytest.mark.parametrize("variant", [0, DIRK3_L_STABLE])
+def test_dirk5_and_dirk3_variants_match_oracle(cuda, variant):
+    """The 5-stage DIRK (Cooper and Sayfy, src/TimeIntegration.cpp:78-89) and the selected 3-stage
+    tableau, reference GMRES mode, against the oracle's steps."""
+    m = vortex_mesh(6)
+    deg = 2
+    spec = EquationSpec.euler(jacobian=1)
+    u = vortex_state(m, deg)
+    solver = SolverSpec(gmres_mode=GMRES_LEFT_REF, restart=20, rtol=1e-11, maxit=4000, pc_kind=PC_DILU,
+                        newton_tol=1e-12, newton_maxit=40, dirk_tableau=variant)
+    orc = O.Oracle(m)
+    E = hip.Equation(hip.Mesh(m), spec, deg)
+    for ns in (5, 3):
+        uo, st = orc.dirk_step(spec, deg, ns, u, 0.4, 0.0, solver)
+        S = hip.Stepper(E, 0, ns, solver)
+        ud, outd = _dev(cuda, u), _dev(cuda, np.zeros_like(u))
+        gs = S.advance(ud, 0.4, 0.0, outd)
+        assert gs.newton_iters == st.newton_iters
+        assert rel(outd.cpu().numpy(), uo) < 1e-9
 
 
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_BF16])
@@ -199,6 +272,21 @@ def test_nonfinite_state_reports_status(cuda, method, ns, pc):
     with pytest.raises(hip.CpgError) as e:
         S.advance(_dev(cuda, u), 0.1, 0.0, _dev(cuda, np.zeros_like(u)))
     assert e.value.status == 5
+
+
+def test_nonphysical_result_reports_its_own_step(cuda):
+    """A forward Euler step with a huge dt from a finite state: the state it produces has
+    negative pressures, and the step itself (not the next one) returns CPG_ERR_NONFINITE."""
+    m = vortex_mesh(4)
+    deg = 1
+    spec = EquationSpec.euler(periodic_trace=1, jacobian=1)
+    u = vortex_state(m, deg)
+    assert np.isfinite(u).all()
+    S = hip.Stepper(hip.Equation(hip.Mesh(m), spec, deg), 5, 1, SolverSpec())
+    with pytest.raises(hip.CpgError) as e:
+        S.advance(_dev(cuda, u), 1e6, 0.0, _dev(cuda, np.zeros_like(u)))
+    assert e.value.status == 5
+    assert "non-physical state" in str(e.value)
 
 
 def test_newton_iteration_limit_reports_status(cuda):

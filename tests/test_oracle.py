@@ -417,6 +417,36 @@ def test_tableaux_order_conditions():
     _order_conditions(*O.irk_tableau(3), 5)
 
 
+def _stability_function(A, b, z):
+    """R(z) = 1 + z b^T (I - z A)^-1 1 of a Runge-Kutta tableau."""
+    n = len(b)
+    return 1 + z * (b @ np.linalg.solve(np.eye(n) - z * A, np.ones(n)))
+
+
+def test_dirk3_stability():
+    """The reference's 3-stage tableau is not A-stable: |R(iy)| > 1 on the imaginary axis and
+    R(-inf) = 1 - b^T A^-1 1 = -1.78, so every stiff mode is amplified by 1.78 per step.
+    Alexander's SDIRK3 (gamma the root of x^3 - 3x^2 + 3x/2 - 1/6 in (1/6, 1/2)) is third
+    order, |R(iy)| <= 1 and R(-inf) = 0 (its last row of A is b)."""
+    A, b, c = O.dirk_tableau(3, 0)
+    assert abs(abs(_stability_function(A, b, 1j)) - 1.13905) < 1e-4
+    assert abs(abs(_stability_function(A, b, 2j)) - 1.44081) < 1e-4
+    assert abs(abs(_stability_function(A, b, 10j)) - 1.76269) < 1e-4
+    rinf = 1 - b @ np.linalg.solve(A, np.ones(3))
+    assert abs(rinf - (-1.782775)) < 1e-5
+    assert abs(_stability_function(A, b, -1e9) - rinf) < 1e-6
+    A, b, c = O.dirk_tableau(3, 1)
+    g = A[0, 0]
+    assert abs(g ** 3 - 3 * g ** 2 + 1.5 * g - 1 / 6) < 1e-15 and 1 / 6 < g < 0.5
+    assert np.allclose(np.diag(A), g, atol=0, rtol=0) and np.array_equal(A[2], b)
+    _order_conditions(A, b, c, 3)
+    assert abs(1 - b @ np.linalg.solve(A, np.ones(3))) < 1e-14
+    ys = np.concatenate([np.linspace(0, 20, 4001), np.geomspace(20, 1e8, 400)])
+    assert max(abs(_stability_function(A, b, 1j * y)) for y in ys) <= 1 + 1e-12
+    xs = -np.geomspace(1e-3, 1e8, 300)
+    assert max(abs(_stability_function(A, b, x)) for x in xs) < 1
+
+
 def _linear_setup():
     m = unit_periodic_mesh(5)
     deg = 2
@@ -432,20 +462,42 @@ def _linear_setup():
     return m, deg, spec, orc, u, M, Jd
 
 
-@pytest.mark.parametrize("ns", [1, 2, 3, 5])
-def test_dirk_linear_advection_dense(ns):
+@pytest.mark.parametrize("ns,variant", [(1, 0), (2, 0), (3, 0), (3, 1), (5, 0)])
+def test_dirk_linear_advection_dense(ns, variant):
     m, deg, spec, orc, u, M, Jd = _linear_setup()
     dt = 0.07
-    A, b, c = O.dirk_tableau(ns)
+    A, b, c = O.dirk_tableau(ns, variant)
     ks = []
     for s in range(ns):
         rhs = Jd @ (u + sum(A[s, j] * ks[j] for j in range(s)))
         ks.append(np.linalg.solve(M - dt * A[s, s] * Jd, dt * rhs))
     want = u + sum(b[i] * ks[i] for i in range(ns))
-    solver = SolverSpec(rtol=1e-13, maxit=3000, newton_tol=1e-13, newton_maxit=5)
+    solver = SolverSpec(rtol=1e-13, maxit=3000, newton_tol=1e-13, newton_maxit=5, dirk_tableau=variant)
     got, st = orc.dirk_step(spec, deg, ns, u, dt, 0.0, solver)
     assert rel(got, want) < 1e-10
     assert st.newton_iters == ns
+
+
+def test_dirk3_step_operator_spectral_radius():
+    """The one-step map of DIRK3 on linear advection at CFL 20 (dt = 20 h / ((2p+1) |beta|)),
+    formed column by column from the stage equations that test_dirk_linear_advection_dense pins
+    the oracle's step to: with the reference tableau its spectral radius exceeds 1 (the step
+    amplifies the under-resolved modes), with Alexander's L-stable SDIRK3 it is at most 1."""
+    m, deg, spec, orc, u, M, Jd = _linear_setup()
+    h = 1.0 / 5
+    dt = 20 * h / ((2 * deg + 1) * math.hypot(1.0, 0.5))
+    N = u.size
+    rho = {}
+    for variant in (0, 1):
+        A, b, c = O.dirk_tableau(3, variant)
+        ks = []
+        for s_ in range(3):
+            rhs = Jd @ (np.eye(N) + sum(A[s_, j] * ks[j] for j in range(s_)))
+            ks.append(np.linalg.solve(M - dt * A[s_, s_] * Jd, dt * rhs))
+        G = np.eye(N) + sum(b[i] * ks[i] for i in range(3))
+        rho[variant] = max(abs(np.linalg.eigvals(G)))
+    assert rho[0] > 1.3
+    assert rho[1] <= 1 + 1e-10
 
 
 def test_irk_linear_advection_dense():
