@@ -406,7 +406,8 @@ def bench_irk(args):
 def pmc_workload(args):
     """Kernel sequence of the HBM-traffic profile: a 1 GiB device copy of known size (calibrates
     FETCH_SIZE / WRITE_SIZE), three FP64 BSR matvecs of the headline Newton matrix, one BF16 DILU
-    apply.  Meant to run under rocprofv3 --pmc; prints nothing but a completion line."""
+    apply, then the matrix-free Newton operator of the same matrix: one prepare (point data) and
+    three products.  Meant to run under rocprofv3 --pmc; prints nothing but a completion line."""
     import torch
 
     from cpolyg_amd import hip, setups
@@ -430,6 +431,9 @@ def pmc_workload(args):
     for _ in range(3):
         J.matvec(x, y)
     P.apply(x)
+    E.newton_prepare(u, 0.0)
+    for _ in range(3):
+        E.newton_apply_prepared(-dt * A_SS[args.dirk3], x, y)
     torch.cuda.synchronize()
     print("pmc workload done", flush=True)
 
