@@ -22,6 +22,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace cpg {
 
@@ -1076,6 +1077,68 @@ void eq_mass_matvec(const cpg_equation_s &E, const double *x, double *y, hipStre
     CPG_DISPATCH_BS(E, launch_mass_matvec, E, x, y, s);
 }
 
+// ---- MeshFn::L2Norm without allocation (cpg_l2norm_noalloc) ----------------------------
+// The per-component sums sum_i u_ic^T M_i u_ic are reduced in two fixed-order stages through
+// a file-scope device array instead of the equation's per-cell buffers: kL2Parts workgroups
+// per component each take a grid-stride share of the cells and leave one partial sum, and
+// k_colsum_final adds the kL2Parts partials of each component in order.  The array is the
+// library's for the life of the process on every GPU it runs on; calls that overlap on one GPU
+// share it and race.  Every call overwrites every entry it reads, so nothing is cleared.
+constexpr int kL2Parts = 1024;                   // partial sums per component
+constexpr int kL2MaxNc = 4;                      // components
+constexpr int kL2Scratch = kL2Parts * kL2MaxNc + kL2MaxNc;   // partials, then the nc sums (32.8 KB)
+static __device__ __align__(256) double scratch[kL2Scratch];
+
+template <int BS, int NC>
+__global__ __launch_bounds__(256) void k_l2norm_parts(int n, const double *__restrict__ M, const double *__restrict__ u,
+                                                      double *__restrict__ parts) {
+    constexpr int BL = NC * BS;
+    __shared__ double red[4];
+    const int c = blockIdx.y;
+    double s = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kL2Parts * 256) {
+        const double *Mi = M + (size_t)i * BS * BS;
+        const double *ui = u + (size_t)i * BL + c * BS;
+        double q = 0.0;
+#pragma unroll
+        for (int k = 0; k < BS; k++) {
+            double t = 0.0;
+#pragma unroll
+            for (int l = 0; l < BS; l++) t = fma(Mi[l * BS + k], ui[l], t);
+            q = fma(ui[k], t, q);
+        }
+        s += q;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) parts[(size_t)c * kL2Parts + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+template <int BS, int NC>
+static void launch_l2norm_parts(const cpg_equation_s &E, const double *u, double *parts, hipStream_t s) {
+    hipLaunchKernelGGL((k_l2norm_parts<BS, NC>), dim3(kL2Parts, NC), dim3(256), 0, s, E.n, E.mass.p, u, parts);
+    CPG_LAUNCH_CHECK();
+}
+
+// device address of `scratch` on the current device, looked up once per device (the array's
+// name in host code is not a device address)
+static double *scratch_ptr() {
+    static std::mutex mu;
+    static std::vector<double *> ptrs;
+    int dev = 0;
+    CPG_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)ptrs.size() <= dev) ptrs.resize(dev + 1, nullptr);
+    if (!ptrs[dev]) {
+        void *p = nullptr;
+        CPG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        CPG_CHECK(p, CPG_ERR_HIP, "no device address for the L2-norm scratch array on device %d", dev);
+        ptrs[dev] = static_cast<double *>(p);
+    }
+    return ptrs[dev];
+}
+
 // r = M k - dt b (k non-null) and returns max_c sqrt(r_c^T M r_c) summed over cells (synchronous);
 // with k null, the norm of b itself.  out_nc (optional) receives the per-component norms.
 double eq_newton_resid(const cpg_equation_s &E, const double *kv, const double *b, double dt, double *r,
@@ -1507,6 +1570,24 @@ int cpg_l2norm(cpg_equation E, const double *u, double *out, void *stream) {
     return guard([&] {
         CPG_CHECK(E && u && out, CPG_ERR_ARG, "null argument");
         eq_newton_resid(*E, nullptr, u, 0.0, nullptr, out, S(stream));
+    });
+}
+
+int cpg_l2norm_noalloc(cpg_equation E, const double *u, double *out, void *stream) {
+    return guard([&] {
+        CPG_CHECK(E && u && out, CPG_ERR_ARG, "null argument");
+        CPG_CHECK(E->nc >= 1 && E->nc <= kL2MaxNc, CPG_ERR_ARG,
+                  "%d components: the allocation-free L2 norm holds partial sums for at most %d", E->nc, kL2MaxNc);
+        hipStream_t s = S(stream);
+        double *parts = scratch_ptr();
+        double *sums = parts + (size_t)kL2Parts * kL2MaxNc;
+        CPG_DISPATCH_BS(*E, launch_l2norm_parts, *E, u, parts, s);
+        hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(64), 0, s, parts, kL2Parts, E->nc, sums);
+        CPG_LAUNCH_CHECK();
+        double h[kL2MaxNc] = {0, 0, 0, 0};
+        CPG_HIP(hipMemcpyAsync(h, sums, sizeof(double) * E->nc, hipMemcpyDeviceToHost, s));
+        CPG_HIP(hipStreamSynchronize(s));
+        for (int c = 0; c < E->nc; c++) out[c] = std::sqrt(h[c]);
     });
 }
 

@@ -344,6 +344,12 @@ class Equation:
         _chk(lib().cpg_l2norm(self.h, _vp(ptr(u)), out.ctypes.data_as(_dp), _stream(stream)))
         return out
 
+    def l2norm_noalloc(self, u, stream=None):
+        """``MeshFn::L2Norm`` through ``cpg_l2norm_noalloc`` (no device allocation)."""
+        out = np.zeros(self.spec.nc)
+        _chk(lib().cpg_l2norm_noalloc(self.h, _vp(ptr(u)), out.ctypes.data_as(_dp), _stream(stream)))
+        return out
+
     @staticmethod
     def _field_par(par):
         p = np.zeros(8)

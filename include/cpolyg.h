@@ -194,6 +194,14 @@ int cpg_mass_matvec(cpg_equation E, const double *x, double *y, void *stream);
 int cpg_mass_download(cpg_equation E, double *host_blocks);   /* n blocks bl x bl, synchronous */
 /* MeshFn::L2Norm (src/MeshFn.cpp:299-312) of u, nc values to host, synchronous */
 int cpg_l2norm(cpg_equation E, const double *u, double *out, void *stream);
+/* cpg_l2norm that allocates nothing.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (tight serving loops, code that forbids hipMalloc).  Its partial
+ * sums live in a fixed 32.8 KB device array of the library (1024 partial sums for each of at most
+ * 4 components, then the sums), held on every GPU the library runs on until the process ends;
+ * any cell count is accepted, an equation with more than 4 components is rejected.  Calls that
+ * overlap on one GPU share the array and race.  Same result as cpg_l2norm up to the order of
+ * the sums; call with E's device current; synchronous. */
+int cpg_l2norm_noalloc(cpg_equation E, const double *u, double *out, void *stream);
 
 /* ---- analytic fields: projection and L2 error on the device ----------------
  * MeshFn::interp (src/MeshFn.cpp:56-145): per cell the least-squares fit of the basis to the
