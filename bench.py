@@ -38,12 +38,21 @@ METRIC = "DIRK steps/s, Euler p=3 65k cells: mixed-prec vs FP64 on GPU at 1e-10 
 HBM_PEAK_GBS = 8000.0
 # diagonal coefficient a_ss of the two 3-stage DIRK tableaux (the Newton matrix is M - dt a_ss J)
 A_SS = {"reference": 0.5 * (1 + 0.435866521508459), "l-stable": 0.43586652150845899942}
-# HBM bytes per BSR-matvec launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
-# kernel (bench.py --workload pmc), FETCH scaled x2 and WRITE x1 by a 1 GiB copy of known size;
-# keyed by (cells, degree).  profiles/bsr_matvec_euler_p3_65k_pmc_traffic.txt has the passes.
-MATVEC_TRAFFIC = {(65536, 3): 6.152e9}
-# the same for one DILU apply (all 8 launches), keyed by (cells, degree, factor precision)
-APPLY_TRAFFIC = {(65536, 3, "bf16"): 1.715e9}
+
+
+def traffic(kernel, cells, deg, precision):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc (profiles/hbm_traffic.json,
+    written from the PMC summaries in profiles/), or None where no profile covers the config."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)["entries"]
+    except (OSError, ValueError, KeyError):
+        return None
+    for e in table:
+        if (e["kernel"], e["cells"], e["deg"], e["precision"]) == (kernel, cells, deg, precision):
+            return e["bytes"]
+    return None
 
 
 def parse():
@@ -74,10 +83,14 @@ def parse():
     p.add_argument("--no-compare", dest="compare_operators", action="store_false",
                    help="skip the runs under the reference refresh policy (matrix-free and assembled operator)")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--cpu-nx", type=int, default=128, help="lattice size of the CPU-baseline sample")
-    p.add_argument("--workload", choices=["dirk3-euler", "batch", "pmc", "irk"], default="dirk3-euler",
+    p.add_argument("--cpu-nx", type=int, default=64,
+                   help="lattice size of the single-thread CPU-baseline sample (0: none)")
+    p.add_argument("--workload", choices=["dirk3-euler", "batch", "pmc", "irk", "rk4-advection"],
+                   default="dirk3-euler",
                    help="batch: 64 independent implicit advection solves advanced as one batch (prints its own "
-                        "line); pmc: the short kernel sequence profiled with rocprofv3 --pmc for HBM traffic")
+                        "line); pmc: the short kernel sequence profiled with rocprofv3 --pmc for HBM traffic; "
+                        "rk4-advection: explicit RK4 linear advection, 1,024 cells, p=2, 100 steps (its own line)")
+    p.add_argument("--rk4-steps", type=int, default=100)
     p.add_argument("--batch-stages", type=int, default=2, help="batch: 1 backward Euler, 2 SDIRK2")
     p.add_argument("--batch-count", type=int, default=64)
     p.add_argument("--no-sequential", action="store_true", help="batch: skip the one-stepper-per-instance timing")
@@ -97,6 +110,23 @@ def workload(nx, deg):
     mesh = periodic_voronoi(jittered_lattice_seeds(nx, nx, 10.0, 10.0, 0.3, 3), 10.0, 10.0)
     u0 = setups.interp(mesh, deg, setups.vortex(beta=5.0, x0=5.0, y0=5.0, uinf=1.0, vinf=1.0, gamma=1.4), 4)
     return mesh, u0
+
+
+def operator_bytes(mesh, deg, nc):
+    """Algorithmic bytes of one matrix-free Newton-operator product (k_cellop<DEG, NC, true>):
+    the point data of the Newton matrix (Euler: 4 doubles per volume quadrature point, 21 per edge
+    quadrature point of every cell's edges), the volume points (x, y, weight, pad: 32 B each),
+    the 88-byte edge records, the cell bounding boxes and point/edge offsets, v in and y out
+    (N doubles each) and the mass blocks (bs^2 doubles per cell).  The neighbours' coefficients
+    of v that the edge points read are counted once, in v."""
+    bs = (deg + 1) * (deg + 2) // 2
+    n = mesh.n_cells
+    nqt = 3 if 2 * deg <= 2 else 6 if 2 * deg <= 4 else 12 if 2 * deg <= 6 else 25
+    nql = 2 if 2 * deg <= 2 else 4 if 2 * deg <= 4 else 6 if 2 * deg <= 6 else 10
+    nv = int(mesh.tptr[-1]) * nqt
+    ne = len(mesh.pv)
+    point = (4 * nv + 21 * ne * nql) * 8 if nc == 4 else 0
+    return point + 32 * nv + 88 * ne + n * (32 + 8) + 2 * n * nc * bs * 8 + n * bs * bs * 8
 
 
 def algorithmic_bytes(n, nnzb, bl, elem, scales):
@@ -190,12 +220,12 @@ def standalone_rates(E, u, t, scale, reps=20):
     return out
 
 
-def cpu_baseline(nx, deg, cfl, counts, n_target, a_ss):
-    """Oracle (single-threaded C port of the reference algorithm) timed per phase on a
-    bounded sample of the same workload at nx^2 cells, scaled linearly to n_target cells
-    and composed with the per-step phase counts of the GPU run."""
-    import numpy as np
-
+def cpu_phase_times(nx, deg, cfl, counts, a_ss, threads, gmres_its):
+    """The oracle's per-phase times on the workload at nx^2 cells with `threads` host threads:
+    one residual, one Jacobian, one DILU factor of the Newton matrix, and one flexible GMRES cycle
+    of exactly `gmres_its` iterations (restart = gmres_its, unreachable tolerance: the oracle rounds
+    maxit up to whole restart cycles), i.e. the iterations of one Newton solve of the GPU run with
+    their Gram-Schmidt lengths (seconds each; the GMRES figure per iteration)."""
     import oracle as O
     from cpolyg_amd import setups
     from cpolyg_amd.spec import EquationSpec
@@ -204,32 +234,129 @@ def cpu_baseline(nx, deg, cfl, counts, n_target, a_ss):
     orc = O.Oracle(mesh)
     spec = EquationSpec.euler(periodic_trace=counts["trace"], jacobian=counts["jacobian"])
     dt = setups.dg_time_step(cfl, 10.0 / nx, deg, setups.euler_max_speed(u, deg))
-    a = a_ss
+    O.set_threads(threads)
+    try:
+        t0 = time.perf_counter()
+        b = orc.assemble(spec, deg, u, 0.0)
+        t_res = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        B = orc.jacobian(spec, deg, u, 0.0)
+        t_jac = time.perf_counter() - t0
+        B *= -dt * a_ss
+        B[orc.rowptr[:-1]] += orc.mass(deg, 4)
+        t0 = time.perf_counter()
+        O.dilu_factor(orc.rowptr, orc.colind, B)
+        t_fac = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.gmres(orc.rowptr, orc.colind, B, b, m=gmres_its, tol=1e-30, maxit=gmres_its, pc_kind=2, mode=1)
+        t_gm = (time.perf_counter() - t0 - t_fac) / gmres_its
+    finally:
+        O.set_threads(1)
+    return mesh.n_cells, t_res, t_jac, t_fac, t_gm
+
+
+def cpu_baseline(nx, deg, cfl, counts, n_target, a_ss, threads, sample_nx):
+    """The oracle (C restatement of the reference algorithm) timed per phase on the host and
+    composed with the per-step phase counts of the GPU run under the reference's refresh policy
+    (one Jacobian and one DILU factor per Newton iteration, a residual per Newton iteration and
+    per stage, the GMRES iterations of the GPU run).  Main figure: all `threads` host threads on
+    the full workload (nx^2 cells, measured directly).  Beside it: one thread on a sample_nx^2-cell
+    sample of the same workload, scaled linearly in cells to nx^2."""
+    def compose(n, t_res, t_jac, t_fac, t_gm, scale=1.0):
+        return scale * (counts["residuals"] * t_res + counts["newton"] * (t_jac + t_fac) + counts["gmres"] * t_gm)
+
+    it = max(1, int(round(counts["gmres"] / max(counts["newton"], 1.0))))   # iterations per Newton solve
+    n, t_res, t_jac, t_fac, t_gm = cpu_phase_times(nx, deg, cfl, counts, a_ss, threads, it)
+    step = compose(n, t_res, t_jac, t_fac, t_gm)
+    out = {"value": 1.0 / step, "unit": "DIRK steps/s", "cores": threads, "kind": "port",
+           "sample": (f"oracle with {threads} threads on the full workload ({n} cells, measured directly): 1 residual "
+                      f"{t_res:.2f}s, 1 Jacobian {t_jac:.2f}s, 1 DILU factor (level-parallel) {t_fac:.2f}s, {it} "
+                      f"flexible GMRES iterations (one solve's worth) {t_gm:.3f}s each; composed with the GPU run's per-step counts "
+                      f"({counts['residuals']:.1f} residuals, {counts['newton']:.1f} Jacobians+factors as the "
+                      f"reference refreshes them, {counts['gmres']:.1f} GMRES iterations)"),
+           "s_per_step": step, **host_cpu()}
+    if sample_nx:
+        n1, r1, j1, f1, g1 = cpu_phase_times(sample_nx, deg, cfl, counts, a_ss, 1, it)
+        step1 = compose(n1, r1, j1, f1, g1, n_target / n1)
+        out["single_core"] = {"value": 1.0 / step1, "s_per_step": step1, "cores": 1,
+                              "sample": (f"oracle on 1 thread at {n1} cells: residual {r1:.2f}s, Jacobian {j1:.2f}s, "
+                                         f"DILU factor {f1:.2f}s, FGMRES iteration {g1:.3f}s; scaled x{n_target / n1:g} "
+                                         f"to {n_target} cells")}
+    return out
+
+
+def bench_rk4(args):
+    """Linear advection with constant velocity (1, 1) on the periodic unit square, Voronoi mesh of
+    1,024 cells from uniform random seeds (seed 1), p = 2, u0 = exp(-100 |x - (1/2, 1/2)|^2), RK4
+    at dt = 0.1 h_min (h_min the smallest sqrt(cell area)), --rk4-steps steps in FP64 with exact
+    periodic traces (RK4::advance, src/TimeIntegration.cpp:14-22, as driven by
+    examples/ExpAdv.cpp:66-83).  Times the steps (one warm-up run of the same length first),
+    reports the L2 error against the advected exact Gaussian, and times the oracle's RK4 step on
+    the host (one core) on the same mesh and state."""
+    import numpy as np
+    import torch
+
+    from cpolyg_amd import hip
+    from cpolyg_amd.mesh import periodic_voronoi, uniform_seeds
+    from cpolyg_amd.spec import FIELD_GAUSSIAN, EquationSpec, SolverSpec
+
+    deg, nsteps = 2, args.rk4_steps
+    mesh = periodic_voronoi(uniform_seeds(1024, 1.0, 1.0, 1), 1.0, 1.0)
+    spec = EquationSpec.constant_advection(1.0, 1.0, periodic_trace=1)
+    E = hip.Equation(hip.Mesh(mesh), spec, deg)
+    dt = 0.1 * mesh.h_min()
+    par = (0.5, 0.5, 100.0, 1.0, 1.0, 1.0, 1.0)
+    u0 = torch.zeros(E.n * E.bl, dtype=torch.float64, device="cuda")
+    E.interp_field(FIELD_GAUSSIAN, par, 0.0, u0)
+    S = hip.Stepper(E, 2, 4, SolverSpec())
+    a, b = u0.clone(), torch.empty_like(u0)
+    for k in range(nsteps):                    # warm-up run
+        S.advance(a, dt, k * dt, b)
+        a, b = b, a
+    a.copy_(u0)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    b = orc.assemble(spec, deg, u, 0.0)
-    t_res = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    J = orc.jacobian(spec, deg, u, 0.0)
-    t_jac = time.perf_counter() - t0
-    M = orc.mass(deg, 4)
-    B = -dt * a * J
-    B[orc.rowptr[:-1]] += M
-    t0 = time.perf_counter()
-    O.dilu_factor(orc.rowptr, orc.colind, B)
-    t_fac = time.perf_counter() - t0
-    it = 6
-    t0 = time.perf_counter()
-    O.gmres(orc.rowptr, orc.colind, B, b, m=50, tol=1e-30, maxit=it, pc_kind=2, mode=1)
-    t_gm = (time.perf_counter() - t0 - t_fac) / it
-    scale = n_target / mesh.n_cells
-    step = scale * (counts["residuals"] * t_res + counts["newton"] * (t_jac + t_fac) + counts["gmres"] * t_gm)
-    sample = (f"oracle on the same workload at {mesh.n_cells} cells (lattice {nx}^2): 1 residual "
-              f"{t_res:.2f}s, 1 Jacobian {t_jac:.2f}s, 1 DILU factor {t_fac:.2f}s, {it} FGMRES(50) iterations "
-              f"{t_gm:.3f}s each; scaled x{scale:g} to {n_target} cells and composed with the GPU run's per-step "
-              f"counts ({counts['residuals']:.1f} residuals, {counts['newton']:.1f} Jacobians+factors, "
-              f"{counts['gmres']:.1f} GMRES iterations)")
-    return {"value": 1.0 / step, "unit": "DIRK steps/s", "cores": 1, "kind": "port", "sample": sample,
-            "s_per_step": step}
+    for k in range(nsteps):
+        S.advance(a, dt, k * dt, b)
+        a, b = b, a
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    err0 = float(E.l2_error_field(u0, FIELD_GAUSSIAN, par, 0.0)[0])
+    err = float(E.l2_error_field(a, FIELD_GAUSSIAN, par, nsteps * dt)[0])
+    out = {"metric": "RK4 steps/s, linear advection p=2, 1,024-cell periodic Voronoi", "value": nsteps / wall,
+           "unit": "RK4 steps/s", "n_gpus": 1, "steps": nsteps, "ms_per_step": 1e3 * wall / nsteps,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"constant advection (1,1), periodic unit square, Voronoi {mesh.n_cells} cells "
+                                  f"(uniform seeds), p={deg}, RK4 dt = 0.1 h_min = {dt:.3e}, {nsteps} steps, exact "
+                                  f"periodic traces", "cells": mesh.n_cells, "deg": deg, "dt": dt},
+           "l2_error_initial_projection": err0, "l2_error_final": err}
+    if not args.no_cpu:
+        import oracle as O
+        orc = O.Oracle(mesh)
+        uc = u0.cpu().numpy()
+        k_cpu = min(nsteps, 20)
+        t0 = time.perf_counter()
+        for k in range(k_cpu):
+            uc = orc.rk4_step(spec, deg, uc, dt, k * dt)
+        t_cpu = (time.perf_counter() - t0) / k_cpu
+        out["cpu_baseline"] = {"value": 1.0 / t_cpu, "unit": "RK4 steps/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle RK4 step (o_rk4_step) on the same mesh and state, {k_cpu} steps",
+                               **host_cpu()}
+    print(json.dumps(out))
+
+
+def host_cpu():
+    """CPU model and logical core count of the host (reported with every CPU number)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
 
 
 def bench_batch(args):
@@ -449,6 +576,12 @@ def main():
             sys.exit(2)
         bench_irk(args)
         return
+    if args.workload == "rk4-advection":
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        bench_rk4(args)
+        return
     if args.workload == "batch":
         if args.gpus != 1:
             print("bench.py runs on one GPU only", file=sys.stderr)
@@ -548,7 +681,8 @@ def main():
     ap_ms = msa / max(na, 1)
     if mixed_pc != PC_DILU:
         res["inner_solve"] = {"iterations": args.inner, "factors": args.mixed, "avg_ms": ap_ms, "launches": na}
-    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9
+    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9   # noqa: E731
+    b_op = operator_bytes(mesh, deg, 4)
     mv_gbs = gbs(b_mv, rates["bsr_matvec_ms"])
     newton = sum(s["newton_iters"] for s in st_mix) / steps
     gm = sum(s["gmres_iters"] for s in st_mix) / steps
@@ -561,14 +695,17 @@ def main():
                                           f"launches: {lev} colour levels x 2 sweeps, forward level 0 a no-op, the last "
                                           f"forward level finishing its rows' backward step)",
                 "achieved": ap_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ap_gbs / HBM_PEAK_GBS,
-                "traffic": APPLY_TRAFFIC.get((n, deg, args.mixed)), "algorithmic_bytes": b_ap, "avg_ms": ap_ms,
+                "traffic": traffic("dilu_apply", n, deg, args.mixed), "algorithmic_bytes": b_ap, "avg_ms": ap_ms,
                 "launches": na}
     else:
         roof = {"bound": "hbm", "kernel": "bsr_matvec (FP64, 40x40 blocks)", "achieved": mv_gbs,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": mv_gbs / HBM_PEAK_GBS,
-                "traffic": MATVEC_TRAFFIC.get((n, deg)), "algorithmic_bytes": b_mv, "avg_ms": rates["bsr_matvec_ms"]}
+                "traffic": traffic("bsr_matvec", n, deg, "fp64"), "algorithmic_bytes": b_mv,
+                "avg_ms": rates["bsr_matvec_ms"]}
+    headline = args.nx == 256 and deg == 3 and args.cfl == 20.0
     out = {
-        "metric": METRIC,
+        "metric": METRIC if headline else (f"DIRK steps/s, Euler p={deg} {n} cells CFL {args.cfl:g}: mixed-prec vs "
+                                           f"FP64 on GPU at 1e-10 resid"),
         "value": steps / wall_mix,
         "unit": "DIRK steps/s",
         "n_gpus": 1,
@@ -594,9 +731,16 @@ def main():
         "in_solver_ms": {"operator_avg_ms": op_ms, "operator_launches": nm, "operator": args.operator,
                          "prec_apply_avg_ms": ap_ms, "prec_applies": na},
         "bsr_matvec": {"algorithmic_bytes": b_mv, "avg_ms": rates["bsr_matvec_ms"], "achieved_gbs": mv_gbs,
-                       "frac": mv_gbs / HBM_PEAK_GBS, "traffic": MATVEC_TRAFFIC.get((n, deg))},
+                       "frac": mv_gbs / HBM_PEAK_GBS, "traffic": traffic("bsr_matvec", n, deg, "fp64")},
         "newton_operator_ms": {"prepare_per_newton_matrix": rates["newton_prepare_ms"],
                                "apply_per_product": rates["newton_apply_ms"]},
+        # the GMRES operator of the headline (replaces BlockMatrix::matvec): per-product rate in the
+        # solver (HIP events on the launch stream) against its algorithmic bytes
+        "newton_operator_roofline": {"bound": "hbm", "kernel": f"k_cellop<{deg},4,true> (matrix-free Newton operator)",
+                                     "algorithmic_bytes": b_op, "avg_ms": op_ms,
+                                     "achieved": gbs(b_op, op_ms), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": gbs(b_op, op_ms) / HBM_PEAK_GBS,
+                                     "traffic": traffic("newton_apply", n, deg, "fp64")},
         "dilu_apply_standalone": {p: {"avg_ms": rates[f"dilu_apply_{p}_ms"], "algorithmic_bytes": b,
                                       "achieved_gbs": gbs(b, rates[f"dilu_apply_{p}_ms"]),
                                       "frac": gbs(b, rates[f"dilu_apply_{p}_ms"]) / HBM_PEAK_GBS}
@@ -611,7 +755,8 @@ def main():
     if not args.no_cpu:
         counts = {"residuals": newton + 3, "newton": newton, "gmres": gm, "trace": args.trace,
                   "jacobian": args.jacobian}
-        out["cpu_baseline"] = cpu_baseline(args.cpu_nx, deg, args.cfl, counts, n, a_ss)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+        out["cpu_baseline"] = cpu_baseline(args.nx, deg, args.cfl, counts, n, a_ss, min(threads, 16), args.cpu_nx)
     print(json.dumps(out))
 
 

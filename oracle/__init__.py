@@ -315,6 +315,19 @@ def rule_tables(deg2):
     return tri, lin
 
 
+def set_threads(n: int):
+    """Host threads of the oracle's loops (1: the serial order the tests pin; more: for timing)."""
+    lib().o_set_threads(int(n))
+
+
+def eval_bc(spec, x, y, t=0.0):
+    """The Dirichlet state of ``spec`` (bc_kind, bc_par) at (x, y, t): ``o_eval_bc``."""
+    e = Oracle.eq(spec)
+    out = np.zeros(spec.nc)
+    lib().o_eval_bc(C.byref(e), C.c_double(x), C.c_double(y), C.c_double(t), _d(out))
+    return out
+
+
 def dirk_tableau(ns, variant=0):
     """variant (3 stages): 0 the reference's tableau, 1 Alexander's L-stable SDIRK3."""
     A, b, c = np.zeros(25), np.zeros(5), np.zeros(5)

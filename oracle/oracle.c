@@ -22,6 +22,14 @@
 #define MAXB 15   /* basis size up to p = 4 */
 #define MAXNC 4
 
+/* Host threads of the per-cell loops, the BSR matvec, the level-scheduled block DILU and the
+ * GMRES reductions.  1 (the default) runs every loop in its serial order, which is what the
+ * tests pin; more threads are for timing the CPU baseline (same arithmetic per cell, row and
+ * block; the DILU factor's Schur sums and the dot products then add in another order). */
+static int g_threads = 1;
+
+void o_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+
 /* ---------------------------------------------------------------------------
  * Quadrature tables, verbatim values of src/Quadrature.cpp:103-171 (the 14-digit
  * triangle rules bias integrals at the 1e-14 level; parity needs the same digits).
@@ -720,6 +728,8 @@ int o_assemble(const o_mesh *m, const o_eq *eq, int deg, const double *u, double
     int nt, nl;
     pick_rules(2 * deg, &tr, &nt, &ln, &nl);
     double phi[MAXB], dxi[MAXB], deta[MAXB], phip[MAXB], Um[MAXNC], Up[MAXNC], Fx[MAXNC], Fy[MAXNC], F[MAXNC];
+#pragma omp parallel for schedule(dynamic, 64) num_threads(g_threads) if (g_threads > 1) \
+    private(phi, dxi, deta, phip, Um, Up, Fx, Fy, F)
     for (int i = 0; i < m->np; i++) {
         const double *Ui = u + (size_t)i * bl;
         double *bi = b + (size_t)i * bl;
@@ -799,6 +809,8 @@ int o_jacobian(const o_mesh *m, const o_eq *eq, int deg, const double *u, double
     double phi[MAXB], dxi[MAXB], deta[MAXB], phip[MAXB], Um[MAXNC], Up[MAXNC];
     double Ax[16], Ay[16], Jd[16], Jo[16];
     size_t bb2 = (size_t)bl * bl;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(g_threads) if (g_threads > 1) \
+    private(phi, dxi, deta, phip, Um, Up, Ax, Ay, Jd, Jo)
     for (int i = 0; i < m->np; i++) {
         const double *Ui = u + (size_t)i * bl;
         double w = m->bb[4 * i + 2], h = m->bb[4 * i + 3];
@@ -893,6 +905,7 @@ void o_bsr_matvec(int n, int bl, const int *rowptr, const int *colind, const dou
                   const double *x, double *y)
 {
     size_t bb2 = (size_t)bl * bl;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int i = 0; i < n; i++) {
         double *yi = y + (size_t)i * bl;
         for (int r = 0; r < bl; r++) yi[r] = 0.0;
@@ -986,11 +999,90 @@ static int find_slot(const int *rowptr, const int *colind, int row, int col, int
 /* BlockILU0::BlockILU0 (src/Preconditioners.cpp:49-123): block DILU in natural row order.
  * F gets A's slot layout: F[diag] = D_i^-1, F[(i,j<i)] = L_ij = A_ij D_j^-1,
  * F[(i,j>i)] = DU_ij = D_i^-1 A_ij. */
+/* Dependency levels of the lower (fwd) or upper (!fwd) triangle of the block pattern: rows of one
+ * level have no lower (upper) neighbour in the same or a later level, so they are independent.
+ * Returns the level count; order[] lists the rows level by level, lptr[] the level starts. */
+static int dilu_levels(int n, const int *rowptr, const int *colind, int fwd, int *order, int *lptr)
+{
+    int *lev = malloc(sizeof(int) * (n > 0 ? n : 1)), nl = 0;
+    for (int t = 0; t < n; t++) {
+        int i = fwd ? t : n - 1 - t, l = 0;
+        for (int s = rowptr[i]; s < rowptr[i + 1]; s++) {
+            int j = colind[s];
+            if ((fwd && j < i) || (!fwd && j > i)) l = lev[j] + 1 > l ? lev[j] + 1 : l;
+        }
+        lev[i] = l;
+        if (l + 1 > nl) nl = l + 1;
+    }
+    for (int l = 0; l <= nl; l++) lptr[l] = 0;
+    for (int i = 0; i < n; i++) lptr[lev[i] + 1]++;
+    for (int l = 0; l < nl; l++) lptr[l + 1] += lptr[l];
+    int *fill = malloc(sizeof(int) * (nl + 1));
+    memcpy(fill, lptr, sizeof(int) * (nl + 1));
+    for (int t = 0; t < n; t++) {
+        int i = fwd ? t : n - 1 - t;
+        order[fill[lev[i]]++] = i;
+    }
+    free(fill);
+    free(lev);
+    return nl;
+}
+
+/* The same factor with host threads: left-looking per row, the rows of one level in parallel
+ * (S_j = A_jj - sum_i L_ji A_ij with L_ji = A_ji D_i^-1 over the lower neighbours i, D_j^-1 =
+ * S_j^-1), then DU_ij = D_i^-1 A_ij for every upper block. */
+static int dilu_factor_threaded(int n, int bl, const int *rowptr, const int *colind, double *F)
+{
+    size_t bb2 = (size_t)bl * bl;
+    int *order = malloc(sizeof(int) * (n > 0 ? n : 1)), *lptr = malloc(sizeof(int) * (n + 2));
+    int nl = dilu_levels(n, rowptr, colind, 1, order, lptr), info = 0;
+    for (int l = 0; l < nl; l++) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(g_threads)
+        for (int t = lptr[l]; t < lptr[l + 1]; t++) {
+            int j = order[t], dj = find_slot(rowptr, colind, j, j, 0);
+            double *T = malloc(sizeof(double) * bb2), *Dn = malloc(sizeof(double) * bb2);
+            for (int s = rowptr[j]; s < rowptr[j + 1]; s++) {
+                int i = colind[s];
+                if (i >= j) continue;
+                int di = find_slot(rowptr, colind, i, i, 0), sij = find_slot(rowptr, colind, i, j, 1);
+                gemm(bl, F + s * bb2, F + di * bb2, T);          /* L_ji = A_ji D_i^-1 */
+                memcpy(F + s * bb2, T, sizeof(double) * bb2);
+                if (sij < 0) continue;
+                gemm(bl, F + s * bb2, F + sij * bb2, T);         /* S_j -= L_ji A_ij */
+                for (size_t e = 0; e < bb2; e++) F[dj * bb2 + e] -= T[e];
+            }
+            int st = o_inv(bl, F + dj * bb2, Dn);
+            memcpy(F + dj * bb2, Dn, sizeof(double) * bb2);
+            if (st) {
+#pragma omp critical
+                if (!info || j + 1 < info) info = j + 1;
+            }
+            free(T);
+            free(Dn);
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 16) num_threads(g_threads)
+    for (int i = 0; i < n; i++) {
+        double *T = malloc(sizeof(double) * bb2);
+        int di = find_slot(rowptr, colind, i, i, 0);
+        for (int s = rowptr[i]; s < rowptr[i + 1]; s++) {
+            if (colind[s] <= i) continue;
+            gemm(bl, F + di * bb2, F + s * bb2, T);              /* DU_ij = D_i^-1 A_ij */
+            memcpy(F + s * bb2, T, sizeof(double) * bb2);
+        }
+        free(T);
+    }
+    free(order);
+    free(lptr);
+    return info;
+}
+
 int o_dilu_factor(int n, int bl, const int *rowptr, const int *colind, const double *A, double *F)
 {
     size_t bb2 = (size_t)bl * bl;
     size_t nnzb = rowptr[n];
     memcpy(F, A, sizeof(double) * bb2 * nnzb);
+    if (g_threads > 1) return dilu_factor_threaded(n, bl, rowptr, colind, F);
     double *D = malloc(sizeof(double) * bb2), *T = malloc(sizeof(double) * bb2);
     int info = 0;
     for (int i = 0; i < n; i++) {
@@ -1018,10 +1110,52 @@ int o_dilu_factor(int n, int bl, const int *rowptr, const int *colind, const dou
     return info;
 }
 
+/* the apply with host threads: each sweep level by level, the rows of a level in parallel, each
+ * row's arithmetic in the serial order (the same result as the serial apply) */
+static void dilu_apply_threaded(int n, int bl, const int *rowptr, const int *colind, const double *F, double *x)
+{
+    size_t bb2 = (size_t)bl * bl;
+    int *order = malloc(sizeof(int) * (n > 0 ? n : 1)), *lptr = malloc(sizeof(int) * (n + 2));
+    for (int fwd = 1; fwd >= 0; fwd--) {
+        int nl = dilu_levels(n, rowptr, colind, fwd, order, lptr);
+        for (int l = 0; l < nl; l++) {
+#pragma omp parallel for schedule(dynamic, 32) num_threads(g_threads)
+            for (int t = lptr[l]; t < lptr[l + 1]; t++) {
+                int i = order[t];
+                for (int s = rowptr[i]; s < rowptr[i + 1]; s++) {
+                    int j = colind[s];
+                    if (fwd ? j >= i : j <= i) continue;
+                    const double *B = F + s * bb2;
+                    for (int c = 0; c < bl; c++)
+                        for (int r = 0; r < bl; r++) x[(size_t)i * bl + r] -= B[c * bl + r] * x[(size_t)j * bl + c];
+                }
+            }
+        }
+        if (fwd) {
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+            for (int i = 0; i < n; i++) {
+                double tmp[64 * MAXNC];
+                int d = find_slot(rowptr, colind, i, i, 0);
+                const double *B = F + d * bb2;
+                for (int r = 0; r < bl; r++) tmp[r] = 0.0;
+                for (int c = 0; c < bl; c++)
+                    for (int r = 0; r < bl; r++) tmp[r] += B[c * bl + r] * x[(size_t)i * bl + c];
+                memcpy(x + (size_t)i * bl, tmp, sizeof(double) * bl);
+            }
+        }
+    }
+    free(order);
+    free(lptr);
+}
+
 /* BlockILU0::solve (src/Preconditioners.cpp:125-158) */
 void o_dilu_apply(int n, int bl, const int *rowptr, const int *colind, const double *F, double *x)
 {
     size_t bb2 = (size_t)bl * bl;
+    if (g_threads > 1 && bl <= 64 * MAXNC) {
+        dilu_apply_threaded(n, bl, rowptr, colind, F, x);
+        return;
+    }
     double *tmp = malloc(sizeof(double) * bl);
     for (int i = 0; i < n; i++)
         for (int s = rowptr[i]; s < rowptr[i + 1]; s++) {
@@ -1084,18 +1218,21 @@ static void pc_solve(const pc_t *pc, double *x)
     else if (pc->kind == O_PC_BJACOBI) o_bjacobi_apply(pc->n, pc->bl, pc->data, pc->piv, x);
 }
 
-static double nrm2(size_t n, const double *x)
-{
-    double s = 0;
-    for (size_t i = 0; i < n; i++) s += x[i] * x[i];
-    return sqrt(s);
-}
-
 static double dot(size_t n, const double *x, const double *y)
 {
     double s = 0;
+    if (g_threads > 1) {
+#pragma omp parallel for schedule(static) num_threads(g_threads) reduction(+ : s)
+        for (size_t i = 0; i < n; i++) s += x[i] * y[i];
+        return s;
+    }
     for (size_t i = 0; i < n; i++) s += x[i] * y[i];
     return s;
+}
+
+static double nrm2(size_t n, const double *x)
+{
+    return sqrt(dot(n, x, x));
 }
 
 /* BlockMatrix::gmres (src/BlockMatrix.cpp:79-163) -- mode O_GMRES_LEFT_REF; and a

@@ -47,6 +47,8 @@ typedef struct {
 } o_eq;
 
 int o_nc(const o_eq *eq);
+/* host threads for timing (1, the default, keeps every loop in its serial order) */
+void o_set_threads(int n);
 
 /* geometry / basis helpers exposed for tests */
 void o_legendre(int p, double x, double *P, double *dP);

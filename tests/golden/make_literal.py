@@ -16,7 +16,8 @@ step, so index, sign or transposition mistakes in the factored oracle show up:
   * MassMatrix (src/MassMatrix.cpp:30-94), BlockILU0 (src/Preconditioners.cpp:49-158)
     with dense inverses, and BlockMatrix::gmres (src/BlockMatrix.cpp:79-163).
 Legendre polynomials come from the Bonnet recurrence (the reference calls
-gsl_sf_legendre_Pl).  Run:  python tests/golden/make_literal.py
+gsl_sf_legendre_Pl).  Run:  python tests/golden/make_literal.py  (p <= 2 cases), then
+python tests/golden/make_literal.py p3  (the p = 3 cases, minutes of pure Python)
 """
 from __future__ import annotations
 
@@ -40,6 +41,11 @@ LIN = {
     6: [(0.6612093864662645, 0.3607615730481386), (-0.6612093864662645, 0.3607615730481386),
         (-0.2386191860831969, 0.4679139345726910), (0.2386191860831969, 0.4679139345726910),
         (-0.9324695142031521, 0.1713244923791704), (0.9324695142031521, 0.1713244923791704)],
+    10: [(0.1488743389816312, 0.2955242247147529), (-0.1488743389816312, 0.2955242247147529),
+         (-0.4333953941292472, 0.2692667193099963), (0.4333953941292472, 0.2692667193099963),
+         (-0.6794095682990244, 0.2190863625159820), (0.6794095682990244, 0.2190863625159820),
+         (-0.8650633666889845, 0.1494513491505806), (0.8650633666889845, 0.1494513491505806),
+         (-0.9739065285171717, 0.0666713443086881), (0.9739065285171717, 0.0666713443086881)],
 }
 TRI = {
     2: [(0.16666666666667, 0.16666666666667, 0.33333333333333), (0.16666666666667, 0.66666666666667, 0.33333333333333),
@@ -47,12 +53,23 @@ TRI = {
     4: [(0.44594849091597, 0.44594849091597, 0.22338158967801), (0.44594849091597, 0.10810301816807, 0.22338158967801),
         (0.10810301816807, 0.44594849091597, 0.22338158967801), (0.09157621350977, 0.09157621350977, 0.10995174365532),
         (0.09157621350977, 0.81684757298046, 0.10995174365532), (0.81684757298046, 0.09157621350977, 0.10995174365532)],
+    6: [(0.24928674517091, 0.24928674517091, 0.11678627572638), (0.24928674517091, 0.50142650965818, 0.11678627572638),
+        (0.50142650965818, 0.24928674517091, 0.11678627572638), (0.06308901449150, 0.06308901449150, 0.05084490637021),
+        (0.06308901449150, 0.87382197101700, 0.05084490637021), (0.87382197101700, 0.06308901449150, 0.05084490637021),
+        (0.31035245103378, 0.63650249912140, 0.08285107561837), (0.63650249912140, 0.05314504984482, 0.08285107561837),
+        (0.05314504984482, 0.31035245103378, 0.08285107561837), (0.63650249912140, 0.31035245103378, 0.08285107561837),
+        (0.31035245103378, 0.05314504984482, 0.08285107561837), (0.05314504984482, 0.63650249912140, 0.08285107561837)],
 }
 
 
 def rules(deg2):
-    k = 2 if deg2 <= 2 else 4 if deg2 <= 4 else 6
-    return TRI[min(k, 4)] if k <= 4 else None, LIN[k]
+    """Quadrature::polygonIntegral / lineIntegral selection (src/Quadrature.cpp:20-27, 74-81):
+    deg <= 2, 4, 6 take the 3-, 6-, 12-point triangle and 2-, 4-, 6-point line rules (the
+    25-point / 10-point pair, for deg > 6, is not needed by the p <= 3 fixtures)."""
+    k = 2 if deg2 <= 2 else 4 if deg2 <= 4 else 6 if deg2 <= 6 else None
+    if k is None:
+        raise ValueError(f"rule of degree {deg2} is not tabulated here")
+    return TRI[k], LIN[k]
 
 
 def Pl(l, x):
@@ -530,8 +547,20 @@ def run_case(name, m, eqn, deg, u, dirichlet, spec):
     print(name, m.n_cells, "cells: residual", np.abs(res).max(), "gmres iters", iters)
 
 
-def main():
+def main(which=None):
     rng = np.random.default_rng(11)
+    if which == "p3":
+        # 5) periodic 3x3 quads, constant advection p=3 (12-point triangle, 6-point line rules)
+        m = periodic_rectangle(3, 3, 1.0, 1.0)
+        u = rng.standard_normal(m.n_cells * 10)
+        run_case("adv_quad3_p3", m, Advection((1.0, 0, 0, 0.5, 0, 0)), 3, u, lambda x, y: [0.0],
+                 dict(kind=0, beta=[1.0, 0, 0, 0.5, 0, 0]))
+        # 6) periodic 3x3 quads, Euler density wave p=3 (reference trace and reference Jacobian)
+        u = setups.interp(m, 3, setups.sine_density(), 4)
+        d = rng.standard_normal(u.shape) * 0.01
+        d.reshape(-1, 4, 10)[:, :, 0] = 0
+        run_case("euler_quad3_p3", m, Euler(1.4), 3, u + d, None, dict(kind=1))
+        return
     # 1) periodic 3x3 quads (reference generator), constant advection p=2
     m = periodic_rectangle(3, 3, 1.0, 1.0)
     u = rng.standard_normal(m.n_cells * 6)
@@ -556,4 +585,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1] if len(sys.argv) > 1 else None)

@@ -417,6 +417,101 @@ def test_tableaux_order_conditions():
     _order_conditions(*O.irk_tableau(3), 5)
 
 
+def _euler_fluxes(U, g=1.4):
+    r, mu, mv, E = U
+    P = (g - 1) * (E - 0.5 * (mu * mu + mv * mv) / r)
+    return (np.array([mu, mu * mu / r + P, mu * mv / r, (E + P) * mu / r]),
+            np.array([mv, mu * mv / r, mv * mv / r + P, (E + P) * mv / r]))
+
+
+def test_dirichlet_isentropic_vortex_is_an_euler_solution():
+    """o_eval_bc's isentropic vortex (Yee, Sandham and Djomehri 1999, moving with the free
+    stream): entropy P / rho^gamma = 1 everywhere, the free stream far away, a tangential
+    velocity perturbation of magnitude beta/(2 pi) r exp((1 - r^2)/2), and an exact solution of
+    the Euler equations (central differences of U_t + F_x + G_y vanish to difference accuracy)."""
+    from cpolyg_amd.spec import BC_ISENTROPIC_VORTEX
+    g, beta, x0, y0, ux, uy = 1.4, 5.0, 5.0, 5.0, 1.0, 1.0
+    spec = EquationSpec.euler(bc_kind=BC_ISENTROPIC_VORTEX, bc_par=(beta, x0, y0, ux, uy, 0, 0, 0))
+    rng = np.random.default_rng(4)
+    for x, y, t in zip(rng.uniform(3, 7, 12), rng.uniform(3, 7, 12), rng.uniform(0, 0.5, 12)):
+        U = O.eval_bc(spec, x, y, t)
+        r, u, v = U[0], U[1] / U[0], U[2] / U[0]
+        P = (g - 1) * (U[3] - 0.5 * r * (u * u + v * v))
+        assert abs(P / r ** g - 1) < 1e-13
+        dx, dy = x - x0 - ux * t, y - y0 - uy * t
+        R = math.hypot(dx, dy)
+        du, dv = u - ux, v - uy
+        assert abs(du * dx + dv * dy) < 1e-13 * max(R, 1)
+        assert abs(math.hypot(du, dv) - beta / (2 * math.pi) * R * math.exp(0.5 * (1 - R * R))) < 1e-13
+        h = 1e-4
+        dU = (O.eval_bc(spec, x, y, t + h) - O.eval_bc(spec, x, y, t - h)) / (2 * h)
+        dF = (_euler_fluxes(O.eval_bc(spec, x + h, y, t))[0] - _euler_fluxes(O.eval_bc(spec, x - h, y, t))[0]) / (2 * h)
+        dG = (_euler_fluxes(O.eval_bc(spec, x, y + h, t))[1] - _euler_fluxes(O.eval_bc(spec, x, y - h, t))[1]) / (2 * h)
+        assert np.max(np.abs(dU + dF + dG)) < 1e-6
+    far = O.eval_bc(spec, 40.0, -30.0, 0.0)
+    np.testing.assert_allclose(far, [1, ux, uy, 1 / (g - 1) + 0.5 * (ux * ux + uy * uy)], rtol=0, atol=1e-14)
+
+
+def test_dirichlet_reference_vortex_properties():
+    """o_eval_bc's reference vortex (VortexSolution, eqns/EulerSetups.cpp:7-67): free stream
+    (cos theta, sin theta) with theta = atan2(1, 2), centre (5, 5) moving with it, pressure equal
+    to density (both are base^(1/(gamma-1)) with rhoInf = pInf = 1), a velocity perturbation
+    tangential about the moving centre of magnitude eps/(2 pi rc) r exp(f/2), f = (1 - r^2)/rc^2,
+    and density (1 - eps^2 (gamma-1) M^2/(8 pi^2) exp f)^(1/(gamma-1))."""
+    from cpolyg_amd.spec import BC_REF_VORTEX
+    g, eps, rc, M = 1.4, 0.3, 1.5, 0.5
+    th = math.atan2(1, 2)
+    ub, vb = math.cos(th), math.sin(th)
+    spec = EquationSpec.euler(bc_kind=BC_REF_VORTEX)
+    rng = np.random.default_rng(6)
+    for x, y, t in zip(rng.uniform(2, 8, 12), rng.uniform(2, 8, 12), rng.uniform(0, 1, 12)):
+        U = O.eval_bc(spec, x, y, t)
+        r, u, v = U[0], U[1] / U[0], U[2] / U[0]
+        P = (g - 1) * (U[3] - 0.5 * r * (u * u + v * v))
+        assert abs(P - r) < 1e-13
+        dx, dy = x - 5 - ub * t, y - 5 - vb * t
+        R2 = dx * dx + dy * dy
+        f = (1 - R2) / (rc * rc)
+        du, dv = u - ub, v - vb
+        assert abs(du * dx + dv * dy) < 1e-13
+        assert abs(math.hypot(du, dv) - eps / (2 * math.pi * rc) * math.sqrt(R2) * math.exp(0.5 * f)) < 1e-13
+        assert abs(r - (1 - eps * eps * (g - 1) * M * M / (8 * math.pi ** 2) * math.exp(f)) ** (1 / (g - 1))) < 1e-13
+    far = O.eval_bc(spec, 60.0, 60.0, 0.0)
+    np.testing.assert_allclose(far[:3], [1, ub, vb], rtol=0, atol=1e-14)
+
+
+def test_threaded_oracle_matches_serial():
+    """The oracle's threaded loops (used only to time the CPU baseline) compute what the serial
+    ones do: residual, Jacobian and DILU apply bitwise, the DILU factor (Schur sums in another
+    order) and GMRES (dot products in another order) to rounding."""
+    m = vortex_mesh(12)
+    deg = 3
+    orc = O.Oracle(m)
+    spec = EquationSpec.euler(periodic_trace=1, jacobian=1)
+    u = vortex_state(m, deg, perturb=0.02)
+    J = orc.jacobian(spec, deg, u)
+    B = -0.1 * J
+    B[orc.rowptr[:-1]] += orc.mass(deg, 4)
+    b = orc.assemble(spec, deg, u)
+    F1, _ = O.dilu_factor(orc.rowptr, orc.colind, B)
+    x1, _, it1 = O.gmres(orc.rowptr, orc.colind, B, b, m=30, tol=1e-10, maxit=500, pc_kind=PC_DILU,
+                         mode=GMRES_FLEXIBLE)
+    try:
+        O.set_threads(4)
+        np.testing.assert_array_equal(orc.assemble(spec, deg, u), b)
+        np.testing.assert_array_equal(orc.jacobian(spec, deg, u), J)
+        F4, _ = O.dilu_factor(orc.rowptr, orc.colind, B)
+        assert rel(F4, F1) < 1e-12
+        np.testing.assert_array_equal(O.dilu_apply(orc.rowptr, orc.colind, F1, b),
+                                      (O.set_threads(1), O.dilu_apply(orc.rowptr, orc.colind, F1, b))[1])
+        O.set_threads(4)
+        x4, _, it4 = O.gmres(orc.rowptr, orc.colind, B, b, m=30, tol=1e-10, maxit=500, pc_kind=PC_DILU,
+                             mode=GMRES_FLEXIBLE)
+        assert abs(it4 - it1) <= 1 and rel(x4, x1) < 1e-8
+    finally:
+        O.set_threads(1)
+
+
 def _stability_function(A, b, z):
     """R(z) = 1 + z b^T (I - z A)^-1 1 of a Runge-Kutta tableau."""
     n = len(b)
