@@ -66,6 +66,7 @@ def parse():
     p.add_argument("--dirk3", choices=["l-stable", "reference"], default="l-stable",
                    help="3-stage DIRK tableau: Alexander's L-stable SDIRK3, or the reference's (not A-stable)")
     p.add_argument("--restart", type=int, default=50)
+    p.add_argument("--gmres-maxit", type=int, default=20000, help="GMRES iteration limit per Newton solve")
     p.add_argument("--mixed", choices=["fp32", "bf16", "inner-fp32", "inner-bf16"], default="bf16",
                    help="fp32/bf16: DILU factors stored in that precision inside FP64 FGMRES; inner-*: FP32 "
                         "GMRES(--inner) with those factors as the FGMRES preconditioner")
@@ -532,13 +533,13 @@ def bench_irk(args):
 
 def pmc_workload(args):
     """Kernel sequence of the HBM-traffic profile: a 1 GiB device copy of known size (calibrates
-    FETCH_SIZE / WRITE_SIZE), three FP64 BSR matvecs of the headline Newton matrix, one BF16 DILU
-    apply, then the matrix-free Newton operator of the same matrix: one prepare (point data) and
+    FETCH_SIZE / WRITE_SIZE), three FP64 BSR matvecs of the headline Newton matrix, one DILU apply
+    with factors in the --mixed precision (BF16 or FP32), then the matrix-free Newton operator of the same matrix: one prepare (point data) and
     three products.  Meant to run under rocprofv3 --pmc; prints nothing but a completion line."""
     import torch
 
     from cpolyg_amd import hip, setups
-    from cpolyg_amd.spec import EquationSpec, PC_DILU, PREC_BF16
+    from cpolyg_amd.spec import EquationSpec, PC_DILU, PREC_BF16, PREC_FP32
 
     mesh, u0 = workload(args.nx, args.deg)
     E = hip.Equation(hip.Mesh(mesh), EquationSpec.euler(periodic_trace=args.trace, jacobian=args.jacobian), args.deg)
@@ -546,7 +547,7 @@ def pmc_workload(args):
     u = torch.from_numpy(u0).cuda()
     dt = setups.dg_time_step(args.cfl, 10.0 / args.nx, args.deg, setups.euler_max_speed(u0, args.deg))
     E.jacobian(u, 0.0, J, scale=-dt * A_SS[args.dirk3], add_mass=True)
-    P = hip.Prec(J, PC_DILU, PREC_BF16)
+    P = hip.Prec(J, PC_DILU, PREC_BF16 if args.mixed == "bf16" else PREC_FP32)
     P.factor(J)
     x = torch.randn(E.n * E.bl, dtype=torch.float64, device="cuda")
     y = torch.empty_like(x)
@@ -616,7 +617,7 @@ def main():
         # each stage's Newton iteration starts from the previous stage's increment and halves an
         # update that would leave the physical states (the reference starts from zero; at CFL 20 its
         # first update on the third stage can overshoot to negative pressure)
-        return SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=args.restart, rtol=1e-10, maxit=20000,
+        return SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=args.restart, rtol=1e-10, maxit=args.gmres_maxit,
                           pc_kind=pc, newton_tol=5e-13, newton_maxit=100, precision=prec, inner_iters=args.inner,
                           newton_guess=1, newton_backtrack=1, dirk_tableau=dirk3,
                           pc_refresh=args.pc_refresh if refresh is None else refresh,
