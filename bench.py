@@ -67,9 +67,10 @@ def parse():
                    help="3-stage DIRK tableau: Alexander's L-stable SDIRK3, or the reference's (not A-stable)")
     p.add_argument("--restart", type=int, default=50)
     p.add_argument("--gmres-maxit", type=int, default=20000, help="GMRES iteration limit per Newton solve")
-    p.add_argument("--mixed", choices=["fp32", "bf16", "inner-fp32", "inner-bf16"], default="bf16",
-                   help="fp32/bf16: DILU factors stored in that precision inside FP64 FGMRES; inner-*: FP32 "
-                        "GMRES(--inner) with those factors as the FGMRES preconditioner")
+    p.add_argument("--mixed", choices=["fp32", "bf16", "bf16-d32", "inner-fp32", "inner-bf16"], default="bf16",
+                   help="fp32/bf16: DILU factors stored in that precision inside FP64 FGMRES; bf16-d32: L and DU "
+                        "blocks in BF16, diagonal inverses in FP32; inner-*: FP32 GMRES(--inner) with those factors "
+                        "as the FGMRES preconditioner")
     p.add_argument("--inner", type=int, default=20)
     p.add_argument("--jacobian", type=int, default=1, help="1: exact off-diagonal Jacobian, 0: reference form")
     p.add_argument("--operator", choices=["matrix-free", "assembled"], default="matrix-free",
@@ -130,11 +131,15 @@ def operator_bytes(mesh, deg, nc):
     return point + 32 * nv + 88 * ne + n * (32 + 8) + 2 * n * nc * bs * 8 + n * bs * bs * 8
 
 
-def algorithmic_bytes(n, nnzb, bl, elem, scales):
-    """matvec: blocks + colind + rowptr + x + y;  DILU apply: factor blocks (+ scales) + x in + x out."""
+def algorithmic_bytes(n, nnzb, bl, elem, scales, diag32=False):
+    """matvec: blocks + colind + rowptr + x + y;  DILU apply: factor blocks (+ scales) + x in + x out
+    (diag32: the n diagonal inverses in FP32, the other blocks at elem bytes with scales)."""
     N = n * bl
     matvec = nnzb * bl * bl * 8 + nnzb * 4 + (n + 1) * 4 + 2 * N * 8
-    apply = nnzb * bl * bl * elem + (nnzb * 4 if scales else 0) + 2 * N * 8
+    if diag32:
+        apply = (nnzb - n) * bl * bl * elem + (nnzb - n) * 4 + n * bl * bl * 4 + 2 * N * 8
+    else:
+        apply = nnzb * bl * bl * elem + (nnzb * 4 if scales else 0) + 2 * N * 8
     return matvec, apply
 
 
@@ -187,7 +192,7 @@ def standalone_rates(E, u, t, scale, reps=20):
     on the Newton matrix M + scale J(u, t)."""
     import torch
     from cpolyg_amd import hip
-    from cpolyg_amd.spec import PC_DILU, PREC_BF16, PREC_FP32, PREC_FP64
+    from cpolyg_amd.spec import PC_DILU, PREC_BF16, PREC_BF16_D32, PREC_FP32, PREC_FP64
 
     stream = torch.cuda.current_stream()
     J = E.jacobian_matrix()
@@ -212,7 +217,7 @@ def standalone_rates(E, u, t, scale, reps=20):
     out = {"bsr_matvec_ms": per_launch(lambda: J.matvec(x, y, stream)),
            "newton_prepare_ms": per_launch(lambda: E.newton_prepare(u, t, stream)),
            "newton_apply_ms": per_launch(lambda: E.newton_apply_prepared(scale, x, y, stream))}
-    for name, prec in (("bf16", PREC_BF16), ("fp32", PREC_FP32), ("fp64", PREC_FP64)):
+    for name, prec in (("bf16", PREC_BF16), ("bf16-d32", PREC_BF16_D32), ("fp32", PREC_FP32), ("fp64", PREC_FP64)):
         P = hip.Prec(J, PC_DILU, prec)
         P.factor(J, stream)
         out[f"dilu_apply_{name}_ms"] = per_launch(lambda: P.apply(y, stream), lambda: y.copy_(x))
@@ -539,7 +544,7 @@ def pmc_workload(args):
     import torch
 
     from cpolyg_amd import hip, setups
-    from cpolyg_amd.spec import EquationSpec, PC_DILU, PREC_BF16, PREC_FP32
+    from cpolyg_amd.spec import EquationSpec, PC_DILU, PREC_BF16, PREC_BF16_D32, PREC_FP32
 
     mesh, u0 = workload(args.nx, args.deg)
     E = hip.Equation(hip.Mesh(mesh), EquationSpec.euler(periodic_trace=args.trace, jacobian=args.jacobian), args.deg)
@@ -547,7 +552,7 @@ def pmc_workload(args):
     u = torch.from_numpy(u0).cuda()
     dt = setups.dg_time_step(args.cfl, 10.0 / args.nx, args.deg, setups.euler_max_speed(u0, args.deg))
     E.jacobian(u, 0.0, J, scale=-dt * A_SS[args.dirk3], add_mass=True)
-    P = hip.Prec(J, PC_DILU, PREC_BF16 if args.mixed == "bf16" else PREC_FP32)
+    P = hip.Prec(J, PC_DILU, {"bf16": PREC_BF16, "bf16-d32": PREC_BF16_D32}.get(args.mixed, PREC_FP32))
     P.factor(J)
     x = torch.randn(E.n * E.bl, dtype=torch.float64, device="cuda")
     y = torch.empty_like(x)
@@ -598,7 +603,8 @@ def main():
 
     from cpolyg_amd import hip, setups
     from cpolyg_amd.spec import (DIRK3_L_STABLE, DIRK3_REFERENCE, EquationSpec, GMRES_FLEXIBLE, OP_ASSEMBLED,
-                                 OP_MATRIX_FREE, PC_DILU, PC_INNER_GMRES, PREC_BF16, PREC_FP32, PREC_FP64, SolverSpec)
+                                 OP_MATRIX_FREE, PC_DILU, PC_INNER_GMRES, PREC_BF16, PREC_BF16_D32, PREC_FP32, PREC_FP64,
+                                 SolverSpec)
 
     deg = args.deg
     t_setup = time.perf_counter()
@@ -624,7 +630,7 @@ def main():
                           newton_operator=(OP_MATRIX_FREE if args.operator == "matrix-free" else OP_ASSEMBLED)
                           if op is None else op)
 
-    mixed_prec = PREC_BF16 if args.mixed.endswith("bf16") else PREC_FP32
+    mixed_prec = {"bf16": PREC_BF16, "bf16-d32": PREC_BF16_D32}.get(args.mixed.replace("inner-", ""), PREC_FP32)
     mixed_pc = PC_INNER_GMRES if args.mixed.startswith("inner") else PC_DILU
     steps = args.steps
     a_ss = A_SS[args.dirk3]
@@ -673,8 +679,8 @@ def main():
             res[key] = cmp_
     # per-launch rates on the last Newton matrix of the mixed run (stage-3 scale)
     rates = standalone_rates(E, u_last, t_last, -dt * a_ss)
-    elem = {PREC_BF16: 2, PREC_FP32: 4}[mixed_prec]
-    b_mv, b_ap = algorithmic_bytes(n, nnzb, bl, elem, mixed_prec == PREC_BF16)
+    elem = {PREC_BF16: 2, PREC_BF16_D32: 2, PREC_FP32: 4}[mixed_prec]
+    b_mv, b_ap = algorithmic_bytes(n, nnzb, bl, elem, mixed_prec != PREC_FP32, mixed_prec == PREC_BF16_D32)
     b_ap32 = algorithmic_bytes(n, nnzb, bl, 4, False)[1]
     b_ap64 = algorithmic_bytes(n, nnzb, bl, 8, False)[1]
     nm, msm, na, msa = kt
@@ -692,9 +698,13 @@ def main():
         # and sweep, forward level 0 and backward level 0 excepted, timed as one unit inside GMRES)
         ap_gbs = gbs(b_ap, ap_ms)
         lev = rates["dilu_levels"]
-        roof = {"bound": "hbm", "kernel": f"dilu_apply ({args.mixed} factors, 40x40 blocks, {2 * lev - 2} row-stream "
-                                          f"launches: {lev} colour levels x 2 sweeps, forward level 0 a no-op, the last "
-                                          f"forward level finishing its rows' backward step)",
+        if mixed_prec == PREC_BF16_D32:
+            shape = (f"{3 * lev - 1} row-stream launches: {lev - 1} forward levels (level 0 a no-op), then per "
+                     f"backward level the FP32 D^-1 products and the BF16 DU stream")
+        else:
+            shape = (f"{2 * lev - 2} row-stream launches: {lev} colour levels x 2 sweeps, forward level 0 a no-op, "
+                     f"the last forward level finishing its rows' backward step")
+        roof = {"bound": "hbm", "kernel": f"dilu_apply ({args.mixed} factors, {bl}x{bl} blocks, {shape})",
                 "achieved": ap_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ap_gbs / HBM_PEAK_GBS,
                 "traffic": traffic("dilu_apply", n, deg, args.mixed), "algorithmic_bytes": b_ap, "avg_ms": ap_ms,
                 "launches": na}
@@ -746,6 +756,7 @@ def main():
                                       "achieved_gbs": gbs(b, rates[f"dilu_apply_{p}_ms"]),
                                       "frac": gbs(b, rates[f"dilu_apply_{p}_ms"]) / HBM_PEAK_GBS}
                                   for p, b in (("bf16", algorithmic_bytes(n, nnzb, bl, 2, True)[1]),
+                                               ("bf16-d32", algorithmic_bytes(n, nnzb, bl, 2, True, True)[1]),
                                                ("fp32", b_ap32), ("fp64", b_ap64))},
         "newton_iters_per_step": newton,
         "gmres_iters_per_step": gm,

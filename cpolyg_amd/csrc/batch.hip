@@ -627,7 +627,9 @@ int cpg_batch_create(int nclass, const cpg_equation *eqs, const int *ninst, cons
                 // launch shapes of this class's block size: matvec (FP64) and sweeps (factor precision)
                 int best = 0;
                 ck(cpg_stream_autotune(C.J, CPG_FP64, 3, &best, nullptr, B->s));
-                if (params->precision != CPG_FP64) ck(cpg_stream_autotune(C.J, params->precision, 3, &best, nullptr, B->s));
+                if (params->precision != CPG_FP64)   // the stream's storage (BF16 for BF16 with FP32 D^-1)
+                    ck(cpg_stream_autotune(C.J, params->precision == CPG_BF16_D32 ? CPG_BF16 : params->precision, 3,
+                                           &best, nullptr, B->s));
                 B->cls.push_back(std::move(C));
             }
             B->N = off;

@@ -53,6 +53,20 @@ __host__ __device__ inline uint16_t f_to_bf16(float f) {
     return (uint16_t)(u >> 16);
 }
 
+// round to nearest even straight from FP64: the value is first rounded to FP32 by rounding to odd
+// (truncate, then set the last bit if anything was dropped), which keeps the information the
+// second rounding needs (FP32 carries 16 more significand bits than BF16), so the result is the
+// correctly rounded BF16 of the FP64 value, not a double rounding through the nearest FP32
+__device__ inline uint16_t d_to_bf16(double v) {
+    const float t = __double2float_rz(v);
+    uint32_t u;
+    memcpy(&u, &t, 4);
+    if ((double)t != v) u |= 1u;
+    float f;
+    memcpy(&f, &u, 4);
+    return f_to_bf16(f);
+}
+
 template <class T>
 struct Store;
 template <>
@@ -78,7 +92,7 @@ struct Store<bf16> {
     using acc = double;
     static constexpr int prec = CPG_BF16;
     __device__ static void put(bf16 *p, size_t e, double v, float inv_scale) {
-        p[e].v = f_to_bf16((float)(v * (double)inv_scale));
+        p[e].v = d_to_bf16(v * (double)inv_scale);
     }
     __device__ static float get(const bf16 *p, size_t e) { return bf16_to_f(p[e].v); }
 };
@@ -100,12 +114,14 @@ __device__ inline double wg_max_abs(double v, double *scratch) {
     return r;
 }
 
-// power-of-two scale s with max|v|/s in (0.5, 1]; 1 for a zero block
+// power-of-two scale s with max|v|/s in (0.5, 1]; 1 for a zero block.  The exponent is kept in
+// FP32's normal range, so s and 1/s are finite and exact; a block whose max lies outside it (beyond
+// ~1e38 or below ~1e-38) is stored relative to the nearest end of the range instead
 __device__ inline float pow2_scale(double mx) {
     if (!(mx > 0.0) || !isfinite(mx)) return 1.0f;
     int e;
     frexp(mx, &e);  // mx = f * 2^e, f in [0.5, 1)
-    return ldexpf(1.0f, e);
+    return ldexpf(1.0f, min(max(e, -125), 127));
 }
 
 // C(entries of this thread) = P * Q, all bl x bl column-major in LDS
@@ -1544,6 +1560,13 @@ struct cpg_prec_s {
     DevBuf<double> DD64;          // FP64 D^-1 (DILU)
     DevBuf<double> ybuf;          // forward-sweep result y
     DevBuf<int> status;
+    // CPG_BF16_D32: L and DU blocks in BF16, the diagonal inverses D_i^-1 in FP32 (DF, row order).
+    // The backward stream then packs only the DU blocks; D_i^-1 (y_i or x_i) goes to hbuf first.
+    bool diag32 = false;
+    DevBuf<float> DF;
+    DevBuf<double> hbuf;
+    DevBuf<int> hcol;             // source tag of each row's D^-1 product (stream_source)
+    std::vector<int> h_diag;      // diagonal slot of each row (host copy)
     size_t elem = 8;
     // inner FP32 GMRES (kind CPG_PC_INNER_GMRES)
     int inner_k = 20;
@@ -1763,7 +1786,8 @@ void apply_t(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s, const int *act
     // contiguous levels the first backward level is the same rows in the same order, and the
     // forward launch of the last level finishes them (FP64 vectors).
     const int nlf = (int)P.flev.size() - 1;
-    const bool fuse = x64 && rowstream_compiled(bl) && nlf >= 2 && P.frow0[nlf - 1] >= 0 && P.brow0.size() > 0 &&
+    const bool fuse = x64 && !P.diag32 && rowstream_compiled(bl) && nlf >= 2 && P.frow0[nlf - 1] >= 0 &&
+                      P.brow0.size() > 0 &&
                       P.brow0[0] == P.frow0[nlf - 1] && P.blev[1] - P.blev[0] == P.flev[nlf] - P.flev[nlf - 1];
     for (size_t lv = x64 ? 1 : 0; lv + 1 < P.flev.size(); lv++) {
         const int r0 = P.flev[lv], nr = P.flev[lv + 1] - r0;
@@ -1791,6 +1815,35 @@ void apply_t(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s, const int *act
             }
         }
         launch_rowstream<T, double>(bl, a, F, sc, s);
+    }
+    if constexpr (x64 && std::is_same<T, bf16>::value) {
+        if (P.diag32) {
+            // per backward level: h_i = D_i^-1 y_i from the FP32 inverses (y_i = x_i where row i has
+            // no lower blocks), then x_i = h_i - sum_j DU_ij x_j from the BF16 stream of DU blocks
+            for (size_t lv = 0; lv + 1 < P.blev.size(); lv++) {
+                const int r0 = P.blev[lv], nr = P.blev[lv + 1] - r0;
+                if (!nr) continue;
+                StreamArgs<double> h;
+                h.S = RowStream{nullptr, nullptr, P.hcol.p, active};
+                h.S.row0 = P.brow0[lv];   // contiguous levels: item w is row row0 + w, DF block row0 + w
+                h.items = nr;
+                h.src0 = y;
+                h.src2 = in;
+                h.c2 = 1.0;
+                h.out = P.hbuf.p;
+                launch_rowstream<float, double>(bl, h, P.DF.p, nullptr, s);
+                StreamArgs<double> a;
+                a.S = stream_of(P.brows, P.brow0, P.bptr, (int)lv, r0, P.scol.p, active);
+                a.items = nr;
+                a.nbmax = P.nbmax_b;
+                a.src1 = out;
+                a.c1 = -1.0;
+                a.base64 = P.hbuf.p;
+                a.out = out;
+                launch_rowstream<T, double>(bl, a, F, sc, s);
+            }
+            return;
+        }
     }
     for (size_t lv = fuse ? 1 : 0; lv + 1 < P.blev.size(); lv++) {
         const int r0 = P.blev[lv], nr = P.blev[lv + 1] - r0;
@@ -1927,10 +1980,14 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
         CPG_CHECK(A && out, CPG_ERR_ARG, "null argument");
         CPG_CHECK(kind == CPG_PC_NONE || kind == CPG_PC_BJACOBI || kind == CPG_PC_DILU || kind == CPG_PC_INNER_GMRES,
                   CPG_ERR_ARG, "bad kind %d", kind);
-        CPG_CHECK(precision >= CPG_FP64 && precision <= CPG_BF16, CPG_ERR_ARG, "bad precision %d", precision);
+        CPG_CHECK(precision >= CPG_FP64 && precision <= CPG_BF16_D32, CPG_ERR_ARG, "bad precision %d", precision);
+        CPG_CHECK(precision != CPG_BF16_D32 || kind == CPG_PC_DILU, CPG_ERR_ARG,
+                  "BF16 factors with FP32 diagonal inverses are a block-DILU storage");
         auto *P = new cpg_prec_s;
         try {
             P->kind = kind;
+            P->diag32 = precision == CPG_BF16_D32;
+            if (P->diag32) precision = CPG_BF16;   // the packed stream's storage
             P->precision = precision;
             P->n = A->n;
             P->bl = A->bl;
@@ -2034,15 +2091,33 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
                 for (int p = 0; p < n; p++) {
                     const int i = brows[p];
                     bptr[p] = cur;
-                    pos[diag[i]] = cur++;
-                    scol.push_back(lptr[i + 1] > lptr[i] ? -1 - i : -1 - (kSrc2Tag + i));
+                    if (!P->diag32) {
+                        pos[diag[i]] = cur++;
+                        scol.push_back(lptr[i + 1] > lptr[i] ? -1 - i : -1 - (kSrc2Tag + i));
+                    }
                     for (int q = uptr[i]; q < uptr[i + 1]; q++) {
                         pos[uslot[q]] = cur++;
                         scol.push_back(ucol[q]);
                     }
-                    P->nbmax_b = std::max(P->nbmax_b, 1 + uptr[i + 1] - uptr[i]);
+                    P->nbmax_b = std::max(P->nbmax_b, (P->diag32 ? 0 : 1) + uptr[i + 1] - uptr[i]);
                 }
                 bptr[n] = cur;
+                if (P->diag32) {
+                    // the BF16 copies of D^-1 (written by the factor, not read by the apply) after the
+                    // streams; the apply's D^-1 products read DF in row order, with these sources
+                    std::vector<int> hcol(n);
+                    for (int i = 0; i < n; i++) {
+                        pos[diag[i]] = cur++;
+                        hcol[i] = lptr[i + 1] > lptr[i] ? -1 - i : -1 - (kSrc2Tag + i);
+                    }
+                    P->hcol.upload(hcol);
+                    bool contiguous = true;
+                    for (int r : P->brow0) contiguous = contiguous && r >= 0;
+                    for (int r : P->frow0) contiguous = contiguous && r >= 0;
+                    CPG_CHECK(contiguous, CPG_ERR_UNSUPPORTED,
+                              "BF16 factors with FP32 diagonal inverses need contiguous colour levels "
+                              "(a colour-ordered matrix)");
+                }
                 // every slot is the diagonal, a lower or an upper block of its row
                 for (int q = 0; q < A->nnzb; q++)
                     CPG_CHECK(pos[q] >= 0, CPG_ERR_ARG, "slot %d is a second diagonal block of its row", q);
@@ -2057,7 +2132,12 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
                 if (precision == CPG_BF16) P->scale.alloc(A->nnzb);
                 P->DD64.alloc((size_t)n * bb);
                 P->ybuf.alloc((size_t)n * bl);   // FP64 intermediate for every factor precision
+                if (P->diag32) {
+                    P->DF.alloc((size_t)n * bb);
+                    P->hbuf.alloc((size_t)n * bl);
+                }
             }
+            P->h_diag = diag;
             if (kind == CPG_PC_INNER_GMRES) {
                 CPG_CHECK(precision != CPG_FP64, CPG_ERR_ARG, "the inner FP32 GMRES needs FP32 or BF16 factors");
                 P->N = (size_t)n * bl;
@@ -2101,6 +2181,10 @@ int cpg_prec_factor(cpg_prec P, cpg_bsr A, void *stream) {
         case CPG_FP64: factor_t<double>(*P, *A, s); break;
         case CPG_FP32: factor_t<float>(*P, *A, s); break;
         default: factor_t<bf16>(*P, *A, s); break;
+        }
+        if (P->diag32) {   // the FP64 D^-1 of every row, rounded once to FP32
+            hipLaunchKernelGGL(k_d2f, dim3(ceil_div(P->DF.n, 256)), dim3(256), 0, s, P->DD64.p, P->DF.p, P->DF.n);
+            CPG_LAUNCH_CHECK();
         }
         if (P->kind == CPG_PC_INNER_GMRES) {
             if (!P->Q.p) {
@@ -2189,6 +2273,12 @@ int cpg_prec_download(cpg_prec P, double *host_blocks) {
         }
         CPG_LAUNCH_CHECK();
         CPG_HIP(hipMemcpy(host_blocks, tmp.p, cnt * sizeof(double), hipMemcpyDeviceToHost));
+        if (P->diag32) {   // the diagonal slots hold the FP32 inverses the apply uses
+            std::vector<float> df(P->DF.n);
+            CPG_HIP(hipMemcpy(df.data(), P->DF.p, df.size() * sizeof(float), hipMemcpyDeviceToHost));
+            for (int i = 0; i < P->n; i++)
+                for (size_t e = 0; e < bb; e++) host_blocks[(size_t)P->h_diag[i] * bb + e] = df[(size_t)i * bb + e];
+        }
     });
 }
 
@@ -2197,6 +2287,7 @@ int cpg_prec_download(cpg_prec P, double *host_blocks) {
 namespace cpg {
 
 void prec_factor_coupled(cpg_prec P, const CoupledBlocks &C, hipStream_t s) {
+    CPG_CHECK(!P->diag32, CPG_ERR_UNSUPPORTED, "the coupled factor stores every block in one precision");
     CPG_CHECK(P && P->kind == CPG_PC_DILU && P->bl == CBL && C.bs > 0 && CSB % C.bs == 0, CPG_ERR_UNSUPPORTED,
               "the coupled DILU takes 3 stages of 40 x 40 blocks");
     CPG_CHECK(P->n == C.n && P->nnzb == C.nnzb, CPG_ERR_ARG, "preconditioner pattern (%d rows, %d slots) does not "

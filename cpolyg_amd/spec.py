@@ -28,6 +28,7 @@ GMRES_FLEXIBLE = 1
 PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16 = 2
+PREC_BF16_D32 = 3    # block DILU: L and DU blocks in BF16, the diagonal inverses in FP32
 
 DIRK3_REFERENCE = 0  # the reference's 3-stage tableau (src/TimeIntegration.cpp:71-77; not A-stable)
 DIRK3_L_STABLE = 1   # Alexander's L-stable SDIRK3 (not in the reference)

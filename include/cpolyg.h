@@ -89,7 +89,11 @@ int cpg_stream_autotune(cpg_bsr A, int precision, int reps, int *best_wpr, float
  * factors are applied with FP64 accumulation, BF16 factors with FP32 per-lane
  * partial sums and an FP64 total.  x is FP64 in and out. */
 enum { CPG_PC_NONE = 0, CPG_PC_BJACOBI = 1, CPG_PC_DILU = 2, CPG_PC_INNER_GMRES = 3 };
-enum { CPG_FP64 = 0, CPG_FP32 = 1, CPG_BF16 = 2 };
+enum { CPG_FP64 = 0, CPG_FP32 = 1, CPG_BF16 = 2, CPG_BF16_D32 = 3 };
+/* CPG_BF16_D32 (block DILU of a colour-ordered matrix only): the L and DU blocks in BF16 with the
+ * per-block scale, the diagonal inverses D_i^-1 in FP32.  The inverses carry the conditioning of
+ * the diagonal blocks: with them in BF16 the p = 4 Euler Newton matrices at CFL 20 leave flexible
+ * GMRES stalled, in FP32 it converges as with FP32 factors.  No reference counterpart. */
 /* CPG_PC_INNER_GMRES: a variable preconditioner for CPG_GMRES_FLEXIBLE -- a fixed number
  * of FP32 GMRES iterations (cpg_prec_set_inner, default 20) on an FP32 copy of A, right-
  * preconditioned by the DILU factors in `precision` (FP32 or BF16); FP64 reductions,

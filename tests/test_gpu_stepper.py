@@ -5,7 +5,7 @@ import pytest
 import oracle as O
 from cpolyg_amd import hip, setups
 from cpolyg_amd.spec import (DIRK3_L_STABLE, EquationSpec, SolverSpec, GMRES_FLEXIBLE, GMRES_LEFT_REF, OP_ASSEMBLED,
-                             OP_MATRIX_FREE, PC_DILU, PC_BJACOBI, PREC_BF16, PREC_FP32, PREC_FP64, bs_of)
+                             OP_MATRIX_FREE, PC_DILU, PC_BJACOBI, PREC_BF16, PREC_BF16_D32, PREC_FP32, PREC_FP64, bs_of)
 from problems import rel, unit_periodic_mesh, vortex_mesh, vortex_state
 
 pytestmark = pytest.mark.gpu
@@ -90,7 +90,7 @@ def test_dirk3_l_stable_25_steps_mixed_bf16(cuda):
     print(f"25 L-stable SDIRK3 steps at CFL 20 (dt {dt:.4f}): max rel. difference from the oracle {worst:.2e}")
 
 
-@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32, PREC_BF16])
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32, PREC_BF16, PREC_BF16_D32])
 def test_dirk3_euler_p4_matches_oracle(cuda, prec):
     """Euler at p = 4 (60 x 60 blocks): one DIRK3 step (L-stable tableau) with FP64, FP32 and BF16
     DILU factors in flexible GMRES, matrix-free Newton operator, against the oracle's step."""
