@@ -163,6 +163,14 @@ __device__ inline void outer_state(const DevEq &eq, const Geo &g, const Edge &E,
 // are added in a fixed order at the end.  The neighbours' coefficients of an edge chunk are
 // staged in LDS with coalesced loads.
 //
+// LDS bandwidth, shared by the CU's waves, bounds much of this work, so the traces of the cell's
+// own coefficients are formed from uniform (scalar) loads of x instead of LDS reads: the 40 (60
+// at p=4) coefficients are the same for every lane.  This also frees the VGPRs that held LDS
+// operands (p=3 operator 168 -> 156, p=4 240 -> 157).  Each edge chunk loads the lanes' edge records
+// first, publishes the neighbour indices through LDS, and then issues the neighbours' coefficients
+// and bounding boxes together, so the neighbour data is one round trip behind the records; the test
+// values of an edge point go to the record as soon as they are formed.
+//
 // The operator (LIN) reads what depends on u alone from the point data k_linprep stored for
 // this u (Euler; advection needs none), so its inner loops hold only traces of v: no divisions,
 // square roots or traces of u per GMRES iteration.
@@ -175,8 +183,7 @@ struct CellOpLayout {
     static constexpr int JG = 64 / BS;                // entry groups of the contraction
     static constexpr int SW = BL;                     // staged doubles per edge (the input's coefficients)
     static constexpr int STG = EPC * SW;
-    static constexpr int SH0 = REC > STG ? REC : STG;
-    static constexpr int SH = SH0 > JG * BL ? SH0 : JG * BL;
+    static constexpr int SH = REC > JG * BL ? REC : JG * BL;
 };
 
 // acc[c] += sum over entries j = jg, jg + JG, ... < nent of D[j][k] G[j][c]
@@ -214,9 +221,14 @@ __global__ __launch_bounds__(64, DEG <= 3 ? 3 : 2) void k_cellop(DevEq eq, Geo g
     constexpr int BS = L::BS, BL = L::BL, NQL = L::NQL, SW = L::SW, JG = L::JG;
     __shared__ __align__(16) double sX[BL];   // the cell's coefficients of x
     __shared__ __align__(16) double sh[L::SH];
+    __shared__ int sNbi[L::EPC];   // the neighbour of each edge of the current edge chunk
+    __shared__ __align__(16) double stg[L::STG];   // the neighbours' coefficients of x for that chunk
     double *D = sh;
     double *G = sh + L::NE * BS;
     const int i = blockIdx.x, lane = threadIdx.x;
+    // the cell's coefficients are the same for every lane: the traces read them through uniform
+    // (scalar) loads, sX keeps a copy for the mass term
+    const double *xc = x + (size_t)i * BL;
     for (int r = lane; r < BL; r += 64) sX[r] = x[(size_t)i * BL + r];
     const double4 bb = g.bb[i];
     const double sx = 2.0 / bb.z, sy = 2.0 / bb.w;
@@ -226,6 +238,9 @@ __global__ __launch_bounds__(64, DEG <= 3 ? 3 : 2) void k_cellop(DevEq eq, Geo g
 #pragma unroll
     for (int c = 0; c < NC; c++) acc[c] = 0.0;
     __syncthreads();
+
+    const double *lr = line_rule<NQL>();
+    const int e0 = g.eptr[i], ne = g.eptr[i + 1] - e0;
 
     // ---- volume points, 64 per chunk
     const int v0 = g.vptr[i], v1 = g.vptr[i + 1];
@@ -240,7 +255,7 @@ __global__ __launch_bounds__(64, DEG <= 3 ? 3 : 2) void k_cellop(DevEq eq, Geo g
             basis<DEG>(xi, eta, phi, dxi, deta);
 #pragma unroll
             for (int k = 0; k < BS; k++) D[lane * BS + k] = dxi[k];
-            eval_state<BS, NC>(sX, phi, X);
+            eval_state<BS, NC>(xc, phi, X);
             if constexpr (LIN) {
                 if constexpr (NC == 4) {
                     double p[LPV];
@@ -278,41 +293,50 @@ __global__ __launch_bounds__(64, DEG <= 3 ? 3 : 2) void k_cellop(DevEq eq, Geo g
     }
 
     // ---- edge points, EPC edges per chunk
-    const double *lr = line_rule<NQL>();
-    const int e0 = g.eptr[i], ne = g.eptr[i + 1] - e0;
     for (int eb = 0; eb < ne; eb += L::EPC) {
         const int nec = min(L::EPC, ne - eb);
-        __syncthreads();   // readers of the previous record are done before the staging overwrites it
+        const int np = nec * NQL;
+        // the lane's edge record first, so that the neighbour loads below depend on one round trip
+        const bool pt = lane < np;
+        const int el = lane / NQL, q = lane - el * NQL;
+        const Edge E = g.edges[e0 + eb + (pt ? el : 0)];
+        // the neighbours' coefficients (neighbour indices from the lanes holding each edge's first
+        // point, through LDS) and the lane's neighbour bounding box
+        double4 nbb = make_double4(0.0, 0.0, 1.0, 1.0);
+        if (pt && E.nb >= 0) nbb = g.bb[E.nb];
+        if (pt && q == 0) sNbi[el] = E.nb;
+        __syncthreads();   // readers of the previous chunk's staging and of the record are done
         for (int idx = lane; idx < nec * SW; idx += 64) {
             const int el = idx / SW, r = idx - el * SW;
-            const int nb = g.edges[e0 + eb + el].nb;   // -1 on a Dirichlet edge
-            sh[idx] = nb >= 0 ? x[(size_t)nb * BL + r] : 0.0;
+            const int nb = sNbi[el];   // -1 on a Dirichlet edge
+            stg[idx] = nb >= 0 ? x[(size_t)nb * BL + r] : 0.0;
         }
         __syncthreads();
-        const int np = nec * NQL;
-        double phi[BS], Ge[NC];
-        if (lane < np) {
-            const int el = lane / NQL, q = lane - el * NQL;
-            const Edge E = g.edges[e0 + eb + el];
+        if (pt) {
             const double s = 0.5 * (lr[2 * q] + 1.0);
             const double px = E.x1 + s * (E.x2 - E.x1), py = E.y1 + s * (E.y2 - E.y1);
             const double W = 0.5 * E.len * lr[2 * q + 1];
             const double xi = 2.0 * (px - bb.x) / bb.z - 1.0, eta = 2.0 * (py - bb.y) / bb.w - 1.0;
-            basis<DEG>(xi, eta, phi, nullptr, nullptr);
             double Xm[NC], Xp[NC];
-            eval_state<BS, NC>(sX, phi, Xm);
+            {
+                double phi[BS];
+                basis<DEG>(xi, eta, phi, nullptr, nullptr);
+#pragma unroll
+                for (int k = 0; k < BS; k++) D[lane * BS + k] = phi[k];   // the entry's test values
+                eval_state<BS, NC>(xc, phi, Xm);
+            }
             if (E.kind == 2) {
                 // Dirichlet: the outer state is data (k_bc_states), independent of u
 #pragma unroll
                 for (int c = 0; c < NC; c++) Xp[c] = LIN ? 0.0 : g.bcst[(size_t)(E.bslot * NQL + q) * NC + c];
             } else {
                 double phin[BS];
-                const double4 b = g.bb[E.nb];
+                const double4 b = nbb;
                 const double xo = E.kind == 0 ? px : px + E.tx, yo = E.kind == 0 ? py : py + E.ty;
                 const double xin = 2.0 * (xo - b.x) / b.z - 1.0, etan = 2.0 * (yo - b.y) / b.w - 1.0;
                 if (E.kind == 0) basis<DEG>(xin, etan, phin, nullptr, nullptr);
                 else trace_basis<DEG>(eq.periodic_trace, xin, etan, phin);
-                eval_state<BS, NC>(sh + el * SW, phin, Xp);
+                eval_state<BS, NC>(stg + el * SW, phin, Xp);
             }
             double F[NC];
             if constexpr (LIN) {
@@ -329,14 +353,7 @@ __global__ __launch_bounds__(64, DEG <= 3 ? 3 : 2) void k_cellop(DevEq eq, Geo g
                 numflux<NC>(eq, Xm, Xp, px, py, E.nx, E.ny, F);
             }
 #pragma unroll
-            for (int c = 0; c < NC; c++) Ge[c] = -W * F[c];
-        }
-        __syncthreads();   // staged coefficients are read before the record overwrites them
-        if (lane < np) {
-#pragma unroll
-            for (int k = 0; k < BS; k++) D[lane * BS + k] = phi[k];
-#pragma unroll
-            for (int c = 0; c < NC; c++) G[lane * NC + c] = Ge[c];
+            for (int c = 0; c < NC; c++) G[lane * NC + c] = -W * F[c];
         }
         __syncthreads();
         if (cl) cellop_contract<BS, NC, JG>(D, G, jg, ck, np, acc);
