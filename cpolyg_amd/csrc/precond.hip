@@ -335,185 +335,6 @@ __global__ __launch_bounds__(WG) void k_bjacobi_factor(int bl, const int *__rest
     store_block<T, NE>(bl, F + (size_t)i * bb, scale ? scale + i : nullptr, c, scratch);
 }
 
-// Block Jacobi for the stage-coupled 120 x 120 blocks, register-tiled: thread (rg, cg) of an
-// RG x CG grid keeps rows [RPT rg, RPT rg + RPT) x columns [CPT cg, CPT cg + CPT) of the block
-// in registers.  Gauss-Jordan in exchange form with partial pivoting (first maximal |pivot|
-// among the rows not yet pivotal): step k exchanges column k with pivot row p_k.  The owners
-// of column k post their best candidate and the column to LDS, every thread picks the pivot
-// from the RG candidates, the owners of row p_k post that row, and each thread updates its
-// tile: X[p][k] = 1/v, X[p][c] = -X[p][c]/v, X[r][k] = X[r][k]/v, X[r][c] -= X[r][k] X[p][c]/v
-// (v = X[p][k]).  Rows are never swapped: the inverse is A^-1[q(r)][p_c] = X[r][c] with
-// q(p_k) = k.  Two barriers per step; the tile is the only large state, so several blocks
-// share a CU (a block held in LDS, 116 KB, leaves room for one: 2.5 us per block against
-// 0.74 us here at 65,536 blocks).
-template <int BL>
-struct RTile;
-template <>
-struct RTile<120> {
-    static constexpr int RG = 12, CG = 20, RPT = 10, CPT = 6;
-};
-
-// inverse of the block at a (column-major, FP64) to outT in precision T (BF16: its scale to
-// *scale_out) and, with OUT64, in FP64 to out64 (which may equal a: every thread has read its
-// tile before the first barrier of the elimination); whole workgroup
-template <class T, int BL, bool OUT64>
-__device__ inline void rt_gauss_jordan(const double *a, double *out64, T *__restrict__ outT,
-                                       float *scale_out, int *__restrict__ status) {
-    using R = RTile<BL>;
-    constexpr int RG = R::RG, CG = R::CG, RPT = R::RPT, CPT = R::CPT;
-    static_assert(RG * RPT == BL && CG * CPT == BL && RG * CG <= WG && RPT <= 32, "tiling");
-    __shared__ double cv[2][RG];
-    __shared__ int cr[2][RG];
-    __shared__ double fcol[2][BL], prow[2][BL];
-    __shared__ int pr[BL], qr[BL];
-    __shared__ double red[WG / 64];
-    const int tid = threadIdx.x;
-    const bool act = tid < RG * CG;
-    const int rg = act ? tid % RG : 0, cg = act ? tid / RG : 0;   // consecutive threads: row groups
-    const int r0 = rg * RPT, c0 = cg * CPT;
-    double x[RPT][CPT];
-#pragma unroll
-    for (int c = 0; c < CPT; c++)
-#pragma unroll
-        for (int r = 0; r < RPT; r++) x[r][c] = act ? a[(size_t)(c0 + c) * BL + r0 + r] : 0.0;
-    unsigned used = 0;   // bit r: row r0 + r has been a pivot row
-    bool ok = true;
-#pragma unroll 1
-    for (int k = 0; k < BL; k++) {
-        const int b = k & 1;
-        const int kg = k / CPT, kc = k - kg * CPT;
-        if (act && cg == kg) {
-            double best = -1.0, bv = 0.0;
-            int br = -1;
-            double col[RPT];
-#pragma unroll
-            for (int c = 0; c < CPT; c++) {
-                if (c == kc) {
-#pragma unroll
-                    for (int r = 0; r < RPT; r++) col[r] = x[r][c];
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < RPT; r++) {
-                const double v = col[r];
-                fcol[b][r0 + r] = v;
-                const double m = fabs(v);
-                if (!((used >> r) & 1u) && m > best) {
-                    best = m;
-                    bv = v;
-                    br = r0 + r;
-                }
-            }
-            cv[b][rg] = bv;
-            cr[b][rg] = br;
-        }
-        __syncthreads();
-        // all candidates loaded first (their reads in flight together), then reduced in order
-        int crr[RG];
-        double cvv[RG];
-#pragma unroll
-        for (int g = 0; g < RG; g++) {
-            crr[g] = cr[b][g];
-            cvv[g] = cv[b][g];
-        }
-        double pa = -1.0, pv = 0.0;
-        int p = -1;
-#pragma unroll
-        for (int g = 0; g < RG; g++) {
-            const double m = fabs(cvv[g]);
-            const bool better = crr[g] >= 0 && m > pa;
-            pa = better ? m : pa;
-            pv = better ? cvv[g] : pv;
-            p = better ? crr[g] : p;
-        }
-        if (p < 0 || !(pa > 0.0) || !isfinite(pv)) {   // the same for every thread
-            ok = false;
-            break;
-        }
-        // p is the same in every lane: scalar, so that the pivot row and column below are
-        // selected by uniform branches and the update itself has none
-        p = __builtin_amdgcn_readfirstlane(p);
-        const int pg = p / RPT, pl = p - pg * RPT;
-        const bool rowhit = act && rg == pg, colhit = act && cg == kg;
-        if (rowhit) {
-#pragma unroll
-            for (int r = 0; r < RPT; r++) {
-                if (r == pl) {
-#pragma unroll
-                    for (int c = 0; c < CPT; c++) prow[b][c0 + c] = x[r][c];
-                }
-            }
-        }
-        if (tid == 0) pr[k] = p;
-        __syncthreads();
-        const double ip = 1.0 / pv;
-        double f[RPT], q[CPT];
-#pragma unroll
-        for (int r = 0; r < RPT; r++) f[r] = fcol[b][r0 + r];
-#pragma unroll
-        for (int c = 0; c < CPT; c++) q[c] = prow[b][c0 + c] * ip;
-        // X[r][c] -= X[r][k] X[p][c] / v everywhere, then the pivot column and row overwritten
-#pragma unroll
-        for (int r = 0; r < RPT; r++)
-#pragma unroll
-            for (int c = 0; c < CPT; c++) x[r][c] = fma(-f[r], q[c], x[r][c]);
-#pragma unroll
-        for (int c = 0; c < CPT; c++) {
-            if (c == kc) {
-#pragma unroll
-                for (int r = 0; r < RPT; r++) x[r][c] = colhit ? f[r] * ip : x[r][c];
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < RPT; r++) {
-            if (r == pl) {
-#pragma unroll
-                for (int c = 0; c < CPT; c++) x[r][c] = rowhit ? (colhit && c == kc ? ip : -q[c]) : x[r][c];
-            }
-        }
-        if (rowhit) used |= 1u << pl;
-    }
-    if (!ok) {
-        if (tid == 0) atomicOr(status, 1);
-        return;
-    }
-    if (tid < BL) qr[pr[tid]] = tid;
-    __syncthreads();
-    float inv = 1.0f;
-    if constexpr (Store<T>::prec == CPG_BF16) {
-        double mx = 0.0;
-#pragma unroll
-        for (int r = 0; r < RPT; r++)
-#pragma unroll
-            for (int c = 0; c < CPT; c++) mx = fmax(mx, act ? fabs(x[r][c]) : 0.0);
-        mx = wg_max_abs(mx, red);
-        const float sc = pow2_scale(mx);
-        inv = 1.0f / sc;
-        if (tid == 0) *scale_out = sc;
-    }
-    if (act) {
-#pragma unroll
-        for (int c = 0; c < CPT; c++) {
-            const size_t col = (size_t)pr[c0 + c] * BL;
-#pragma unroll
-            for (int r = 0; r < RPT; r++) {
-                Store<T>::put(outT, col + qr[r0 + r], x[r][c], inv);
-                if constexpr (OUT64) out64[col + qr[r0 + r]] = x[r][c];
-            }
-        }
-    }
-}
-
-template <class T, int BL>
-__global__ __launch_bounds__(WG) void k_bjacobi_factor_rt(const int *__restrict__ diag, const double *__restrict__ A,
-                                                          T *__restrict__ F, float *__restrict__ scale,
-                                                          int *__restrict__ status) {
-    constexpr size_t bb = (size_t)BL * BL;
-    const int i = blockIdx.x;
-    rt_gauss_jordan<T, BL, false>(A + (size_t)diag[i] * bb, nullptr, F + (size_t)i * bb,
-                                  scale ? scale + i : nullptr, status);
-}
-
 // ---- wavefront-per-row factor: FP64 MFMA block products, in-register Gauss-Jordan -------
 //
 // Per level, one 64-lane wavefront per block row forms the L blocks and the Schur
@@ -755,6 +576,311 @@ __device__ inline bool wave_gauss_jordan(double *X) {
     }
     __syncthreads();
     return ok;
+}
+
+// ---- blocked Gauss-Jordan inverse on FP64 MFMA (bl = 120) ------------------------------
+//
+// Gauss-Jordan in exchange form with partial pivoting: step k takes the pivot row p among the
+// rows not yet pivotal (largest |X[p][k]|, the lower row on ties) and sets, with v = X[p][k],
+//     X[p][k] = 1/v,  X[p][c] = -X[p][c]/v,  X[r][k] = X[r][k]/v,  X[r][c] -= X[r][k] X[p][c]/v.
+// Rows are never swapped.  Here the steps are organised by panels of GJP columns.  The block
+// lives in MFMA accumulator tiles (the v_mfma_f64_16x16x4f64 result layout) over the padded
+// NP x NP matrix: wavefront w of the workgroup holds the tile columns w and w + 4.  Per panel:
+//  1. the wavefront holding the panel's columns copies them to LDS and runs the GJP exchange
+//     steps on them alone, two matrix rows per lane (rows l and l + 64): the pivot search is a
+//     wave-wide max, the pivot row comes by readlane, no barrier inside;
+//  2. the steps' effect on every other column c is
+//         X[:, c] <- X[:, c] - E_P X[P, c] - Q X[P, c],
+//     with P the panel's pivot rows, E_P their identity columns and Q the panel's columns after
+//     its steps: the rows P are cleared and a rank-GJP product is subtracted, which all four
+//     wavefronts do on MFMA with -Q and the previous rows P staged in LDS.
+// The arithmetic is that of the unblocked elimination except for the order in which each
+// element's rank-1 updates are summed.  The inverse is A^-1[s(r)][p(c)] = X[r][c] (s(r) the step
+// at which row r was the pivot row, p(c) the pivot row of step c), written out through LDS in
+// chunks of OC destination columns so that the global stores are contiguous.
+constexpr int GJP = 8;
+
+template <int BL>
+struct MG {
+    static constexpr int NP = (BL + 15) / 16 * 16;   // padded size
+    static constexpr int TM = NP / 16;               // tile rows and tile columns
+    static constexpr int PLD = GJP + 1;              // panel row stride
+    static constexpr int BLD = NP + 16;              // pivot-row stride: the four k rows of one
+                                                     // operand read fall on both bank halves
+    static constexpr int OC = 32;                    // destination columns per output chunk
+    static constexpr int OLD = BL + 1;
+    static constexpr int PB = NP * PLD + GJP * BLD;
+    static constexpr int STAGE = PB > OC * OLD ? PB : OC * OLD;
+};
+
+// pivot key: used rows < NaN < numbers; among numbers the larger |v| compared to its exponent and
+// leading 13 mantissa bits (a candidate within a factor 1 - 2^-13 of the largest may be chosen
+// over it), then the lower row (r < 128)
+__device__ inline unsigned gj_key(double v, bool used, int r) {
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(v) & 0x7fffffffffffffffull;
+    const unsigned tag = 127u - (unsigned)r;
+    const unsigned num = ((unsigned)(bits >> 39) + 2u) << 7;   // |v| < 2^1024: bits >> 39 < 2^24
+    const unsigned k = v != v ? 128u : num;
+    return (used ? 0u : k) | tag;
+}
+
+// one DPP step of a 32-bit max over the rows in RM
+template <int CTRL, int RM>
+__device__ inline unsigned dpp_max_u32(unsigned v) {
+    const unsigned m = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, RM, 0xF, false);
+    return m > v ? m : v;
+}
+
+// maximum over the 64 lanes (uniform result), same lane pattern as wave_max_u64
+__device__ inline unsigned wave_max_u32(unsigned v) {
+    v = dpp_max_u32<0xB1, 0xF>(v);
+    v = dpp_max_u32<0x4E, 0xF>(v);
+    v = dpp_max_u32<0x141, 0xF>(v);
+    v = dpp_max_u32<0x140, 0xF>(v);
+    v = dpp_max_u32<0x142, 0xA>(v);
+    v = dpp_max_u32<0x143, 0xC>(v);
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// inverse of the block at a (column-major, FP64) to outT in precision T (BF16: its scale to
+// *scale_out) and, with OUT64, in FP64 to out64 (which may equal a: the block is read into
+// registers before the first barrier); whole workgroup of 4 wavefronts
+template <class T, int BL, bool OUT64>
+__device__ inline void mfma_gauss_jordan(const double *a, double *out64, T *__restrict__ outT,
+                                         float *scale_out, int *__restrict__ status) {
+    using G = MG<BL>;
+    constexpr int NP = G::NP, TM = G::TM, PLD = G::PLD, BLD = G::BLD, OC = G::OC, OLD = G::OLD;
+    static_assert(TM <= 8 && NP <= 128 && BL % GJP == 0 && GJP % 4 == 0, "blocked inverse tiling");
+    __shared__ double stage[G::STAGE];
+    __shared__ int rowstep[NP];   // step at which row r was the pivot row, -1 before
+    __shared__ int pr[NP];        // pivot row of step k
+    __shared__ double red[WG / 64];
+    double *Pan = stage, *Brow = stage + NP * PLD;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    dx4 acc[TM][2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int col = 16 * (wv + 4 * h) + lr;
+#pragma unroll
+        for (int tm = 0; tm < TM; tm++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int row = 16 * tm + lq + 4 * i;
+                acc[tm][h][i] = (row < BL && col < BL) ? a[(size_t)col * BL + row] : 0.0;
+            }
+    }
+    if (tid < NP) rowstep[tid] = -1;
+    bool ok = true;   // false in the wavefront that met a zero or non-finite pivot
+#pragma unroll 1
+    for (int k0 = 0; k0 < BL; k0 += GJP) {
+        const int t = k0 >> 4, hf = (k0 >> 3) & 1;   // tile column of the panel, and its half
+        const int owner = t & 3, oh = t >> 2;
+        __syncthreads();   // the previous update is done with Pan and Brow
+        if (wv == owner && (lr >> 3) == hf) {
+            const int c = lr & 7;
+            if (oh) {
+#pragma unroll
+                for (int tm = 0; tm < TM; tm++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) Pan[(16 * tm + lq + 4 * i) * PLD + c] = acc[tm][1][i];
+            } else {
+#pragma unroll
+                for (int tm = 0; tm < TM; tm++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) Pan[(16 * tm + lq + 4 * i) * PLD + c] = acc[tm][0][i];
+            }
+        }
+        if (wv == owner) {
+            // the owner's own writes above, complete before its reads below
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int ra = lane, rb = lane + 64;
+            double xa[GJP], xb[GJP];
+#pragma unroll
+            for (int c = 0; c < GJP; c++) {
+                xa[c] = Pan[ra * PLD + c];
+                xb[c] = rb < NP ? Pan[rb * PLD + c] : 0.0;
+            }
+            bool ua = ra >= BL || rowstep[ra] >= 0;
+            bool ub = rb >= BL || rowstep[rb] >= 0;
+#pragma unroll
+            for (int j = 0; j < GJP; j++) {
+                const unsigned ka = gj_key(xa[j], ua, ra), kb = gj_key(xb[j], ub, rb);
+                const unsigned K = wave_max_u32(ka > kb ? ka : kb);
+                const int rho = 127 - (int)(K & 127u);
+                const bool hi = rho >= 64;
+                const int rl = rho & 63;
+                double q[GJP];
+                if (hi) {
+#pragma unroll
+                    for (int c = 0; c < GJP; c++) q[c] = readlane_d(xb[c], rl);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < GJP; c++) q[c] = readlane_d(xa[c], rl);
+                }
+                const double v = q[j];
+                const bool good = K >= 256u && v != 0.0 && isfinite(v);
+                ok = ok && good;
+                const double inv = good ? 1.0 / v : 1.0;
+#pragma unroll
+                for (int c = 0; c < GJP; c++) q[c] *= inv;
+                const double fa = xa[j], fb = xb[j];
+#pragma unroll
+                for (int c = 0; c < GJP; c++) {
+                    xa[c] = fma(-fa, q[c], xa[c]);
+                    xb[c] = fma(-fb, q[c], xb[c]);
+                }
+                xa[j] = fa * inv;
+                xb[j] = fb * inv;
+                // the pivot row: -q, and 1 / v in column j
+                if (lane == rl) {
+                    if (hi) {
+#pragma unroll
+                        for (int c = 0; c < GJP; c++) xb[c] = c == j ? inv : -q[c];
+                        ub = true;
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < GJP; c++) xa[c] = c == j ? inv : -q[c];
+                        ua = true;
+                    }
+                }
+                if (lane == 0) {
+                    rowstep[rho] = k0 + j;
+                    pr[k0 + j] = rho;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < GJP; c++) {
+                Pan[ra * PLD + c] = xa[c];
+                if (rb < NP) Pan[rb * PLD + c] = xb[c];
+            }
+        }
+        __syncthreads();
+        // panel columns take Q; elsewhere the rows P go to Brow (in step order) and are cleared;
+        // Brow is zero in the panel columns, so the product below leaves them as they are.
+        // Row 16 tm + lq + 4 i of a lane is its slot 4 tm + i; the slots holding pivot rows of
+        // this panel are found from the (uniform) pivot rows, not by a lookup per slot.
+        {
+            unsigned smask = 0;                   // slots of the panel's pivot rows (uniform)
+            unsigned mine = 0;                    // ... held by this lane (lq matches)
+            unsigned tj[4] = {0u, 0u, 0u, 0u};    // step of each of my slots, 4 bits a slot
+#pragma unroll
+            for (int j = 0; j < GJP; j++) {
+                const int r = __builtin_amdgcn_readfirstlane(pr[k0 + j]);
+                const int sl = (r >> 4) * 4 + ((r >> 2) & 3);
+                smask |= 1u << sl;
+                const bool me = (r & 3) == lq;
+                mine |= me ? 1u << sl : 0u;
+#pragma unroll
+                for (int w = 0; w < 4; w++) tj[w] |= (me && (sl >> 3) == w) ? (unsigned)j << (4 * (sl & 7)) : 0u;
+            }
+            smask = __builtin_amdgcn_readfirstlane(smask);
+#pragma unroll
+            for (int tm = 0; tm < TM; tm++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int sl = 4 * tm + i;
+                    if ((smask >> sl) & 1u) {
+                        if ((mine >> sl) & 1u) {
+                            const int js = (int)((tj[sl >> 3] >> (4 * (sl & 7))) & 15u);
+#pragma unroll
+                            for (int h = 0; h < 2; h++) {
+                                const int col = 16 * (wv + 4 * h) + lr;
+                                const bool pc = (col >> 3) == (k0 >> 3);
+                                Brow[js * BLD + col] = pc ? 0.0 : acc[tm][h][i];
+                                acc[tm][h][i] = 0.0;
+                            }
+                        }
+                    }
+                }
+            if (wv == owner) {
+                const int c = lr & 7;
+                if ((lr >> 3) == hf) {
+                    if (oh) {
+#pragma unroll
+                        for (int tm = 0; tm < TM; tm++)
+#pragma unroll
+                            for (int i = 0; i < 4; i++) acc[tm][1][i] = Pan[(16 * tm + lq + 4 * i) * PLD + c];
+                    } else {
+#pragma unroll
+                        for (int tm = 0; tm < TM; tm++)
+#pragma unroll
+                            for (int i = 0; i < 4; i++) acc[tm][0][i] = Pan[(16 * tm + lq + 4 * i) * PLD + c];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // X -= Q Brow
+#pragma unroll
+        for (int ks = 0; ks < GJP / 4; ks++) {
+            const int k = 4 * ks + lq;
+            const double b0 = Brow[k * BLD + 16 * wv + lr];
+            const double b1 = Brow[k * BLD + 16 * (wv + 4) + lr];
+#pragma unroll
+            for (int tm = 0; tm < TM; tm++) {
+                const double am = -Pan[(16 * tm + lr) * PLD + k];
+                acc[tm][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(am, b0, acc[tm][0], 0, 0, 0);
+                acc[tm][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(am, b1, acc[tm][1], 0, 0, 0);
+            }
+        }
+    }
+    __syncthreads();
+    float inv = 1.0f;
+    if constexpr (Store<T>::prec == CPG_BF16) {
+        double mx = 0.0;
+#pragma unroll
+        for (int tm = 0; tm < TM; tm++)
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) mx = fmax(mx, fabs(acc[tm][h][i]));
+        mx = wg_max_abs(mx, red);
+        const float sc = pow2_scale(mx);
+        inv = 1.0f / sc;
+        if (tid == 0) *scale_out = sc;
+    }
+    double *Out = stage;
+#pragma unroll 1
+    for (int d0 = 0; d0 < BL; d0 += OC) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int col = 16 * (wv + 4 * h) + lr;
+            const int d = col < BL ? pr[col] - d0 : -1;
+            if (d >= 0 && d < OC) {
+#pragma unroll
+                for (int tm = 0; tm < TM; tm++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int row = 16 * tm + lq + 4 * i;
+                        if (row < BL) Out[d * OLD + rowstep[row]] = acc[tm][h][i];
+                    }
+            }
+        }
+        __syncthreads();
+        const int ne = min(OC, BL - d0) * BL;
+        for (int e = tid; e < ne; e += WG) {
+            const int dc = e / BL, r = e - dc * BL;
+            const double v = Out[dc * OLD + r];
+            const size_t g = (size_t)d0 * BL + e;
+            Store<T>::put(outT, g, v, inv);
+            if constexpr (OUT64) out64[g] = v;
+        }
+        __syncthreads();
+    }
+    if (!ok && lane == 0) atomicOr(status, 1);
+}
+
+template <class T, int BL>
+__global__ __launch_bounds__(WG, 2) void k_bjacobi_factor_rt(const int *__restrict__ diag, const double *__restrict__ A,
+                                                             T *__restrict__ F, float *__restrict__ scale,
+                                                             int *__restrict__ status) {
+    constexpr size_t bb = (size_t)BL * BL;
+    const int i = blockIdx.x;
+    mfma_gauss_jordan<T, BL, false>(A + (size_t)diag[i] * bb, nullptr, F + (size_t)i * bb,
+                                    scale ? scale + i : nullptr, status);
 }
 
 // ---- DILU factor, one level: two kernels per level, then one launch for all DU blocks.
@@ -1270,7 +1396,7 @@ __global__ __launch_bounds__(IWG) void k_f32_combine(const float *__restrict__ Q
 //   Schur (one workgroup per row i of a level): per lower neighbour k: X <- (coef x I) D_k^-1,
 //     T = BD(J_ik) X (registers), X <- T (stored as L_ik), accY += X BD(J_ki) (a second set of
 //     accumulators); then S_i = C_ii - accY (coef x I) to DD64[i] element by element;
-//   inverse: D_i^-1 = S_i^-1 by the register-tiled Gauss-Jordan, to DD64[i] and the factors;
+//   inverse: D_i^-1 = S_i^-1 by the blocked Gauss-Jordan on MFMA, to DD64[i] and the factors;
 //   upper (after all levels): DU_ij = (D_i^-1 BD(J_ij)) (coef x I).
 // The block-diagonal products take 180 (BD X) and at most 320 (X BD) MFMAs per wavefront
 // against 480 for a dense 120 x 120 product.
@@ -1485,13 +1611,13 @@ __global__ __launch_bounds__(FWG) void k_cdilu_schur(RowLists L, CoupledSrc src,
 }
 
 template <class T>
-__global__ __launch_bounds__(WG) void k_cdilu_inv(const int *__restrict__ rows, const int *__restrict__ diag,
-                                                  const int *__restrict__ pos, double *DD64, T *__restrict__ F,
-                                                  float *__restrict__ scale, int *__restrict__ status) {
+__global__ __launch_bounds__(WG, 2) void k_cdilu_inv(const int *__restrict__ rows, const int *__restrict__ diag,
+                                                     const int *__restrict__ pos, double *DD64, T *__restrict__ F,
+                                                     float *__restrict__ scale, int *__restrict__ status) {
     const int i = rows[blockIdx.x];
     const int pd = pos[diag[i]];
     double *S = DD64 + (size_t)i * CBB;
-    rt_gauss_jordan<T, CBL, true>(S, S, F + (size_t)pd * CBB, scale ? scale + pd : nullptr, status);
+    mfma_gauss_jordan<T, CBL, true>(S, S, F + (size_t)pd * CBB, scale ? scale + pd : nullptr, status);
 }
 
 template <class T>

@@ -290,6 +290,68 @@ def test_singular_block_reports_status(cuda):
     assert e.value.status == 4
 
 
+def _blocks_120(n):
+    """n 120 x 120 blocks that exercise the pivot search: Gaussian, row-permuted with a zero
+    diagonal, graded over 16 orders of magnitude, random +-1 (ties), and diagonally dominant."""
+    rng = np.random.default_rng(11)
+    out = []
+    for q in range(n):
+        kind = q % 5
+        if kind == 0:
+            X = rng.standard_normal((120, 120))
+        elif kind == 1:
+            X = rng.standard_normal((120, 120))
+            X = X[rng.permutation(120)]
+            np.fill_diagonal(X, 0.0)
+        elif kind == 2:
+            d1 = 10.0 ** rng.uniform(-8, 8, 120)
+            d2 = 10.0 ** rng.uniform(-8, 8, 120)
+            X = d1[:, None] * rng.standard_normal((120, 120)) * d2[None, :]
+        elif kind == 3:
+            X = rng.choice([-1.0, 1.0], size=(120, 120))
+        else:
+            X = rng.standard_normal((120, 120)) + 120.0 * np.eye(120)
+        out.append(X)
+    return np.array(out)
+
+
+def test_blocked_inverse_120(cuda):
+    """The 120 x 120 inverses (blocked Gauss-Jordan on FP64 MFMA) against LAPACK's, per block
+    relative to cond(block) * 2^-52; the FP32 and BF16 factors are the FP64 ones rounded once
+    (bitwise); a block with a zero column reports the singular status."""
+    m = vortex_mesh(4)
+    orc = O.Oracle(m)
+    n, nnzb = m.n_cells, len(orc.colind)
+    B = np.zeros((nnzb, 120, 120))
+    diag = np.array([orc.rowptr[i] + list(orc.colind[orc.rowptr[i]:orc.rowptr[i + 1]]).index(i)
+                     for i in range(n)])
+    B[diag] = _blocks_120(n)
+    A = hip.Bsr(orc.rowptr, orc.colind, 120)
+    A.upload(B)
+    P64 = hip.Prec(A, PC_BJACOBI, PREC_FP64)
+    P64.factor()
+    F64 = P64.download()
+    for i in range(n):
+        ref = np.linalg.inv(B[diag[i]])
+        err = np.abs(F64[i] - ref).max() / np.abs(ref).max()
+        bound = 120 * np.linalg.cond(B[diag[i]]) * 2.0 ** -52
+        assert err < bound, (i, i % 5, err, bound)
+    P32 = hip.Prec(A, PC_BJACOBI, PREC_FP32)
+    P32.factor()
+    np.testing.assert_array_equal(P32.download(), F64.astype(np.float32).astype(np.float64))
+    P16 = hip.Prec(A, PC_BJACOBI, PREC_BF16)
+    P16.factor()
+    mx = np.abs(F64).reshape(n, -1).max(1)
+    s = np.ldexp(1.0, np.clip(np.frexp(mx)[1], -125, 127))[:, None, None]
+    np.testing.assert_array_equal(P16.download(), _rne_bf16(F64 / s) * s)
+    B[diag[5], 37, :] = 0.0     # download layout [c][r]: column 37 of block 5 is zero
+    A.upload(B)
+    P = hip.Prec(A, PC_BJACOBI, PREC_FP64)
+    with pytest.raises(hip.CpgError) as e:
+        P.factor()
+    assert e.value.status == 4
+
+
 @pytest.mark.parametrize("prec", [PREC_FP32, PREC_BF16])
 def test_inner_gmres_preconditioner_in_flexible_gmres(cuda, prec):
     """FP64 flexible GMRES preconditioned by FP32 GMRES(k) + DILU reaches the FP64 tolerance;
