@@ -96,6 +96,14 @@ def parse():
     p.add_argument("--batch-stages", type=int, default=2, help="batch: 1 backward Euler, 2 SDIRK2")
     p.add_argument("--batch-count", type=int, default=64)
     p.add_argument("--no-sequential", action="store_true", help="batch: skip the one-stepper-per-instance timing")
+    p.add_argument("--batch-mode", choices=["persistent", "graph"], default="persistent",
+                   help="batch: the timed solver (one cooperative launch per batched solve, or hipGraph-replayed "
+                        "launches per phase); the other one is timed beside it unless --no-batch-compare")
+    p.add_argument("--no-batch-compare", action="store_true")
+    p.add_argument("--batch-team-detail", action="store_true",
+                   help="batch, persistent: per instance [cells, degree, team size, team ms, Arnoldi steps]")
+    p.add_argument("--batch-wg-per-cu", type=int, default=0,
+                   help="batch, persistent: workgroups per CU (0: autotuned over the first solves)")
     p.add_argument("--irk-nx", type=int, default=512, help="irk: lattice size (cells = nx^2)")
     p.add_argument("--irk-cfl", type=float, default=10.0)
     p.add_argument("--irk-restart", type=int, default=60)
@@ -369,8 +377,10 @@ def bench_batch(args):
     """Batched implicit advection: 64 instances (Voronoi meshes of the unit square with
     log-uniform 500..20,000 cells, degrees 1..4, Gaussian bumps, rotating field, Dirichlet
     inflow), CFL 5 each, SDIRK2 (or backward Euler), FP32 block DILU in FP64 flexible
-    GMRES(30) to 1e-10, one batch per step with hipGraph-captured restart cycles.  Also times
-    the same instances stepped one stepper at a time (same solver) for comparison."""
+    GMRES(30) to 1e-10, one batch per step.  The timed solver is the persistent one (every
+    restart cycle of every instance in one cooperative launch per batched solve) or the
+    launch-per-phase one with hipGraph-captured restart cycles; the other is timed beside it,
+    and the same instances stepped one stepper at a time (same solver) for comparison."""
     import numpy as np
     import torch
 
@@ -384,40 +394,72 @@ def bench_batch(args):
     solver = SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=30, rtol=1e-10, maxit=6000, pc_kind=PC_DILU,
                         precision=PREC_FP32, newton_tol=1e-12, newton_maxit=10)
     eqs = [(hip.Equation(hip.Mesh(um), spec, p), cp) for p, um, cp, _, _, _ in classes]
-    t0 = time.perf_counter()
-    B = hip.Batch(eqs, args.batch_stages, solver, use_graph=True)
-    t_create = time.perf_counter() - t0
-    us = [torch.from_numpy(u0).cuda() for _, _, _, u0, _, _ in classes]
-    outs = [torch.zeros_like(u) for u in us]
     dts = [d for _, _, d in meta]
-    t = 0.0
-    for _ in range(args.warmup):
-        B.advance(us, dts, t, outs)
-        us, outs = outs, us
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    its = []
-    st = None
-    for _ in range(args.steps):
-        st, it = B.advance(us, dts, t, outs)
-        its.append(it)
-        us, outs = outs, us
-    torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) / args.steps
-    its = np.asarray(its)
+    modes = {"persistent": hip.Batch.PERSISTENT, "graph": hip.Batch.GRAPH}
+
+    def run(mode_name):
+        t0 = time.perf_counter()
+        B = hip.Batch(eqs, args.batch_stages, solver, mode=modes[mode_name])
+        t_create = time.perf_counter() - t0
+        if mode_name == "persistent" and args.batch_wg_per_cu:
+            B.set_teams(args.batch_wg_per_cu)
+        us = [torch.from_numpy(u0).cuda() for _, _, _, u0, _, _ in classes]
+        outs = [torch.zeros_like(u) for u in us]
+        t = 0.0
+        # warmup: at least the persistent solver's autotuner trials (up to 4 candidates, two solves per step)
+        for _ in range(max(args.warmup, 3 if mode_name == "persistent" else 0)):
+            B.advance(us, dts, t, outs)
+            us, outs = outs, us
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its = []
+        st = None
+        for _ in range(args.steps):
+            st, it = B.advance(us, dts, t, outs)
+            its.append(it)
+            us, outs = outs, us
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        final = [u.cpu().numpy() for u in us]
+        return B, wall, np.asarray(its), st, t_create, final
+
+    B, wall, its, st, t_create, final = run(args.batch_mode)
     cells = sum(n for n, _, _ in meta)
     dofs = sum(n * (p + 1) * (p + 2) // 2 for n, p, _ in meta)
+    solver_desc = ("one cooperative launch per batched solve (persistent teams)" if args.batch_mode == "persistent"
+                   else "hipGraph-replayed restart cycles")
     out = {"metric": "batched implicit steps: 64 advection solves, instance-steps/s", "value": len(meta) / wall,
            "unit": "instance-steps/s", "ms_per_batch_step": wall * 1e3, "steps": args.steps, "warmup": args.warmup,
            "config": {"workload": f"{len(meta)} rotating-field advection solves, Voronoi unit squares, "
                                   f"{min(n for n, _, _ in meta)}..{max(n for n, _, _ in meta)} cells, p 1..4, "
                                   f"{'SDIRK2' if args.batch_stages == 2 else 'backward Euler'} at CFL 5, FP32 DILU in "
-                                  f"FP64 FGMRES(30) rtol 1e-10, hipGraph restart cycles",
-                      "instances": len(meta), "cells": cells, "dofs": dofs,
+                                  f"FP64 FGMRES(30) rtol 1e-10, {solver_desc}",
+                      "instances": len(meta), "cells": cells, "dofs": dofs, "mode": args.batch_mode,
                       "classes": [{"deg": p, "instances": len(cp) - 1, "cells": int(um.n_cells)}
                                   for p, um, cp, _, _, _ in classes]},
            "gmres_steps_per_instance_step": {"min": int(its.min()), "max": int(its.max()), "mean": float(its.mean())},
-           "newton_iters_per_step": st.newton_iters, "graph": B.info(), "setup_s": t_setup, "create_s": t_create}
+           "newton_iters_per_step": st.newton_iters, "setup_s": t_setup, "create_s": t_create}
+    if args.batch_mode == "persistent":
+        tm = B.teams()
+        out["teams"] = {"grid": tm["grid"], "wg_per_cu": tm["wg_per_cu"],
+                        "team_wg": {"min": int(tm["team_wg"].min()), "max": int(tm["team_wg"].max())},
+                        "team_ms_last_solve": {"min": float(tm["team_ms"].min()), "max": float(tm["team_ms"].max()),
+                                               "mean": float(tm["team_ms"].mean())}}
+        if args.batch_team_detail:
+            out["teams"]["detail"] = [[int(n), int(d), int(w), round(float(ms), 3), int(i)] for (n, d, _), w, ms, i in
+                                      zip(meta, tm["team_wg"], tm["team_ms"], its[-1])]
+    else:
+        out["graph"] = B.info()
+    del B
+    if not args.no_batch_compare:
+        other = "graph" if args.batch_mode == "persistent" else "persistent"
+        B2, wall2, its2, _, _, final2 = run(other)
+        diff = max(float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)) for a, b in zip(final, final2))
+        out[other] = {"ms_per_batch_step": wall2 * 1e3, "value": len(meta) / wall2,
+                      "gmres_steps_per_instance_step_mean": float(its2.mean()),
+                      "max_rel_diff_vs_timed": diff}
+        out[f"speedup_vs_{other}"] = wall2 / wall
+        del B2
     if not args.no_sequential:
         # the same instances one stepper at a time (same solver, same kernels, one launch per operation
         # per instance)

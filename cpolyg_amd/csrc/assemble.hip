@@ -198,12 +198,15 @@ __device__ inline void cellop_contract(const double *D, const double *G, int jg,
 }
 
 // Point data of the Euler Newton operator: nv volume points (LPV fields each) and nep edge
-// points (LPE fields each), stored field-major so that a wavefront's loads are contiguous; read
-// once per product with non-temporal loads, so that they do not evict the Krylov vectors that
-// the preconditioner sweeps gather next
+// points (LPE fields each), cell by cell and field-major inside a cell, so that a wavefront's
+// loads are contiguous and a cell's data shares cache lines with its two neighbours in memory
+// only at its ends (field-major over all points, every field of a cell began and ended inside
+// lines that neighbouring cells, usually on other XCDs, fetched as well); read once per product
+// with non-temporal loads, so that they do not evict the Krylov vectors that the preconditioner
+// sweeps gather next.  Cell i with volume points [v0, v1) and edges [e0, e0 + ne):
 struct LinData {
-    const double *vol;   // vol[f * nv + q]
-    const double *edg;   // edg[f * nep + e * NQL + q] for edge e (global edge index) and point q
+    const double *vol;   // vol[LPV v0 + f (v1 - v0) + (q - v0)] for volume point q of cell i
+    const double *edg;   // edg[LPE NQL e0 + f ne NQL + (e - e0) NQL + k] for point k of edge e
     int nv, nep;
 };
 
@@ -261,7 +264,7 @@ __global__ __launch_bounds__(64, DEG <= 3 ? 3 : 2) void k_cellop(DevEq eq, Geo g
                     double p[LPV];
 #pragma unroll
                     for (int f = 0; f < LPV; f++)
-                        p[f] = __builtin_nontemporal_load(ld.vol + (size_t)f * ld.nv + q0 + lane);
+                        p[f] = __builtin_nontemporal_load(ld.vol + (size_t)LPV * v0 + (size_t)f * (v1 - v0) + (q0 - v0) + lane);
                     euler_jvp_p(eq.gamma, p, X, eq.jacobian == 0, Fx, Gy);
                 } else {
                     double bx, by;
@@ -342,9 +345,9 @@ __global__ __launch_bounds__(64, DEG <= 3 ? 3 : 2) void k_cellop(DevEq eq, Geo g
             if constexpr (LIN) {
                 if constexpr (NC == 4) {
                     double rec[LPE];
-                    const size_t ep = (size_t)(e0 + eb + el) * NQL + q;
+                    const double *ec = ld.edg + (size_t)LPE * NQL * e0 + (eb + el) * NQL + q;
 #pragma unroll
-                    for (int f = 0; f < LPE; f++) rec[f] = __builtin_nontemporal_load(ld.edg + (size_t)f * ld.nep + ep);
+                    for (int f = 0; f < LPE; f++) rec[f] = __builtin_nontemporal_load(ec + (size_t)f * ne * NQL);
                     euler_numflux_jvp_p(eq, rec, Xm, Xp, E.nx, E.ny, F);
                 } else {
                     F[0] = adv_numflux_jvp(eq, Xm[0], Xp[0], px, py, E.nx, E.ny);
@@ -410,7 +413,7 @@ __global__ __launch_bounds__(64) void k_linprep(DevEq eq, Geo g, const double *_
         eval_state<BS, NC>(sU, phi, U);
         euler_prim4(eq.gamma, U, p);
 #pragma unroll
-        for (int f = 0; f < LPV; f++) vol[(size_t)f * nv + q] = p[f];
+        for (int f = 0; f < LPV; f++) vol[(size_t)LPV * v0 + (size_t)f * (v1 - v0) + (q - v0)] = p[f];
     }
     const double *lr = line_rule<NQL>();
     const int e0 = g.eptr[i], ne = g.eptr[i + 1] - e0;
@@ -445,9 +448,9 @@ __global__ __launch_bounds__(64) void k_linprep(DevEq eq, Geo g, const double *_
                 eval_state<BS, NC>(stg + el * BL, phin, Up);
             }
             euler_edge_prep(eq.gamma, Um, Up, E.nx, E.ny, rec);
-            const size_t ep = (size_t)(e0 + eb + el) * NQL + q;
+            double *ec = edg + (size_t)LPE * NQL * e0 + (eb + el) * NQL + q;
 #pragma unroll
-            for (int f = 0; f < LPE; f++) edg[(size_t)f * nep + ep] = rec[f];
+            for (int f = 0; f < LPE; f++) ec[(size_t)f * ne * NQL] = rec[f];
         }
     }
 }

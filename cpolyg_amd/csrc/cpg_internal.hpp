@@ -130,6 +130,21 @@ void set_stream_wpr(int elem_bytes, int bl, int wpr);
 void prec_apply_to(cpg_prec P, const double *in, double *out, hipStream_t s, const int *active = nullptr);
 // the preconditioner's generation (next_generation): changes when its buffers are reallocated
 uint64_t prec_generation(cpg_prec P);
+// The factor streams of a DILU or block-Jacobi preconditioner as its sweeps read them: the
+// forward sweep takes the rows frows[flev[l] .. flev[l+1]) of level l, row frows[p] from the
+// packed blocks [fptr[p], fptr[p+1]) of F; the backward sweep likewise with blev, brows, bptr;
+// scol[b] is the source tag of packed block b (stream_source in rowstream.hpp).  Block Jacobi:
+// block i of F is D_i^-1 and the lists are unused.  The host vectors live as long as P.
+struct PrecTables {
+    int kind = 0, precision = 0, n = 0, bl = 0;
+    bool diag32 = false;
+    const void *F = nullptr;
+    const float *scale = nullptr;
+    const int *scol = nullptr;
+    const std::vector<int> *flev = nullptr, *frows = nullptr, *fptr = nullptr;
+    const std::vector<int> *blev = nullptr, *brows = nullptr, *bptr = nullptr;
+};
+PrecTables prec_tables(cpg_prec P);
 
 // y = A x for the Krylov solvers; `active` as for bsr_matvec
 struct LinOp {

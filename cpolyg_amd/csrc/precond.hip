@@ -1679,6 +1679,7 @@ struct cpg_prec_s {
     // backward), -1-i for D_i^-1 y_i (src0 = y), or the input vector (src2) where the source
     // row has no lower blocks, so that y = x there and forward level 0 needs no launch
     DevBuf<int> pos, fptr, bptr, scol;
+    std::vector<int> h_frows, h_brows, h_fptr, h_bptr;   // host copies (persistent batched solves)
     int nbmax_f = 1, nbmax_b = 1;
     // storage
     DevBuf<unsigned char> F;      // factor blocks in `precision`
@@ -1721,6 +1722,25 @@ struct cpg_prec_s {
 
 namespace cpg {
 uint64_t prec_generation(cpg_prec P) { return P ? P->gen : 0; }
+
+PrecTables prec_tables(cpg_prec P) {
+    PrecTables t;
+    t.kind = P->kind;
+    t.precision = P->precision;
+    t.n = P->n;
+    t.bl = P->bl;
+    t.diag32 = P->diag32;
+    t.F = P->F.p;
+    t.scale = P->scale.p;
+    t.scol = P->scol.p;
+    t.flev = &P->flev;
+    t.frows = &P->h_frows;
+    t.fptr = &P->h_fptr;
+    t.blev = &P->blev;
+    t.brows = &P->h_brows;
+    t.bptr = &P->h_bptr;
+    return t;
+}
 }  // namespace cpg
 
 namespace {
@@ -2251,7 +2271,11 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
                 P->fptr.upload(fptr);
                 P->bptr.upload(bptr);
                 P->scol.upload(scol);
+                P->h_fptr = fptr;
+                P->h_bptr = bptr;
             }
+            P->h_frows = frows;
+            P->h_brows = brows;
             P->status.alloc(1);
             if (P->dilu()) {
                 P->F.alloc((size_t)A->nnzb * bb * P->elem);

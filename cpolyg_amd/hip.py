@@ -416,7 +416,15 @@ class Batch:
     ``classes`` is a list of ``(Equation, cell_ptr)``: each equation is built on a union mesh
     (:func:`cpolyg_amd.mesh.union_mesh`) whose parts are the instances of that class."""
 
-    def __init__(self, classes, nstages: int, solver: _spec.SolverSpec, use_graph: bool = True):
+    EAGER, GRAPH, PERSISTENT = 0, 1, 2
+
+    def __init__(self, classes, nstages: int, solver: _spec.SolverSpec, use_graph: bool = True, mode=None):
+        """``mode``: ``Batch.EAGER``, ``Batch.GRAPH`` (restart cycles replayed from hipGraphs) or
+        ``Batch.PERSISTENT`` (one cooperative launch per batched solve); by default ``use_graph``
+        picks between the first two."""
+        if mode is None:
+            mode = self.GRAPH if use_graph else self.EAGER
+        self.mode = int(mode)
         self.classes = classes
         self.sp = solver_params(solver)
         n = len(classes)
@@ -425,7 +433,7 @@ class Batch:
         ninst = (C.c_int * n)(*[len(cp) - 1 for cp in self._cptr])
         cps = (_ip * n)(*[cp.ctypes.data_as(_ip) for cp in self._cptr])
         h = _vp()
-        _chk(lib().cpg_batch_create(n, eqs, ninst, cps, int(nstages), C.byref(self.sp), int(use_graph), C.byref(h)))
+        _chk(lib().cpg_batch_create(n, eqs, ninst, cps, int(nstages), C.byref(self.sp), self.mode, C.byref(h)))
         self.h = h
         self.ninst = sum(len(cp) - 1 for cp in self._cptr)
 
@@ -438,6 +446,20 @@ class Batch:
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         _chk(lib().cpg_batch_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return {"instances": a.value, "chunks": b.value, "graph_captured": bool(c.value)}
+
+    def set_teams(self, wg_per_cu: int = 0, max_team: int = 0, spin_limit: int = 0):
+        """Persistent mode: workgroups per CU (0: autotuned over the first solves), the most
+        workgroups per instance (0: as many as its rows can use) and the team barriers' wait
+        limit in sleep rounds (0: the default)."""
+        _chk(lib().cpg_batch_set_teams(self.h, int(wg_per_cu), int(max_team), C.c_uint(int(spin_limit))))
+
+    def teams(self):
+        """Persistent mode, after a solve: {"grid", "wg_per_cu", "team_wg", "team_ms" (per instance)}."""
+        g, w = C.c_int(), C.c_int()
+        tw = np.zeros(self.ninst, np.int32)
+        tm = np.zeros(self.ninst, np.float64)
+        _chk(lib().cpg_batch_teams(self.h, C.byref(g), C.byref(w), tw.ctypes.data_as(_ip), tm.ctypes.data_as(_dp)))
+        return {"grid": g.value, "wg_per_cu": w.value, "team_wg": tw, "team_ms": tm}
 
     def advance(self, us, dts, t, outs, stream=None):
         """One step of every instance; ``us``/``outs`` are per-class device vectors, ``dts``

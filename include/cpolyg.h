@@ -310,10 +310,21 @@ int cpg_stepper_kernel_timing(cpg_stepper S, int *n_matvec, double *ms_matvec, i
  * batched flexible GMRES(params->restart): matvec, DILU factor and sweeps cover whole
  * classes, inner products are segmented per instance, Hessenberg/Givens/least squares run
  * on the device per instance (an instance that converges stops while the others go on),
- * and with use_graph a restart cycle is captured once into hipGraphs (start, blocks of 5
- * Arnoldi steps, finish) and replayed, the host ending a cycle early once every instance
- * has converged.
- * The row-stream launch shapes of every class's block size are autotuned at creation.
+ * and the solve runs in one of three modes:
+ *   CPG_BATCH_EAGER       launches per phase (matvec and sweeps per class, segmented CGS);
+ *   CPG_BATCH_GRAPH       the same, a restart cycle captured once into hipGraphs (start, blocks
+ *                         of 5 Arnoldi steps, finish) and replayed, the host ending a cycle
+ *                         early once every instance has converged;
+ *   CPG_BATCH_PERSISTENT  one cooperative launch per batched solve, every restart cycle of
+ *                         every instance inside it: instance q is solved by its own team of
+ *                         workgroups (sized by its bytes per Arnoldi step times its Arnoldi
+ *                         count in the previous solve), with team barriers between the DILU
+ *                         levels, the matvec and the Gram-Schmidt reductions, so a converged
+ *                         instance leaves while the others go on and no instance waits on a
+ *                         launch boundary.  DILU (not the BF16 factors with FP32 inverses),
+ *                         block-Jacobi or no preconditioner; block sizes up to 64.
+ * The row-stream launch shapes of every class's block size are autotuned at creation (modes
+ * 0 and 1); the persistent mode tunes its workgroups per CU over its first solves.
  * DIRK with nstages 1 (backward Euler) or 2 (SDIRK2) and the other tableaux of
  * cpg_stepper_create; each instance has its own dt (dt[q]); the stage time passed to the
  * residual is t + c_s dt of a class's first instance, so the equations must have steady
@@ -321,10 +332,21 @@ int cpg_stepper_kernel_timing(cpg_stepper S, int *n_matvec, double *ms_matvec, i
  * class c.  Synchronous.  inst_gmres_iters (optional, ninst entries): Arnoldi steps per
  * instance in this step. */
 typedef struct cpg_batch_s *cpg_batch;
+enum { CPG_BATCH_EAGER = 0, CPG_BATCH_GRAPH = 1, CPG_BATCH_PERSISTENT = 2 };
 int cpg_batch_create(int nclass, const cpg_equation *eqs, const int *ninst, const int *const *cell_ptr, int nstages,
-                     const cpg_solver_params *params, int use_graph, cpg_batch *out);
+                     const cpg_solver_params *params, int mode, cpg_batch *out);
 int cpg_batch_destroy(cpg_batch B);
 int cpg_batch_info(cpg_batch B, int *ninst, int *nchunk, int *graph_captured);
+/* Persistent mode: workgroups per CU of the cooperative launch (0: autotune over the first
+ * solves, the default), the most workgroups one instance's team may have (0: as many as its
+ * rows can use) and the wait limit of a team barrier in sleep rounds (0: the default, a few
+ * seconds; a tiny limit forces the abort path: the solve then fails with CPG_ERR_HIP and every
+ * workgroup leaves the kernel). */
+int cpg_batch_set_teams(cpg_batch B, int wg_per_cu, int max_team, unsigned spin_limit);
+/* Persistent mode, after a solve: the grid, the workgroups per CU it used, each instance's team
+ * size and the time its team took in the last solve in ms (team_wg, team_ms: ninst entries each,
+ * optional). */
+int cpg_batch_teams(cpg_batch B, int *grid, int *wg_per_cu, int *team_wg, double *team_ms);
 int cpg_batch_advance(cpg_batch B, const double *const *u, const double *dt, double t, double *const *unp1,
                       cpg_step_stats *stats, int *inst_gmres_iters, void *stream);
 
