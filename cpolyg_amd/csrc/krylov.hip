@@ -14,8 +14,8 @@
 //    (iteration-count parity).
 //  * CPG_GMRES_FLEXIBLE: right-preconditioned flexible GMRES with everything on the device.
 //    The Hessenberg column, Givens rotations (c = rd/dd, s = -hnew/dd as in
-//    src/BlockMatrix.cpp:137-145), least squares and the convergence flag are FP64 kernels of
-//    one thread; a restart cycle is a start part, blocks of KBLK Arnoldi steps and a finish
+//    src/BlockMatrix.cpp:137-145) and the convergence flag are FP64 kernels of one thread, the
+//    least squares one wavefront; a restart cycle is a start part, blocks of KBLK Arnoldi steps and a finish
 //    part, each captured once into a hipGraph and replayed for every later solve with the same
 //    operands.  The host launches block b+1 before it reads the flag that block b left, so the
 //    device never waits on it; once the flag reads "converged" every kernel of a replayed step
@@ -33,9 +33,13 @@ namespace cpg {
 constexpr int KWG = 256;
 constexpr int KPT = 8;  // elements per thread in the vector kernels
 constexpr int KCH = KWG * KPT;
-constexpr int KBLK = 2;  // Arnoldi steps per captured block (flexible driver)
+constexpr int KBLK = 1;  // Arnoldi steps per captured block (flexible driver): the look-ahead block
+                         // queued after the step that converges runs as no-ops (~15 launches)
 
-// partial[b*nv + j] = sum over chunk b of V[j] . w
+// partial[b*nv + j] = sum over chunk b of V[j] . w.  Two basis vectors at a time, the next two
+// vectors' loads issued before this pair's products and wave reductions (one vector at a time,
+// every vector cost a full memory round trip, and the reductions stalled the next loads); the
+// summation order of every product is that of one vector at a time.
 __global__ __launch_bounds__(KWG) void k_mdot(const double *__restrict__ V, size_t N, int nv,
                                               const double *__restrict__ w, double *__restrict__ partial,
                                               const int *__restrict__ active) {
@@ -51,16 +55,40 @@ __global__ __launch_bounds__(KWG) void k_mdot(const double *__restrict__ V, size
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int j0 = 0; j0 < nv; j0 += 64) {
         const int jn = min(64, nv - j0);
-        for (int jj = 0; jj < jn; jj++) {
-            const double *v = V + (size_t)(j0 + jj) * N;
-            double s = 0.0;
+        // software pipeline over pairs of vectors: the next pair's loads are issued before this
+        // pair's products and reductions
+        double cur[2][KPT], nxt[2][KPT];
+        auto load = [&](int jj, double (&buf)[2][KPT]) {
 #pragma unroll
-            for (int k = 0; k < KPT; k++) {
-                const size_t e = base + (size_t)k * KWG;
-                if (e < N) s = fma(v[e], wv[k], s);
+            for (int u = 0; u < 2; u++) {
+                const bool in = jj + u < jn;
+                const double *v = V + (size_t)(j0 + (in ? jj + u : 0)) * N;
+#pragma unroll
+                for (int k = 0; k < KPT; k++) {
+                    const size_t e = base + (size_t)k * KWG;
+                    buf[u][k] = (in && e < N) ? v[e] : 0.0;
+                }
             }
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-            if (lane == 0) red[wid][jj] = s;
+        };
+        load(0, cur);
+        for (int jj = 0; jj < jn; jj += 2) {
+            if (jj + 2 < jn) load(jj + 2, nxt);
+            double s[2] = {0.0, 0.0};
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+#pragma unroll
+                for (int k = 0; k < KPT; k++) s[u] = fma(cur[u][k], wv[k], s[u]);
+            for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+                for (int u = 0; u < 2; u++) s[u] += __shfl_xor(s[u], off);
+            if (lane == 0)
+#pragma unroll
+                for (int u = 0; u < 2; u++)
+                    if (jj + u < jn) red[wid][jj + u] = s[u];
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+#pragma unroll
+                for (int k = 0; k < KPT; k++) cur[u][k] = nxt[u][k];
         }
         __syncthreads();
         if (threadIdx.x < jn) {
@@ -104,14 +132,26 @@ __global__ __launch_bounds__(KWG) void k_orth_norm(const double *__restrict__ V,
         const size_t e = base + (size_t)k * KWG;
         wv[k] = e < N ? w[e] : 0.0;
     }
-    for (int j = 0; j < nv; j++) {
-        const double hj = h[j];
-        const double *v = V + (size_t)j * N;
+    // four basis vectors' loads in flight together; w is updated vector by vector in order
+    for (int j0 = 0; j0 < nv; j0 += 4) {
+        double vv[4][KPT];
 #pragma unroll
-        for (int k = 0; k < KPT; k++) {
-            const size_t e = base + (size_t)k * KWG;
-            if (e < N) wv[k] = fma(-hj, v[e], wv[k]);
-        }
+        for (int u = 0; u < 4; u++)
+            if (j0 + u < nv) {
+                const double *v = V + (size_t)(j0 + u) * N;
+#pragma unroll
+                for (int k = 0; k < KPT; k++) {
+                    const size_t e = base + (size_t)k * KWG;
+                    vv[u][k] = e < N ? v[e] : 0.0;
+                }
+            }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (j0 + u < nv) {
+                const double hj = h[j0 + u];
+#pragma unroll
+                for (int k = 0; k < KPT; k++) wv[k] = fma(-hj, vv[u][k], wv[k]);
+            }
     }
     double s = 0.0;
 #pragma unroll
@@ -173,14 +213,25 @@ __global__ __launch_bounds__(KWG) void k_update(const double *__restrict__ B, si
         const size_t e = base + (size_t)k * KWG;
         xv[k] = e < N ? x[e] : 0.0;
     }
-    for (int j = 0; j < nk; j++) {
-        const double yj = y[j];
-        const double *b = B + (size_t)j * N;
+    for (int j0 = 0; j0 < nk; j0 += 4) {
+        double bv[4][KPT];
 #pragma unroll
-        for (int k = 0; k < KPT; k++) {
-            const size_t e = base + (size_t)k * KWG;
-            if (e < N) xv[k] = fma(-yj, b[e], xv[k]);
-        }
+        for (int u = 0; u < 4; u++)
+            if (j0 + u < nk) {
+                const double *b = B + (size_t)(j0 + u) * N;
+#pragma unroll
+                for (int k = 0; k < KPT; k++) {
+                    const size_t e = base + (size_t)k * KWG;
+                    bv[u][k] = e < N ? b[e] : 0.0;
+                }
+            }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (j0 + u < nk) {
+                const double yj = y[j0 + u];
+#pragma unroll
+                for (int k = 0; k < KPT; k++) xv[k] = fma(-yj, bv[u][k], xv[k]);
+            }
     }
 #pragma unroll
     for (int k = 0; k < KPT; k++) {
@@ -218,14 +269,13 @@ __global__ void k_fg_cycle_init(FgState S) {
     S.scal[3] = b;
 }
 
-// Arnoldi step i: store the column, apply the previous rotations, form the new one, update g
-// and test |g_{i+1}| against the tolerance (a non-finite estimate also ends the cycle)
-__global__ void k_fg_givens(int i, FgState S) {
-    if (!S.flags[0]) return;
+// Arnoldi step i: store the column (h_0..h_i from hcol, hnew = ||w||), apply the previous
+// rotations, form the new one, update g and test |g_{i+1}| against the tolerance (a non-finite
+// estimate also ends the cycle).  One thread.
+__device__ inline void fg_givens(int i, const FgState &S, double hnew) {
     const int m = S.m;
     double *h = S.H + (size_t)i * (m + 1);
     for (int r = 0; r <= i; r++) h[r] = S.hcol[r];
-    const double hnew = S.hcol[i + 1];
     for (int k = 0; k < i; k++) {
         const double t = S.cs[k] * h[k] - S.sn[k] * h[k + 1];
         h[k + 1] = S.sn[k] * h[k] + S.cs[k] * h[k + 1];
@@ -248,15 +298,72 @@ __global__ void k_fg_givens(int i, FgState S) {
     if (i + 1 == m) S.flags[0] = 0;
 }
 
-// R y = g over the Krylov dimension of the cycle
-__global__ void k_fg_solve(FgState S) {
-    const int kd = S.flags[1];
-    for (int r = 0; r < kd; r++) S.y[r] = S.g[r];
-    for (int c = kd - 1; c >= 0; c--) {
-        const double *h = S.H + (size_t)c * (S.m + 1);
-        S.y[c] /= h[c];
-        for (int r = 0; r < c; r++) S.y[r] -= h[r] * S.y[c];
+// Step i's tail in one launch: ||w||^2 is the sum of the orthogonalisation's chunk partial sums,
+// which every workgroup adds in k_reduce's order (so all hold the same norm, equal to the one
+// k_reduce gives); each workgroup scales its chunk, v_{i+1} = w / ||w||; workgroup 0 then stores
+// ||w|| in the Hessenberg column and runs the Givens update (fg_givens), and writes the resulting active
+// flag to `hflag` (host memory the driver polls; null: none).  A workgroup that starts after
+// workgroup 0 cleared the flag skips its chunk; the flag is cleared only when the cycle ends at
+// this step, and v_{i+1} is then never read.
+__global__ __launch_bounds__(KWG) void k_fg_norm_step(const double *__restrict__ partial, int nb, double *v, size_t N,
+                                                      int i, FgState S, int *hflag) {
+    __shared__ double red[KWG / 64];
+    if (!S.flags[0]) {
+        if (blockIdx.x == 0 && threadIdx.x == 0 && hflag) *hflag = 0;
+        return;
     }
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += KWG) s += partial[b];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double t = 0.0;
+    for (int q = 0; q < KWG / 64; q++) t += red[q];
+    const double nrm = sqrt(t);
+    const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    const size_t base = (size_t)blockIdx.x * KCH + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < KPT; k++) {
+        const size_t e = base + (size_t)k * KWG;
+        if (e < N) v[e] = v[e] * inv;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S.hcol[i + 1] = nrm;
+        fg_givens(i, S, nrm);
+        if (hflag) *hflag = S.flags[0];
+    }
+}
+
+// R y = g over the Krylov dimension of the cycle.  Up to 64 columns: one wavefront, the
+// triangle staged in LDS, lane r holding y_r (the operations and their order per entry are the
+// serial back substitution's); longer ones: one thread.
+__global__ __launch_bounds__(64) void k_fg_solve(FgState S) {
+    __shared__ double R[64 * 64];
+    const int kd = S.flags[1];
+    const int lane = threadIdx.x;
+    if (kd > 64) {
+        if (lane == 0) {
+            for (int r = 0; r < kd; r++) S.y[r] = S.g[r];
+            for (int c = kd - 1; c >= 0; c--) {
+                const double *h = S.H + (size_t)c * (S.m + 1);
+                S.y[c] /= h[c];
+                for (int r = 0; r < c; r++) S.y[r] -= h[r] * S.y[c];
+            }
+        }
+        return;
+    }
+    for (int c = 0; c < kd; c++) {
+        const double *h = S.H + (size_t)c * (S.m + 1);
+        for (int r = lane; r <= c; r += 64) R[c * 64 + r] = h[r];
+    }
+    double y = lane < kd ? S.g[lane] : 0.0;
+    __syncthreads();
+    for (int c = kd - 1; c >= 0; c--) {
+        const double yc = __shfl(y, c) / R[c * 64 + c];
+        if (lane == c) y = yc;
+        if (lane < c) y -= R[c * 64 + lane] * yc;
+    }
+    if (lane < kd) S.y[lane] = y;
 }
 
 }  // namespace cpg
@@ -273,6 +380,7 @@ struct cpg_krylov_s {
     DevBuf<double> fH, fcs, fsn, fg, fy;
     DevBuf<int> fflags;
     int *h_flags = nullptr;   // pinned: one active flag per block, then the final 4 flags
+    int *d_flags = nullptr;   // the device's address of h_flags (the step kernels write the block flags)
     std::vector<hipEvent_t> fev;
     // the flexible driver runs on its own stream (capture is not possible on the null stream),
     // ordered after the caller's stream by an event
@@ -551,18 +659,16 @@ void fg_steps(cpg_krylov_s &K, KOps &op, int i0, int i1) {
         CPG_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_orth_norm, dim3(K.nblk), dim3(KWG), 0, s, V, N, i + 1, K.hdev.p, vi1, K.partial.p, act);
         CPG_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(KWG), 0, s, K.partial.p, K.nblk, 1, K.hdev.p + i + 1, 1, act);
-        CPG_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_scale_dev, dim3(ceil_div(N, 256)), dim3(256), 0, s, vi1, vi1, N, K.hdev.p + i + 1, act);
-        CPG_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_fg_givens, dim3(1), dim3(1), 0, s, i, K.fstate());
+        // ||w||, v_{i+1} = w / ||w||, Givens, and the active flag into the host's look-ahead slot
+        hipLaunchKernelGGL(k_fg_norm_step, dim3(K.nblk), dim3(KWG), 0, s, K.partial.p, K.nblk, vi1, N, i, K.fstate(),
+                           K.d_flags ? K.d_flags + i / KBLK : nullptr);
         CPG_LAUNCH_CHECK();
     }
 }
 
 // y = R^-1 g ; x -= Z y
 void fg_finish(cpg_krylov_s &K, KOps &op, double *x) {
-    hipLaunchKernelGGL(k_fg_solve, dim3(1), dim3(1), 0, op.s, K.fstate());
+    hipLaunchKernelGGL(k_fg_solve, dim3(1), dim3(64), 0, op.s, K.fstate());
     CPG_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_update, dim3(K.nblk), dim3(KWG), 0, op.s, K.Z.p, K.N, 0, K.fflags.p + 1, K.fy.p, x);
     CPG_LAUNCH_CHECK();
@@ -640,7 +746,6 @@ void gmres_flexible(cpg_krylov_s *K, const LinOp &A, cpg_prec P, const PcOp *M, 
         for (int bk = 0; bk < nblk; bk++) {
             const int i0 = bk * KBLK, i1 = std::min(m, i0 + KBLK);
             run_part(*K, s, K->g_blocks[bk], [&] { fg_steps(*K, op, i0, i1); });
-            CPG_HIP(hipMemcpyAsync(hf + bk, K->fflags.p, sizeof(int), hipMemcpyDeviceToHost, s));
             CPG_HIP(hipEventRecord(K->fev[bk], s));
             // look-ahead: block bk is queued; stop if block bk-1 already ended the cycle
             if (bk > 0) {
@@ -752,8 +857,9 @@ int cpg_krylov_create(int n_rows, int bl, int restart, int mode, cpg_krylov *out
                 K->fg.alloc(restart + 1);
                 K->fy.alloc(restart);
                 K->fflags.alloc(4);
-                CPG_HIP(hipHostMalloc((void **)&K->h_flags, sizeof(int) * (nb + 4), hipHostMallocDefault));
+                CPG_HIP(hipHostMalloc((void **)&K->h_flags, sizeof(int) * (nb + 4), hipHostMallocMapped));
                 std::memset(K->h_flags, 0, sizeof(int) * (nb + 4));
+                CPG_HIP(hipHostGetDevicePointer((void **)&K->d_flags, K->h_flags, 0));
                 K->fev.assign(nb, nullptr);
                 for (auto &e : K->fev) CPG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
                 K->g_blocks.assign(nb, nullptr);
