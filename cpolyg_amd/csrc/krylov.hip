@@ -33,8 +33,10 @@ namespace cpg {
 constexpr int KWG = 256;
 constexpr int KPT = 8;  // elements per thread in the vector kernels
 constexpr int KCH = KWG * KPT;
-constexpr int KBLK = 1;  // Arnoldi steps per captured block (flexible driver): the look-ahead block
-                         // queued after the step that converges runs as no-ops (~15 launches)
+// Arnoldi steps per captured block (flexible driver).  Each graph launch costs a ~13 us gap before
+// its first kernel, and the look-ahead block queued after the step that converges runs as no-ops
+// (~15 launches of ~4.5 us per step): two steps per block balance the two for ~19-step solves.
+constexpr int KBLK = 2;
 
 // partial[b*nv + j] = sum over chunk b of V[j] . w.  Two basis vectors at a time, the next two
 // vectors' loads issued before this pair's products and wave reductions (one vector at a time,
