@@ -937,6 +937,9 @@ struct cpg_batch_s {
         std::vector<double> bytes;          // bytes per Arnoldi step of each instance
         std::vector<int> cap;               // most workgroups an instance can use
         std::vector<int> last_iters;        // Arnoldi steps of each instance in the last solve
+        // the same per solve slot (stage, Newton iteration): a step's solves differ from one another
+        // more than from the same solve of the previous step
+        std::vector<std::vector<int>> slot_iters;
         // measured cost of each instance: team time x team size per Arnoldi step (0: not yet)
         std::vector<double> cost;
         DevBuf<unsigned long long> clk;
@@ -1279,15 +1282,17 @@ void persist_setup(cpg_batch_s &B) {
 // time as that work over its team size.  Every team starts with one workgroup and the next
 // workgroup goes to the team with the longest predicted time below its cap (cap[q], max_team),
 // which minimises the longest team under that model; all teams run in the one launch.
-void persist_plan(cpg_batch_s &B, int wg_per_cu) {
+void persist_plan(cpg_batch_s &B, int wg_per_cu, int slot) {
     auto &P = B.P;
+    const std::vector<int> &its =
+        slot >= 0 && slot < (int)P.slot_iters.size() && !P.slot_iters[slot].empty() ? P.slot_iters[slot] : P.last_iters;
     const int W = std::max(B.ninst, std::min(wg_per_cu, P.occupancy) * P.cus);
     bool measured = true;
     for (int q = 0; q < B.ninst; q++) measured = measured && P.cost[q] > 0.0;
     std::vector<double> work(B.ninst);
     std::vector<int> n(B.ninst, 1), lim(B.ninst);
     for (int q = 0; q < B.ninst; q++) {
-        work[q] = (measured ? P.cost[q] : P.bytes[q]) * std::max(1, P.last_iters[q]);
+        work[q] = (measured ? P.cost[q] : P.bytes[q]) * std::max(1, its[q]);
         lim[q] = P.max_team > 0 ? std::min(P.cap[q], P.max_team) : P.cap[q];
     }
     std::vector<std::pair<double, int>> heap;
@@ -1361,7 +1366,8 @@ void persist_launch(cpg_batch_s &B) {
 
 // solve A x = r for every instance in one cooperative launch (x = 0 start, FGMRES tolerance
 // rtol * ||r_q||)
-void batch_solve_persistent(cpg_batch_s &B, cpg_step_stats &st) {
+// slot: the solve's place in the step (stage x Newton iteration), keying the Arnoldi-count history
+void batch_solve_persistent(cpg_batch_s &B, cpg_step_stats &st, int slot) {
     auto &P = B.P;
     // launch shape: the fixed choice, or the next autotuner candidate, or the best one measured
     int wpc = P.wg_per_cu;
@@ -1377,7 +1383,7 @@ void batch_solve_persistent(cpg_batch_s &B, cpg_step_stats &st) {
         }
     }
     wpc = std::min(wpc, P.occupancy);
-    persist_plan(B, wpc);
+    persist_plan(B, wpc, slot);
     CPG_HIP(hipMemsetAsync(P.abort_flag.p, 0, sizeof(int), B.s));
     CPG_HIP(hipEventRecord(P.e0, B.s));
     switch (B.sp.precision) {
@@ -1407,6 +1413,10 @@ void batch_solve_persistent(cpg_batch_s &B, cpg_step_stats &st) {
         mx = std::max(mx, B.h_iters[q]);
         work += P.bytes[q] * B.h_iters[q];
         P.last_iters[q] = B.h_iters[q];
+        if (slot >= 0) {
+            if ((int)P.slot_iters.size() <= slot) P.slot_iters.resize(slot + 1);
+            P.slot_iters[slot].assign(B.h_iters.begin(), B.h_iters.end());
+        }
         if (B.h_iters[q] > 0 && P.h_clk[2 * q + 1] > P.h_clk[2 * q])
             P.cost[q] = (double)(P.h_clk[2 * q + 1] - P.h_clk[2 * q]) * P.h_inst[q].nwg / B.h_iters[q];
         CPG_CHECK(std::isfinite(B.h_resid[q]), CPG_ERR_NONFINITE, "non-finite GMRES residual in batch instance %d", q);
@@ -1664,7 +1674,7 @@ int cpg_batch_advance(cpg_batch B, const double *const *u, const double *dt, dou
                 CPG_HIP(hipMemcpyAsync(B->done.p, B->h_done.data(), B->ninst * sizeof(int), hipMemcpyHostToDevice, B->s));
                 hipLaunchKernelGGL(k_bzero_done, dim3(B->nchunk), dim3(BWG), 0, B->s, B->chunks(), B->done.p, B->r.p);
                 CPG_LAUNCH_CHECK();
-                if (B->mode == CPG_BATCH_PERSISTENT) batch_solve_persistent(*B, st);
+                if (B->mode == CPG_BATCH_PERSISTENT) batch_solve_persistent(*B, st, stg * B->sp.newton_maxit + it);
                 else batch_solve(*B, st);
                 if (inst_gmres_iters)
                     for (int q = 0; q < B->ninst; q++) inst_gmres_iters[q] += B->h_iters[q];
