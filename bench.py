@@ -81,6 +81,9 @@ def parse():
                    help="0: Newton matrix and preconditioner at every Newton iteration (reference); 1: once per stage; "
                         "2: once per step; 3: kept across steps, renewed when GMRES slows (matrix-free operator "
                         "only)")
+    p.add_argument("--dilu-apply", choices=["single", "levels"], default="single",
+                   help="block DILU apply of FP64 vectors: one launch per apply (rows wait on device-side "
+                        "completion counters) or one launch per colour level")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-compare", dest="compare_operators", action="store_false",
                    help="skip the runs under the reference refresh policy (matrix-free and assembled operator)")
@@ -194,7 +197,7 @@ def run_path(E, u0, dt, steps, warmup, solver, keep=False, label=""):
     return u.cpu().numpy(), wall, stats, kt, u, t
 
 
-def standalone_rates(E, u, t, scale, reps=20):
+def standalone_rates(E, u, t, scale, reps=20, single_launch=True):
     """Per-launch HIP-event timing on torch's current stream (the stream the library launches on)
     of the BSR matvec, the matrix-free Newton apply and the DILU apply in each factor precision,
     on the Newton matrix M + scale J(u, t)."""
@@ -227,6 +230,7 @@ def standalone_rates(E, u, t, scale, reps=20):
            "newton_apply_ms": per_launch(lambda: E.newton_apply_prepared(scale, x, y, stream))}
     for name, prec in (("bf16", PREC_BF16), ("bf16-d32", PREC_BF16_D32), ("fp32", PREC_FP32), ("fp64", PREC_FP64)):
         P = hip.Prec(J, PC_DILU, prec)
+        P.set_flow(int(single_launch))
         P.factor(J, stream)
         out[f"dilu_apply_{name}_ms"] = per_launch(lambda: P.apply(y, stream), lambda: y.copy_(x))
         out["dilu_levels"] = P.levels()
@@ -668,6 +672,7 @@ def main():
         return SolverSpec(gmres_mode=GMRES_FLEXIBLE, restart=args.restart, rtol=1e-10, maxit=args.gmres_maxit,
                           pc_kind=pc, newton_tol=5e-13, newton_maxit=100, precision=prec, inner_iters=args.inner,
                           newton_guess=1, newton_backtrack=1, dirk_tableau=dirk3,
+                          pc_level_launches=int(args.dilu_apply == "levels"),
                           pc_refresh=args.pc_refresh if refresh is None else refresh,
                           newton_operator=(OP_MATRIX_FREE if args.operator == "matrix-free" else OP_ASSEMBLED)
                           if op is None else op)
@@ -720,7 +725,7 @@ def main():
                 print(f"# {key}: {e}", file=sys.stderr, flush=True)
             res[key] = cmp_
     # per-launch rates on the last Newton matrix of the mixed run (stage-3 scale)
-    rates = standalone_rates(E, u_last, t_last, -dt * a_ss)
+    rates = standalone_rates(E, u_last, t_last, -dt * a_ss, single_launch=args.dilu_apply == "single")
     elem = {PREC_BF16: 2, PREC_BF16_D32: 2, PREC_FP32: 4}[mixed_prec]
     b_mv, b_ap = algorithmic_bytes(n, nnzb, bl, elem, mixed_prec != PREC_FP32, mixed_prec == PREC_BF16_D32)
     b_ap32 = algorithmic_bytes(n, nnzb, bl, 4, False)[1]
@@ -736,13 +741,18 @@ def main():
     newton = sum(s["newton_iters"] for s in st_mix) / steps
     gm = sum(s["gmres_iters"] for s in st_mix) / steps
     if mixed_pc == PC_DILU:
-        # dominant kernel of the mixed step: the DILU apply (one row-stream launch per colour level
-        # and sweep, forward level 0 and backward level 0 excepted, timed as one unit inside GMRES)
+        # dominant kernel of the mixed step: the DILU apply (one launch, or one row-stream launch per
+        # colour level and sweep, forward level 0 and backward level 0 excepted; timed as one unit
+        # inside GMRES)
         ap_gbs = gbs(b_ap, ap_ms)
         lev = rates["dilu_levels"]
         if mixed_prec == PREC_BF16_D32:
             shape = (f"{3 * lev - 1} row-stream launches: {lev - 1} forward levels (level 0 a no-op), then per "
                      f"backward level the FP32 D^-1 products and the BF16 DU stream")
+        elif args.dilu_apply == "single":
+            shape = (f"one k_dilu_flow launch: the rows of {lev - 1} forward levels (level 0 a no-op), the last "
+                     f"finishing its rows' backward step, then of {lev - 1} backward levels, each row waiting on "
+                     f"completion counters of the rows it reads")
         else:
             shape = (f"{2 * lev - 2} row-stream launches: {lev} colour levels x 2 sweeps, forward level 0 a no-op, "
                      f"the last forward level finishing its rows' backward step")

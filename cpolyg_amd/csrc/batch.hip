@@ -1487,6 +1487,7 @@ int cpg_batch_create(int nclass, const cpg_equation *eqs, const int *ninst, cons
                 CPG_CHECK(off < (size_t)1 << 31, CPG_ERR_UNSUPPORTED, "batch vector longer than 2^31");
                 ck(cpg_jacobian_create(C.E, &C.J));
                 ck(cpg_prec_create(C.J, params->pc_kind, params->precision, &C.P));
+                ck(cpg_prec_set_flow(C.P, params->pc_level_launches ? 0 : -1, -1));
                 C.cdt.alloc(C.n);
                 C.d_cptr.upload(C.cptr);
                 // launch shapes of this class's block size: matvec (FP64) and sweeps (factor precision)

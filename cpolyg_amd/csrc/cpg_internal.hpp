@@ -128,6 +128,8 @@ void set_stream_wpr(int elem_bytes, int bl, int wpr);
 // out = M^-1 in for a factored preconditioner (in and out distinct); `active` as for bsr_matvec
 // (DILU and block-Jacobi sweeps only)
 void prec_apply_to(cpg_prec P, const double *in, double *out, hipStream_t s, const int *active = nullptr);
+// reports (CPG_ERR_HIP) a single-launch block DILU apply whose dependency wait ran out since the last check
+void prec_flow_check(cpg_prec P);
 // the preconditioner's generation (next_generation): changes when its buffers are reallocated
 uint64_t prec_generation(cpg_prec P);
 // The factor streams of a DILU or block-Jacobi preconditioner as its sweeps read them: the

@@ -620,6 +620,7 @@ void gmres_left_ref(cpg_krylov_s *K, const LinOp &A, cpg_prec P, const double *b
     *maxit = m * j + i;
     *tol = std::fabs(y[i]) / nrm2b;
     K->harvest();
+    prec_flow_check(P);
     CPG_CHECK(std::isfinite(*tol), CPG_ERR_NONFINITE, "non-finite GMRES residual estimate");
 }
 
@@ -766,6 +767,7 @@ void gmres_flexible(cpg_krylov_s *K, const LinOp &A, cpg_prec P, const PcOp *M, 
     *maxit = hf[nblk + 3];
     *tol = resid / nb;
     K->harvest();
+    prec_flow_check(P);
     CPG_CHECK(std::isfinite(*tol), CPG_ERR_NONFINITE,
               "non-finite GMRES residual estimate (%d Arnoldi steps, ||b|| %g)", *maxit, nb);
 }

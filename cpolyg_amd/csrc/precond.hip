@@ -27,6 +27,7 @@
 #include "cpg_internal.hpp"
 #include "rowstream.hpp"
 
+#include <climits>
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -1658,6 +1659,115 @@ __global__ __launch_bounds__(FWG) void k_cdilu_upper(const int *__restrict__ uro
 
 }  // namespace cpg
 
+namespace cpg {
+
+// ---- block DILU apply in one launch (FP64 vectors) ------------------------------------------------
+// The level launches of apply_t order the sweep's rows by launch boundaries, so every level pays a
+// ramp and a tail.  Here one workgroup per row task takes the tasks in level order by workgroup index
+// (forward rows of levels 1 on, then backward rows): a task issues its row's first block loads, waits
+// until every row it reads has finished in this apply, runs the row-stream body and then marks its own
+// row finished, so a level's tail overlaps the next level's start.  Completion counters, one per row
+// and sweep direction, rise by one per apply; a task reads its own row's counter g (the applies so
+// far) and waits for g + 1 on the rows it reads.  Workgroups of one launch are dispatched to each XCD
+// in index order and a task waits only on tasks of lower index, so every wait ends; the waits are
+// still bounded (spin_max polls): one that runs out sets the abort word and gives up, the task goes on
+// (the counters stay consistent) and the host reports the apply as failed.
+constexpr int kFlowLevels = 32;
+
+// level l: tasks [first[l], first[l + 1]), rows and packed streams S[l] as the level's launch takes
+// them, kind[l] 0 forward, 1 forward finished by D^-1 (writes x), 2 backward; unused levels have
+// first = INT_MAX
+struct FlowTable {
+    int first[kFlowLevels];
+    int kind[kFlowLevels];
+    RowStream S[kFlowLevels];
+    unsigned *fdone = nullptr, *bdone = nullptr;   // completions of each row's forward / backward step
+    unsigned *abort = nullptr;                     // host-mapped word, set by a wait that ran out
+    int spin_max = 0;
+};
+
+struct FlowHooks {
+    static constexpr bool coherent = true;
+    unsigned *own;          // completion counters of this task's step
+    const unsigned *dep1;   // counters of the src1 rows (y forward, x backward)
+    const unsigned *dep0;   // counters of the src0 rows (y, backward)
+    unsigned *abort;
+    int spin_max;
+    unsigned g = 0;         // this task's completions before this apply
+    __device__ void wait(int i, int b0, int nb, const int *bcol, int tid, int nt) {
+        g = __hip_atomic_load(own + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int b = tid; b < nb; b += nt) {
+            const int j = bcol[b0 + b];
+            const int u = -1 - j;
+            const unsigned *f = j >= 0 ? dep1 : u < kSrc2Tag ? dep0 : nullptr;   // src2 (the input) needs none
+            if (!f) continue;
+            const int r = j >= 0 ? j : u;
+            for (int k = 0;; k++) {
+                if (k >= spin_max ||
+                    ((k & 255) == 255 && __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))) {
+                    __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    break;
+                }
+                if (__hip_atomic_load(f + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g + 1u) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __syncthreads();
+    }
+    __device__ void publish(int i, int tid) {
+        // every wave's result stores (device scope) complete before the counter is raised
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(own + i, g + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
+// x = M^-1 in through y (forward result); in may equal x (the pattern is structurally symmetric, so
+// the backward step of a row that forward rows read as input comes after theirs)
+template <class T, int BL, int WPR>
+__global__ __launch_bounds__(64 * WPR) void k_dilu_flow(FlowTable A, const int *__restrict__ active,
+                                                        const T *__restrict__ F, const float *__restrict__ scale,
+                                                        const double *in, double *y, double *x) {
+    if (active && !*active) return;
+    const int t = blockIdx.x;
+    int L = 0;
+#pragma unroll
+    for (int l = 1; l < kFlowLevels; l++) L += t >= A.first[l];
+    const RowStream S = A.S[L];
+    const int first = A.first[L], kind = A.kind[L];
+    const bool bwd = kind == 2;
+    FlowHooks fl{kind == 0 ? A.fdone : A.bdone, bwd ? A.bdone : A.fdone, bwd ? A.fdone : nullptr, A.abort,
+                 A.spin_max};
+    // forward: y_i = in_i - sum L_ij y_j (level-0 sources read in); backward: x_i = D_i^-1 y_i - sum DU_ij x_j
+    rowstream_row<T, BL, WPR, double>(S, t - first, F, scale, bwd ? y : nullptr, bwd ? x : y, -1.0, in,
+                                      bwd ? 1.0 : -1.0, bwd ? nullptr : in, nullptr, kind == 0 ? y : x, fl);
+}
+
+inline bool flow_compiled(int bl) {
+    return bl == 3 || bl == 6 || bl == 10 || bl == 12 || bl == 15 || bl == 24 || bl == 40 || bl == 60;
+}
+
+// a host word the device can write
+struct MappedWord {
+    unsigned *h = nullptr, *d = nullptr;
+    MappedWord() = default;
+    MappedWord(const MappedWord &) = delete;
+    MappedWord &operator=(const MappedWord &) = delete;
+    ~MappedWord() {
+        if (h) (void)hipHostFree(h);
+    }
+    void alloc() {
+        if (h) return;
+        CPG_HIP(hipHostMalloc((void **)&h, sizeof(unsigned), hipHostMallocMapped));
+        *h = 0;
+        CPG_HIP(hipHostGetDevicePointer((void **)&d, h, 0));
+    }
+};
+
+}  // namespace cpg
+
 using namespace cpg;
 
 // ---- handle --------------------------------------------------------------------------
@@ -1706,6 +1816,14 @@ struct cpg_prec_s {
     // iterations), basis (k+1) x N, work vectors
     DevBuf<float> A32, Q, iw, it;
     DevBuf<double> ipart, H, g, cs, sn, y, nrm;
+    // block DILU apply in one launch (k_dilu_flow): level table, completion counters, abort word
+    FlowTable flow;
+    int flow_tasks = 0;
+    bool flow_ok = false;        // the table applies to this preconditioner
+    bool flow_enabled = true;    // cpg_prec_set_flow
+    int flow_spin = 1 << 22;
+    DevBuf<unsigned> fdone, bdone;
+    MappedWord flow_abort;
 
     bool dilu() const { return kind == CPG_PC_DILU || kind == CPG_PC_INNER_GMRES; }
 
@@ -1902,6 +2020,106 @@ void factor_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
     CPG_CHECK(false, CPG_ERR_UNSUPPORTED, "block size %d too large", P.bl);
 }
 
+// The single-launch apply's level table (FP64 vectors; factors without FP32 diagonal inverses, a block
+// size with a compiled k_dilu_flow): the forward levels from 1 (level 0 has y = x), the last one
+// finishing its rows with D^-1 where the first backward level is the same rows, as apply_t fuses them,
+// then the backward levels.  More levels than the table holds keep the level launches.
+void flow_setup(cpg_prec_s &P) {
+    P.flow_ok = false;
+    P.flow_tasks = 0;
+    if (P.kind != CPG_PC_DILU || P.diag32 || !flow_compiled(P.bl)) return;
+    const int nlf = (int)P.flev.size() - 1;
+    const bool fuse = nlf >= 2 && P.frow0[nlf - 1] >= 0 && P.brow0.size() > 0 && P.brow0[0] == P.frow0[nlf - 1] &&
+                      P.blev[1] - P.blev[0] == P.flev[nlf] - P.flev[nlf - 1];
+    FlowTable T;
+    int tasks = 0, nl = 0;
+    for (int l = 0; l < kFlowLevels; l++) {
+        T.first[l] = INT_MAX;
+        T.kind[l] = 0;
+        T.S[l] = RowStream{nullptr, nullptr, nullptr};
+    }
+    auto add = [&](const DevBuf<int> &rows, const std::vector<int> &row0, const DevBuf<int> &ptr,
+                   const std::vector<int> &lev, int lv, int kind) {
+        const int r0 = lev[lv], nr = lev[lv + 1] - r0;
+        if (!nr) return true;
+        if (nl == kFlowLevels) return false;
+        T.S[nl] = RowStream{row0[lv] >= 0 ? nullptr : rows.p + r0, ptr.p + r0, P.scol.p, nullptr};
+        T.S[nl].row0 = row0[lv] >= 0 ? row0[lv] : 0;
+        if (kind == 1) T.S[nl].dptr = P.bptr.p + P.blev[0];
+        T.first[nl] = tasks;
+        T.kind[nl] = kind;
+        nl++;
+        tasks += nr;
+        return true;
+    };
+    for (int lv = 1; lv < nlf; lv++)
+        if (!add(P.frows, P.frow0, P.fptr, P.flev, lv, fuse && lv == nlf - 1 ? 1 : 0)) return;
+    for (int lv = fuse ? 1 : 0; lv + 1 < (int)P.blev.size(); lv++)
+        if (!add(P.brows, P.brow0, P.bptr, P.blev, lv, 2)) return;
+    if (!tasks) return;
+    if (P.fdone.n != (size_t)P.n) {
+        P.fdone.alloc(P.n);
+        P.bdone.alloc(P.n);
+        CPG_HIP(hipMemset(P.fdone.p, 0, (size_t)P.n * sizeof(unsigned)));
+        CPG_HIP(hipMemset(P.bdone.p, 0, (size_t)P.n * sizeof(unsigned)));
+    }
+    P.flow_abort.alloc();
+    T.fdone = P.fdone.p;
+    T.bdone = P.bdone.p;
+    T.abort = P.flow_abort.d;
+    P.flow = T;
+    P.flow_tasks = tasks;
+    P.flow_ok = true;
+}
+
+// A single-launch apply whose wait ran out gave a wrong result: report it, and take the level
+// launches from then on (graphs that captured the single launch are captured again)
+void flow_check(cpg_prec_s &P) {
+    if (!P.flow_abort.h || !*(volatile unsigned *)P.flow_abort.h) return;
+    *(volatile unsigned *)P.flow_abort.h = 0;
+    P.flow_enabled = false;
+    P.gen = next_generation();
+    CPG_CHECK(false, CPG_ERR_HIP,
+              "block DILU apply: a row waited more than %d polls for the rows it reads; that apply's result is "
+              "invalid, and the preconditioner now applies one launch per level",
+              P.flow_spin);
+}
+
+template <class T, int BL, int WPR>
+void launch_flow_w(cpg_prec_s &P, const double *in, double *out, hipStream_t s, const int *active) {
+    const size_t lds = rowstream_lds<T, BL, WPR>(std::max(P.nbmax_f, P.nbmax_b));
+    set_lds(k_dilu_flow<T, BL, WPR>, lds);
+    FlowTable A = P.flow;
+    A.spin_max = P.flow_spin;
+    const float *sc = P.precision == CPG_BF16 ? P.scale.p : nullptr;
+    hipLaunchKernelGGL((k_dilu_flow<T, BL, WPR>), dim3(P.flow_tasks), dim3(64 * WPR), lds, s, A, active, Fp<T>(P),
+                       sc, in, P.ybuf.p, out);
+    CPG_LAUNCH_CHECK();
+}
+
+template <class T, int BL>
+void launch_flow_bl(cpg_prec_s &P, const double *in, double *out, hipStream_t s, const int *active) {
+    switch (stream_wpr((int)sizeof(T), BL)) {
+    case 1: launch_flow_w<T, BL, 1>(P, in, out, s, active); break;
+    case 2: launch_flow_w<T, BL, 2>(P, in, out, s, active); break;
+    default: launch_flow_w<T, BL, 4>(P, in, out, s, active); break;
+    }
+}
+
+template <class T>
+void launch_flow(cpg_prec_s &P, const double *in, double *out, hipStream_t s, const int *active) {
+    switch (P.bl) {
+    case 3: launch_flow_bl<T, 3>(P, in, out, s, active); return;
+    case 6: launch_flow_bl<T, 6>(P, in, out, s, active); return;
+    case 10: launch_flow_bl<T, 10>(P, in, out, s, active); return;
+    case 12: launch_flow_bl<T, 12>(P, in, out, s, active); return;
+    case 15: launch_flow_bl<T, 15>(P, in, out, s, active); return;
+    case 24: launch_flow_bl<T, 24>(P, in, out, s, active); return;
+    case 40: launch_flow_bl<T, 40>(P, in, out, s, active); return;
+    default: launch_flow_bl<T, 60>(P, in, out, s, active); return;
+    }
+}
+
 // out = M^-1 in (in may equal out)
 template <class T, class XT>
 void apply_t(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s, const int *active) {
@@ -1917,8 +2135,15 @@ void apply_t(cpg_prec_s &P, const XT *in, XT *out, hipStream_t s, const int *act
         launch_rowstream<T, XT>(bl, a, F, sc, s);
         return;
     }
-    double *y = P.ybuf.p;
     constexpr bool x64 = std::is_same<XT, double>::value;
+    if constexpr (x64) {
+        flow_check(P);
+        if (P.flow_ok && P.flow_enabled) {
+            launch_flow<T>(P, in, out, s, active);
+            return;
+        }
+    }
+    double *y = P.ybuf.p;
     auto stream_of = [](const DevBuf<int> &rows, const std::vector<int> &row0, const DevBuf<int> &ptr, int lv, int r0,
                         const int *scol, const int *active) {
         RowStream S{row0[lv] >= 0 ? nullptr : rows.p + r0, ptr.p + r0, scol, active};
@@ -2301,6 +2526,7 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
                 P->F.alloc((size_t)n * bb * P->elem);
                 if (precision == CPG_BF16) P->scale.alloc(n);
             }
+            flow_setup(*P);
         } catch (...) {
             delete P;
             throw;
@@ -2311,6 +2537,24 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
 
 int cpg_prec_destroy(cpg_prec P) {
     return guard([&] { delete P; });
+}
+
+int cpg_prec_set_flow(cpg_prec P, int enable, int spin_limit) {
+    return guard([&] {
+        CPG_CHECK(P, CPG_ERR_ARG, "null argument");
+        if (enable >= 0 && (enable != 0) != P->flow_enabled) {
+            P->flow_enabled = enable != 0;
+            P->gen = next_generation();   // graphs that captured the other form are captured again
+        }
+        if (spin_limit >= 0) P->flow_spin = spin_limit;
+    });
+}
+
+int cpg_prec_flow_state(cpg_prec P, int *in_use) {
+    return guard([&] {
+        CPG_CHECK(P && in_use, CPG_ERR_ARG, "null argument");
+        *in_use = P->flow_ok && P->flow_enabled ? 1 : 0;
+    });
 }
 
 int cpg_prec_levels(cpg_prec P, int *n_levels) {
@@ -2371,6 +2615,10 @@ int cpg_prec_apply(cpg_prec P, double *x, void *stream) {
 }  // extern "C"
 
 namespace cpg {
+void prec_flow_check(cpg_prec P) {
+    if (P) flow_check(*P);
+}
+
 // out = M^-1 in, in and out distinct (used by flexible GMRES for z_i = M^-1 v_i)
 void prec_apply_to(cpg_prec P, const double *in, double *out, hipStream_t s, const int *active) {
     if (P->kind == CPG_PC_NONE || P->n == 0) {

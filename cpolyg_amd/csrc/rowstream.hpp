@@ -53,7 +53,8 @@ struct RowStream {
 // reads src0[u]; -1 - (kSrc2Tag + k) reads c2 * src2[k]
 constexpr int kSrc2Tag = 1 << 30;
 
-template <class XT>
+// COH: the sources are read at device scope (vectors that other workgroups of the same launch write)
+template <class XT, bool COH = false>
 __device__ inline double stream_source(int j, int c, int bl, const double *__restrict__ src0, const XT *src1,
                                        double c1, const XT *src2, double c2) {
     if constexpr (sizeof(XT) == 8) {
@@ -63,6 +64,8 @@ __device__ inline double stream_source(int j, int c, int bl, const double *__res
         const double *base = s1 ? (const double *)src1 : s2 ? (const double *)src2 : src0;
         const int r = s1 ? j : s2 ? u - kSrc2Tag : u;
         const double cf = s1 ? c1 : s2 ? c2 : 1.0;
+        if constexpr (COH)
+            return cf * __hip_atomic_load(base + (size_t)r * bl + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return cf * base[(size_t)r * bl + c];
     }
     if (j >= 0) return c1 * (double)src1[(size_t)j * bl + c];
@@ -145,16 +148,29 @@ inline size_t rowstream_lds(int nbmax) {
     return (size_t)(G + (nbmax > 0 ? nbmax : 1)) * BL * sizeof(A);
 }
 
-// out may alias src1 and src2 (rows written by one launch are never sources of the same
-// launch, and a row that is its own source is gathered before it is written).  Exactly one
-// of base64 / base32 may be non-null.
-template <class T, int BL, int WPR, class XT>
-__global__ __launch_bounds__(64 * WPR) void k_rowstream(RowStream S, const T *__restrict__ F,
-                                                        const float *__restrict__ scale,
-                                                        const double *__restrict__ src0, const XT *src1, double c1,
-                                                        const XT *src2, double c2,
-                                                        const double *__restrict__ base64,
-                                                        const float *__restrict__ base32, XT *out) {
+// Hooks of a sweep whose rows wait for the rows they read (none: the level launches order them).
+// coherent: the row's sources and its result are read and written at device scope.
+struct NoFlow {
+    static constexpr bool coherent = false;
+    __device__ void wait(int, int, int, const int *, int, int) {}
+    __device__ void publish(int, int) {}
+};
+
+template <class XT, bool COH>
+__device__ inline void stream_store(XT *p, XT v) {
+    if constexpr (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+
+// Item w of the sweep S (one workgroup of WPR wavefronts).  out may alias src1 and src2 (rows
+// written by one sweep are never sources of the same level, and a row that is its own source is
+// gathered before it is written).  Exactly one of base64 / base32 may be non-null.
+template <class T, int BL, int WPR, class XT, class Flow>
+__device__ __forceinline__ void rowstream_row(const RowStream &S, int w, const T *__restrict__ F,
+                                              const float *__restrict__ scale, const double *__restrict__ src0,
+                                              const XT *src1, double c1, const XT *src2, double c2,
+                                              const double *__restrict__ base64, const float *__restrict__ base32,
+                                              XT *out, Flow &fl) {
     using A = typename StreamAcc<T>::type;
     constexpr int VW = stream_vw<T, BL>();
     constexpr int TPC = BL / VW;
@@ -167,8 +183,6 @@ __global__ __launch_bounds__(64 * WPR) void k_rowstream(RowStream S, const T *__
     extern __shared__ double lds_d[];
     A *red = reinterpret_cast<A *>(lds_d);
     A *xs = red + G * BL;
-    if (S.active && !*S.active) return;
-    const int w = blockIdx.x;
     const int tid = threadIdx.x;
     const int i = S.rows ? S.rows[w] : S.row0 + w;
     const int b0 = S.bptr ? S.bptr[w] : S.row0 + w;
@@ -184,7 +198,7 @@ __global__ __launch_bounds__(64 * WPR) void k_rowstream(RowStream S, const T *__
     // the first chunk is issued before the gather, so its latency overlaps the gather's
     constexpr int PF = 4 / LPW;
     constexpr int CH = G * PF;   // columns per chunk
-    stream_u4 cur[LPW][PF];
+    stream_u4 cur[LPW][PF] = {};
     auto issue = [&](stream_u4 (&r)[LPW][PF], int base) {
 #pragma unroll
         for (int k = 0; k < PF; k++)
@@ -195,10 +209,11 @@ __global__ __launch_bounds__(64 * WPR) void k_rowstream(RowStream S, const T *__
             }
     };
     if (gl < GW) issue(cur, 0);
+    fl.wait(i, b0, nb, S.bcol, tid, NT);
     for (int e = tid; e < ncol; e += NT) {
         const int b = e / BL, c = e - b * BL;
         const int j = S.bcol ? S.bcol[b0 + b] : i;
-        double v = stream_source<XT>(j, c, BL, src0, src1, c1, src2, c2);
+        double v = stream_source<XT, Flow::coherent>(j, c, BL, src0, src1, c1, src2, c2);
         if constexpr (sizeof(T) == 2) v *= (double)scale[b0 + b];
         xs[e] = (A)v;
     }
@@ -210,7 +225,7 @@ __global__ __launch_bounds__(64 * WPR) void k_rowstream(RowStream S, const T *__
 #pragma unroll
             for (int v = 0; v < VW; v++) acc[l][v] = 0;
         for (int base = 0; base < ncol; base += CH) {
-            stream_u4 nxt[LPW][PF];
+            stream_u4 nxt[LPW][PF] = {};
             if (base + CH < ncol) issue(nxt, base + CH);
 #pragma unroll
             for (int k = 0; k < PF; k++)
@@ -242,8 +257,9 @@ __global__ __launch_bounds__(64 * WPR) void k_rowstream(RowStream S, const T *__
             double r = 0.0;
 #pragma unroll
             for (int gg = 0; gg < G; gg++) r += (double)red[gg * BL + t];
-            out[(size_t)i * BL + t] = (XT)(s + r);
+            stream_store<XT, Flow::coherent>(out + (size_t)i * BL + t, (XT)(s + r));
         }
+        fl.publish(i, tid);
         return;
     }
     // the row result becomes the source of one more block: gathered (and BF16-scaled) exactly as
@@ -289,8 +305,22 @@ __global__ __launch_bounds__(64 * WPR) void k_rowstream(RowStream S, const T *__
         double r = 0.0;
 #pragma unroll
         for (int gg = 0; gg < G; gg++) r += (double)red[gg * BL + t];
-        out[(size_t)i * BL + t] = (XT)r;
+        stream_store<XT, Flow::coherent>(out + (size_t)i * BL + t, (XT)r);
     }
+    fl.publish(i, tid);
+}
+
+// one launch per level: item w = blockIdx.x
+template <class T, int BL, int WPR, class XT>
+__global__ __launch_bounds__(64 * WPR) void k_rowstream(RowStream S, const T *__restrict__ F,
+                                                        const float *__restrict__ scale,
+                                                        const double *__restrict__ src0, const XT *src1, double c1,
+                                                        const XT *src2, double c2,
+                                                        const double *__restrict__ base64,
+                                                        const float *__restrict__ base32, XT *out) {
+    if (S.active && !*S.active) return;
+    NoFlow fl;
+    rowstream_row<T, BL, WPR, XT>(S, (int)blockIdx.x, F, scale, src0, src1, c1, src2, c2, base64, base32, out, fl);
 }
 
 // Any block size up to 256 (no compiled specialisation): one thread per row entry walks all

@@ -502,6 +502,7 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
                           "bad solver parameters");
                 check(cpg_jacobian_create(E, &S->J));
                 check(cpg_prec_create(S->J, params->pc_kind, params->precision, &S->P));
+                check(cpg_prec_set_flow(S->P, params->pc_level_launches ? 0 : -1, -1));
                 if (params->pc_kind == CPG_PC_INNER_GMRES) {
                     CPG_CHECK(params->gmres_mode == CPG_GMRES_FLEXIBLE, CPG_ERR_ARG,
                               "the inner-GMRES preconditioner varies between iterations: use flexible GMRES");
@@ -551,7 +552,10 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
                     for (int i = 0; i < ns; i++) S->btAinv[j] += S->b[i] * S->Ainv[i * ns + j];
                 }
                 check(cpg_jacobian_create(E, &S->J));
-                for (int st = 0; st < ns; st++) check(cpg_prec_create(S->J, params->pc_kind, params->precision, &S->Pw[st]));
+                for (int st = 0; st < ns; st++) {
+                    check(cpg_prec_create(S->J, params->pc_kind, params->precision, &S->Pw[st]));
+                    check(cpg_prec_set_flow(S->Pw[st], params->pc_level_launches ? 0 : -1, -1));
+                }
                 check(cpg_krylov_create(ns * n, bl, params->restart, CPG_GMRES_FLEXIBLE, &S->K));
                 S->bigR.alloc(S->N * ns);
                 S->bigK.alloc(S->N * ns);
@@ -597,6 +601,7 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
                     C->rowptr.upload(C->h_rowptr);
                     C->colind.upload(C->h_colind);
                     check(cpg_prec_create(S->Cpat, CPG_PC_DILU, params->precision, &S->P));
+                    check(cpg_prec_set_flow(S->P, params->pc_level_launches ? 0 : -1, -1));
                 } else {
                     std::vector<int> rp(n + 1), ci(n);
                     for (int i = 0; i <= n; i++) rp[i] = i;
@@ -645,6 +650,7 @@ int cpg_stepper_create(cpg_equation E, int method, int nstages, const cpg_solver
                 check(cpg_bsr_create(ns * n, bl, (int)bcol.size(), brow.data(), bcol.data(), &S->big));
                 S->slot_row.upload(srow);
                 check(cpg_prec_create(S->big, params->pc_kind, params->precision, &S->P));
+                check(cpg_prec_set_flow(S->P, params->pc_level_launches ? 0 : -1, -1));
                 check(cpg_krylov_create(ns * n, bl, params->restart, params->gmres_mode, &S->K));
                 S->bigR.alloc(S->N * ns);
                 S->bigK.alloc(S->N * ns);

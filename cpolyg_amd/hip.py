@@ -42,7 +42,7 @@ class SolverParams(C.Structure):
                 ("pc_kind", C.c_int), ("precision", C.c_int), ("newton_tol", C.c_double),
                 ("newton_maxit", C.c_int), ("inner_iters", C.c_int), ("newton_guess", C.c_int),
                 ("newton_backtrack", C.c_int), ("newton_operator", C.c_int), ("pc_refresh", C.c_int),
-                ("dirk_tableau", C.c_int)]
+                ("dirk_tableau", C.c_int), ("pc_level_launches", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -211,6 +211,16 @@ class Prec:
         _chk(lib().cpg_prec_levels(self.h, C.byref(n)))
         return n.value
 
+    def set_flow(self, enable=-1, spin_limit=-1):
+        """Block DILU apply in one launch (1) or one launch per level (0); spin_limit bounds each
+        row's wait for the rows it reads (0: give up at once).  -1 leaves a setting unchanged."""
+        _chk(lib().cpg_prec_set_flow(self.h, int(enable), int(spin_limit)))
+
+    def flow_in_use(self):
+        v = C.c_int()
+        _chk(lib().cpg_prec_flow_state(self.h, C.byref(v)))
+        return bool(v.value)
+
 
 class Krylov:
     def __init__(self, n, bl, restart, mode):
@@ -256,7 +266,7 @@ def eq_params(s: _spec.EquationSpec) -> EqParams:
 def solver_params(s: _spec.SolverSpec) -> SolverParams:
     return SolverParams(s.gmres_mode, s.restart, s.rtol, s.maxit, s.pc_kind, s.precision, s.newton_tol,
                         s.newton_maxit, s.inner_iters, s.newton_guess, s.newton_backtrack, s.newton_operator,
-                        s.pc_refresh, s.dirk_tableau)
+                        s.pc_refresh, s.dirk_tableau, s.pc_level_launches)
 
 
 class Mesh:

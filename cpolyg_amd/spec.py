@@ -105,6 +105,7 @@ class SolverSpec:
     newton_operator: int = OP_ASSEMBLED
     pc_refresh: int = 0          # 1: with OP_MATRIX_FREE, refresh the preconditioner once per stage
     dirk_tableau: int = DIRK3_REFERENCE   # 3-stage DIRK: reference tableau or Alexander's L-stable SDIRK3
+    pc_level_launches: int = 0   # 1: block DILU applies one launch per colour level (else one launch per apply)
 
 
 def bs_of(deg: int) -> int:

@@ -120,6 +120,18 @@ int cpg_prec_factor_coupled(cpg_prec P, cpg_equation E, cpg_bsr J0, cpg_bsr J1, 
  * (DILU) or n inverse diagonal blocks (block Jacobi); synchronous */
 int cpg_prec_download(cpg_prec P, double *host_blocks);
 int cpg_prec_levels(cpg_prec P, int *n_levels);
+/* Block DILU apply on FP64 vectors in one launch: the rows of every level, forward then backward,
+ * each waiting on device-side completion counters of the rows it reads, where the level launches
+ * order them by launch boundaries (the same arithmetic per row: bitwise the same result).  Used by
+ * default where it applies (CPG_PC_DILU with factors other than CPG_BF16_D32, block sizes 3, 6, 10,
+ * 12, 15, 24, 40 and 60, at most 32 level launches).  enable: 1 on, 0 one launch per level, < 0 unchanged.
+ * spin_limit >= 0 bounds each wait (polls; default 2^22; 0 gives up at once, for tests of the
+ * failure path), < 0 unchanged.  A wait that runs out makes that apply's result invalid: the next
+ * apply or the GMRES solve that used it returns CPG_ERR_HIP, and the level launches are used from
+ * then on.  No reference counterpart (BlockILU0::solve, src/Preconditioners.cpp:125-158, is a
+ * sequential loop). */
+int cpg_prec_set_flow(cpg_prec P, int enable, int spin_limit);
+int cpg_prec_flow_state(cpg_prec P, int *in_use);
 
 /* ---- GMRES ------------------------------------------------------------------
  * Replaces BlockMatrix::gmres (src/BlockMatrix.cpp:79-163) and Jacobian::solve
@@ -278,6 +290,8 @@ typedef struct {
                                (src/TimeIntegration.cpp:71-77: second order, not A-stable, R(-inf) = -1.78);
                                CPG_DIRK3_L_STABLE Alexander's L-stable SDIRK3 (third order, R(-inf) = 0;
                                no reference counterpart) */
+    int pc_level_launches;  /* block DILU apply on FP64 vectors: 0 one launch per apply where it applies
+                               (cpg_prec_set_flow), 1 one launch per colour level */
 } cpg_solver_params;
 
 typedef struct {

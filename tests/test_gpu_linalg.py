@@ -70,6 +70,99 @@ def test_dilu_fp64_factor_and_apply(cuda, deg):
     assert rel(xd.cpu().numpy(), O.dilu_apply(orc.rowptr, orc.colind, Fo, x)) < 1e-11
 
 
+def _permuted(orc, B, seed):
+    """The same matrix with rows and columns renumbered by a random permutation (each row's diagonal
+    block first, then the others by column): DILU levels that are not colour segments."""
+    rp, ci = np.asarray(orc.rowptr), np.asarray(orc.colind)
+    n = len(rp) - 1
+    perm = np.random.default_rng(seed).permutation(n)   # new row r is old row perm[r]
+    inv = np.argsort(perm)
+    rp2, ci2, blk = [0], [], []
+    for r in range(n):
+        i = perm[r]
+        ent = sorted(range(rp[i], rp[i + 1]), key=lambda q: (inv[ci[q]] != r, inv[ci[q]]))
+        ci2 += [int(inv[ci[q]]) for q in ent]
+        blk += [B[q] for q in ent]
+        rp2.append(len(ci2))
+    return np.array(rp2, np.int32), np.array(ci2, np.int32), np.array(blk)
+
+
+@pytest.mark.parametrize("case", ["colour-ordered", "permuted"])
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32, PREC_BF16])
+def test_dilu_single_launch_matches_level_launches(cuda, case, prec):
+    """The one-launch DILU apply (rows waiting on completion counters) against one launch per level:
+    bitwise equal over alternating applies (the counters advance by one per apply), on a
+    colour-ordered matrix (contiguous levels, the last forward level finishing its rows) and on a
+    randomly renumbered one (levels of scattered rows); FP64 also against the oracle."""
+    if case == "colour-ordered":
+        m, orc, B = _euler_system(3)
+        rp, ci = orc.rowptr, orc.colind
+    else:
+        m, orc, B = _euler_system(1, nx=6)
+        rp, ci, B = _permuted(orc, B, 1)
+    bl = B.shape[1]
+    A = hip.Bsr(rp, ci, bl)
+    A.upload(B)
+    P = hip.Prec(A, PC_DILU, prec)
+    P.factor()
+    assert P.flow_in_use()
+    if case == "permuted":
+        assert P.levels() > m.n_colours   # not the colour segments
+    x = np.random.default_rng(7).standard_normal(m.n_cells * bl)
+    outs = []
+    for mode in (1, 0, 1, 1, 0, 1):
+        P.set_flow(mode)
+        xd = _dev(cuda, x)
+        P.apply(xd)
+        outs.append(xd.cpu().numpy())
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
+    if prec == PREC_FP64:
+        Fo, info = O.dilu_factor(rp, ci, B)
+        assert info == 0
+        assert rel(outs[0], O.dilu_apply(rp, ci, Fo, x)) < 1e-11
+
+
+def test_dilu_single_launch_not_used_with_fp32_inverses(cuda):
+    m, orc, B = _euler_system(3)
+    A = hip.Bsr(orc.rowptr, orc.colind, B.shape[1])
+    A.upload(B)
+    assert not hip.Prec(A, PC_DILU, PREC_BF16_D32).flow_in_use()
+    assert hip.Prec(A, PC_DILU, PREC_BF16).flow_in_use()
+
+
+def test_dilu_single_launch_wait_limit_reports_and_recovers(cuda):
+    """A wait limit of 0 polls makes every row that reads another row give up at once: the kernel
+    still ends (the counters stay consistent), the next apply reports the failed one, and the
+    preconditioner continues with one launch per level until the single launch is enabled again."""
+    import torch
+    m, orc, B = _euler_system(3)
+    A = hip.Bsr(orc.rowptr, orc.colind, B.shape[1])
+    A.upload(B)
+    P = hip.Prec(A, PC_DILU, PREC_FP64)
+    P.factor()
+    x = np.random.default_rng(8).standard_normal(m.n_cells * B.shape[1])
+    P.set_flow(0)
+    xd = _dev(cuda, x)
+    P.apply(xd)
+    ref = xd.cpu().numpy()
+    P.set_flow(1, 0)
+    P.apply(_dev(cuda, x))
+    torch.cuda.synchronize()
+    with pytest.raises(hip.CpgError, match="waited more than 0 polls"):
+        P.apply(_dev(cuda, x))
+    assert not P.flow_in_use()
+    xd = _dev(cuda, x)
+    P.apply(xd)
+    np.testing.assert_array_equal(xd.cpu().numpy(), ref)
+    P.set_flow(1, 1 << 22)
+    assert P.flow_in_use()
+    for _ in range(2):
+        xd = _dev(cuda, x)
+        P.apply(xd)
+        np.testing.assert_array_equal(xd.cpu().numpy(), ref)
+
+
 @pytest.mark.parametrize("deg", [3, 4])
 @pytest.mark.parametrize("prec,tol", [(PREC_FP32, 5e-6), (PREC_BF16, 2e-2), (PREC_BF16_D32, 2e-2)])
 def test_dilu_reduced_precision(cuda, prec, tol, deg):
