@@ -1668,19 +1668,24 @@ namespace cpg {
 // until every row it reads has finished in this apply, runs the row-stream body and then marks its own
 // row finished, so a level's tail overlaps the next level's start.  Completion counters, one per row
 // and sweep direction, rise by one per apply; a task reads its own row's counter g (the applies so
-// far) and waits for g + 1 on the rows it reads.  Workgroups of one launch are dispatched to each XCD
-// in index order and a task waits only on tasks of lower index, so every wait ends; the waits are
-// still bounded (spin_max polls): one that runs out sets the abort word and gives up, the task goes on
-// (the counters stay consistent) and the host reports the apply as failed.
-constexpr int kFlowLevels = 32;
+// far) and waits for g + 1 on the rows it reads.  The vectors written inside the launch (y, x) are
+// stored and gathered at device scope, and a row's stores complete before its counter rises, so no
+// cache writeback or invalidation is needed.  Workgroups of one launch are dispatched to each XCD in
+// index order and a task waits only on tasks of lower index, so every wait ends; the waits are still
+// bounded (spin_max polls): one that runs out sets the abort word and gives up, the task goes on (the
+// counters stay consistent) and the host reports the apply as failed.
+//
+// Task t's record (rs ints at rec + t rs, built with the preconditioner): row, first packed block,
+// block count, the fused D^-1 block (the last forward level, whose rows have no upper blocks, or -1),
+// then the source tags of its blocks (stream_source), padded with tags of the input vector.  The
+// record's header comes through scalar loads and its tags through one vector load per lane, neither
+// depending on other data, so a task's dependency counters are one round trip behind its start.
+constexpr int kFlowMaxNb = 32;   // blocks per row
 
-// level l: tasks [first[l], first[l + 1]), rows and packed streams S[l] as the level's launch takes
-// them, kind[l] 0 forward, 1 forward finished by D^-1 (writes x), 2 backward; unused levels have
-// first = INT_MAX
 struct FlowTable {
-    int first[kFlowLevels];
-    int kind[kFlowLevels];
-    RowStream S[kFlowLevels];
+    const int *rec = nullptr;
+    int rs = 0;                                    // ints per record: 4 + the padded tag count
+    int bwd0 = 0;                                  // first backward task
     unsigned *fdone = nullptr, *bdone = nullptr;   // completions of each row's forward / backward step
     unsigned *abort = nullptr;                     // host-mapped word, set by a wait that ran out
     int spin_max = 0;
@@ -1688,28 +1693,47 @@ struct FlowTable {
 
 struct FlowHooks {
     static constexpr bool coherent = true;
+    const int *rec;         // this task's record
+    int *stag;              // LDS: the row's source tags
+    int ntag;               // padded tag count
     unsigned *own;          // completion counters of this task's step
     const unsigned *dep1;   // counters of the src1 rows (y forward, x backward)
     const unsigned *dep0;   // counters of the src0 rows (y, backward)
     unsigned *abort;
     int spin_max;
     unsigned g = 0;         // this task's completions before this apply
-    __device__ void wait(int i, int b0, int nb, const int *bcol, int tid, int nt) {
+    int jt = 0;             // the thread's source tag (threads < ntag)
+    const unsigned *fc = nullptr;   // and the counter it waits on (null: none)
+    unsigned fv = 0;        // that counter's first poll
+    __device__ void row(const RowStream &, int, int &i, int &b0, int &nb) const {
+        i = rec[0];
+        b0 = rec[1];
+        nb = rec[2];
+    }
+    __device__ int tag(const RowStream &, int, int b, int) const { return stag[b]; }
+    __device__ bool has_d(const RowStream &) const { return rec[3] >= 0; }
+    __device__ int dblock(const RowStream &, int) const { return rec[3]; }
+    // issued before the row's first block loads, so that waiting for them leaves those in flight
+    // the thread's tag (one per thread: ntag <= kFlowMaxNb < the workgroup size), its counter's
+    // first poll and the task's own counter, issued before the row's first block loads
+    __device__ void prefetch(int i, int tid) {
+        jt = tid < ntag ? rec[4 + tid] : -1 - kSrc2Tag;
         g = __hip_atomic_load(own + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (int b = tid; b < nb; b += nt) {
-            const int j = bcol[b0 + b];
-            const int u = -1 - j;
-            const unsigned *f = j >= 0 ? dep1 : u < kSrc2Tag ? dep0 : nullptr;   // src2 (the input) needs none
-            if (!f) continue;
-            const int r = j >= 0 ? j : u;
-            for (int k = 0;; k++) {
+        const int u = -1 - jt;
+        fc = jt >= 0 ? dep1 + jt : u < kSrc2Tag ? dep0 + u : nullptr;   // src2 (the input) needs none
+        if (fc) fv = __hip_atomic_load(fc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ void wait(int, int, int, const int *, int tid, int) {
+        if (tid < ntag) {
+            stag[tid] = jt;
+            for (int k = 0; fc && fv != g + 1u; k++) {
                 if (k >= spin_max ||
                     ((k & 255) == 255 && __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))) {
                     __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                     break;
                 }
-                if (__hip_atomic_load(f + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g + 1u) break;
                 __builtin_amdgcn_s_sleep(1);
+                fv = __hip_atomic_load(fc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -1730,19 +1754,17 @@ template <class T, int BL, int WPR>
 __global__ __launch_bounds__(64 * WPR) void k_dilu_flow(FlowTable A, const int *__restrict__ active,
                                                         const T *__restrict__ F, const float *__restrict__ scale,
                                                         const double *in, double *y, double *x) {
+    __shared__ int stag[kFlowMaxNb];
     if (active && !*active) return;
     const int t = blockIdx.x;
-    int L = 0;
-#pragma unroll
-    for (int l = 1; l < kFlowLevels; l++) L += t >= A.first[l];
-    const RowStream S = A.S[L];
-    const int first = A.first[L], kind = A.kind[L];
-    const bool bwd = kind == 2;
-    FlowHooks fl{kind == 0 ? A.fdone : A.bdone, bwd ? A.bdone : A.fdone, bwd ? A.fdone : nullptr, A.abort,
-                 A.spin_max};
+    const int *rec = A.rec + (size_t)t * A.rs;
+    const bool bwd = t >= A.bwd0, fin = rec[3] >= 0;   // backward; forward finished by D^-1
+    FlowHooks fl{rec, stag, A.rs - 4, bwd || fin ? A.bdone : A.fdone, bwd ? A.bdone : A.fdone,
+                 bwd ? A.fdone : nullptr, A.abort, A.spin_max};
+    const RowStream S{nullptr, nullptr, nullptr};
     // forward: y_i = in_i - sum L_ij y_j (level-0 sources read in); backward: x_i = D_i^-1 y_i - sum DU_ij x_j
-    rowstream_row<T, BL, WPR, double>(S, t - first, F, scale, bwd ? y : nullptr, bwd ? x : y, -1.0, in,
-                                      bwd ? 1.0 : -1.0, bwd ? nullptr : in, nullptr, kind == 0 ? y : x, fl);
+    rowstream_row<T, BL, WPR, double>(S, 0, F, scale, bwd ? y : nullptr, bwd ? x : y, -1.0, in, bwd ? 1.0 : -1.0,
+                                      bwd ? nullptr : in, nullptr, bwd || fin ? x : y, fl);
 }
 
 inline bool flow_compiled(int bl) {
@@ -1789,7 +1811,8 @@ struct cpg_prec_s {
     // backward), -1-i for D_i^-1 y_i (src0 = y), or the input vector (src2) where the source
     // row has no lower blocks, so that y = x there and forward level 0 needs no launch
     DevBuf<int> pos, fptr, bptr, scol;
-    std::vector<int> h_frows, h_brows, h_fptr, h_bptr;   // host copies (persistent batched solves)
+    std::vector<int> h_frows, h_brows, h_fptr, h_bptr, h_scol;   // host copies (persistent batched solves,
+                                                                 // single-launch apply records)
     int nbmax_f = 1, nbmax_b = 1;
     // storage
     DevBuf<unsigned char> F;      // factor blocks in `precision`
@@ -1823,6 +1846,7 @@ struct cpg_prec_s {
     bool flow_enabled = true;    // cpg_prec_set_flow
     int flow_spin = 1 << 22;
     DevBuf<unsigned> fdone, bdone;
+    DevBuf<int> flow_rec;        // task records
     MappedWord flow_abort;
 
     bool dilu() const { return kind == CPG_PC_DILU || kind == CPG_PC_INNER_GMRES; }
@@ -2028,35 +2052,30 @@ void flow_setup(cpg_prec_s &P) {
     P.flow_ok = false;
     P.flow_tasks = 0;
     if (P.kind != CPG_PC_DILU || P.diag32 || !flow_compiled(P.bl)) return;
+    const int nbm = std::max(P.nbmax_f, P.nbmax_b);
+    if (nbm > kFlowMaxNb) return;
     const int nlf = (int)P.flev.size() - 1;
     const bool fuse = nlf >= 2 && P.frow0[nlf - 1] >= 0 && P.brow0.size() > 0 && P.brow0[0] == P.frow0[nlf - 1] &&
                       P.blev[1] - P.blev[0] == P.flev[nlf] - P.flev[nlf - 1];
-    FlowTable T;
-    int tasks = 0, nl = 0;
-    for (int l = 0; l < kFlowLevels; l++) {
-        T.first[l] = INT_MAX;
-        T.kind[l] = 0;
-        T.S[l] = RowStream{nullptr, nullptr, nullptr};
-    }
-    auto add = [&](const DevBuf<int> &rows, const std::vector<int> &row0, const DevBuf<int> &ptr,
-                   const std::vector<int> &lev, int lv, int kind) {
-        const int r0 = lev[lv], nr = lev[lv + 1] - r0;
-        if (!nr) return true;
-        if (nl == kFlowLevels) return false;
-        T.S[nl] = RowStream{row0[lv] >= 0 ? nullptr : rows.p + r0, ptr.p + r0, P.scol.p, nullptr};
-        T.S[nl].row0 = row0[lv] >= 0 ? row0[lv] : 0;
-        if (kind == 1) T.S[nl].dptr = P.bptr.p + P.blev[0];
-        T.first[nl] = tasks;
-        T.kind[nl] = kind;
-        nl++;
-        tasks += nr;
-        return true;
+    const int rs = 4 + ((nbm + 3) & ~3);   // 16-byte records
+    std::vector<int> rec;
+    auto add = [&](const std::vector<int> &rows, const std::vector<int> &ptr, int p0, int p1, bool fin) {
+        for (int p = p0; p < p1; p++) {
+            const int i = rows[p], b0 = ptr[p], nb = ptr[p + 1] - b0;
+            rec.push_back(i);
+            rec.push_back(b0);
+            rec.push_back(nb);
+            rec.push_back(fin ? P.h_bptr[P.blev[0] + (p - p0)] : -1);
+            for (int b = 0; b < rs - 4; b++) rec.push_back(b < nb ? P.h_scol[b0 + b] : -1 - (kSrc2Tag + i));
+        }
     };
-    for (int lv = 1; lv < nlf; lv++)
-        if (!add(P.frows, P.frow0, P.fptr, P.flev, lv, fuse && lv == nlf - 1 ? 1 : 0)) return;
+    for (int lv = 1; lv < nlf; lv++) add(P.h_frows, P.h_fptr, P.flev[lv], P.flev[lv + 1], fuse && lv == nlf - 1);
+    const int bwd0 = (int)(rec.size() / rs);
     for (int lv = fuse ? 1 : 0; lv + 1 < (int)P.blev.size(); lv++)
-        if (!add(P.brows, P.brow0, P.bptr, P.blev, lv, 2)) return;
+        add(P.h_brows, P.h_bptr, P.blev[lv], P.blev[lv + 1], false);
+    const int tasks = (int)(rec.size() / rs);
     if (!tasks) return;
+    P.flow_rec.upload(rec);
     if (P.fdone.n != (size_t)P.n) {
         P.fdone.alloc(P.n);
         P.bdone.alloc(P.n);
@@ -2064,6 +2083,10 @@ void flow_setup(cpg_prec_s &P) {
         CPG_HIP(hipMemset(P.bdone.p, 0, (size_t)P.n * sizeof(unsigned)));
     }
     P.flow_abort.alloc();
+    FlowTable T;
+    T.rec = P.flow_rec.p;
+    T.rs = rs;
+    T.bwd0 = bwd0;
     T.fdone = P.fdone.p;
     T.bdone = P.bdone.p;
     T.abort = P.flow_abort.d;
@@ -2498,6 +2521,7 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
                 P->scol.upload(scol);
                 P->h_fptr = fptr;
                 P->h_bptr = bptr;
+                P->h_scol = scol;
             }
             P->h_frows = frows;
             P->h_brows = brows;

@@ -150,8 +150,19 @@ inline size_t rowstream_lds(int nbmax) {
 
 // Hooks of a sweep whose rows wait for the rows they read (none: the level launches order them).
 // coherent: the row's sources and its result are read and written at device scope.
+// row(): the item's row i, first packed block b0 and block count nb; tag(): the source tag of
+// packed block b0 + b; has_d() / dblock(): the fused D^-1 block (RowStream::dptr).
 struct NoFlow {
     static constexpr bool coherent = false;
+    __device__ void row(const RowStream &S, int w, int &i, int &b0, int &nb) const {
+        i = S.rows ? S.rows[w] : S.row0 + w;
+        b0 = S.bptr ? S.bptr[w] : S.row0 + w;
+        nb = S.bptr ? S.bptr[w + 1] - b0 : 1;
+    }
+    __device__ int tag(const RowStream &S, int b0, int b, int i) const { return S.bcol ? S.bcol[b0 + b] : i; }
+    __device__ bool has_d(const RowStream &S) const { return S.dptr != nullptr; }
+    __device__ int dblock(const RowStream &S, int w) const { return S.dptr[w]; }
+    __device__ void prefetch(int, int) {}
     __device__ void wait(int, int, int, const int *, int, int) {}
     __device__ void publish(int, int) {}
 };
@@ -184,9 +195,8 @@ __device__ __forceinline__ void rowstream_row(const RowStream &S, int w, const T
     A *red = reinterpret_cast<A *>(lds_d);
     A *xs = red + G * BL;
     const int tid = threadIdx.x;
-    const int i = S.rows ? S.rows[w] : S.row0 + w;
-    const int b0 = S.bptr ? S.bptr[w] : S.row0 + w;
-    const int nb = S.bptr ? S.bptr[w + 1] - b0 : 1;
+    int i, b0, nb;
+    fl.row(S, w, i, b0, nb);
     const int ncol = nb * BL;
     const int lane = tid & 63, wv = tid >> 6;
     const int gl = lane / TPC;
@@ -208,11 +218,12 @@ __device__ __forceinline__ void rowstream_row(const RowStream &S, int w, const T
                 if (cg < ncol) r[l][k] = raw_load<T, VW>(p + (size_t)cg * BL);
             }
     };
+    fl.prefetch(i, tid);
     if (gl < GW) issue(cur, 0);
     fl.wait(i, b0, nb, S.bcol, tid, NT);
     for (int e = tid; e < ncol; e += NT) {
         const int b = e / BL, c = e - b * BL;
-        const int j = S.bcol ? S.bcol[b0 + b] : i;
+        const int j = fl.tag(S, b0, b, i);
         double v = stream_source<XT, Flow::coherent>(j, c, BL, src0, src1, c1, src2, c2);
         if constexpr (sizeof(T) == 2) v *= (double)scale[b0 + b];
         xs[e] = (A)v;
@@ -251,7 +262,7 @@ __device__ __forceinline__ void rowstream_row(const RowStream &S, int w, const T
             for (int v = 0; v < VW; v++) red[(g0 + GW * WPR * l) * BL + rr + v] = acc[l][v];
     }
     __syncthreads();
-    if (!S.dptr) {
+    if (!fl.has_d(S)) {
         for (int t = tid; t < BL; t += NT) {
             double s = base64 ? base64[(size_t)i * BL + t] : base32 ? (double)base32[(size_t)i * BL + t] : 0.0;
             double r = 0.0;
@@ -264,7 +275,7 @@ __device__ __forceinline__ void rowstream_row(const RowStream &S, int w, const T
     }
     // the row result becomes the source of one more block: gathered (and BF16-scaled) exactly as
     // a separate launch would gather it, then streamed and reduced in the same order
-    const int bd = S.dptr[w];
+    const int bd = fl.dblock(S, w);
     double dsc = 1.0;
     if constexpr (sizeof(T) == 2) dsc = (double)scale[bd];
     for (int t = tid; t < BL; t += NT) {
