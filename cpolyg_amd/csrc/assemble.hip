@@ -977,8 +977,8 @@ static void launch_newton_apply(const cpg_equation_s &E, double scale, const dou
                                 const double *xm, double *y, const int *active, const double *vol, const double *edg,
                                 hipStream_t s) {
     const LinData ld{vol ? vol : E.lin_vol.p, vol ? edg : E.lin_edg.p, E.nvpts, E.nedges * Deg<DEG>::NQL};
-    hipLaunchKernelGGL((k_cellop<DEG, NC, true>), dim3(E.n), dim3(64), 0, s, E.eq, E.geo(), v, ld, scale, scale_dev,
-                       active, E.mass.p, xm, y);
+    launch_timed(k_cellop<DEG, NC, true>, dim3(E.n), dim3(64), 0, s, E.eq, E.geo(), v, ld, scale, scale_dev, active,
+                 (const double *)E.mass.p, xm, y);
     CPG_LAUNCH_CHECK();
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <atomic>
 #include <cstdint>
@@ -178,4 +179,26 @@ void prec_factor_coupled(cpg_prec P, const CoupledBlocks &C, hipStream_t s);
 // flexible GMRES (K in CPG_GMRES_FLEXIBLE mode) with the preconditioner as a PcOp
 void gmres_flexible_op(cpg_krylov K, const LinOp &A, const PcOp &M, const double *b, double *x, bool x_zero,
                        double *tol, int *maxit, hipStream_t s);
+
+// Kernel timing from a dispatch's own start and end timestamps (hipExtLaunchKernel events, no
+// marker packets between kernels): the caller arms a pair of events and the next launch that goes
+// through launch_timed takes them (and sets used); other launches leave them armed.
+struct DispatchTiming {
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool used = false;
+};
+DispatchTiming &dispatch_timing();   // per host thread
+
+template <class F, class... Args>
+void launch_timed(F kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s, Args... args) {
+    DispatchTiming &t = dispatch_timing();
+    if (t.start) {
+        hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)lds, s, t.start, t.stop, 0u, args...);
+        t.used = true;
+        t.start = t.stop = nullptr;
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+    }
+}
+
 }  // namespace cpg

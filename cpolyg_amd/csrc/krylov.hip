@@ -405,6 +405,10 @@ struct cpg_krylov_s {
     std::vector<hipEvent_t> ev;
     std::vector<int> evkind;  // 0 operator, 1 preconditioner
     size_t evused = 0;
+    // per marker pair: a dispatch-timestamp pair (launch_timed) that replaces it where the timed
+    // launch took it
+    std::vector<hipEvent_t> dev;
+    std::vector<char> dused;
     int n_matvec = 0, n_apply = 0;
     double ms_matvec = 0, ms_apply = 0;
 
@@ -424,6 +428,7 @@ struct cpg_krylov_s {
     ~cpg_krylov_s() {
         drop_graphs();
         for (auto e : ev) (void)hipEventDestroy(e);
+        for (auto e : dev) (void)hipEventDestroy(e);
         for (auto e : fev) (void)hipEventDestroy(e);
         if (h_flags) (void)hipHostFree(h_flags);
         if (cev) (void)hipEventDestroy(cev);
@@ -442,11 +447,30 @@ struct cpg_krylov_s {
         CPG_HIP(hipEventRecord(ev[evused], s));
         evused++;
     }
+    // before an operator / preconditioner application (after its opening mark): its timed launch,
+    // if it has exactly one, takes the next dispatch-timestamp pair
+    void arm() {
+        if (!timing) return;
+        const size_t p = evused / 2;
+        while (dev.size() < 2 * (p + 1)) {
+            hipEvent_t e;
+            CPG_HIP(hipEventCreate(&e));
+            dev.push_back(e);
+        }
+        if (dused.size() < p + 1) dused.resize(p + 1);
+        dispatch_timing() = DispatchTiming{dev[2 * p], dev[2 * p + 1], false};
+    }
+    void disarm() {
+        if (!timing) return;
+        dused[evused / 2] = dispatch_timing().used;
+        dispatch_timing() = DispatchTiming{};
+    }
     void harvest() {
         if (!timing) return;
         for (size_t k = 0; k + 1 < evused; k += 2) {
             float ms = 0;
-            CPG_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            if (k / 2 < dused.size() && dused[k / 2]) CPG_HIP(hipEventElapsedTime(&ms, dev[k], dev[k + 1]));
+            else CPG_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
             if (evkind[k] == 0) { ms_matvec += ms; n_matvec++; }
             else { ms_apply += ms; n_apply++; }
         }
@@ -465,13 +489,17 @@ struct KOps {
 
     void matvec(const double *x, double *y, const int *active = nullptr) {
         K.mark(0, s);
+        K.arm();
         A.fn(A.ctx, x, y, active, s);
+        K.disarm();
         K.mark(0, s);
     }
     void pc(double *x) {
         if (!P) return;
         K.mark(1, s);
+        K.arm();
         int st = cpg_prec_apply(P, x, s);
+        K.disarm();
         if (st) throw Error{st};
         K.mark(1, s);
     }
@@ -488,7 +516,9 @@ struct KOps {
             return;
         }
         K.mark(1, s);
+        K.arm();
         prec_apply_to(P, in, out, s, active);
+        K.disarm();
         K.mark(1, s);
     }
     // ||x|| into dst (device), two fixed-order stages
@@ -936,6 +966,11 @@ int cpg_jacobi(cpg_krylov K, cpg_bsr A, cpg_prec P, const double *b, double *x, 
 
 // timing accessors used by the stepper
 namespace cpg {
+DispatchTiming &dispatch_timing() {
+    static thread_local DispatchTiming t;
+    return t;
+}
+
 void krylov_timing(cpg_krylov K, bool on) {
     K->timing = on;
     K->n_matvec = K->n_apply = 0;

@@ -2115,8 +2115,8 @@ void launch_flow_w(cpg_prec_s &P, const double *in, double *out, hipStream_t s, 
     FlowTable A = P.flow;
     A.spin_max = P.flow_spin;
     const float *sc = P.precision == CPG_BF16 ? P.scale.p : nullptr;
-    hipLaunchKernelGGL((k_dilu_flow<T, BL, WPR>), dim3(P.flow_tasks), dim3(64 * WPR), lds, s, A, active, Fp<T>(P),
-                       sc, in, P.ybuf.p, out);
+    launch_timed(k_dilu_flow<T, BL, WPR>, dim3(P.flow_tasks), dim3(64 * WPR), lds, s, A, active,
+                 (const T *)Fp<T>(P), sc, in, P.ybuf.p, out);
     CPG_LAUNCH_CHECK();
 }
 
@@ -2561,6 +2561,15 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
 
 int cpg_prec_destroy(cpg_prec P) {
     return guard([&] { delete P; });
+}
+
+int cpg_prec_apply_to(cpg_prec P, const double *in, double *out, void *stream) {
+    return guard([&] {
+        CPG_CHECK(P, CPG_ERR_ARG, "null argument");
+        if (P->n == 0) return;
+        CPG_CHECK(in && out && in != out, CPG_ERR_ARG, "in and out must be distinct device vectors");
+        prec_apply_to(P, in, out, S(stream));
+    });
 }
 
 int cpg_prec_set_flow(cpg_prec P, int enable, int spin_limit) {

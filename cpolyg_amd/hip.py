@@ -211,6 +211,10 @@ class Prec:
         _chk(lib().cpg_prec_levels(self.h, C.byref(n)))
         return n.value
 
+    def apply_to(self, x, out, stream=None):
+        """out = M^-1 x (distinct vectors)."""
+        _chk(lib().cpg_prec_apply_to(self.h, _vp(ptr(x)), _vp(ptr(out)), _stream(stream)))
+
     def set_flow(self, enable=-1, spin_limit=-1):
         """Block DILU apply in one launch (1) or one launch per level (0); spin_limit bounds each
         row's wait for the rows it reads (0: give up at once).  -1 leaves a setting unchanged."""

@@ -105,6 +105,10 @@ int cpg_prec_set_inner(cpg_prec P, int iterations);
 int cpg_prec_destroy(cpg_prec P);
 int cpg_prec_factor(cpg_prec P, cpg_bsr A, void *stream);
 int cpg_prec_apply(cpg_prec P, double *x, void *stream);   /* x <- M^-1 x, in place */
+/* out <- M^-1 in (distinct vectors; CPG_PC_NONE copies): the form flexible GMRES applies,
+ * z_i = M^-1 v_i.  The reference's Preconditioner::solve (src/Preconditioners.h:10-15) works
+ * in place only. */
+int cpg_prec_apply_to(cpg_prec P, const double *in, double *out, void *stream);
 /* Block DILU of the Radau IIA stage-coupled Newton matrix in cell-interleaved order (3 stages
  * x bl = 40: 120-wide blocks), with the coupled blocks formed inside the factor: block (s, t)
  * of slot q is coef[3 s + t] * J_s[q], plus E's mass block on the diagonal slots when s == t.

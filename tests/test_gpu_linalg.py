@@ -123,6 +123,28 @@ def test_dilu_single_launch_matches_level_launches(cuda, case, prec):
         assert rel(outs[0], O.dilu_apply(rp, ci, Fo, x)) < 1e-11
 
 
+@pytest.mark.parametrize("kind,prec", [(PC_DILU, PREC_BF16), (PC_DILU, PREC_FP64), (PC_DILU, PREC_BF16_D32),
+                                       (PC_BJACOBI, PREC_FP32)])
+def test_prec_apply_out_of_place_matches_in_place(cuda, kind, prec):
+    """cpg_prec_apply_to (z = M^-1 x, the flexible-GMRES form) leaves x alone and gives the in-place
+    result bitwise."""
+    import torch
+    m, orc, B = _euler_system(3)
+    A = hip.Bsr(orc.rowptr, orc.colind, B.shape[1])
+    A.upload(B)
+    P = hip.Prec(A, kind, prec)
+    P.factor()
+    x = np.random.default_rng(9).standard_normal(m.n_cells * B.shape[1])
+    xd = _dev(cuda, x)
+    z = torch.full_like(xd, np.nan)
+    P.apply_to(xd, z)
+    np.testing.assert_array_equal(xd.cpu().numpy(), x)
+    P.apply(xd)
+    np.testing.assert_array_equal(z.cpu().numpy(), xd.cpu().numpy())
+    with pytest.raises(hip.CpgError):
+        P.apply_to(xd, xd)
+
+
 def test_dilu_single_launch_not_used_with_fp32_inverses(cuda):
     m, orc, B = _euler_system(3)
     A = hip.Bsr(orc.rowptr, orc.colind, B.shape[1])
