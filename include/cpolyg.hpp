@@ -208,6 +208,8 @@ class FactoredPreconditioner : public Preconditioner {
         check(cpg_prec_factor(h, A.handle(), nullptr));
     }
     void solve(double *x) override { check(cpg_prec_apply(h_.get(), x, nullptr)); }
+    // out = M^-1 in on distinct device vectors (the reference's solve works in place only)
+    void solve(const double *in, double *out) { check(cpg_prec_apply_to(h_.get(), in, out, nullptr)); }
     cpg_prec handle() const override { return h_.get(); }
     void refactor(BlockMatrix &A) { check(cpg_prec_factor(h_.get(), A.handle(), nullptr)); }
 
