@@ -413,8 +413,9 @@ def bench_batch(args):
         us = [torch.from_numpy(u0).cuda() for _, _, _, u0, _, _ in classes]
         outs = [torch.zeros_like(u) for u in us]
         t = 0.0
-        # warmup: at least the persistent solver's autotuner trials (up to 4 candidates, two solves per step)
-        for _ in range(max(args.warmup, 3 if mode_name == "persistent" else 0)):
+        # warmup: at least the persistent solver's autotuner trials (up to 4 candidates, two solves per step);
+        # both modes take the same steps, so that their final states compare
+        for _ in range(max(args.warmup, 3)):
             B.advance(us, dts, t, outs)
             us, outs = outs, us
         torch.cuda.synchronize()
