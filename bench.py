@@ -127,7 +127,7 @@ def workload(nx, deg):
 
 def operator_bytes(mesh, deg, nc):
     """Algorithmic bytes of one matrix-free Newton-operator product (k_cellop<DEG, NC, true>):
-    the point data of the Newton matrix (Euler: 4 doubles per volume quadrature point, 21 per edge
+    the point data of the Newton matrix (Euler: 4 doubles per volume quadrature point, 17 per edge
     quadrature point of every cell's edges), the volume points (x, y, weight, pad: 32 B each),
     the 88-byte edge records, the cell bounding boxes and point/edge offsets, v in and y out
     (N doubles each) and the mass blocks (bs^2 doubles per cell).  The neighbours' coefficients
@@ -138,7 +138,7 @@ def operator_bytes(mesh, deg, nc):
     nql = 2 if 2 * deg <= 2 else 4 if 2 * deg <= 4 else 6 if 2 * deg <= 6 else 10
     nv = int(mesh.tptr[-1]) * nqt
     ne = len(mesh.pv)
-    point = (4 * nv + 21 * ne * nql) * 8 if nc == 4 else 0
+    point = (4 * nv + 17 * ne * nql) * 8 if nc == 4 else 0
     return point + 32 * nv + 88 * ne + n * (32 + 8) + 2 * n * nc * bs * 8 + n * bs * bs * 8
 
 

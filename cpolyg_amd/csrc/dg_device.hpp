@@ -334,7 +334,7 @@ __device__ inline EulerPrim euler_prim(double g, const double *U) {
 //   edge point:   p(U-), p(U+), U- - U+, the LF speed alpha, and alpha's derivative with respect
 //                 to U- and to U+ (one of the two is zero: alpha belongs to one state).
 constexpr int LPV = 4;    // doubles per volume point
-constexpr int LPE = 21;   // doubles per edge point: pm[4] pp[4] dU[4] alpha dm[4] dp[4]
+constexpr int LPE = 17;   // doubles per edge point: pm[4] pp[4] dU[4] alpha d[4] (euler_edge_prep)
 
 __device__ inline void euler_prim4(double g, const double *U, double *p) {
     const EulerPrim q = euler_prim(g, U);
@@ -370,7 +370,9 @@ __device__ inline void euler_an_v_p(double g, const double *p, const double *V, 
 
 // The u-dependent edge-point data: LF speed alpha = first maximum of |vn-c|, |vn|, |vn+c| of
 // Um, then of Up (the tie rule of euler_alpha_deriv / euler_alpha_deriv_plus), and d alpha
-// with respect to the state that owns it (Euler::alphaDerivative), written to rec[LPE]
+// with respect to the state that owns it (Euler::alphaDerivative), written to rec[LPE]: the
+// derivative once, its side in the sign of the stored alpha (alpha >= 0; -alpha, -0.0 included,
+// when Up owns it)
 __device__ inline void euler_edge_prep(double g, const double *Um, const double *Up, double nx, double ny,
                                        double *rec) {
     const EulerPrim pm = euler_prim(g, Um), pp = euler_prim(g, Up);
@@ -397,12 +399,9 @@ __device__ inline void euler_edge_prep(double g, const double *Um, const double 
     rec[4] = pp.u; rec[5] = pp.v; rec[6] = pp.k; rec[7] = pp.H;
 #pragma unroll
     for (int c = 0; c < 4; c++) rec[8 + c] = Um[c] - Up[c];
-    rec[12] = alpha;
+    rec[12] = side ? -alpha : alpha;
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        rec[13 + c] = side ? 0.0 : d[c];
-        rec[17 + c] = side ? d[c] : 0.0;
-    }
+    for (int c = 0; c < 4; c++) rec[13 + c] = d[c];
 }
 
 // Derivative of the Euler numerical flux along (Vm, Vp) from the edge-point data: Jd Vm + Jo Vp
@@ -411,14 +410,15 @@ __device__ inline void euler_edge_prep(double g, const double *Um, const double 
 // form).  Vp = 0 on Dirichlet edges.
 __device__ inline void euler_numflux_jvp_p(const DevEq &eq, const double *rec, const double *Vm, const double *Vp,
                                            double nx, double ny, double *f) {
-    const double g = eq.gamma, alpha = rec[12];
-    const double *dU = rec + 8, *dm = rec + 13, *dp = rec + 17;
+    const double g = eq.gamma, alpha = fabs(rec[12]);
+    const bool plus = signbit(rec[12]);   // alpha's derivative is with respect to Up
+    const double *dU = rec + 8, *d = rec + 13;
     double fm[4], fp[4], dv = 0.0;
     if (eq.jacobian) {
         euler_an_v_p(g, rec, Vm, 0, nx, ny, fm);
         euler_an_v_p(g, rec + 4, Vp, 0, nx, ny, fp);
 #pragma unroll
-        for (int c = 0; c < 4; c++) dv = fma(dm[c], Vm[c], fma(dp[c], Vp[c], dv));
+        for (int c = 0; c < 4; c++) dv = fma(d[c], plus ? Vp[c] : Vm[c], dv);
     } else {
         double Vs[4];
 #pragma unroll
@@ -427,7 +427,7 @@ __device__ inline void euler_numflux_jvp_p(const DevEq &eq, const double *rec, c
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             fp[c] = 0.0;
-            dv = fma(dm[c], Vs[c], dv);
+            dv = plus ? dv : fma(d[c], Vs[c], dv);
         }
     }
 #pragma unroll
