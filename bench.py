@@ -757,7 +757,11 @@ def main():
         ap_sus = rates[f"dilu_apply_{args.mixed}_ms"]
         ap_gbs = gbs(b_ap, ap_sus)
         lev = rates["dilu_levels"]
-        if mixed_prec == PREC_BF16_D32:
+        if mixed_prec == PREC_BF16_D32 and args.dilu_apply == "single":
+            shape = (f"two launches: the rows of {lev - 1} forward levels (level 0 a no-op), then of {lev} backward "
+                     f"levels, each backward row forming h = D^-1 y from the FP32 inverse and x = h - DU x from the "
+                     f"BF16 stream; rows wait on completion counters of the rows they read")
+        elif mixed_prec == PREC_BF16_D32:
             shape = (f"{3 * lev - 1} row-stream launches: {lev - 1} forward levels (level 0 a no-op), then per "
                      f"backward level the FP32 D^-1 products and the BF16 DU stream")
         elif args.dilu_apply == "single":

@@ -1689,10 +1689,17 @@ struct FlowTable {
     unsigned *fdone = nullptr, *bdone = nullptr;   // completions of each row's forward / backward step
     unsigned *abort = nullptr;                     // host-mapped word, set by a wait that ran out
     int spin_max = 0;
+    const float *DF = nullptr;                     // CPG_BF16_D32: the FP32 inverses (row order)
+    double *hbuf = nullptr;                        // and the products h = D^-1 y
 };
 
+// MODE 0: a task's row stream.  For factors with FP32 diagonal inverses (CPG_BF16_D32) a backward
+// task runs two: MODE 1 forms h_i = D_i^-1 (y_i or in_i) from the FP32 inverse (the record's fourth
+// entry is that product's source tag) after waiting for every source of the task, MODE 2 then
+// streams the BF16 DU blocks onto h_i and raises the task's counter.
+template <int MODE>
 struct FlowHooks {
-    static constexpr bool coherent = true;
+    static constexpr bool coh_src = true, coh_out = MODE != 1;
     const int *rec;         // this task's record
     int *stag;              // LDS: the row's source tags
     int ntag;               // padded tag count
@@ -1702,30 +1709,32 @@ struct FlowHooks {
     unsigned *abort;
     int spin_max;
     unsigned g = 0;         // this task's completions before this apply
-    int jt = 0;             // the thread's source tag (threads < ntag)
+    int jt = 0;             // the thread's source tag (threads < ntag; MODE 1: thread ntag the D^-1 source)
     const unsigned *fc = nullptr;   // and the counter it waits on (null: none)
     unsigned fv = 0;        // that counter's first poll
     __device__ void row(const RowStream &, int, int &i, int &b0, int &nb) const {
         i = rec[0];
-        b0 = rec[1];
-        nb = rec[2];
+        b0 = MODE == 1 ? i : rec[1];   // FP32 inverses in row order
+        nb = MODE == 1 ? 1 : rec[2];
     }
-    __device__ int tag(const RowStream &, int, int b, int) const { return stag[b]; }
-    __device__ bool has_d(const RowStream &) const { return rec[3] >= 0; }
+    __device__ int tag(const RowStream &, int, int b, int) const { return MODE == 1 ? rec[3] : stag[b]; }
+    __device__ bool has_d(const RowStream &) const { return MODE == 0 && rec[3] >= 0; }
     __device__ int dblock(const RowStream &, int) const { return rec[3]; }
-    // issued before the row's first block loads, so that waiting for them leaves those in flight
     // the thread's tag (one per thread: ntag <= kFlowMaxNb < the workgroup size), its counter's
-    // first poll and the task's own counter, issued before the row's first block loads
+    // first poll and the task's own counter, issued before the row's first block loads, so that
+    // waiting for them leaves those in flight
     __device__ void prefetch(int i, int tid) {
-        jt = tid < ntag ? rec[4 + tid] : -1 - kSrc2Tag;
+        if constexpr (MODE == 2) return;
+        jt = tid < ntag ? rec[4 + tid] : MODE == 1 && tid == ntag ? rec[3] : -1 - kSrc2Tag;
         g = __hip_atomic_load(own + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const int u = -1 - jt;
         fc = jt >= 0 ? dep1 + jt : u < kSrc2Tag ? dep0 + u : nullptr;   // src2 (the input) needs none
         if (fc) fv = __hip_atomic_load(fc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __device__ void wait(int, int, int, const int *, int tid, int) {
-        if (tid < ntag) {
-            stag[tid] = jt;
+        if constexpr (MODE == 2) return;   // MODE 1 waited for this task's sources
+        if (tid <= ntag) {
+            if (tid < ntag) stag[tid] = jt;
             for (int k = 0; fc && fv != g + 1u; k++) {
                 if (k >= spin_max ||
                     ((k & 255) == 255 && __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))) {
@@ -1740,6 +1749,10 @@ struct FlowHooks {
         __syncthreads();
     }
     __device__ void publish(int i, int tid) {
+        if constexpr (MODE == 1) {
+            __syncthreads();   // h_i (this workgroup's plain stores) before the DU stream reads it
+            return;
+        }
         // every wave's result stores (device scope) complete before the counter is raised
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         __builtin_amdgcn_s_waitcnt(0);
@@ -1749,7 +1762,9 @@ struct FlowHooks {
 };
 
 // x = M^-1 in through y (forward result); in may equal x (the pattern is structurally symmetric, so
-// the backward step of a row that forward rows read as input comes after theirs)
+// the backward step of a row that forward rows read as input comes after theirs).  Tasks [t0, t1):
+// every task, or for factors with FP32 diagonal inverses the forward ones (k_dilu_flow_d32b runs
+// the backward sweep)
 template <class T, int BL, int WPR>
 __global__ __launch_bounds__(64 * WPR) void k_dilu_flow(FlowTable A, const int *__restrict__ active,
                                                         const T *__restrict__ F, const float *__restrict__ scale,
@@ -1759,12 +1774,33 @@ __global__ __launch_bounds__(64 * WPR) void k_dilu_flow(FlowTable A, const int *
     const int t = blockIdx.x;
     const int *rec = A.rec + (size_t)t * A.rs;
     const bool bwd = t >= A.bwd0, fin = rec[3] >= 0;   // backward; forward finished by D^-1
-    FlowHooks fl{rec, stag, A.rs - 4, bwd || fin ? A.bdone : A.fdone, bwd ? A.bdone : A.fdone,
-                 bwd ? A.fdone : nullptr, A.abort, A.spin_max};
+    FlowHooks<0> fl{rec, stag, A.rs - 4, bwd || fin ? A.bdone : A.fdone, bwd ? A.bdone : A.fdone,
+                    bwd ? A.fdone : nullptr, A.abort, A.spin_max};
     const RowStream S{nullptr, nullptr, nullptr};
     // forward: y_i = in_i - sum L_ij y_j (level-0 sources read in); backward: x_i = D_i^-1 y_i - sum DU_ij x_j
     rowstream_row<T, BL, WPR, double>(S, 0, F, scale, bwd ? y : nullptr, bwd ? x : y, -1.0, in, bwd ? 1.0 : -1.0,
                                       bwd ? nullptr : in, nullptr, bwd || fin ? x : y, fl);
+}
+
+// The backward sweep of factors with FP32 diagonal inverses in one launch (task A.bwd0 + blockIdx.x):
+// h_i = D_i^-1 y_i (in_i where row i has no lower blocks) from the FP32 inverse, then
+// x_i = h_i - sum DU_ij x_j from the BF16 stream.  A launch of its own: with these two row-stream
+// bodies inlined beside the forward task's in one kernel, the compiler's CFG simplification crashes,
+// and as a called function the kernel spilled to a stack at half the occupancy.
+template <class T, int BL, int WPR>
+__global__ __launch_bounds__(64 * WPR) void k_dilu_flow_d32b(FlowTable A, const int *__restrict__ active,
+                                                             const T *__restrict__ F, const float *__restrict__ scale,
+                                                             const double *in, double *y, double *x) {
+    __shared__ int stag[kFlowMaxNb];
+    if (active && !*active) return;
+    const int *rec = A.rec + (size_t)(A.bwd0 + (int)blockIdx.x) * A.rs;
+    const RowStream S{nullptr, nullptr, nullptr};
+    FlowHooks<1> f1{rec, stag, A.rs - 4, A.bdone, A.bdone, A.fdone, A.abort, A.spin_max};
+    rowstream_row<float, BL, WPR, double>(S, 0, A.DF, nullptr, y, nullptr, 0.0, in, 1.0, nullptr, nullptr, A.hbuf,
+                                          f1);
+    FlowHooks<2> f2{rec, stag, A.rs - 4, A.bdone, A.bdone, A.fdone, A.abort, A.spin_max};
+    f2.g = f1.g;
+    rowstream_row<T, BL, WPR, double>(S, 0, F, scale, nullptr, x, -1.0, nullptr, 0.0, A.hbuf, nullptr, x, f2);
 }
 
 inline bool flow_compiled(int bl) {
@@ -1811,8 +1847,8 @@ struct cpg_prec_s {
     // backward), -1-i for D_i^-1 y_i (src0 = y), or the input vector (src2) where the source
     // row has no lower blocks, so that y = x there and forward level 0 needs no launch
     DevBuf<int> pos, fptr, bptr, scol;
-    std::vector<int> h_frows, h_brows, h_fptr, h_bptr, h_scol;   // host copies (persistent batched solves,
-                                                                 // single-launch apply records)
+    std::vector<int> h_frows, h_brows, h_fptr, h_bptr, h_scol, h_hcol;   // host copies (persistent batched
+                                                                         // solves, single-launch records)
     int nbmax_f = 1, nbmax_b = 1;
     // storage
     DevBuf<unsigned char> F;      // factor blocks in `precision`
@@ -2051,28 +2087,31 @@ void factor_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
 void flow_setup(cpg_prec_s &P) {
     P.flow_ok = false;
     P.flow_tasks = 0;
-    if (P.kind != CPG_PC_DILU || P.diag32 || !flow_compiled(P.bl)) return;
+    if (P.kind != CPG_PC_DILU || !flow_compiled(P.bl)) return;
     const int nbm = std::max(P.nbmax_f, P.nbmax_b);
     if (nbm > kFlowMaxNb) return;
     const int nlf = (int)P.flev.size() - 1;
-    const bool fuse = nlf >= 2 && P.frow0[nlf - 1] >= 0 && P.brow0.size() > 0 && P.brow0[0] == P.frow0[nlf - 1] &&
-                      P.blev[1] - P.blev[0] == P.flev[nlf] - P.flev[nlf - 1];
+    const bool fuse = !P.diag32 && nlf >= 2 && P.frow0[nlf - 1] >= 0 && P.brow0.size() > 0 &&
+                      P.brow0[0] == P.frow0[nlf - 1] && P.blev[1] - P.blev[0] == P.flev[nlf] - P.flev[nlf - 1];
     const int rs = 4 + ((nbm + 3) & ~3);   // 16-byte records
     std::vector<int> rec;
-    auto add = [&](const std::vector<int> &rows, const std::vector<int> &ptr, int p0, int p1, bool fin) {
+    // the fourth entry: the fused D^-1 block of a finishing forward row (else -1), or for a backward row
+    // of factors with FP32 inverses the source tag of its product D_i^-1 y_i
+    auto add = [&](const std::vector<int> &rows, const std::vector<int> &ptr, int p0, int p1, bool fin, bool bwd) {
         for (int p = p0; p < p1; p++) {
             const int i = rows[p], b0 = ptr[p], nb = ptr[p + 1] - b0;
             rec.push_back(i);
             rec.push_back(b0);
             rec.push_back(nb);
-            rec.push_back(fin ? P.h_bptr[P.blev[0] + (p - p0)] : -1);
+            rec.push_back(fin ? P.h_bptr[P.blev[0] + (p - p0)] : bwd && P.diag32 ? P.h_hcol[i] : -1);
             for (int b = 0; b < rs - 4; b++) rec.push_back(b < nb ? P.h_scol[b0 + b] : -1 - (kSrc2Tag + i));
         }
     };
-    for (int lv = 1; lv < nlf; lv++) add(P.h_frows, P.h_fptr, P.flev[lv], P.flev[lv + 1], fuse && lv == nlf - 1);
+    for (int lv = 1; lv < nlf; lv++)
+        add(P.h_frows, P.h_fptr, P.flev[lv], P.flev[lv + 1], fuse && lv == nlf - 1, false);
     const int bwd0 = (int)(rec.size() / rs);
     for (int lv = fuse ? 1 : 0; lv + 1 < (int)P.blev.size(); lv++)
-        add(P.h_brows, P.h_bptr, P.blev[lv], P.blev[lv + 1], false);
+        add(P.h_brows, P.h_bptr, P.blev[lv], P.blev[lv + 1], false, true);
     const int tasks = (int)(rec.size() / rs);
     if (!tasks) return;
     P.flow_rec.upload(rec);
@@ -2090,6 +2129,8 @@ void flow_setup(cpg_prec_s &P) {
     T.fdone = P.fdone.p;
     T.bdone = P.bdone.p;
     T.abort = P.flow_abort.d;
+    T.DF = P.diag32 ? P.DF.p : nullptr;
+    T.hbuf = P.diag32 ? P.hbuf.p : nullptr;
     P.flow = T;
     P.flow_tasks = tasks;
     P.flow_ok = true;
@@ -2111,10 +2152,28 @@ void flow_check(cpg_prec_s &P) {
 template <class T, int BL, int WPR>
 void launch_flow_w(cpg_prec_s &P, const double *in, double *out, hipStream_t s, const int *active) {
     const size_t lds = rowstream_lds<T, BL, WPR>(std::max(P.nbmax_f, P.nbmax_b));
-    set_lds(k_dilu_flow<T, BL, WPR>, lds);
     FlowTable A = P.flow;
     A.spin_max = P.flow_spin;
     const float *sc = P.precision == CPG_BF16 ? P.scale.p : nullptr;
+    if constexpr (std::is_same<T, bf16>::value) {
+        if (P.diag32) {
+            // forward sweep, then backward sweep: one launch each
+            const int nb = P.flow_tasks - A.bwd0;
+            if (A.bwd0 > 0) {
+                set_lds(k_dilu_flow<T, BL, WPR>, lds);
+                hipLaunchKernelGGL((k_dilu_flow<T, BL, WPR>), dim3(A.bwd0), dim3(64 * WPR), lds, s, A, active,
+                                   (const T *)Fp<T>(P), sc, in, P.ybuf.p, out);
+                CPG_LAUNCH_CHECK();
+            }
+            const size_t lb = std::max(lds, rowstream_lds<float, BL, WPR>(1));
+            set_lds(k_dilu_flow_d32b<T, BL, WPR>, lb);
+            hipLaunchKernelGGL((k_dilu_flow_d32b<T, BL, WPR>), dim3(nb), dim3(64 * WPR), lb, s, A, active,
+                               (const T *)Fp<T>(P), sc, in, P.ybuf.p, out);
+            CPG_LAUNCH_CHECK();
+            return;
+        }
+    }
+    set_lds(k_dilu_flow<T, BL, WPR>, lds);
     launch_timed(k_dilu_flow<T, BL, WPR>, dim3(P.flow_tasks), dim3(64 * WPR), lds, s, A, active,
                  (const T *)Fp<T>(P), sc, in, P.ybuf.p, out);
     CPG_LAUNCH_CHECK();
@@ -2505,6 +2564,7 @@ int cpg_prec_create(cpg_bsr A, int kind, int precision, cpg_prec *out) {
                         hcol[i] = lptr[i + 1] > lptr[i] ? -1 - i : -1 - (kSrc2Tag + i);
                     }
                     P->hcol.upload(hcol);
+                    P->h_hcol = hcol;
                     bool contiguous = true;
                     for (int r : P->brow0) contiguous = contiguous && r >= 0;
                     for (int r : P->frow0) contiguous = contiguous && r >= 0;

@@ -151,9 +151,10 @@ inline size_t rowstream_lds(int nbmax) {
 // Hooks of a sweep whose rows wait for the rows they read (none: the level launches order them).
 // coherent: the row's sources and its result are read and written at device scope.
 // row(): the item's row i, first packed block b0 and block count nb; tag(): the source tag of
-// packed block b0 + b; has_d() / dblock(): the fused D^-1 block (RowStream::dptr).
+// packed block b0 + b; has_d() / dblock(): the fused D^-1 block (RowStream::dptr).  coh_src /
+// coh_out: the row's sources are read / its result written at device scope.
 struct NoFlow {
-    static constexpr bool coherent = false;
+    static constexpr bool coh_src = false, coh_out = false;
     __device__ void row(const RowStream &S, int w, int &i, int &b0, int &nb) const {
         i = S.rows ? S.rows[w] : S.row0 + w;
         b0 = S.bptr ? S.bptr[w] : S.row0 + w;
@@ -224,7 +225,7 @@ __device__ __forceinline__ void rowstream_row(const RowStream &S, int w, const T
     for (int e = tid; e < ncol; e += NT) {
         const int b = e / BL, c = e - b * BL;
         const int j = fl.tag(S, b0, b, i);
-        double v = stream_source<XT, Flow::coherent>(j, c, BL, src0, src1, c1, src2, c2);
+        double v = stream_source<XT, Flow::coh_src>(j, c, BL, src0, src1, c1, src2, c2);
         if constexpr (sizeof(T) == 2) v *= (double)scale[b0 + b];
         xs[e] = (A)v;
     }
@@ -268,7 +269,7 @@ __device__ __forceinline__ void rowstream_row(const RowStream &S, int w, const T
             double r = 0.0;
 #pragma unroll
             for (int gg = 0; gg < G; gg++) r += (double)red[gg * BL + t];
-            stream_store<XT, Flow::coherent>(out + (size_t)i * BL + t, (XT)(s + r));
+            stream_store<XT, Flow::coh_out>(out + (size_t)i * BL + t, (XT)(s + r));
         }
         fl.publish(i, tid);
         return;
@@ -316,7 +317,7 @@ __device__ __forceinline__ void rowstream_row(const RowStream &S, int w, const T
         double r = 0.0;
 #pragma unroll
         for (int gg = 0; gg < G; gg++) r += (double)red[gg * BL + t];
-        stream_store<XT, Flow::coherent>(out + (size_t)i * BL + t, (XT)r);
+        stream_store<XT, Flow::coh_out>(out + (size_t)i * BL + t, (XT)r);
     }
     fl.publish(i, tid);
 }

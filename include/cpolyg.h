@@ -127,8 +127,10 @@ int cpg_prec_levels(cpg_prec P, int *n_levels);
 /* Block DILU apply on FP64 vectors in one launch: the rows of every level, forward then backward,
  * each waiting on device-side completion counters of the rows it reads, where the level launches
  * order them by launch boundaries (the same arithmetic per row: bitwise the same result).  Used by
- * default where it applies (CPG_PC_DILU with factors other than CPG_BF16_D32, block sizes 3, 6, 10,
- * 12, 15, 24, 40 and 60, at most 32 level launches).  enable: 1 on, 0 one launch per level, < 0 unchanged.
+ * default where it applies (CPG_PC_DILU, block sizes 3, 6, 10, 12, 15, 24, 40 and 60, at most 32
+ * blocks per row; CPG_BF16_D32 factors take one launch per sweep, the backward one forming
+ * h = D^-1 y from the FP32 inverses and x = h - sum DU x in each row's workgroup).  enable: 1 on,
+ * 0 one launch per level, < 0 unchanged.
  * spin_limit >= 0 bounds each wait (polls; default 2^22; 0 gives up at once, for tests of the
  * failure path), < 0 unchanged.  A wait that runs out makes that apply's result invalid: the next
  * apply or the GMRES solve that used it returns CPG_ERR_HIP, and the level launches are used from

@@ -87,14 +87,19 @@ def _permuted(orc, B, seed):
     return np.array(rp2, np.int32), np.array(ci2, np.int32), np.array(blk)
 
 
-@pytest.mark.parametrize("case", ["colour-ordered", "permuted"])
-@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32, PREC_BF16])
+@pytest.mark.parametrize("case,prec", [(c, p) for c in ("colour-ordered", "permuted")
+                                       for p in (PREC_FP64, PREC_FP32, PREC_BF16)]
+                         + [("colour-ordered", PREC_BF16_D32), ("colour-ordered-p4", PREC_BF16_D32)])
 def test_dilu_single_launch_matches_level_launches(cuda, case, prec):
-    """The one-launch DILU apply (rows waiting on completion counters) against one launch per level:
-    bitwise equal over alternating applies (the counters advance by one per apply), on a
-    colour-ordered matrix (contiguous levels, the last forward level finishing its rows) and on a
-    randomly renumbered one (levels of scattered rows); FP64 also against the oracle."""
-    if case == "colour-ordered":
+    """The one-launch DILU apply (rows waiting on completion counters; with FP32 diagonal inverses one
+    launch per sweep) against one launch per level: bitwise equal over alternating applies (the
+    counters advance by one per apply), on a colour-ordered matrix (contiguous levels, the last
+    forward level finishing its rows) and on a randomly renumbered one (levels of scattered rows);
+    FP64 also against the oracle."""
+    if case == "colour-ordered-p4":
+        m, orc, B = _euler_system(4, nx=6)
+        rp, ci = orc.rowptr, orc.colind
+    elif case == "colour-ordered":
         m, orc, B = _euler_system(3)
         rp, ci = orc.rowptr, orc.colind
     else:
@@ -143,14 +148,6 @@ def test_prec_apply_out_of_place_matches_in_place(cuda, kind, prec):
     np.testing.assert_array_equal(z.cpu().numpy(), xd.cpu().numpy())
     with pytest.raises(hip.CpgError):
         P.apply_to(xd, xd)
-
-
-def test_dilu_single_launch_not_used_with_fp32_inverses(cuda):
-    m, orc, B = _euler_system(3)
-    A = hip.Bsr(orc.rowptr, orc.colind, B.shape[1])
-    A.upload(B)
-    assert not hip.Prec(A, PC_DILU, PREC_BF16_D32).flow_in_use()
-    assert hip.Prec(A, PC_DILU, PREC_BF16).flow_in_use()
 
 
 def test_dilu_single_launch_wait_limit_reports_and_recovers(cuda):
