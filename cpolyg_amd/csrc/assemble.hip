@@ -309,10 +309,23 @@ __global__ __launch_bounds__(64, DEG <= 3 ? 3 : 2) void k_cellop(DevEq eq, Geo g
         if (pt && E.nb >= 0) nbb = g.bb[E.nb];
         if (pt && q == 0) sNbi[el] = E.nb;
         __syncthreads();   // readers of the previous chunk's staging and of the record are done
-        for (int idx = lane; idx < nec * SW; idx += 64) {
-            const int el = idx / SW, r = idx - el * SW;
-            const int nb = sNbi[el];   // -1 on a Dirichlet edge
-            stg[idx] = nb >= 0 ? x[(size_t)nb * BL + r] : 0.0;
+        {
+            // every gather in flight before the first LDS write: one round trip per chunk (a loop
+            // of load-then-store waited for each load in turn); Dirichlet edges (nb = -1) and idle
+            // lanes read x[0] and store 0
+            constexpr int GI = (L::STG + 63) / 64;
+            double gv[GI];
+#pragma unroll
+            for (int t = 0; t < GI; t++) {
+                const int idx = lane + 64 * t;
+                const int el = idx / SW, r = idx - el * SW;
+                const int nb = idx < nec * SW ? sNbi[el] : -1;
+                const double v = x[nb >= 0 ? (size_t)nb * BL + r : 0];
+                gv[t] = nb >= 0 ? v : 0.0;
+            }
+#pragma unroll
+            for (int t = 0; t < GI; t++)
+                if (lane + 64 * t < nec * SW) stg[lane + 64 * t] = gv[t];
         }
         __syncthreads();
         if (pt) {
