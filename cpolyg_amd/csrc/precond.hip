@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 namespace cpg {
 
@@ -947,17 +948,18 @@ __device__ inline void wg_block_to_lds(double *X, const double *__restrict__ g) 
     }
 }
 
-// the block in X to global memory in precision T, all threads (BF16: one power-of-two scale
-// for the block from a workgroup max, written to *scale_out)
-template <class T, int BL>
-__device__ inline void wg_lds_block_store(const double *X, T *__restrict__ g, float *scale_out, double *red) {
+// the block in X (FP64, or FP32 for the coupled factor's FP32 products) to global memory in
+// precision T, all threads (BF16: one power-of-two scale for the block from a workgroup max,
+// written to *scale_out)
+template <class T, int BL, class XT>
+__device__ inline void wg_lds_block_store(const XT *X, T *__restrict__ g, float *scale_out, double *red) {
     using W = WF<BL>;
     float inv = 1.0f;
     if constexpr (Store<T>::prec == CPG_BF16) {
         double mx = 0.0;
         for (int e = threadIdx.x; e < W::BB; e += FWG) {
             const int c = e / BL, r = e - c * BL;
-            mx = fmax(mx, fabs(X[c * W::LD + r]));
+            mx = fmax(mx, fabs((double)X[c * W::LD + r]));
         }
         mx = wave_max(mx);
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
@@ -969,7 +971,7 @@ __device__ inline void wg_lds_block_store(const double *X, T *__restrict__ g, fl
     }
     for (int e = threadIdx.x; e < W::BB; e += FWG) {
         const int c = e / BL, r = e - c * BL;
-        Store<T>::put(g, e, X[c * W::LD + r], inv);
+        Store<T>::put(g, e, (double)X[c * W::LD + r], inv);
     }
 }
 
@@ -1391,9 +1393,10 @@ __global__ __launch_bounds__(IWG) void k_f32_combine(const float *__restrict__ Q
 //   C_q[(s,r),(t,c)] = coef[s][t] J_s[q](r,c)  (+ m_i(r mod bs, c mod bs) when i = j, s = t and
 //   r, c lie in the same component),
 // built from the three stage Jacobians (40 x 40 column-major blocks of one pattern) and the
-// per-cell mass blocks.  The factor is the DILU above on these blocks, in FP64 on
-// v_mfma_f64_16x16x4f64 (8 x 8 tiles of 16 over the padded 128 x 128 block; wavefront w of the
-// workgroup owns tile columns w and w + 4, 16 tiles):
+// per-cell mass blocks.  The factor is the DILU above on these blocks, on 16x16x4 MFMA (8 x 8
+// tiles of 16 over the padded 128 x 128 block; wavefront w of the workgroup owns tile columns w
+// and w + 4, 16 tiles): FP64 products for FP64 factors, FP32 products for FP32 and BF16 factors
+// (k_cdilu_schur32 / k_cdilu_upper32 below), the inverses in FP64 for every precision:
 //   Schur (one workgroup per row i of a level): per lower neighbour k: X <- (coef x I) D_k^-1,
 //     T = BD(J_ik) X (registers), X <- T (stored as L_ik), accY += X BD(J_ki) (a second set of
 //     accumulators); then S_i = C_ii - accY (coef x I) to DD64[i] element by element;
@@ -1652,6 +1655,250 @@ __global__ __launch_bounds__(FWG) void k_cdilu_upper(const int *__restrict__ uro
         for (int j = 0; j < 16; j++) tile_to_lds<CBL>(X, j >> 1, wv + 4 * (j & 1), acc[j]);
         __syncthreads();
         coupled_mix_cols(X, src);   // DU_ij = Y (coef x I)
+        __syncthreads();
+        wg_lds_block_store<T, CBL>(X, F + (size_t)pos[sl] * CBB, scale ? scale + pos[sl] : nullptr, red);
+    }
+}
+
+// ---- the coupled factor with FP32 products, for FP32 and BF16 factors ----------------------
+// Factors stored in FP32 or BF16 take their Schur products and DU blocks on
+// v_mfma_f32_16x16x4f32, at twice the FP64 MFMA rate, from FP32 copies of D_k^-1 and of the stage
+// blocks staged in LDS: 76 KB per workgroup, so two workgroups share a CU where the FP64 kernels
+// hold one.  The stage mixing, S_i = C_ii - accY (coef x I) and the Gauss-Jordan inverse stay in
+// FP64; the FP32 rounding of the products (about 1e-7 relative to |L_ik| |BD(J_ki)|) is below the
+// FP32 storage of the factor it feeds.  The FP32 16x16x4 MFMA takes its inputs like the FP64 one
+// (lane l: A row l % 16, B column l % 16, k-step l / 16) but returns row 4 (l / 16) + e of column
+// l % 16 in register e (the FP64 one: row l / 16 + 4 e).
+typedef float fx4 __attribute__((ext_vector_type(4)));
+constexpr size_t CLDS32 = ((size_t)CBL * CXL + 3 * CSTAGE) * sizeof(float);
+
+__device__ inline void coupled_stage_store32(float *Jl, const double2 (&v)[CSU]) {
+    constexpr int SB = CSB * CSB, NU = 3 * SB / 2;
+#pragma unroll
+    for (int j = 0; j < CSU; j++) {
+        const int u = (int)threadIdx.x + FWG * j;
+        if (u < NU) {
+            const int st = (2 * u) / SB, rem = 2 * u - st * SB, c = rem / CSB, r = rem - c * CSB;
+            Jl[st * CSTAGE + c * CLD + r] = (float)v[j].x;
+            Jl[st * CSTAGE + c * CLD + r + 1] = (float)v[j].y;
+        }
+    }
+}
+
+// the three stage blocks of slot q straight to Jl (FP32), all threads, loads batched
+__device__ inline void coupled_stage_to_lds32(float *Jl, const CoupledSrc &src, int q) {
+    constexpr int SB = CSB * CSB, NU = 3 * SB / 2;
+    wg_batched<NU>(
+        [&](int u) {
+            const int st = (2 * u) / SB, rem = 2 * u - st * SB;
+            const double *J = st == 0 ? src.J0 : st == 1 ? src.J1 : src.J2;
+            return *reinterpret_cast<const double2 *>(J + (size_t)q * SB + rem);
+        },
+        [&](int u, double2 v) {
+            const int st = (2 * u) / SB, rem = 2 * u - st * SB, c = rem / CSB, r = rem - c * CSB;
+            Jl[st * CSTAGE + c * CLD + r] = (float)v.x;
+            Jl[st * CSTAGE + c * CLD + r + 1] = (float)v.y;
+        });
+}
+
+// X (FP32, column stride CXL) <- the FP64 120 x 120 block at g
+__device__ inline void coupled_block_to_lds32(float *X, const double *__restrict__ g) {
+    wg_batched<(int)CBB / 2>([&](int u) { return *reinterpret_cast<const double2 *>(g + 2 * u); },
+                             [&](int u, double2 a) {
+                                 const int c = (2 * u) / CBL, r = 2 * u - c * CBL;
+                                 X[c * CXL + r] = (float)a.x;
+                                 X[c * CXL + r + 1] = (float)a.y;
+                             });
+}
+
+__device__ inline void coupled_mix_rows32(float *X, const CoupledSrc &src) {
+    for (int e = threadIdx.x; e < CSB * CBL; e += FWG) {
+        const int n = e / CSB, r = e - n * CSB;
+        float *x = X + (size_t)n * CXL + r;
+        const double x0 = x[0], x1 = x[CSB], x2 = x[2 * CSB];
+#pragma unroll
+        for (int st = 0; st < 3; st++)
+            x[st * CSB] = (float)fma(src.coef[3 * st], x0, fma(src.coef[3 * st + 1], x1, src.coef[3 * st + 2] * x2));
+    }
+}
+
+__device__ inline void coupled_mix_cols32(float *Y, const CoupledSrc &src) {
+    for (int e = threadIdx.x; e < CSB * CBL; e += FWG) {
+        const int c = e / CBL, m = e - c * CBL;
+        float *y = Y + (size_t)c * CXL + m;
+        const double y0 = y[0], y1 = y[(size_t)CSB * CXL], y2 = y[(size_t)2 * CSB * CXL];
+#pragma unroll
+        for (int t = 0; t < 3; t++)
+            y[(size_t)t * CSB * CXL] = (float)fma(src.coef[t], y0, fma(src.coef[3 + t], y1, src.coef[6 + t] * y2));
+    }
+}
+
+__device__ inline double coupled_mix_col32(const float *Y, const CoupledSrc &src, int e) {
+    const int col = e / CBL, m = e - col * CBL, t = col / CSB, c = col - t * CSB;
+    const float *y = Y + (size_t)c * CXL + m;
+    return fma(coupled_coef(src, 0, t), (double)y[0],
+               fma(coupled_coef(src, 1, t), (double)y[(size_t)CSB * CXL],
+                   coupled_coef(src, 2, t) * (double)y[(size_t)2 * CSB * CXL]));
+}
+
+// tile (tm, tn) of an FP32 16x16x4 accumulator to X
+__device__ inline void tile_to_lds32(float *X, int tm, int tn, const fx4 &t) {
+    const int l = threadIdx.x & 63;
+    const int col = 16 * tn + (l & 15);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int row = 16 * tm + 4 * (l >> 4) + e;
+        if (row < CBL && col < CBL) X[col * CXL + row] = t[e];
+    }
+}
+
+// acc[2 tm + h] = tile (tm, w + 4 h) of BD(J) X (bd_times_x on FP32)
+__device__ inline void bd_times_x32(const float *Jl, const float *X, fx4 (&acc)[16]) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < 16; j++) acc[j] = fx4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int n0 = 16 * wv + lr, n1 = 16 * (wv + 4) + lr;
+    const bool v1 = n1 < CBL;
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+#pragma unroll
+        for (int kk = 0; kk < CSB / 4; kk++) {
+            const int c = 4 * kk + lk, k = CSB * t + c;
+            const float b0 = X[n0 * CXL + k];
+            const float b1 = v1 ? X[n1 * CXL + k] : 0.0f;
+#pragma unroll
+            for (int tm = 0; tm < CTM; tm++) {
+                if ((16 * tm) / CSB > t || (16 * tm + 15) / CSB < t) continue;
+                const int m = 16 * tm + lr;
+                const float a = (m < CBL && m / CSB == t) ? Jl[t * CSTAGE + c * CLD + (m - t * CSB)] : 0.0f;
+                acc[2 * tm] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc[2 * tm], 0, 0, 0);
+                acc[2 * tm + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc[2 * tm + 1], 0, 0, 0);
+            }
+        }
+    }
+}
+
+template <bool H0, bool H1>
+__device__ inline void x_bd_stage32(const float *X, const float *Jl, int s, int c0, int t0, int c1, int t1, int lr,
+                                    int lk, fx4 (&acc)[16]) {
+#pragma unroll
+    for (int kk = 0; kk < CSB / 4; kk++) {
+        const int r = 4 * kk + lk, k = CSB * s + r;
+        const float b0 = H0 && t0 == s ? Jl[s * CSTAGE + c0 * CLD + r] : 0.0f;
+        const float b1 = H1 && t1 == s ? Jl[s * CSTAGE + c1 * CLD + r] : 0.0f;
+#pragma unroll
+        for (int tm = 0; tm < CTM; tm++) {
+            const int m = 16 * tm + lr;
+            const float a = m < CBL ? X[k * CXL + m] : 0.0f;
+            if (H0) acc[2 * tm] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc[2 * tm], 0, 0, 0);
+            if (H1) acc[2 * tm + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc[2 * tm + 1], 0, 0, 0);
+        }
+    }
+}
+
+// acc[2 tm + h] += tile (tm, w + 4 h) of X BD(J) (x_times_bd on FP32)
+__device__ inline void x_times_bd32(const float *X, const float *Jl, fx4 (&acc)[16]) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int n0 = 16 * wv + lr, n1 = 16 * (wv + 4) + lr;
+    const bool v1 = n1 < CBL;
+    const int t0 = n0 / CSB, c0 = n0 - t0 * CSB;
+    const int t1 = v1 ? n1 / CSB : 3, c1 = v1 ? n1 - t1 * CSB : 0;
+    for (int s = 0; s < 3; s++) {
+        const bool h0 = (16 * wv) / CSB <= s && s <= (16 * wv + 15) / CSB;
+        const bool h1 = (16 * (wv + 4)) / CSB <= s && s <= (16 * (wv + 4) + 15) / CSB;
+        if (h0 && h1) x_bd_stage32<true, true>(X, Jl, s, c0, t0, c1, t1, lr, lk, acc);
+        else if (h0) x_bd_stage32<true, false>(X, Jl, s, c0, t0, c1, t1, lr, lk, acc);
+        else if (h1) x_bd_stage32<false, true>(X, Jl, s, c0, t0, c1, t1, lr, lk, acc);
+    }
+}
+
+// k_cdilu_schur with FP32 products (T = float or bf16)
+template <class T>
+__global__ __launch_bounds__(FWG, 2) void k_cdilu_schur32(RowLists L, CoupledSrc src, T *__restrict__ F,
+                                                         float *__restrict__ scale, double *__restrict__ DD64) {
+    extern __shared__ float lds_f[];
+    float *X = lds_f;
+    float *Jl = X + CBL * CXL;
+    __shared__ double red[4];
+    const int wv = threadIdx.x >> 6;
+    const int i = L.rows[blockIdx.x];
+    fx4 accY[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) accY[j] = fx4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int q0 = L.lptr[i], q1 = L.lptr[i + 1];
+    for (int q = q0; q < q1; q++) {
+        const int k = L.lcol[q], st = L.ltrans[q];
+        __syncthreads();   // X and Jl free
+        coupled_stage_to_lds32(Jl, src, L.lslot[q]);   // J_ik's stage blocks
+        coupled_block_to_lds32(X, DD64 + (size_t)k * CBB);
+        __syncthreads();
+        coupled_mix_rows32(X, src);   // W = (coef x I) D_k^-1
+        __syncthreads();
+        fx4 acc[16];
+        bd_times_x32(Jl, X, acc);     // L_ik = BD(J_ik) W
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 16; j++) tile_to_lds32(X, j >> 1, wv + 4 * (j & 1), acc[j]);
+        coupled_stage_to_lds32(Jl, src, st);   // J_ki's
+        __syncthreads();
+        const int sl = L.lslot[q];
+        wg_lds_block_store<T, CBL>(X, F + (size_t)L.pos[sl] * CBB, scale ? scale + L.pos[sl] : nullptr, red);
+        x_times_bd32(X, Jl, accY);    // accY += L_ik BD(J_ki)
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; j++) tile_to_lds32(X, j >> 1, wv + 4 * (j & 1), accY[j]);
+    __syncthreads();
+    const int d = L.diag[i];
+    const int bs = src.bs;
+    double *S = DD64 + (size_t)i * CBB;
+    wg_batched<(int)CBB>(
+        [&](int e) {
+            const int col = e / CBL, m = e - col * CBL;
+            const int st = m / CSB, r = m - st * CSB, c = col % CSB;
+            const double *J = st == 0 ? src.J0 : st == 1 ? src.J1 : src.J2;
+            return double2{J[(size_t)d * CSB * CSB + c * CSB + r], src.mass[(size_t)i * bs * bs + (c % bs) * bs + (r % bs)]};
+        },
+        [&](int e, double2 a) {
+            const int col = e / CBL, m = e - col * CBL;
+            const int st = m / CSB, r = m - st * CSB, t = col / CSB, c = col - t * CSB;
+            S[e] = fma(coupled_coef(src, st, t), a.x, st == t && r / bs == c / bs ? a.y : 0.0) - coupled_mix_col32(X, src, e);
+        });
+}
+
+// k_cdilu_upper with FP32 products (T = float or bf16)
+template <class T>
+__global__ __launch_bounds__(FWG, 2) void k_cdilu_upper32(const int *__restrict__ urows, const int *__restrict__ uptr,
+                                                         const int *__restrict__ uslot, const int *__restrict__ pos,
+                                                         CoupledSrc src, const double *__restrict__ DD64,
+                                                         T *__restrict__ F, float *__restrict__ scale) {
+    extern __shared__ float lds_f[];
+    float *X = lds_f;
+    float *Jl = X + CBL * CXL;
+    __shared__ double red[4];
+    const int wv = threadIdx.x >> 6;
+    const int i = urows[blockIdx.x];
+    double2 nxt[CSU];
+    const int q0 = uptr[i], q1 = uptr[i + 1];
+    if (q0 < q1) coupled_stage_load(src, uslot[q0], nxt);
+    for (int q = q0; q < q1; q++) {
+        const int sl = uslot[q];
+        __syncthreads();   // X, Jl and red free
+        coupled_stage_store32(Jl, nxt);
+        coupled_block_to_lds32(X, DD64 + (size_t)i * CBB);
+        __syncthreads();
+        if (q + 1 < q1) coupled_stage_load(src, uslot[q + 1], nxt);
+        fx4 acc[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc[j] = fx4{0.0f, 0.0f, 0.0f, 0.0f};
+        x_times_bd32(X, Jl, acc);   // Y = D_i^-1 BD(J_ij)
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 16; j++) tile_to_lds32(X, j >> 1, wv + 4 * (j & 1), acc[j]);
+        __syncthreads();
+        coupled_mix_cols32(X, src);   // DU_ij = Y (coef x I)
         __syncthreads();
         wg_lds_block_store<T, CBL>(X, F + (size_t)pos[sl] * CBB, scale ? scale + pos[sl] : nullptr, red);
     }
@@ -2019,9 +2266,22 @@ void factor_wf_t(cpg_prec_s &P, const cpg_bsr_s &A, hipStream_t s) {
 
 // block DILU of the stage-coupled system: per level the Schur and inverse kernels, then the DU
 // blocks of every row
+// FP32 and BF16 factors take the FP32-product kernels unless CPOLYG_COUPLED_FP64_PRODUCTS=1 (read
+// at every factor call)
+bool coupled_fp64_products() {
+    const char *e = std::getenv("CPOLYG_COUPLED_FP64_PRODUCTS");
+    return e && e[0] == '1';
+}
+
 template <class T>
 void factor_coupled_t(cpg_prec_s &P, const CoupledSrc &src, hipStream_t s) {
     float *sc = P.precision == CPG_BF16 ? P.scale.p : nullptr;
+    bool f32 = false;
+    if constexpr (!std::is_same<T, double>::value) f32 = !coupled_fp64_products();
+    if constexpr (!std::is_same<T, double>::value) {
+        set_lds(k_cdilu_schur32<T>, CLDS32);
+        set_lds(k_cdilu_upper32<T>, CLDS32);
+    }
     set_lds(k_cdilu_schur<T>, CLDS);
     set_lds(k_cdilu_upper<T>, CLDS);
     RowLists L = P.lists(true);
@@ -2030,15 +2290,24 @@ void factor_coupled_t(cpg_prec_s &P, const CoupledSrc &src, hipStream_t s) {
         if (!nr) continue;
         RowLists Ll = L;
         Ll.rows = P.frows.p + P.flev[lv];
-        hipLaunchKernelGGL(k_cdilu_schur<T>, dim3(nr), dim3(FWG), CLDS, s, Ll, src, Fp<T>(P), sc, P.DD64.p);
+        if constexpr (!std::is_same<T, double>::value) {
+            if (f32) hipLaunchKernelGGL(k_cdilu_schur32<T>, dim3(nr), dim3(FWG), CLDS32, s, Ll, src, Fp<T>(P), sc, P.DD64.p);
+        }
+        if (!f32) hipLaunchKernelGGL(k_cdilu_schur<T>, dim3(nr), dim3(FWG), CLDS, s, Ll, src, Fp<T>(P), sc, P.DD64.p);
         CPG_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_cdilu_inv<T>, dim3(nr), dim3(WG), 0, s, Ll.rows, P.diag.p, P.pos.p, P.DD64.p, Fp<T>(P), sc,
                            P.status.p);
         CPG_LAUNCH_CHECK();
     }
     if (P.nup > 0) {
-        hipLaunchKernelGGL(k_cdilu_upper<T>, dim3(P.nurows), dim3(FWG), CLDS, s, P.urows.p, P.uptr.p, P.uslot.p,
-                           P.pos.p, src, P.DD64.p, Fp<T>(P), sc);
+        if constexpr (!std::is_same<T, double>::value) {
+            if (f32)
+                hipLaunchKernelGGL(k_cdilu_upper32<T>, dim3(P.nurows), dim3(FWG), CLDS32, s, P.urows.p, P.uptr.p,
+                                   P.uslot.p, P.pos.p, src, P.DD64.p, Fp<T>(P), sc);
+        }
+        if (!f32)
+            hipLaunchKernelGGL(k_cdilu_upper<T>, dim3(P.nurows), dim3(FWG), CLDS, s, P.urows.p, P.uptr.p, P.uslot.p,
+                               P.pos.p, src, P.DD64.p, Fp<T>(P), sc);
         CPG_LAUNCH_CHECK();
     }
 }

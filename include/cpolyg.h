@@ -117,7 +117,10 @@ int cpg_prec_apply_to(cpg_prec P, const double *in, double *out, void *stream);
  * (src/TimeIntegration.cpp:233-245), on the interleaved ordering (a different DILU from the
  * stage-major one) and without storing the coupled matrix.  P: kind CPG_PC_DILU created over a
  * BSR of block size 120 with the stage Jacobians' pattern; J0..J2: that pattern at block size
- * 40; coef: 9 host doubles (-dt A for the Newton matrix M - dt (A x J)); synchronous. */
+ * 40; coef: 9 host doubles (-dt A for the Newton matrix M - dt (A x J)); synchronous.  FP32 and
+ * BF16 factors form their Schur and DU products on FP32 MFMA (FP32 factor blocks within about
+ * 3e-6 of the FP64 factor's); the environment variable CPOLYG_COUPLED_FP64_PRODUCTS=1, read at
+ * each call, selects FP64 products (within 1e-6).  The inverses are FP64 in every precision. */
 int cpg_prec_factor_coupled(cpg_prec P, cpg_equation E, cpg_bsr J0, cpg_bsr J1, cpg_bsr J2, const double *coef,
                             void *stream);
 /* factors converted to FP64 in A's slot layout: diag = D^-1, lower = L, upper = DU
