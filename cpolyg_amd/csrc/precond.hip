@@ -1722,15 +1722,21 @@ __device__ inline void coupled_mix_rows32(float *X, const CoupledSrc &src) {
     }
 }
 
-__device__ inline void coupled_mix_cols32(float *Y, const CoupledSrc &src) {
+// Y <- Y (coef x I); returns this thread's max of |mixed value|
+__device__ inline double coupled_mix_cols32(float *Y, const CoupledSrc &src) {
+    double pm = 0.0;
     for (int e = threadIdx.x; e < CSB * CBL; e += FWG) {
         const int c = e / CBL, m = e - c * CBL;
         float *y = Y + (size_t)c * CXL + m;
         const double y0 = y[0], y1 = y[(size_t)CSB * CXL], y2 = y[(size_t)2 * CSB * CXL];
 #pragma unroll
-        for (int t = 0; t < 3; t++)
-            y[(size_t)t * CSB * CXL] = (float)fma(src.coef[t], y0, fma(src.coef[3 + t], y1, src.coef[6 + t] * y2));
+        for (int t = 0; t < 3; t++) {
+            const float v = (float)fma(src.coef[t], y0, fma(src.coef[3 + t], y1, src.coef[6 + t] * y2));
+            y[(size_t)t * CSB * CXL] = v;
+            pm = fmax(pm, fabs((double)v));
+        }
     }
+    return pm;
 }
 
 __device__ inline double coupled_mix_col32(const float *Y, const CoupledSrc &src, int e) {
@@ -1739,6 +1745,27 @@ __device__ inline double coupled_mix_col32(const float *Y, const CoupledSrc &src
     return fma(coupled_coef(src, 0, t), (double)y[0],
                fma(coupled_coef(src, 1, t), (double)y[(size_t)CSB * CXL],
                    coupled_coef(src, 2, t) * (double)y[(size_t)2 * CSB * CXL]));
+}
+
+// wg_lds_block_store with the BF16 block max given: pm is this thread's max of |stored value| over
+// its share of the block (taken where the values were formed, which saves a pass over X)
+template <class T>
+__device__ inline void coupled_store32(const float *X, T *__restrict__ g, float *scale_out, double *red, double pm) {
+    using W = WF<CBL>;
+    float inv = 1.0f;
+    if constexpr (Store<T>::prec == CPG_BF16) {
+        pm = wave_max(pm);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = pm;
+        __syncthreads();
+        const double mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        const float sc = pow2_scale(mx);
+        inv = 1.0f / sc;
+        if (threadIdx.x == 0) *scale_out = sc;
+    }
+    for (int e = threadIdx.x; e < W::BB; e += FWG) {
+        const int c = e / CBL, r = e - c * CBL;
+        Store<T>::put(g, e, (double)X[c * W::LD + r], inv);
+    }
 }
 
 // tile (tm, tn) of an FP32 16x16x4 accumulator to X
@@ -1838,13 +1865,20 @@ __global__ __launch_bounds__(FWG, 2) void k_cdilu_schur32(RowLists L, CoupledSrc
         __syncthreads();
         fx4 acc[16];
         bd_times_x32(Jl, X, acc);     // L_ik = BD(J_ik) W
+        double pm = 0.0;   // BF16 scale: the block max from the accumulators (padding holds zeros)
+        if constexpr (Store<T>::prec == CPG_BF16) {
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) pm = fmax(pm, fabs((double)acc[j][e]));
+        }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < 16; j++) tile_to_lds32(X, j >> 1, wv + 4 * (j & 1), acc[j]);
         coupled_stage_to_lds32(Jl, src, st);   // J_ki's
         __syncthreads();
         const int sl = L.lslot[q];
-        wg_lds_block_store<T, CBL>(X, F + (size_t)L.pos[sl] * CBB, scale ? scale + L.pos[sl] : nullptr, red);
+        coupled_store32<T>(X, F + (size_t)L.pos[sl] * CBB, scale ? scale + L.pos[sl] : nullptr, red, pm);
         x_times_bd32(X, Jl, accY);    // accY += L_ik BD(J_ki)
     }
     __syncthreads();
@@ -1898,9 +1932,9 @@ __global__ __launch_bounds__(FWG, 2) void k_cdilu_upper32(const int *__restrict_
 #pragma unroll
         for (int j = 0; j < 16; j++) tile_to_lds32(X, j >> 1, wv + 4 * (j & 1), acc[j]);
         __syncthreads();
-        coupled_mix_cols32(X, src);   // DU_ij = Y (coef x I)
+        const double pm = coupled_mix_cols32(X, src);   // DU_ij = Y (coef x I)
         __syncthreads();
-        wg_lds_block_store<T, CBL>(X, F + (size_t)pos[sl] * CBB, scale ? scale + pos[sl] : nullptr, red);
+        coupled_store32<T>(X, F + (size_t)pos[sl] * CBB, scale ? scale + pos[sl] : nullptr, red, pm);
     }
 }
 
