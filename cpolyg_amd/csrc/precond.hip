@@ -1711,6 +1711,26 @@ __device__ inline void coupled_block_to_lds32(float *X, const double *__restrict
                              });
 }
 
+// X <- D^-1 of the row whose diagonal slot is pd (FP32): FP32 factors hold it rounded to FP32 in
+// that slot, the same values as rounding DD64's, at half the bytes; BF16 factors round DD64's
+template <class T>
+__device__ inline void coupled_dinv_to_lds32(float *X, const double *__restrict__ DD64, const T *F, int row, int pd) {
+    if constexpr (std::is_same<T, float>::value) {
+        const float *g = F + (size_t)pd * CBB;
+        wg_batched<(int)CBB / 4>([&](int u) { return *reinterpret_cast<const float4 *>(g + 4 * u); },
+                                 [&](int u, float4 a) {
+                                     const int c = (4 * u) / CBL, r = 4 * u - c * CBL;
+                                     float *x = X + c * CXL + r;
+                                     x[0] = a.x;
+                                     x[1] = a.y;
+                                     x[2] = a.z;
+                                     x[3] = a.w;
+                                 });
+    } else {
+        coupled_block_to_lds32(X, DD64 + (size_t)row * CBB);
+    }
+}
+
 __device__ inline void coupled_mix_rows32(float *X, const CoupledSrc &src) {
     for (int e = threadIdx.x; e < CSB * CBL; e += FWG) {
         const int n = e / CSB, r = e - n * CSB;
@@ -1859,7 +1879,7 @@ __global__ __launch_bounds__(FWG, 2) void k_cdilu_schur32(RowLists L, CoupledSrc
         const int k = L.lcol[q], st = L.ltrans[q];
         __syncthreads();   // X and Jl free
         coupled_stage_to_lds32(Jl, src, L.lslot[q]);   // J_ik's stage blocks
-        coupled_block_to_lds32(X, DD64 + (size_t)k * CBB);
+        coupled_dinv_to_lds32<T>(X, DD64, F, k, L.pos[L.diag[k]]);
         __syncthreads();
         coupled_mix_rows32(X, src);   // W = (coef x I) D_k^-1
         __syncthreads();
@@ -1906,8 +1926,9 @@ __global__ __launch_bounds__(FWG, 2) void k_cdilu_schur32(RowLists L, CoupledSrc
 template <class T>
 __global__ __launch_bounds__(FWG, 2) void k_cdilu_upper32(const int *__restrict__ urows, const int *__restrict__ uptr,
                                                          const int *__restrict__ uslot, const int *__restrict__ pos,
-                                                         CoupledSrc src, const double *__restrict__ DD64,
-                                                         T *__restrict__ F, float *__restrict__ scale) {
+                                                         const int *__restrict__ diag, CoupledSrc src,
+                                                         const double *__restrict__ DD64, T *__restrict__ F,
+                                                         float *__restrict__ scale) {
     extern __shared__ float lds_f[];
     float *X = lds_f;
     float *Jl = X + CBL * CXL;
@@ -1921,7 +1942,7 @@ __global__ __launch_bounds__(FWG, 2) void k_cdilu_upper32(const int *__restrict_
         const int sl = uslot[q];
         __syncthreads();   // X, Jl and red free
         coupled_stage_store32(Jl, nxt);
-        coupled_block_to_lds32(X, DD64 + (size_t)i * CBB);
+        coupled_dinv_to_lds32<T>(X, DD64, F, i, pos[diag[i]]);
         __syncthreads();
         if (q + 1 < q1) coupled_stage_load(src, uslot[q + 1], nxt);
         fx4 acc[16];
@@ -2337,7 +2358,7 @@ void factor_coupled_t(cpg_prec_s &P, const CoupledSrc &src, hipStream_t s) {
         if constexpr (!std::is_same<T, double>::value) {
             if (f32)
                 hipLaunchKernelGGL(k_cdilu_upper32<T>, dim3(P.nurows), dim3(FWG), CLDS32, s, P.urows.p, P.uptr.p,
-                                   P.uslot.p, P.pos.p, src, P.DD64.p, Fp<T>(P), sc);
+                                   P.uslot.p, P.pos.p, P.diag.p, src, P.DD64.p, Fp<T>(P), sc);
         }
         if (!f32)
             hipLaunchKernelGGL(k_cdilu_upper<T>, dim3(P.nurows), dim3(FWG), CLDS, s, P.urows.p, P.uptr.p, P.uslot.p,
